@@ -1,0 +1,108 @@
+/*
+ * CPU oracle for the Binius64 B128 RS-encode + Merkle-commit + FRI-fold path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/, the
+ * smoke() entry and bench.py's CPU-baseline leg may load it.  The product
+ * (binius64_amd / libb64hip.so) never links, calls or falls back to it.
+ *
+ * Every function restates the algorithm of the reference file:line it cites
+ * (paths relative to the reference repository root).  Field elements cross the
+ * ABI as pairs of little-endian u64 words (lo, hi), i.e. the u128 LE byte order
+ * of verifier/utils/src/serialization.rs:66-72.
+ */
+#ifndef B64_ORACLE_H
+#define B64_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bo_domain bo_domain;
+
+/* ---- field ------------------------------------------------------------- */
+/* variant: 0 = PCLMUL (arch/shared/ghash.rs:19-45), 1 = portable BearSSL
+ * (arch/portable/packed_ghash_128.rs:33-97), 2 = bit-serial textbook. */
+void bo_gf_mul_many(const uint64_t *a, const uint64_t *b, uint64_t *out, size_t n, int variant);
+void bo_gf_square_many(const uint64_t *a, uint64_t *out, size_t n);
+void bo_gf_inv_many(const uint64_t *a, uint64_t *out, size_t n);
+void bo_gf_mul_x_many(const uint64_t *a, uint64_t *out, size_t n);
+void bo_gf_mul_inv_x_many(const uint64_t *a, uint64_t *out, size_t n);
+void bo_gf_pow(const uint64_t a[2], const uint64_t e[2], uint64_t out[2]);
+
+/* ---- domain / twiddles (verifier/math/src/ntt/domain_context.rs) -------- */
+bo_domain *bo_domain_new(const uint64_t *basis, int dim);
+bo_domain *bo_domain_new_standard(int dim);
+void bo_domain_free(bo_domain *d);
+int bo_domain_dim(const bo_domain *d);
+/* evals[i][k] (k < dim - i) written at out[2*(i*dim + k)] */
+void bo_domain_evals(const bo_domain *d, uint64_t *out);
+void bo_domain_twiddle(const bo_domain *d, int layer, uint64_t block, uint64_t out[2]);
+/* basis of subspace(i): i elements */
+void bo_domain_subspace(const bo_domain *d, int i, uint64_t *out);
+
+/* ---- NTT ----------------------------------------------------------------- */
+int bo_ntt_forward_ref(const bo_domain *d, uint64_t *data, int log_d, int skip_early, int skip_late);
+int bo_ntt_inverse_ref(const bo_domain *d, uint64_t *data, int log_d, int skip_early, int skip_late);
+int bo_ntt_forward_mt(const bo_domain *d, uint64_t *data, int log_d, int skip_early, int skip_late,
+                      int log_num_shares, int nthreads);
+
+/* ---- RS encode (verifier/math/src/reed_solomon.rs:128-201) -------------- */
+int bo_encode_batch(const bo_domain *d, const uint64_t *msg, int log_dim, int log_inv_rate,
+                    int log_batch, uint64_t *out, int nthreads);
+
+/* ---- hashing ------------------------------------------------------------- */
+void bo_sha256(const uint8_t *data, size_t len, uint8_t out[32]);
+void bo_sha256_compress_block(const uint32_t iv[8], const uint8_t block[64], uint32_t state_out[8]);
+void bo_binius_compress(const uint8_t left[32], const uint8_t right[32], uint8_t out[32]);
+/* tree_out holds (2*n_leaves - 1) digests; returns 0 or an error code */
+int bo_merkle_build(const uint64_t *elems, size_t n_elems, size_t batch, uint8_t *tree_out, int nthreads);
+int bo_merkle_branch(const uint8_t *tree, int log_len, size_t index, int layer_depth, uint8_t *out);
+
+/* ---- FRI ----------------------------------------------------------------- */
+size_t bo_n_test_queries(int security_bits, int log_inv_rate);
+int bo_estimate_optimal_arity(int log_block_length, int digest_size, int field_size);
+/* Verifier::setup -> FRIParams::choose_with_constant_fold_arity.  Writes
+ * [log_batch, log_dim, rs_log_len, n_queries, n_arities, arities...] into out
+ * (cap >= 8 + 64). */
+int bo_fri_params_setup(int log_msg_len, int log_inv_rate, int security_bits, int *out);
+void bo_eq_ind(const uint64_t *point, int n, uint64_t *out);
+int bo_fold_interleaved(const uint64_t *cw, int log_len, int log_batch, const uint64_t *chal,
+                        uint64_t *out, int nthreads);
+int bo_fold_codeword(const bo_domain *d, const uint64_t *cw, int log_len, const uint64_t *chal,
+                     int n_chal, uint64_t *out, int nthreads);
+void bo_evaluate_mle(const uint64_t *evals, int log_n, const uint64_t *point, uint64_t out[2]);
+void bo_pack_witness(const uint64_t *words, size_t n_elems, uint64_t *out);
+
+/* Full FRI-only prover flow (prover/prover/src/fri/tests.rs:58-89 with
+ * FRIFoldProver, fri/fold.rs:117-244, and FRIQueryProver, fri/query.rs:51-134).
+ * challenges_in == NULL: challenges are sampled from the transcript; otherwise
+ * the given log_msg_len challenges are used (they are still not observed).
+ * Outputs: the proof tape, every committed root, every challenge used, the
+ * terminal codeword. Returns 0 on success. */
+int bo_fri_prove(const bo_domain *d, const uint64_t *msg, int log_dim, int log_inv_rate,
+                 int log_batch, const int *arities, int n_arities, int n_queries,
+                 const uint64_t *challenges_in, uint8_t *tape, size_t tape_cap,
+                 size_t *tape_len, uint8_t *roots_out, uint64_t *challenges_out,
+                 uint64_t *terminal_out, int nthreads);
+
+/* ---- transcript primitives (verifier/transcript/src) --------------------- */
+typedef struct bo_transcript bo_transcript;
+bo_transcript *bo_transcript_new(void);
+void bo_transcript_free(bo_transcript *t);
+void bo_transcript_message(bo_transcript *t, const uint8_t *bytes, size_t n);
+void bo_transcript_observe(bo_transcript *t, const uint8_t *bytes, size_t n);
+void bo_transcript_decommit(bo_transcript *t, const uint8_t *bytes, size_t n);
+void bo_transcript_sample(bo_transcript *t, uint8_t *out, size_t n);
+uint32_t bo_transcript_sample_bits(bo_transcript *t, int bits);
+size_t bo_transcript_tape(const bo_transcript *t, uint8_t *out, size_t cap);
+
+/* ---- seeded input generator (SplitMix64; fixed for the whole repository) -- */
+void bo_splitmix_fill(uint64_t seed, uint64_t *out, size_t n_words);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
