@@ -1,0 +1,190 @@
+"""Benchmark: RS-encode + Merkle commit + first FRI fold of one B128 job on one GPU.
+
+Workload (one "step"): a 2^log_n B128 message resident in HBM is Reed-Solomon
+encoded at rate 1/2 with the interleaved batch 2^4 chosen by Verifier::setup
+(codeword 2^(log_n+1) elements), committed with the SHA-256 Merkle tree (16-element
+leaves), and folded once by the interleaved FRI fold with 4 pre-seeded
+challenges.  Default log_n = 24 (256 MiB message, 512 MiB codeword).
+
+Prints one JSON line.  `value` is message GB/s (16 * 2^log_n bytes / step time),
+the convention of the reference's pcs/ntt benches.  `roofline` covers the
+dominant kernel (the NTT pass), measured live with HIP events on the stream the
+kernels run on.  `cpu_baseline` times the CPU oracle (a restatement of the
+reference's rayon algorithm) on the host cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log-n", type=int, default=24)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-log-n", type=int, default=24, help="CPU baseline sample size")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is a design note, not a measured path)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import binius64_amd as b64
+
+    L = b64.lib()
+    if b64.device_count() == 0:
+        raise SystemExit("no HIP device")
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    sp = C.c_void_p(stream.cuda_stream)
+
+    n = args.log_n
+    params, dom = b64.setup_commit_params(n, 1, 96)
+    lbs, log_dim, lir = params.log_batch_size, params.log_dim, params.log_inv_rate
+    rs_log_len = params.rs_log_len()
+    log_cw = rs_log_len + lbs
+    g = torch.Generator(device=dev)
+    g.manual_seed(24)
+    msg = torch.randint(-(2 ** 63), 2 ** 63 - 1, (2 << n,), dtype=torch.int64, device=dev, generator=g)
+    cw = torch.empty((2 << log_cw,), dtype=torch.int64, device=dev)
+    tree = torch.empty((4 * ((2 << rs_log_len) - 1),), dtype=torch.int64, device=dev)
+    folded = torch.empty((2 << rs_log_len,), dtype=torch.int64, device=dev)
+    chal = np.random.default_rng(4).integers(0, 2 ** 63, size=(lbs, 2), dtype=np.uint64)
+    chal_p = chal.ctypes.data_as(C.POINTER(C.c_uint64))
+
+    def step():
+        b64.check(L.b64_rs_encode_batch(dom.ptr, log_dim, lir, msg.data_ptr(), lbs, cw.data_ptr(), sp), "encode")
+        b64.check(L.b64_merkle_commit(cw.data_ptr(), 1 << log_cw, 1 << lbs, tree.data_ptr(), None, sp), "merkle")
+        b64.check(L.b64_fri_fold_interleaved(cw.data_ptr(), rs_log_len, lbs, chal_p, folded.data_ptr(), sp), "fold")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    t0.record(stream)
+    for _ in range(args.steps):
+        step()
+    t1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = t0.elapsed_time(t1) / args.steps
+    msg_bytes = 16 << n
+    value = msg_bytes / (ms * 1e-3) / 1e9
+
+    # ---- per-stage timing (HIP events on the same stream) -------------------
+    def timed(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    scratch = cw.clone()
+    n_layers = rs_log_len - lir  # layers lir .. log_cw - lbs
+    max_k = 7
+    n_passes = (n_layers + max_k - 1) // max_k
+    stages = {
+        "encode_ms": timed(lambda: b64.check(L.b64_rs_encode_batch(dom.ptr, log_dim, lir, msg.data_ptr(), lbs,
+                                                                   cw.data_ptr(), sp))),
+        "ntt_ms": timed(lambda: b64.check(L.b64_ntt_forward(dom.ptr, scratch.data_ptr(), log_cw, lir, lbs, sp))),
+        "merkle_ms": timed(lambda: b64.check(L.b64_merkle_commit(cw.data_ptr(), 1 << log_cw, 1 << lbs,
+                                                                 tree.data_ptr(), None, sp))),
+        "fold_ms": timed(lambda: b64.check(L.b64_fri_fold_interleaved(cw.data_ptr(), rs_log_len, lbs, chal_p,
+                                                                      folded.data_ptr(), sp))),
+    }
+    ntt_pass_ms = stages["ntt_ms"] / n_passes
+    pass_bytes = 2 * (16 << log_cw)  # each pass reads and writes the codeword once
+    achieved = pass_bytes / (ntt_pass_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "k_ntt_pass", "avg_launch_ms": round(ntt_pass_ms, 4),
+                "algorithmic_bytes_per_launch": pass_bytes}
+    step_bytes = 58 * (1 << n)  # algorithmic HBM bytes of the whole step
+    path_achieved = step_bytes / (ms * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.cpu_log_n)
+
+    out = {
+        "metric": "RS-encode+commit GB/s (B128, log_n=24, rate 1/2)",
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "b128",
+        "data": "synthetic (uniform random B128 message, seeded)",
+        "config": {"workload": f"encode+merkle+interleaved-fold, log_n={n}, rate 1/2, batch 2^{lbs}",
+                   "log_n": n, "log_inv_rate": lir, "log_batch_size": lbs, "rs_log_len": rs_log_len,
+                   "fold_arities": params.fold_arities, "n_test_queries": params.n_test_queries},
+        "roofline": roofline,
+        "path_hbm": {"algorithmic_bytes_per_step": step_bytes, "achieved_GBs": round(path_achieved, 2),
+                     "frac": round(path_achieved / HBM_PEAK_GBS, 5)},
+        "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+        "wall_s_timed_region": round(wall, 4),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(log_n: int):
+    """The oracle's multithreaded restatement of NeighborsLastMultiThread +
+    BinaryMerkleTreeProver + fold_interleaved on the host cores, on a bounded
+    sample of the same workload (log_n elements instead of 2^24)."""
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    cores = min(16, os.cpu_count() or 1)
+    p = orc.fri_params_setup(log_n, 1, 96)
+    std = orc.Domain(dim=log_n + 1)
+    dom = orc.Domain(std.subspace(p["rs_log_len"]))
+    msg = orc.random_elems(22, 1 << log_n)
+    chal = orc.random_elems(23, p["log_batch_size"])
+    lbs = p["log_batch_size"]
+    reps, times = 2, []
+    for _ in range(reps):
+        t = time.perf_counter()
+        cw = orc.encode_batch(dom, msg, p["log_dim"], 1, lbs, threads=cores)
+        orc.merkle_build(cw, 1 << lbs, threads=cores)
+        orc.fold_interleaved(cw, p["rs_log_len"], lbs, chal, threads=cores)
+        times.append(time.perf_counter() - t)
+    best = min(times)
+    return {"value": round((16 << log_n) / best / 1e9, 4), "unit": "GB/s", "cores": cores, "kind": "port",
+            "sample": f"log_n={log_n} (2^{log_n} B128 message, same encode+merkle+fold step), best of {reps}",
+            "seconds": round(best, 3)}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,461 @@
+"""binius64_amd: MI355X implementation of Binius64's B128 polynomial-commitment
+hot path (additive NTT / Reed-Solomon encode, SHA-256 Merkle commit, FRI folds).
+
+This Python layer is a thin host mirror over the C ABI of libb64hip.so
+(include/b64hip.h).  Names follow the reference's Rust interfaces:
+  DomainContext / GenericPreExpanded  verifier/math/src/ntt/{mod.rs,domain_context.rs}
+  AdditiveNTT                         verifier/math/src/ntt/mod.rs:58-113
+  ReedSolomonCode.encode_batch        verifier/math/src/reed_solomon.rs:128-201
+  BinaryMerkleTreeProver              prover/prover/src/merkle_tree/prover.rs
+  FRIParams                           verifier/verifier/src/fri/common.rs
+  commit_interleaved                  prover/prover/src/fri/commit.rs:25-75
+  FRIFoldProver                       prover/prover/src/fri/fold.rs
+  ProverTranscript                    verifier/transcript/src/transcript.rs
+Field elements on the host are numpy uint64 arrays of shape (n, 2) = (lo, hi).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from ._abi import B64Error, FriParamsC, LIB_PATH, check, lib
+
+__all__ = [
+    "B64Error", "DeviceBuffer", "DomainContext", "AdditiveNTT", "ReedSolomonCode",
+    "BinaryMerkleTreeProver", "FRIParams", "CommitOutput", "commit_interleaved", "setup_commit_params",
+    "ProverTranscript", "FRIFoldProver", "gf_mul", "gf_inv", "gf_mul_device", "gf_mul_const_device",
+    "pack_witness", "device_count", "LIB_PATH", "lib",
+]
+
+
+def _p64(a: np.ndarray):
+    assert a.dtype == np.uint64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+def _p8(a: np.ndarray):
+    assert a.dtype == np.uint8 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def _stream(stream) -> Optional[int]:
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return int(getattr(stream, "cuda_stream", stream))
+
+
+def device_count() -> int:
+    return lib().b64_device_count()
+
+
+def _elems(x) -> np.ndarray:
+    a = np.ascontiguousarray(x, dtype=np.uint64)
+    return a.reshape(-1, 2)
+
+
+def _u128(v: int) -> np.ndarray:
+    return np.array([v & 0xFFFFFFFFFFFFFFFF, v >> 64], dtype=np.uint64)
+
+
+# ---------------------------------------------------------------------------
+class DeviceBuffer:
+    """An HBM allocation owned by the library's allocator."""
+
+    def __init__(self, nbytes: int):
+        ptr = C.c_void_p()
+        check(lib().b64_device_alloc(nbytes, C.byref(ptr)), "device_alloc")
+        self.ptr = ptr.value
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_host(cls, arr: np.ndarray, stream=None) -> "DeviceBuffer":
+        a = np.ascontiguousarray(arr)
+        buf = cls(max(a.nbytes, 16))
+        check(lib().b64_memcpy_htod(buf.ptr, a.ctypes.data, a.nbytes, _stream(stream)), "htod")
+        return buf
+
+    def to_host(self, n_elems: Optional[int] = None, stream=None) -> np.ndarray:
+        n = n_elems if n_elems is not None else self.nbytes // 16
+        out = np.zeros((n, 2), dtype=np.uint64)
+        check(lib().b64_memcpy_dtoh(out.ctypes.data, self.ptr, 16 * n, _stream(stream)), "dtoh")
+        return out
+
+    def to_bytes(self, nbytes: Optional[int] = None, stream=None) -> bytes:
+        n = nbytes if nbytes is not None else self.nbytes
+        out = np.zeros(n, dtype=np.uint8)
+        check(lib().b64_memcpy_dtoh(out.ctypes.data, self.ptr, n, _stream(stream)), "dtoh")
+        return out.tobytes()
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().b64_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+def gf_mul(a: int, b: int) -> int:
+    out = np.zeros(2, dtype=np.uint64)
+    lib().b64_gf_mul(_p64(_u128(a)), _p64(_u128(b)), _p64(out))
+    return int(out[0]) | (int(out[1]) << 64)
+
+
+def gf_inv(a: int) -> int:
+    out = np.zeros(2, dtype=np.uint64)
+    lib().b64_gf_inv(_p64(_u128(a)), _p64(out))
+    return int(out[0]) | (int(out[1]) << 64)
+
+
+def gf_mul_device(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    a, b = _elems(a), _elems(b)
+    da, db = DeviceBuffer.from_host(a), DeviceBuffer.from_host(b)
+    do = DeviceBuffer(max(a.nbytes, 16))
+    check(lib().b64_gf_mul_device(da.ptr, db.ptr, do.ptr, a.shape[0], None), "gf_mul_device")
+    return do.to_host(a.shape[0])
+
+
+def gf_mul_const_device(a: np.ndarray, c: int) -> np.ndarray:
+    a = _elems(a)
+    da = DeviceBuffer.from_host(a)
+    do = DeviceBuffer(max(a.nbytes, 16))
+    check(lib().b64_gf_mul_const_device(da.ptr, _p64(_u128(c)), do.ptr, a.shape[0], None), "gf_mul_const")
+    return do.to_host(a.shape[0])
+
+
+# ---------------------------------------------------------------------------
+class DomainContext:
+    """GenericPreExpanded domain (ntt/domain_context.rs:99-159)."""
+
+    def __init__(self, basis=None, dim: Optional[int] = None, _ptr=None):
+        if _ptr is not None:
+            self.ptr = _ptr
+        else:
+            out = C.c_void_p()
+            if basis is None:
+                check(lib().b64_domain_create_standard(dim, C.byref(out)), "domain_create_standard")
+            else:
+                b = _elems(basis)
+                check(lib().b64_domain_create(_p64(b), b.shape[0], C.byref(out)), "domain_create")
+            self.ptr = out.value
+
+    @classmethod
+    def standard(cls, dim: int) -> "DomainContext":
+        return cls(dim=dim)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().b64_domain_destroy(self.ptr)
+            self.ptr = None
+
+    def log_domain_size(self) -> int:
+        return lib().b64_domain_log_size(self.ptr)
+
+    def subspace(self, i: int) -> np.ndarray:
+        out = np.zeros((max(i, 1), 2), dtype=np.uint64)
+        check(lib().b64_domain_subspace(self.ptr, i, _p64(out)), "subspace")
+        return out[:i]
+
+    def twiddle(self, layer: int, block: int) -> int:
+        out = np.zeros(2, dtype=np.uint64)
+        check(lib().b64_domain_twiddle(self.ptr, layer, block, _p64(out)), "twiddle")
+        return int(out[0]) | (int(out[1]) << 64)
+
+
+class AdditiveNTT:
+    """The AdditiveNTT trait (ntt/mod.rs:58-113) on device buffers."""
+
+    def __init__(self, domain_context: DomainContext):
+        self.domain_context = domain_context
+
+    def log_domain_size(self) -> int:
+        return self.domain_context.log_domain_size()
+
+    def subspace(self, i: int) -> np.ndarray:
+        return self.domain_context.subspace(i)
+
+    def twiddle(self, layer: int, block: int) -> int:
+        return self.domain_context.twiddle(layer, block)
+
+    def forward_transform(self, data: DeviceBuffer, log_len: int, skip_early: int, skip_late: int, stream=None):
+        check(lib().b64_ntt_forward(self.domain_context.ptr, data.ptr, log_len, skip_early, skip_late,
+                                    _stream(stream)), "ntt_forward")
+
+    def inverse_transform(self, data: DeviceBuffer, log_len: int, skip_early: int, skip_late: int, stream=None):
+        check(lib().b64_ntt_inverse(self.domain_context.ptr, data.ptr, log_len, skip_early, skip_late,
+                                    _stream(stream)), "ntt_inverse")
+
+
+class ReedSolomonCode:
+    """ReedSolomonCode::with_domain_context_subspace (reed_solomon.rs:54-65)."""
+
+    def __init__(self, domain_context: DomainContext, log_dim: int, log_inv_rate: int):
+        self.domain_context = domain_context
+        self.log_dimension = log_dim
+        self.log_inv_rate = log_inv_rate
+
+    def log_dim(self) -> int:
+        return self.log_dimension
+
+    def log_len(self) -> int:
+        return self.log_dimension + self.log_inv_rate
+
+    def encode_batch(self, data: DeviceBuffer, log_batch_size: int, out: Optional[DeviceBuffer] = None,
+                     stream=None) -> DeviceBuffer:
+        n_out = 1 << (self.log_len() + log_batch_size)
+        out = out or DeviceBuffer(16 * n_out)
+        check(lib().b64_rs_encode_batch(self.domain_context.ptr, self.log_dimension, self.log_inv_rate, data.ptr,
+                                        log_batch_size, out.ptr, _stream(stream)), "encode_batch")
+        return out
+
+
+class BinaryMerkleTreeProver:
+    """BinaryMerkleTreeProver<B128, Sha256, Sha256Compression> (merkle_tree/prover.rs)."""
+
+    @staticmethod
+    def commit(data: DeviceBuffer, n_elems: int, batch_size: int, tree: Optional[DeviceBuffer] = None,
+               stream=None):
+        n_leaves = n_elems // batch_size if batch_size else 0
+        tree = tree or DeviceBuffer(32 * max(2 * n_leaves - 1, 1))
+        root = np.zeros(32, dtype=np.uint8)
+        check(lib().b64_merkle_commit(data.ptr, n_elems, batch_size, tree.ptr, _p8(root), _stream(stream)),
+              "merkle_commit")
+        return root.tobytes(), tree
+
+    @staticmethod
+    def layer(tree: DeviceBuffer, log_len: int, depth: int, stream=None) -> List[bytes]:
+        out = np.zeros(32 << depth, dtype=np.uint8)
+        check(lib().b64_merkle_layer(tree.ptr, log_len, depth, _p8(out), _stream(stream)), "merkle_layer")
+        b = out.tobytes()
+        return [b[32 * i:32 * i + 32] for i in range(1 << depth)]
+
+    @staticmethod
+    def branch(tree: DeviceBuffer, log_len: int, index: int, depth: int, stream=None) -> List[bytes]:
+        n = max(log_len - depth, 1)
+        out = np.zeros(32 * n, dtype=np.uint8)
+        check(lib().b64_merkle_branch(tree.ptr, log_len, index, depth, _p8(out), _stream(stream)), "merkle_branch")
+        b = out.tobytes()
+        return [b[32 * i:32 * i + 32] for i in range(log_len - depth)]
+
+
+# ---------------------------------------------------------------------------
+@dataclass
+class FRIParams:
+    log_dim: int
+    log_inv_rate: int
+    log_batch_size: int
+    n_test_queries: int
+    fold_arities: List[int] = field(default_factory=list)
+    log_terminal_dim: int = 0
+
+    @classmethod
+    def from_c(cls, c: FriParamsC) -> "FRIParams":
+        return cls(c.log_dim, c.log_inv_rate, c.log_batch_size, c.n_test_queries,
+                   [c.fold_arities[i] for i in range(c.n_fold_arities)], c.log_terminal_dim)
+
+    def to_c(self) -> FriParamsC:
+        c = FriParamsC()
+        arr = (C.c_uint32 * max(len(self.fold_arities), 1))(*self.fold_arities)
+        check(lib().b64_fri_params_new(self.log_dim, self.log_inv_rate, self.log_batch_size, arr,
+                                       len(self.fold_arities), self.n_test_queries, C.byref(c)), "fri_params_new")
+        return c
+
+    @classmethod
+    def new(cls, log_dim, log_inv_rate, log_batch_size, fold_arities, n_test_queries) -> "FRIParams":
+        p = cls(log_dim, log_inv_rate, log_batch_size, n_test_queries, list(fold_arities))
+        return cls.from_c(p.to_c())
+
+    @classmethod
+    def choose_with_constant_fold_arity(cls, domain: DomainContext, log_msg_len: int, security_bits: int,
+                                        log_inv_rate: int, arity: int) -> "FRIParams":
+        c = FriParamsC()
+        check(lib().b64_fri_params_choose(domain.ptr, log_msg_len, security_bits, log_inv_rate, arity,
+                                          C.byref(c)), "fri_params_choose")
+        return cls.from_c(c)
+
+    # accessors named as in fri/common.rs:117-150
+    def rs_log_len(self) -> int:
+        return self.log_dim + self.log_inv_rate
+
+    def log_len(self) -> int:
+        return self.rs_log_len() + self.log_batch_size
+
+    def log_msg_len(self) -> int:
+        return self.log_dim + self.log_batch_size
+
+    def n_fold_rounds(self) -> int:
+        return self.log_msg_len()
+
+    def n_oracles(self) -> int:
+        return 1 + len(self.fold_arities)
+
+    def index_bits(self) -> int:
+        return self.rs_log_len()
+
+    def n_final_challenges(self) -> int:
+        return self.log_terminal_dim
+
+
+def calculate_n_test_queries(security_bits: int, log_inv_rate: int) -> int:
+    return lib().b64_calculate_n_test_queries(security_bits, log_inv_rate)
+
+
+def estimate_optimal_arity(log_block_length: int, digest_size: int, field_size: int) -> int:
+    return lib().b64_estimate_optimal_arity(log_block_length, digest_size, field_size)
+
+
+def setup_commit_params(log_witness_elems: int, log_inv_rate: int = 1, security_bits: int = 96):
+    """Verifier::setup + Prover::setup chain (verify.rs:61-106, prove.rs:93-115)."""
+    c = FriParamsC()
+    dom = C.c_void_p()
+    check(lib().b64_setup_commit_params(log_witness_elems, log_inv_rate, security_bits, C.byref(c), C.byref(dom)),
+          "setup_commit_params")
+    return FRIParams.from_c(c), DomainContext(_ptr=dom.value)
+
+
+@dataclass
+class CommitOutput:
+    commitment: bytes
+    codeword: DeviceBuffer
+    tree: DeviceBuffer
+
+
+def commit_interleaved(params: FRIParams, domain: DomainContext, message: DeviceBuffer, stream=None) -> CommitOutput:
+    cw = DeviceBuffer(16 << params.log_len())
+    tree = DeviceBuffer(32 * ((2 << params.rs_log_len()) - 1))
+    root = np.zeros(32, dtype=np.uint8)
+    pc = params.to_c()
+    check(lib().b64_fri_commit_interleaved(C.byref(pc), domain.ptr, message.ptr, cw.ptr, tree.ptr, _p8(root),
+                                           _stream(stream)), "commit_interleaved")
+    return CommitOutput(root.tobytes(), cw, tree)
+
+
+class ProverTranscript:
+    """ProverTranscript<HasherChallenger<Sha256>> (transcript.rs:205-420)."""
+
+    def __init__(self):
+        out = C.c_void_p()
+        check(lib().b64_transcript_create(C.byref(out)), "transcript_create")
+        self.ptr = out.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().b64_transcript_destroy(self.ptr)
+            self.ptr = None
+
+    def _buf(self, data: bytes):
+        return np.frombuffer(data, dtype=np.uint8).copy() if data else np.zeros(1, dtype=np.uint8)
+
+    def message(self, data: bytes):
+        lib().b64_transcript_message(self.ptr, _p8(self._buf(data)), len(data))
+
+    def observe(self, data: bytes):
+        lib().b64_transcript_observe(self.ptr, _p8(self._buf(data)), len(data))
+
+    def decommit(self, data: bytes):
+        lib().b64_transcript_decommit(self.ptr, _p8(self._buf(data)), len(data))
+
+    def sample_bytes(self, n: int) -> bytes:
+        out = np.zeros(max(n, 1), dtype=np.uint8)
+        lib().b64_transcript_sample(self.ptr, _p8(out), n)
+        return out[:n].tobytes()
+
+    def sample(self) -> int:
+        """CanSample<B128>: 16 sampled bytes as a little-endian u128."""
+        return int.from_bytes(self.sample_bytes(16), "little")
+
+    def sample_bits(self, bits: int) -> int:
+        return lib().b64_transcript_sample_bits(self.ptr, bits)
+
+    def finalize(self) -> bytes:
+        n = lib().b64_transcript_tape(self.ptr, None, 0)
+        out = np.zeros(max(n, 1), dtype=np.uint8)
+        lib().b64_transcript_tape(self.ptr, _p8(out), n)
+        return out[:n].tobytes()
+
+
+class FRIFoldProver:
+    """FRIFoldProver (prover/prover/src/fri/fold.rs:31-245) over device-resident oracles."""
+
+    def __init__(self, params: FRIParams, domain: DomainContext, codeword: DeviceBuffer, tree: DeviceBuffer,
+                 stream=None):
+        self._keep = (domain, codeword, tree)
+        self._pc = params.to_c()
+        out = C.c_void_p()
+        check(lib().b64_fri_prover_create(C.byref(self._pc), domain.ptr, codeword.ptr, tree.ptr, _stream(stream),
+                                          C.byref(out)), "fri_prover_create")
+        self.ptr = out.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().b64_fri_prover_destroy(self.ptr)
+            self.ptr = None
+
+    def n_rounds(self) -> int:
+        return lib().b64_fri_prover_n_rounds(self.ptr)
+
+    def curr_round(self) -> int:
+        return lib().b64_fri_prover_curr_round(self.ptr)
+
+    def receive_challenge(self, challenge: int):
+        check(lib().b64_fri_prover_receive_challenge(self.ptr, _p64(_u128(challenge))), "receive_challenge")
+
+    def execute_fold_round(self) -> Optional[bytes]:
+        root = np.zeros(32, dtype=np.uint8)
+        committed = C.c_int(0)
+        check(lib().b64_fri_prover_execute_fold_round(self.ptr, _p8(root), C.byref(committed)), "fold_round")
+        return root.tobytes() if committed.value else None
+
+    def finish_proof(self, transcript: ProverTranscript):
+        check(lib().b64_fri_prover_finish_proof(self.ptr, transcript.ptr), "finish_proof")
+
+    def oracle(self, k: int) -> np.ndarray:
+        ptr, log_len, tree = C.c_void_p(), C.c_uint32(), C.c_void_p()
+        check(lib().b64_fri_prover_oracle(self.ptr, k, C.byref(ptr), C.byref(log_len), C.byref(tree)), "oracle")
+        n = 1 << log_len.value
+        out = np.zeros((n, 2), dtype=np.uint64)
+        check(lib().b64_memcpy_dtoh(out.ctypes.data, ptr.value, 16 * n, None), "dtoh")
+        return out
+
+
+def pack_witness(words: DeviceBuffer, n_words: int, log_elems: int, out: Optional[DeviceBuffer] = None,
+                 stream=None) -> DeviceBuffer:
+    out = out or DeviceBuffer(16 << log_elems)
+    check(lib().b64_pack_witness(words.ptr, n_words, log_elems, out.ptr, _stream(stream)), "pack_witness")
+    return out
+
+
+def fri_prove(params: FRIParams, domain: DomainContext, message: np.ndarray, challenges=None):
+    """FRI-only prover flow (prover/prover/src/fri/tests.rs:58-89): commit, fold
+    rounds with transcript challenges (or the given ones), finish_proof.
+    Returns (proof tape, roots, challenges)."""
+    msg = DeviceBuffer.from_host(_elems(message))
+    co = commit_interleaved(params, domain, msg)
+    prover = FRIFoldProver(params, domain, co.codeword, co.tree)
+    t = ProverTranscript()
+    t.message(co.commitment)
+    roots = [co.commitment]
+    r = prover.execute_fold_round()
+    if r is not None:
+        t.message(r)
+        roots.append(r)
+    used = []
+    for i in range(params.n_fold_rounds()):
+        ch = t.sample() if challenges is None else int(challenges[i])
+        used.append(ch)
+        prover.receive_challenge(ch)
+        r = prover.execute_fold_round()
+        if r is not None:
+            t.message(r)
+            roots.append(r)
+    prover.finish_proof(t)
+    return t.finalize(), roots, used

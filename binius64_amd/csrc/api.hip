@@ -1,0 +1,764 @@
+// C ABI of libb64hip (include/b64hip.h): domain setup, NTT / RS encode, Merkle
+// commit, FRI folds, FRI parameters, the Fiat-Shamir transcript and the FRI
+// fold / query prover.  Host orchestration is C++; all bulk work runs in the
+// gfx950 kernels of kernels.hip.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "host_util.hpp"
+#include "internal.hpp"
+
+using b64::host::u128;
+
+#define B64_VERSION "0.1.0"
+
+namespace {
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+inline u128 ld128(const uint64_t *p) { return ((u128)p[1] << 64) | p[0]; }
+inline void st128(uint64_t *p, u128 v) {
+    p[0] = (uint64_t)v;
+    p[1] = (uint64_t)(v >> 64);
+}
+inline uint4 to_u4(u128 v) {
+    return make_uint4((uint32_t)v, (uint32_t)(v >> 32), (uint32_t)(v >> 64), (uint32_t)(v >> 96));
+}
+
+int device_ok() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
+    return 1;
+}
+
+#define HIP_TRY(expr)                                                   \
+    do {                                                                \
+        hipError_t _e = (expr);                                         \
+        if (_e != hipSuccess) return _e == hipErrorOutOfMemory ? B64_ERR_OUT_OF_MEMORY : B64_ERR_HIP; \
+    } while (0)
+
+#define B64_TRY(expr)              \
+    do {                           \
+        int _rc = (expr);          \
+        if (_rc != B64_OK) return _rc; \
+    } while (0)
+
+int log2_ceil_u64(uint64_t x) {
+    int r = 0;
+    while ((1ull << r) < x) r++;
+    return r;
+}
+
+b64::CompressIV compress_iv() {
+    static b64::CompressIV iv;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        static const char tag[] = "BINIUS SHA-256 COMPRESS";
+        uint8_t d[32];
+        b64::host::Sha256::digest((const uint8_t *)tag, sizeof(tag) - 1, d);
+        // compress.rs:37-45: the digest bytes reinterpreted as native (LE) u32 words
+        for (int k = 0; k < 8; k++)
+            iv.w[k] = (uint32_t)d[4 * k] | ((uint32_t)d[4 * k + 1] << 8) | ((uint32_t)d[4 * k + 2] << 16) |
+                      ((uint32_t)d[4 * k + 3] << 24);
+    });
+    return iv;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Domain
+// ===========================================================================
+struct b64_domain {
+    int dim = 0;
+    std::vector<u128> evals;  // evals[i*dim + k], k < dim - i
+    std::vector<uint4> W;     // dim * W_STRIDE
+    uint4 *d_W = nullptr;
+    std::mutex mu;
+
+    u128 eval(int i, int k) const { return evals[(size_t)i * dim + k]; }
+    // twiddle(layer, block) = subset sum of evals[dim-1-layer][1..] (domain_context.rs:91-94,149-151)
+    u128 twiddle(int layer, uint64_t block) const {
+        u128 t = 0;
+        for (int b = 0; b < layer; b++)
+            if ((block >> b) & 1) t ^= eval(dim - 1 - layer, 1 + b);
+        return t;
+    }
+    int device_view(b64::DeviceDomain &dd) {
+        std::lock_guard<std::mutex> g(mu);
+        if (!d_W) {
+            if (!device_ok()) return B64_ERR_NO_DEVICE;
+            HIP_TRY(hipMalloc(&d_W, W.size() * sizeof(uint4)));
+            HIP_TRY(hipMemcpy(d_W, W.data(), W.size() * sizeof(uint4), hipMemcpyHostToDevice));
+        }
+        dd.dim = dim;
+        dd.W = d_W;
+        return B64_OK;
+    }
+};
+
+extern "C" {
+
+const char *b64_version(void) { return B64_VERSION; }
+
+const char *b64_status_string(int s) {
+    switch (s) {
+        case B64_OK: return "ok";
+        case B64_ERR_INVALID_ARGS: return "invalid arguments";
+        case B64_ERR_ENCODER_SUBSPACE_MISMATCH: return "the evaluation domain of the code does not match the subspace of the NTT encoder";
+        case B64_ERR_SUBSPACE_DIMENSION_MISMATCH: return "the dimension of the evaluation domain of the code does not match the parameters";
+        case B64_ERR_INCORRECT_BATCH_SIZE: return "incorrect batch size";
+        case B64_ERR_POWER_OF_TWO_LENGTH_REQUIRED: return "power of two length required";
+        case B64_ERR_EARLY_PROVER_FINISH: return "prover finished early";
+        case B64_ERR_INCORRECT_LAYER_DEPTH: return "incorrect layer depth";
+        case B64_ERR_INDEX_OUT_OF_RANGE: return "index out of range";
+        case B64_ERR_INVALID_FOLD_ARITY_SEQUENCE: return "invalid fold arity sequence";
+        case B64_ERR_BUFFER_TOO_SMALL: return "buffer too small";
+        case B64_ERR_NOT_COMMIT_ROUND: return "not a commit round";
+        case B64_ERR_NO_DEVICE: return "no HIP device";
+        case B64_ERR_HIP: return "HIP runtime error";
+        case B64_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case B64_ERR_ADMISSION_REJECTED: return "admission rejected";
+        default: return "unknown status";
+    }
+}
+
+int b64_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int b64_device_alloc(size_t bytes, void **d_ptr) {
+    if (!d_ptr) return B64_ERR_INVALID_ARGS;
+    if (!device_ok()) return B64_ERR_NO_DEVICE;
+    HIP_TRY(hipMalloc(d_ptr, bytes ? bytes : 16));
+    return B64_OK;
+}
+
+int b64_device_free(void *d_ptr) {
+    if (!d_ptr) return B64_OK;
+    HIP_TRY(hipFree(d_ptr));
+    return B64_OK;
+}
+
+int b64_memcpy_htod(void *d_dst, const void *src, size_t bytes, void *stream) {
+    if (!bytes) return B64_OK;
+    HIP_TRY(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+    HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    return B64_OK;
+}
+
+int b64_memcpy_dtoh(void *dst, const void *d_src, size_t bytes, void *stream) {
+    if (!bytes) return B64_OK;
+    HIP_TRY(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+    HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    return B64_OK;
+}
+
+int b64_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes, void *stream) {
+    if (!bytes) return B64_OK;
+    HIP_TRY(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
+    return B64_OK;
+}
+
+int b64_stream_synchronize(void *stream) {
+    HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    return B64_OK;
+}
+
+void b64_gf_mul(const uint64_t a[2], const uint64_t b[2], uint64_t out[2]) {
+    st128(out, b64::host::gf_mul(ld128(a), ld128(b)));
+}
+
+void b64_gf_inv(const uint64_t a[2], uint64_t out[2]) { st128(out, b64::host::gf_inv(ld128(a))); }
+
+int b64_gf_mul_device(const void *d_a, const void *d_b, void *d_out, uint64_t n, void *stream) {
+    if (!device_ok()) return B64_ERR_NO_DEVICE;
+    HIP_TRY(b64::launch_gf_mul((const uint4 *)d_a, (const uint4 *)d_b, (uint4 *)d_out, n, as_stream(stream)));
+    return B64_OK;
+}
+
+int b64_gf_mul_const_device(const void *d_a, const uint64_t c[2], void *d_out, uint64_t n, void *stream) {
+    if (!device_ok()) return B64_ERR_NO_DEVICE;
+    HIP_TRY(b64::launch_gf_mul_const((const uint4 *)d_a, to_u4(ld128(c)), (uint4 *)d_out, n, as_stream(stream)));
+    return B64_OK;
+}
+
+// generate_evals_from_subspace (ntt/domain_context.rs:19-46); the W table holds
+// each layer's twiddle basis (GenericPreExpanded's expansion basis, :114-132).
+int b64_domain_create(const uint64_t *basis, uint32_t dim, b64_domain **out) {
+    if (!out || dim > (uint32_t)b64::MAX_DIM || (dim && !basis)) return B64_ERR_INVALID_ARGS;
+    b64_domain *d = new (std::nothrow) b64_domain();
+    if (!d) return B64_ERR_OUT_OF_MEMORY;
+    d->dim = (int)dim;
+    d->evals.assign((size_t)dim * dim, 0);
+    for (uint32_t k = 0; k < dim; k++) d->evals[k] = ld128(basis + 2 * k);
+    for (int i = 1; i < (int)dim; i++) {
+        const int prev_len = (int)dim - (i - 1);
+        for (int k = 1; k < prev_len; k++) {
+            const u128 pk = d->eval(i - 1, k), p0 = d->eval(i - 1, 0);
+            d->evals[(size_t)i * dim + (k - 1)] = b64::host::gf_mul(pk, pk ^ p0);
+        }
+    }
+    for (int i = 0; i < (int)dim; i++) {
+        const u128 inv = b64::host::gf_inv(d->eval(i, 0));
+        for (int k = 0; k < (int)dim - i; k++)
+            d->evals[(size_t)i * dim + k] = b64::host::gf_mul(d->eval(i, k), inv);
+    }
+    d->W.assign((size_t)(dim ? dim : 1) * b64::W_STRIDE, make_uint4(0, 0, 0, 0));
+    for (int layer = 0; layer < (int)dim; layer++)
+        for (int b = 0; b < layer; b++)
+            d->W[(size_t)layer * b64::W_STRIDE + b] = to_u4(d->eval((int)dim - 1 - layer, 1 + b));
+    *out = d;
+    return B64_OK;
+}
+
+int b64_domain_create_standard(uint32_t dim, b64_domain **out) {
+    std::vector<uint64_t> basis(2 * (size_t)(dim ? dim : 1), 0);
+    for (uint32_t i = 0; i < dim; i++) st128(&basis[2 * i], (u128)1 << i);
+    return b64_domain_create(basis.data(), dim, out);
+}
+
+void b64_domain_destroy(b64_domain *d) {
+    if (!d) return;
+    if (d->d_W) (void)hipFree(d->d_W);
+    delete d;
+}
+
+uint32_t b64_domain_log_size(const b64_domain *d) { return d ? (uint32_t)d->dim : 0; }
+
+int b64_domain_subspace(const b64_domain *d, uint32_t i, uint64_t *basis_out) {
+    if (!d || i > (uint32_t)d->dim) return B64_ERR_INVALID_ARGS;
+    for (uint32_t k = 0; k < i; k++) st128(basis_out + 2 * k, d->eval(d->dim - (int)i, (int)k));
+    return B64_OK;
+}
+
+int b64_domain_twiddle(const b64_domain *d, uint32_t layer, uint64_t block, uint64_t out[2]) {
+    if (!d || layer >= (uint32_t)d->dim || (layer < 64 && (block >> layer) != 0)) return B64_ERR_INVALID_ARGS;
+    st128(out, d->twiddle((int)layer, block));
+    return B64_OK;
+}
+
+// ===========================================================================
+// NTT / RS encode
+// ===========================================================================
+static int ntt_common(const b64_domain *dc, void *d_data, uint32_t log_len, uint32_t se, uint32_t sl,
+                      bool inverse, void *stream) {
+    b64_domain *d = const_cast<b64_domain *>(dc);
+    if (!d || !d_data) return B64_ERR_INVALID_ARGS;
+    // input_check (ntt/reference.rs:89-100)
+    if (se + sl > log_len || log_len - sl > (uint32_t)d->dim || log_len > 40) return B64_ERR_INVALID_ARGS;
+    b64::DeviceDomain dd;
+    B64_TRY(d->device_view(dd));
+    HIP_TRY(b64::launch_ntt(dd, (uint4 *)d_data, (int)log_len, (int)se, (int)sl, inverse, as_stream(stream)));
+    return B64_OK;
+}
+
+int b64_ntt_forward(const b64_domain *d, void *d_data, uint32_t log_len, uint32_t se, uint32_t sl, void *stream) {
+    return ntt_common(d, d_data, log_len, se, sl, false, stream);
+}
+
+int b64_ntt_inverse(const b64_domain *d, void *d_data, uint32_t log_len, uint32_t se, uint32_t sl, void *stream) {
+    return ntt_common(d, d_data, log_len, se, sl, true, stream);
+}
+
+int b64_rs_encode_batch(const b64_domain *dc, uint32_t log_dim, uint32_t lir, const void *d_msg,
+                        uint32_t log_batch, void *d_out, void *stream) {
+    b64_domain *d = const_cast<b64_domain *>(dc);
+    if (!d || !d_msg || !d_out) return B64_ERR_INVALID_ARGS;
+    if (log_dim + lir > (uint32_t)d->dim) return B64_ERR_SUBSPACE_DIMENSION_MISMATCH;
+    const uint32_t m = log_dim + log_batch;
+    if (m + lir > 40 || lir > 8) return B64_ERR_INVALID_ARGS;
+    b64::DeviceDomain dd;
+    B64_TRY(d->device_view(dd));
+    hipStream_t s = as_stream(stream);
+    HIP_TRY(b64::launch_bitrev_replicate((const uint4 *)d_msg, (uint4 *)d_out, (int)m, (int)lir, s));
+    HIP_TRY(b64::launch_ntt(dd, (uint4 *)d_out, (int)(m + lir), (int)lir, (int)log_batch, false, s));
+    return B64_OK;
+}
+
+// ===========================================================================
+// Merkle
+// ===========================================================================
+static int merkle_commit_impl(const uint4 *elems, uint64_t n_elems, uint64_t batch, uint4 *tree, hipStream_t s) {
+    if (batch == 0 || n_elems % batch) return B64_ERR_INCORRECT_BATCH_SIZE;
+    const uint64_t n_leaves = n_elems / batch;
+    if (n_leaves == 0 || (n_leaves & (n_leaves - 1))) return B64_ERR_POWER_OF_TWO_LENGTH_REQUIRED;
+    HIP_TRY(b64::launch_leaf_hash(elems, n_leaves, 0, batch, tree, s));
+    HIP_TRY(b64::launch_merkle_levels(tree, n_leaves, compress_iv(), s));
+    return B64_OK;
+}
+
+int b64_merkle_commit(const void *d_elems, uint64_t n_elems, uint64_t batch, void *d_tree, uint8_t root_out[32],
+                      void *stream) {
+    if (!d_elems || !d_tree) return B64_ERR_INVALID_ARGS;
+    if (!device_ok()) return B64_ERR_NO_DEVICE;
+    hipStream_t s = as_stream(stream);
+    B64_TRY(merkle_commit_impl((const uint4 *)d_elems, n_elems, batch, (uint4 *)d_tree, s));
+    if (root_out) {
+        const uint64_t n_leaves = n_elems / batch;
+        const uint8_t *root = (const uint8_t *)d_tree + 32 * (2 * n_leaves - 2);
+        HIP_TRY(hipMemcpyAsync(root_out, root, 32, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    return B64_OK;
+}
+
+int b64_merkle_layer(const void *d_tree, uint32_t log_len, uint32_t depth, uint8_t *out, void *stream) {
+    if (!d_tree || !out) return B64_ERR_INVALID_ARGS;
+    if (depth > log_len) return B64_ERR_INCORRECT_LAYER_DEPTH;
+    const uint64_t total = (2ull << log_len) - 1;
+    const uint64_t start = total + 1 - (2ull << depth);
+    hipStream_t s = as_stream(stream);
+    HIP_TRY(hipMemcpyAsync(out, (const uint8_t *)d_tree + 32 * start, 32ull << depth, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return B64_OK;
+}
+
+int b64_merkle_branch(const void *d_tree, uint32_t log_len, uint64_t index, uint32_t depth, uint8_t *out,
+                      void *stream) {
+    if (!d_tree || !out) return B64_ERR_INVALID_ARGS;
+    if (index >= (1ull << log_len) || depth > log_len) return B64_ERR_INDEX_OUT_OF_RANGE;
+    hipStream_t s = as_stream(stream);
+    for (uint32_t j = 0; j < log_len - depth; j++) {
+        const uint64_t node = (((1ull << j) - 1) << (log_len + 1 - j)) | ((index >> j) ^ 1);
+        HIP_TRY(hipMemcpyAsync(out + 32 * j, (const uint8_t *)d_tree + 32 * node, 32, hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return B64_OK;
+}
+
+// ===========================================================================
+// Folds
+// ===========================================================================
+int b64_fri_fold_interleaved(const void *d_cw, uint32_t log_len, uint32_t log_batch, const uint64_t *chal,
+                             void *d_out, void *stream) {
+    if (!d_cw || !d_out || (log_batch && !chal) || log_batch > 6 || log_len > 40) return B64_ERR_INVALID_ARGS;
+    if (!device_ok()) return B64_ERR_NO_DEVICE;
+    uint4 ch[16];
+    for (uint32_t t = 0; t < log_batch; t++) ch[t] = to_u4(ld128(chal + 2 * t));
+    HIP_TRY(b64::launch_fold_interleaved((const uint4 *)d_cw, (uint4 *)d_out, 1ull << log_len, (int)log_batch, ch,
+                                         as_stream(stream)));
+    return B64_OK;
+}
+
+int b64_fri_fold_codeword(const b64_domain *dc, const void *d_cw, uint32_t log_len, const uint64_t *chal,
+                          uint32_t n_chal, void *d_out, void *stream) {
+    b64_domain *d = const_cast<b64_domain *>(dc);
+    if (!d || !d_cw || !d_out || (n_chal && !chal) || n_chal > log_len || n_chal > 8) return B64_ERR_INVALID_ARGS;
+    if (log_len > (uint32_t)d->dim) return B64_ERR_INVALID_ARGS;
+    b64::DeviceDomain dd;
+    B64_TRY(d->device_view(dd));
+    uint4 ch[16];
+    for (uint32_t t = 0; t < n_chal; t++) ch[t] = to_u4(ld128(chal + 2 * t));
+    HIP_TRY(b64::launch_fold_codeword(dd, (const uint4 *)d_cw, (uint4 *)d_out, (int)log_len, ch, (int)n_chal,
+                                      as_stream(stream)));
+    return B64_OK;
+}
+
+int b64_pack_witness(const void *d_words, uint64_t n_words, uint32_t log_elems, void *d_out, void *stream) {
+    if (!d_words || !d_out || log_elems > 40 || n_words > (2ull << log_elems)) return B64_ERR_INVALID_ARGS;
+    if (!device_ok()) return B64_ERR_NO_DEVICE;
+    HIP_TRY(b64::launch_pack_witness((const uint64_t *)d_words, n_words, (uint4 *)d_out, 1ull << log_elems,
+                                     as_stream(stream)));
+    return B64_OK;
+}
+
+// ===========================================================================
+// FRI parameters (verifier/verifier/src/fri/common.rs)
+// ===========================================================================
+uint32_t b64_calculate_n_test_queries(uint32_t security_bits, uint32_t lir) {
+    const double rate = pow(2.0, -(double)lir);
+    const double per_query_err = 0.5 * (1.0 + rate);
+    return (uint32_t)ceil((double)security_bits / -log2(per_query_err));
+}
+
+uint32_t b64_estimate_optimal_arity(uint32_t L, uint32_t digest_size, uint32_t field_size) {
+    bool have = false;
+    uint64_t old = 0;
+    uint32_t last = 1;
+    for (uint32_t arity = 1; arity <= L; arity++) {
+        const uint64_t est = (((uint64_t)(L / 2) * digest_size + (1ull << arity) * field_size) * (L - arity)) / arity;
+        if (have && est > old) break;
+        have = true;
+        old = est;
+        last = arity;
+    }
+    return last;
+}
+
+int b64_fri_params_new(uint32_t log_dim, uint32_t lir, uint32_t lbs, const uint32_t *arities, uint32_t n_ar,
+                       uint32_t n_queries, b64_fri_params *out) {
+    if (!out || n_ar > B64_MAX_FOLD_ARITIES || (n_ar && !arities)) return B64_ERR_INVALID_ARGS;
+    uint32_t sum = 0;
+    for (uint32_t i = 0; i < n_ar; i++) sum += arities[i];
+    if (sum > log_dim) return B64_ERR_INVALID_FOLD_ARITY_SEQUENCE;
+    memset(out, 0, sizeof(*out));
+    out->log_dim = log_dim;
+    out->log_inv_rate = lir;
+    out->log_batch_size = lbs;
+    out->n_test_queries = n_queries;
+    out->n_fold_arities = n_ar;
+    for (uint32_t i = 0; i < n_ar; i++) out->fold_arities[i] = arities[i];
+    out->log_terminal_dim = log_dim - sum;
+    return B64_OK;
+}
+
+int b64_fri_params_choose(const b64_domain *d, uint32_t log_msg_len, uint32_t security_bits, uint32_t lir,
+                          uint32_t arity, b64_fri_params *out) {
+    if (!d || !out || arity == 0) return B64_ERR_INVALID_ARGS;
+    const uint32_t lbs = log_msg_len < arity ? log_msg_len : arity;
+    const uint32_t log_dim = log_msg_len - lbs;
+    const uint32_t rs_log_len = log_dim + lir;
+    if (rs_log_len > (uint32_t)d->dim) return B64_ERR_SUBSPACE_DIMENSION_MISMATCH;
+    const uint32_t nq = b64_calculate_n_test_queries(security_bits, lir);
+    const uint32_t cap = (uint32_t)log2_ceil_u64(nq);
+    uint32_t term = cap < lir ? lir : cap;
+    if (term > rs_log_len) term = rs_log_len;
+    const uint32_t q = (rs_log_len - term) / arity, rem = (rs_log_len - term) % arity;
+    uint32_t ar[B64_MAX_FOLD_ARITIES];
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < q && n < B64_MAX_FOLD_ARITIES; i++) ar[n++] = arity;
+    if (rem && n < B64_MAX_FOLD_ARITIES) ar[n++] = rem;
+    return b64_fri_params_new(log_dim, lir, lbs, ar, n, nq, out);
+}
+
+int b64_setup_commit_params(uint32_t log_witness_elems, uint32_t lir, uint32_t security_bits,
+                            b64_fri_params *params_out, b64_domain **domain_out) {
+    if (!params_out || !domain_out) return B64_ERR_INVALID_ARGS;
+    // Verifier::setup (verify.rs:76-96): digest 32 B, field element 16 B
+    const uint32_t log_code_len = log_witness_elems + lir;
+    if (log_code_len > (uint32_t)b64::MAX_DIM) return B64_ERR_INVALID_ARGS;
+    const uint32_t arity = b64_estimate_optimal_arity(log_code_len, 32, 16);
+    b64_domain *std_dom = nullptr;
+    B64_TRY(b64_domain_create_standard(log_code_len, &std_dom));
+    int rc = b64_fri_params_choose(std_dom, log_witness_elems, security_bits, lir, arity, params_out);
+    if (rc == B64_OK) {
+        // Prover::setup (prove.rs:98-99): GenericPreExpanded over rs_code.subspace()
+        const uint32_t rs_log_len = params_out->log_dim + lir;
+        std::vector<uint64_t> basis(2 * (size_t)(rs_log_len ? rs_log_len : 1));
+        rc = b64_domain_subspace(std_dom, rs_log_len, basis.data());
+        if (rc == B64_OK) rc = b64_domain_create(basis.data(), rs_log_len, domain_out);
+    }
+    b64_domain_destroy(std_dom);
+    return rc;
+}
+
+int b64_fri_commit_interleaved(const b64_fri_params *p, const b64_domain *d, const void *d_msg, void *d_codeword,
+                               void *d_tree, uint8_t root_out[32], void *stream) {
+    if (!p || !d || !d_msg || !d_codeword || !d_tree) return B64_ERR_INVALID_ARGS;
+    // encode_batch requires ntt.subspace(rs_log_len) == rs_code.subspace(); the
+    // code built over `d` satisfies it by construction (reed_solomon.rs:54-65).
+    B64_TRY(b64_rs_encode_batch(d, p->log_dim, p->log_inv_rate, d_msg, p->log_batch_size, d_codeword, stream));
+    const uint64_t n = 1ull << (p->log_dim + p->log_inv_rate + p->log_batch_size);
+    return b64_merkle_commit(d_codeword, n, 1ull << p->log_batch_size, d_tree, root_out, stream);
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Transcript: HasherChallenger<Sha256> + proof tape
+// (verifier/transcript/src/fiat_shamir/hasher_challenger.rs, transcript.rs)
+// ===========================================================================
+struct b64_transcript {
+    b64::host::Sha256 hasher;
+    bool sampler = true;
+    uint8_t buffer[32];
+    size_t index = 0;
+    std::vector<uint8_t> tape;
+
+    b64_transcript() {
+        uint8_t e[32];
+        b64::host::Sha256::digest(nullptr, 0, e);
+        hasher.update(e, 32);
+        memcpy(buffer, e, 32);
+    }
+    void observe(const uint8_t *b, size_t n) {
+        if (sampler) {  // Sampler -> Observer: absorb the LE usize read index
+            uint8_t le[8];
+            for (int i = 0; i < 8; i++) le[i] = (uint8_t)((uint64_t)index >> (8 * i));
+            hasher.update(le, 8);
+            sampler = false;
+        }
+        hasher.update(b, n);
+    }
+    void sample(uint8_t *out, size_t n) {
+        if (!sampler) {  // Observer -> Sampler: buffer exhausted, refill on first read
+            sampler = true;
+            index = 32;
+        }
+        while (n) {
+            if (index == 32) {
+                uint8_t d[32];
+                hasher.finalize(d);
+                hasher.reset();
+                hasher.update(d, 32);
+                memcpy(buffer, d, 32);
+                index = 0;
+            }
+            size_t take = 32 - index;
+            if (take > n) take = n;
+            memcpy(out, buffer + index, take);
+            index += take;
+            out += take;
+            n -= take;
+        }
+    }
+};
+
+extern "C" {
+
+int b64_transcript_create(b64_transcript **out) {
+    if (!out) return B64_ERR_INVALID_ARGS;
+    *out = new (std::nothrow) b64_transcript();
+    return *out ? B64_OK : B64_ERR_OUT_OF_MEMORY;
+}
+void b64_transcript_destroy(b64_transcript *t) { delete t; }
+void b64_transcript_message(b64_transcript *t, const uint8_t *b, size_t n) {
+    t->tape.insert(t->tape.end(), b, b + n);
+    t->observe(b, n);
+}
+void b64_transcript_observe(b64_transcript *t, const uint8_t *b, size_t n) { t->observe(b, n); }
+void b64_transcript_decommit(b64_transcript *t, const uint8_t *b, size_t n) { t->tape.insert(t->tape.end(), b, b + n); }
+void b64_transcript_sample(b64_transcript *t, uint8_t *out, size_t n) { t->sample(out, n); }
+uint32_t b64_transcript_sample_bits(b64_transcript *t, uint32_t bits) {
+    if (bits > 32) bits = 32;
+    uint8_t b[4];
+    t->sample(b, 4);
+    const uint32_t v = (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) | ((uint32_t)b[3] << 24);
+    return bits >= 32 ? v : (v & ((1u << bits) - 1));
+}
+size_t b64_transcript_tape(const b64_transcript *t, uint8_t *out, size_t cap) {
+    if (out && cap >= t->tape.size()) memcpy(out, t->tape.data(), t->tape.size());
+    return t->tape.size();
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// FRI fold / query prover (prover/prover/src/fri/fold.rs, query.rs)
+// ===========================================================================
+struct b64_fri_prover {
+    struct Oracle {
+        uint4 *cw = nullptr;    // device codeword
+        uint4 *tree = nullptr;  // device tree (2*n_leaves-1 digests)
+        int log_len = 0;        // log2 codeword length
+        int tree_log_len = 0;   // log2 number of leaves
+    };
+    b64_fri_params p;
+    b64_domain *d = nullptr;
+    Oracle first;  // the committed interleaved codeword (not owned)
+    std::vector<Oracle> committed;
+    hipStream_t s = nullptr;
+    uint32_t curr_round = 0;
+    bool has_next = true;
+    uint32_t next_commit = 0;
+    std::vector<u128> unprocessed;
+
+    ~b64_fri_prover() {
+        for (auto &o : committed) {
+            if (o.cw) (void)hipFree(o.cw);
+            if (o.tree) (void)hipFree(o.tree);
+        }
+    }
+    uint32_t n_rounds() const { return p.log_dim + p.log_batch_size; }
+    uint32_t rs_log_len() const { return p.log_dim + p.log_inv_rate; }
+};
+
+namespace {
+
+int fetch(void *dst, const void *d_src, size_t bytes, hipStream_t s) {
+    HIP_TRY(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, s));
+    return B64_OK;
+}
+
+// Gather openings for all queries of one oracle with a single device->host
+// transfer per oracle: the coset values and the branch digests.
+int gather_rows(const b64_fri_prover::Oracle &o, const std::vector<uint64_t> &coset_idx, int log_coset, int depth,
+                std::vector<uint8_t> &vals, std::vector<uint8_t> &branches, hipStream_t s) {
+    const size_t q = coset_idx.size();
+    const size_t coset_bytes = 16ull << log_coset;
+    const int br_len = o.tree_log_len - depth;
+    vals.resize(q * coset_bytes);
+    branches.resize(q * 32 * (size_t)(br_len > 0 ? br_len : 0));
+    for (size_t i = 0; i < q; i++) {
+        B64_TRY(fetch(vals.data() + i * coset_bytes, (const uint8_t *)o.cw + (coset_idx[i] << log_coset) * 16,
+                      coset_bytes, s));
+        for (int j = 0; j < br_len; j++) {
+            const uint64_t node = (((1ull << j) - 1) << (o.tree_log_len + 1 - j)) | ((coset_idx[i] >> j) ^ 1);
+            B64_TRY(fetch(branches.data() + (i * br_len + j) * 32, (const uint8_t *)o.tree + 32 * node, 32, s));
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return B64_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int b64_fri_prover_create(const b64_fri_params *p, const b64_domain *d, const void *d_codeword, const void *d_tree,
+                          void *stream, b64_fri_prover **out) {
+    if (!p || !d || !d_codeword || !d_tree || !out) return B64_ERR_INVALID_ARGS;
+    if (p->log_dim + p->log_inv_rate > (uint32_t)d->dim) return B64_ERR_INVALID_ARGS;
+    b64_fri_prover *fp = new (std::nothrow) b64_fri_prover();
+    if (!fp) return B64_ERR_OUT_OF_MEMORY;
+    fp->p = *p;
+    fp->d = const_cast<b64_domain *>(d);
+    fp->first.cw = (uint4 *)d_codeword;
+    fp->first.tree = (uint4 *)d_tree;
+    fp->first.log_len = (int)(p->log_dim + p->log_inv_rate + p->log_batch_size);
+    fp->first.tree_log_len = (int)(p->log_dim + p->log_inv_rate);
+    fp->s = as_stream(stream);
+    fp->next_commit = p->log_batch_size;  // FRIFoldProver::new (fold.rs:70)
+    *out = fp;
+    return B64_OK;
+}
+
+void b64_fri_prover_destroy(b64_fri_prover *fp) { delete fp; }
+uint32_t b64_fri_prover_n_rounds(const b64_fri_prover *fp) { return fp->n_rounds(); }
+uint32_t b64_fri_prover_curr_round(const b64_fri_prover *fp) { return fp->curr_round; }
+
+int b64_fri_prover_receive_challenge(b64_fri_prover *fp, const uint64_t ch[2]) {
+    if (!fp || !ch) return B64_ERR_INVALID_ARGS;
+    fp->unprocessed.push_back(ld128(ch));
+    fp->curr_round++;
+    return B64_OK;
+}
+
+int b64_fri_prover_execute_fold_round(b64_fri_prover *fp, uint8_t root_out[32], int *committed) {
+    if (!fp || !committed) return B64_ERR_INVALID_ARGS;
+    *committed = 0;
+    if (!(fp->has_next && fp->next_commit == fp->curr_round)) return B64_OK;
+    b64::DeviceDomain dd;
+    B64_TRY(fp->d->device_view(dd));
+    b64_fri_prover::Oracle o;
+    std::vector<uint4> ch(fp->unprocessed.size() + 1);
+    for (size_t i = 0; i < fp->unprocessed.size(); i++) ch[i] = to_u4(fp->unprocessed[i]);
+    const size_t n_ch = fp->unprocessed.size();
+    if (fp->committed.empty()) {
+        // fold_interleaved (fold.rs:260-282)
+        o.log_len = (int)fp->rs_log_len();
+        if (n_ch != fp->p.log_batch_size || n_ch > 6) return B64_ERR_INVALID_ARGS;
+        HIP_TRY(hipMalloc(&o.cw, 16ull << o.log_len));
+        hipError_t e = b64::launch_fold_interleaved(fp->first.cw, o.cw, 1ull << o.log_len, (int)n_ch, ch.data(), fp->s);
+        if (e != hipSuccess) {
+            (void)hipFree(o.cw);
+            return B64_ERR_HIP;
+        }
+    } else {
+        // fold_codeword (fold.rs:297-318)
+        const b64_fri_prover::Oracle &prev = fp->committed.back();
+        if ((int)n_ch > prev.log_len || n_ch > 8) return B64_ERR_INVALID_ARGS;
+        o.log_len = prev.log_len - (int)n_ch;
+        HIP_TRY(hipMalloc(&o.cw, 16ull << o.log_len));
+        hipError_t e = b64::launch_fold_codeword(dd, prev.cw, o.cw, prev.log_len, ch.data(), (int)n_ch, fp->s);
+        if (e != hipSuccess) {
+            (void)hipFree(o.cw);
+            return B64_ERR_HIP;
+        }
+    }
+    fp->unprocessed.clear();
+    const size_t k = fp->committed.size();
+    const bool has_arity = k < fp->p.n_fold_arities;
+    const int log_coset = has_arity ? (int)fp->p.fold_arities[k] : (int)fp->p.log_terminal_dim;
+    if (log_coset > o.log_len) {
+        (void)hipFree(o.cw);
+        return B64_ERR_INCORRECT_BATCH_SIZE;
+    }
+    o.tree_log_len = o.log_len - log_coset;
+    if (hipMalloc(&o.tree, 32ull * ((2ull << o.tree_log_len) - 1)) != hipSuccess) {
+        (void)hipFree(o.cw);
+        return B64_ERR_OUT_OF_MEMORY;
+    }
+    fp->committed.push_back(o);  // owned from here on
+    B64_TRY(merkle_commit_impl(o.cw, 1ull << o.log_len, 1ull << log_coset, o.tree, fp->s));
+    if (has_arity)
+        fp->next_commit += fp->p.fold_arities[k];
+    else
+        fp->has_next = false;
+    if (root_out) {
+        B64_TRY(fetch(root_out, (const uint8_t *)o.tree + 32 * ((2ull << o.tree_log_len) - 2), 32, fp->s));
+        HIP_TRY(hipStreamSynchronize(fp->s));
+    }
+    *committed = 1;
+    return B64_OK;
+}
+
+int b64_fri_prover_oracle(const b64_fri_prover *fp, uint32_t k, const void **d_cw, uint32_t *log_len,
+                          const void **d_tree) {
+    if (!fp || k >= fp->committed.size()) return B64_ERR_INDEX_OUT_OF_RANGE;
+    if (d_cw) *d_cw = fp->committed[k].cw;
+    if (log_len) *log_len = (uint32_t)fp->committed[k].log_len;
+    if (d_tree) *d_tree = fp->committed[k].tree;
+    return B64_OK;
+}
+
+// finish_proof (fold.rs:221-244) with vcs_optimal_layers (query.rs:93-109) and
+// prove_query (query.rs:51-91).
+int b64_fri_prover_finish_proof(b64_fri_prover *fp, b64_transcript *t) {
+    if (!fp || !t) return B64_ERR_INVALID_ARGS;
+    if (fp->curr_round != fp->n_rounds()) return B64_ERR_EARLY_PROVER_FINISH;  // finalize (:190-192)
+    const b64_fri_params &p = fp->p;
+    const hipStream_t s = fp->s;
+    // terminate codeword: the last committed oracle, or the codeword itself
+    const b64_fri_prover::Oracle &term = fp->committed.empty() ? fp->first : fp->committed.back();
+    std::vector<uint8_t> buf(16ull << term.log_len);
+    B64_TRY(fetch(buf.data(), term.cw, buf.size(), s));
+    HIP_TRY(hipStreamSynchronize(s));
+    b64_transcript_decommit(t, buf.data(), buf.size());
+
+    // optimal layer depths (verifier fri/common.rs:154-168, scheme.rs:70-72)
+    const int cap = log2_ceil_u64(p.n_test_queries);
+    std::vector<int> depths;
+    int log_n_cosets = (int)(p.log_dim + p.log_inv_rate + p.log_batch_size);
+    for (uint32_t i = 0; i <= p.n_fold_arities; i++) {
+        log_n_cosets -= (int)(i == 0 ? p.log_batch_size : p.fold_arities[i - 1]);
+        depths.push_back(cap < log_n_cosets ? cap : log_n_cosets);
+    }
+    // the first oracle and every committed oracle but the terminal one
+    std::vector<const b64_fri_prover::Oracle *> opened;
+    opened.push_back(&fp->first);
+    for (size_t i = 0; i + 1 < fp->committed.size(); i++) opened.push_back(&fp->committed[i]);
+    const size_t n_open = opened.size() < depths.size() ? opened.size() : depths.size();
+    for (size_t i = 0; i < n_open; i++) {
+        const b64_fri_prover::Oracle &o = *opened[i];
+        const int depth = depths[i];
+        const uint64_t total = (2ull << o.tree_log_len) - 1;
+        const uint64_t start = total + 1 - (2ull << depth);
+        std::vector<uint8_t> layer(32ull << depth);
+        B64_TRY(fetch(layer.data(), (const uint8_t *)o.tree + 32 * start, layer.size(), s));
+        HIP_TRY(hipStreamSynchronize(s));
+        b64_transcript_decommit(t, layer.data(), layer.size());
+    }
+    // queries: decommitments are not observed, so every index can be sampled first
+    const uint32_t nq = p.n_test_queries;
+    std::vector<uint64_t> idx0(nq);
+    for (uint32_t q = 0; q < nq; q++) idx0[q] = b64_transcript_sample_bits(t, fp->rs_log_len());
+    std::vector<std::vector<uint8_t>> vals(n_open), brs(n_open);
+    std::vector<int> log_cosets(n_open);
+    std::vector<uint64_t> cur = idx0;
+    for (size_t i = 0; i < n_open; i++) {
+        if (i > 0)
+            for (auto &x : cur) x >>= p.fold_arities[i - 1];
+        log_cosets[i] = (int)(i == 0 ? p.log_batch_size : p.fold_arities[i - 1]);
+        B64_TRY(gather_rows(*opened[i], cur, log_cosets[i], depths[i], vals[i], brs[i], s));
+    }
+    for (uint32_t q = 0; q < nq; q++) {
+        for (size_t i = 0; i < n_open; i++) {
+            const size_t cb = 16ull << log_cosets[i];
+            b64_transcript_decommit(t, vals[i].data() + q * cb, cb);
+            const int br_len = opened[i]->tree_log_len - depths[i];
+            if (br_len > 0) b64_transcript_decommit(t, brs[i].data() + (size_t)q * br_len * 32, 32ull * br_len);
+        }
+    }
+    return B64_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,153 @@
+// GF(2^128) arithmetic for gfx950 kernels: the GHASH field of
+// verifier/field/src/ghash.rs (modulus x^128 + x^7 + x^2 + x + 1, bit i of the
+// little-endian u128 is the coefficient of x^i).  An element lives in a uint4:
+// .x = bits 0..31, .y = 32..63, .z = 64..95, .w = 96..127, which is exactly the
+// 16-byte HBM image, so elements move with single 16-byte loads and stores.
+//
+// Two multipliers:
+//  * gf_mul      -- general product.  CDNA4 has no carry-less multiply, so the
+//                   32x32 carry-less products come from integer multiplies on
+//                   operands with 3-bit "holes" (the method of the reference's
+//                   portable backend, arch/portable/univariate_mul_utils_128.rs:120-141,
+//                   re-cut for 32-bit lanes), combined by two Karatsuba levels and a
+//                   word-wise reduction (packed_ghash_128.rs:84-97).
+//  * WinTable    -- product by a constant c shared by many lanes: a 4-bit window
+//                   table T[k][n] = c * n * x^(4k) (32 x 16 entries, 8 KiB) in LDS;
+//                   v*c is the XOR of 32 table rows selected by v's nibbles.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace b64 {
+
+__device__ __forceinline__ uint4 gxor(uint4 a, uint4 b) {
+    return make_uint4(a.x ^ b.x, a.y ^ b.y, a.z ^ b.z, a.w ^ b.w);
+}
+__device__ __forceinline__ bool gzero(uint4 a) { return (a.x | a.y | a.z | a.w) == 0u; }
+
+// 32x32 -> 64 carry-less product: operands split into four masks with 3-bit
+// holes; each integer product sums at most 8 one-bit terms per kept position,
+// so carries stay inside the holes and are masked away.
+__device__ __forceinline__ uint64_t clmul32(uint32_t a, uint32_t b) {
+    const uint32_t m0 = 0x11111111u, m1 = 0x22222222u, m2 = 0x44444444u, m3 = 0x88888888u;
+    const uint32_t a0 = a & m0, a1 = a & m1, a2 = a & m2, a3 = a & m3;
+    const uint32_t b0 = b & m0, b1 = b & m1, b2 = b & m2, b3 = b & m3;
+    uint64_t z0 = ((uint64_t)a0 * b0) ^ ((uint64_t)a1 * b3) ^ ((uint64_t)a2 * b2) ^ ((uint64_t)a3 * b1);
+    uint64_t z1 = ((uint64_t)a0 * b1) ^ ((uint64_t)a1 * b0) ^ ((uint64_t)a2 * b3) ^ ((uint64_t)a3 * b2);
+    uint64_t z2 = ((uint64_t)a0 * b2) ^ ((uint64_t)a1 * b1) ^ ((uint64_t)a2 * b0) ^ ((uint64_t)a3 * b3);
+    uint64_t z3 = ((uint64_t)a0 * b3) ^ ((uint64_t)a1 * b2) ^ ((uint64_t)a2 * b1) ^ ((uint64_t)a3 * b0);
+    return (z0 & 0x1111111111111111ull) | (z1 & 0x2222222222222222ull) |
+           (z2 & 0x4444444444444444ull) | (z3 & 0x8888888888888888ull);
+}
+
+// 64x64 -> 128 carry-less product (one Karatsuba level over clmul32).
+__device__ __forceinline__ void clmul64(uint64_t a, uint64_t b, uint64_t &lo, uint64_t &hi) {
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+    const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    const uint64_t p0 = clmul32(a0, b0);
+    const uint64_t p2 = clmul32(a1, b1);
+    const uint64_t p1 = clmul32(a0 ^ a1, b0 ^ b1) ^ p0 ^ p2;
+    lo = p0 ^ (p1 << 32);
+    hi = p2 ^ (p1 >> 32);
+}
+
+// Reduce the 256-bit product (v0..v3, 64-bit words, v0 lowest) modulo
+// x^128 + x^7 + x^2 + x + 1.
+__device__ __forceinline__ uint4 gf_reduce(uint64_t v0, uint64_t v1, uint64_t v2, uint64_t v3) {
+    v1 ^= v3 ^ (v3 << 1) ^ (v3 << 2) ^ (v3 << 7);
+    v2 ^= (v3 >> 63) ^ (v3 >> 62) ^ (v3 >> 57);
+    v0 ^= v2 ^ (v2 << 1) ^ (v2 << 2) ^ (v2 << 7);
+    v1 ^= (v2 >> 63) ^ (v2 >> 62) ^ (v2 >> 57);
+    return make_uint4((uint32_t)v0, (uint32_t)(v0 >> 32), (uint32_t)v1, (uint32_t)(v1 >> 32));
+}
+
+__device__ __forceinline__ uint4 gf_mul(uint4 a, uint4 b) {
+    const uint64_t a0 = ((uint64_t)a.y << 32) | a.x, a1 = ((uint64_t)a.w << 32) | a.z;
+    const uint64_t b0 = ((uint64_t)b.y << 32) | b.x, b1 = ((uint64_t)b.w << 32) | b.z;
+    uint64_t l0, h0, l2, h2, l1, h1;
+    clmul64(a0, b0, l0, h0);
+    clmul64(a1, b1, l2, h2);
+    clmul64(a0 ^ a1, b0 ^ b1, l1, h1);
+    l1 ^= l0 ^ l2;
+    h1 ^= h0 ^ h2;
+    return gf_reduce(l0, h0 ^ l1, l2 ^ h1, h2);
+}
+
+// v * x (ghash.rs:69-80)
+__device__ __forceinline__ uint4 gf_mul_x(uint4 v) {
+    const uint32_t carry = (v.w >> 31) ? 0x87u : 0u;
+    return make_uint4((v.x << 1) ^ carry, (v.y << 1) | (v.x >> 31), (v.z << 1) | (v.y >> 31),
+                      (v.w << 1) | (v.z >> 31));
+}
+
+// v * x^s for 0 <= s < 128 by shifting and folding the overflow through
+// x^128 = x^7 + x^2 + x + 1 twice.
+__device__ inline uint4 gf_mul_xpow(uint4 v, int s) {
+    unsigned __int128 t = ((unsigned __int128)(((uint64_t)v.w << 32) | v.z) << 64) |
+                          (((uint64_t)v.y << 32) | v.x);
+    if (s == 0) return v;
+    unsigned __int128 lo = t << s;
+    unsigned __int128 of = t >> (128 - s);  // < 2^s
+    // of * (1 + x + x^2 + x^7): low part and the part that overflows again
+    unsigned __int128 r = of ^ (of << 1) ^ (of << 2) ^ (of << 7);
+    unsigned __int128 of2 = (of >> 127) ^ (of >> 126) ^ (of >> 121);  // bits shifted past 2^128
+    r ^= of2 ^ (of2 << 1) ^ (of2 << 2) ^ (of2 << 7);
+    unsigned __int128 out = lo ^ r;
+    const uint64_t o0 = (uint64_t)out, o1 = (uint64_t)(out >> 64);
+    return make_uint4((uint32_t)o0, (uint32_t)(o0 >> 32), (uint32_t)o1, (uint32_t)(o1 >> 32));
+}
+
+// ---- 4-bit window tables ---------------------------------------------------
+constexpr int WIN_ROWS = 32;
+constexpr int WIN_ENTRIES = WIN_ROWS * 16;  // uint4 entries per table (8 KiB)
+
+// Builds row `row` (0..31) of the window table of constant c: entries
+// c * n * x^(4*row) for n = 0..15.  32 lanes build a whole table.
+__device__ inline void win_build_row(uint4 *__restrict__ tab, uint4 c, int row) {
+    const uint4 p0 = gf_mul_xpow(c, 4 * row);
+    const uint4 p1 = gf_mul_x(p0);
+    const uint4 p2 = gf_mul_x(p1);
+    const uint4 p3 = gf_mul_x(p2);
+    uint4 *r = tab + row * 16;
+    const uint4 z = make_uint4(0, 0, 0, 0);
+    r[0] = z;
+    r[1] = p0;
+    r[2] = p1;
+    const uint4 p01 = gxor(p0, p1);
+    r[3] = p01;
+    r[4] = p2;
+    r[5] = gxor(p2, p0);
+    r[6] = gxor(p2, p1);
+    r[7] = gxor(p2, p01);
+    const uint4 p3_01 = gxor(p3, p01);
+    r[8] = p3;
+    r[9] = gxor(p3, p0);
+    r[10] = gxor(p3, p1);
+    r[11] = p3_01;
+    const uint4 p32 = gxor(p3, p2);
+    r[12] = p32;
+    r[13] = gxor(p32, p0);
+    r[14] = gxor(p32, p1);
+    r[15] = gxor(p32, p01);
+}
+
+__device__ __forceinline__ uint4 win_word(const uint4 *__restrict__ tab, uint32_t w, int row0, uint4 acc) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const uint4 e = tab[(row0 + j) * 16 + ((w >> (4 * j)) & 15u)];
+        acc = gxor(acc, e);
+    }
+    return acc;
+}
+
+// v * c using c's window table.
+__device__ __forceinline__ uint4 win_mul(const uint4 *__restrict__ tab, uint4 v) {
+    uint4 acc = make_uint4(0, 0, 0, 0);
+    acc = win_word(tab, v.x, 0, acc);
+    acc = win_word(tab, v.y, 8, acc);
+    acc = win_word(tab, v.z, 16, acc);
+    acc = win_word(tab, v.w, 24, acc);
+    return acc;
+}
+
+}  // namespace b64

@@ -1,0 +1,145 @@
+// Host-side arithmetic for the library: GF(2^128) (setup of twiddle bases,
+// fold challenges) and SHA-256 (Fiat-Shamir transcript, compression IV).
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+#include <string.h>
+
+namespace b64 {
+namespace host {
+
+typedef unsigned __int128 u128;
+
+static inline u128 clmul64(uint64_t a, uint64_t b) {
+    u128 r = 0;
+    for (int i = 0; i < 64; i++)
+        if ((b >> i) & 1) r ^= (u128)a << i;
+    return r;
+}
+
+// GHASH product (verifier/field/src/ghash.rs:188-195): schoolbook carry-less
+// product, then reduction by x^128 = x^7 + x^2 + x + 1.
+static inline u128 gf_mul(u128 a, u128 b) {
+    const uint64_t a0 = (uint64_t)a, a1 = (uint64_t)(a >> 64);
+    const uint64_t b0 = (uint64_t)b, b1 = (uint64_t)(b >> 64);
+    const u128 lo = clmul64(a0, b0), hi = clmul64(a1, b1);
+    const u128 mid = clmul64(a0, b1) ^ clmul64(a1, b0);
+    uint64_t v0 = (uint64_t)lo, v1 = (uint64_t)(lo >> 64) ^ (uint64_t)mid;
+    uint64_t v2 = (uint64_t)hi ^ (uint64_t)(mid >> 64), v3 = (uint64_t)(hi >> 64);
+    v1 ^= v3 ^ (v3 << 1) ^ (v3 << 2) ^ (v3 << 7);
+    v2 ^= (v3 >> 63) ^ (v3 >> 62) ^ (v3 >> 57);
+    v0 ^= v2 ^ (v2 << 1) ^ (v2 << 2) ^ (v2 << 7);
+    v1 ^= (v2 >> 63) ^ (v2 >> 62) ^ (v2 >> 57);
+    return ((u128)v1 << 64) | v0;
+}
+
+// a^(2^128 - 2): the inverse, and 0 for 0 (invert_or_zero).
+static inline u128 gf_inv(u128 a) {
+    u128 r = 1, base = a;
+    u128 e = ~(u128)0 - 1;
+    while (e) {
+        if (e & 1) r = gf_mul(r, base);
+        base = gf_mul(base, base);
+        e >>= 1;
+    }
+    return r;
+}
+
+class Sha256 {
+   public:
+    Sha256() { reset(); }
+    void reset() {
+        static const uint32_t iv[8] = {0x6a09e667, 0xbb67ae85, 0x3c6ef372, 0xa54ff53a,
+                                       0x510e527f, 0x9b05688c, 0x1f83d9ab, 0x5be0cd19};
+        memcpy(h_, iv, sizeof(iv));
+        len_ = 0;
+        total_ = 0;
+    }
+    void update(const uint8_t *p, size_t n) {
+        total_ += n;
+        while (n) {
+            size_t take = 64 - len_;
+            if (take > n) take = n;
+            memcpy(buf_ + len_, p, take);
+            len_ += take;
+            p += take;
+            n -= take;
+            if (len_ == 64) {
+                block(h_, buf_);
+                len_ = 0;
+            }
+        }
+    }
+    void finalize(uint8_t out[32]) const {
+        Sha256 c = *this;
+        const uint64_t bits = c.total_ * 8;
+        const uint8_t pad = 0x80, z = 0;
+        c.update(&pad, 1);
+        while (c.len_ != 56) c.update(&z, 1);
+        uint8_t len[8];
+        for (int i = 0; i < 8; i++) len[i] = (uint8_t)(bits >> (56 - 8 * i));
+        c.update(len, 8);
+        for (int i = 0; i < 8; i++) {
+            out[4 * i] = (uint8_t)(c.h_[i] >> 24);
+            out[4 * i + 1] = (uint8_t)(c.h_[i] >> 16);
+            out[4 * i + 2] = (uint8_t)(c.h_[i] >> 8);
+            out[4 * i + 3] = (uint8_t)c.h_[i];
+        }
+    }
+    static void digest(const uint8_t *p, size_t n, uint8_t out[32]) {
+        Sha256 s;
+        s.update(p, n);
+        s.finalize(out);
+    }
+
+   private:
+    static inline uint32_t ror(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+    static void block(uint32_t h[8], const uint8_t *b) {
+        static const uint32_t K[64] = {
+            0x428a2f98, 0x71374491, 0xb5c0fbcf, 0xe9b5dba5, 0x3956c25b, 0x59f111f1, 0x923f82a4, 0xab1c5ed5,
+            0xd807aa98, 0x12835b01, 0x243185be, 0x550c7dc3, 0x72be5d74, 0x80deb1fe, 0x9bdc06a7, 0xc19bf174,
+            0xe49b69c1, 0xefbe4786, 0x0fc19dc6, 0x240ca1cc, 0x2de92c6f, 0x4a7484aa, 0x5cb0a9dc, 0x76f988da,
+            0x983e5152, 0xa831c66d, 0xb00327c8, 0xbf597fc7, 0xc6e00bf3, 0xd5a79147, 0x06ca6351, 0x14292967,
+            0x27b70a85, 0x2e1b2138, 0x4d2c6dfc, 0x53380d13, 0x650a7354, 0x766a0abb, 0x81c2c92e, 0x92722c85,
+            0xa2bfe8a1, 0xa81a664b, 0xc24b8b70, 0xc76c51a3, 0xd192e819, 0xd6990624, 0xf40e3585, 0x106aa070,
+            0x19a4c116, 0x1e376c08, 0x2748774c, 0x34b0bcb5, 0x391c0cb3, 0x4ed8aa4a, 0x5b9cca4f, 0x682e6ff3,
+            0x748f82ee, 0x78a5636f, 0x84c87814, 0x8cc70208, 0x90befffa, 0xa4506ceb, 0xbef9a3f7, 0xc67178f2};
+        uint32_t w[64];
+        for (int i = 0; i < 16; i++)
+            w[i] = ((uint32_t)b[4 * i] << 24) | ((uint32_t)b[4 * i + 1] << 16) | ((uint32_t)b[4 * i + 2] << 8) |
+                   b[4 * i + 3];
+        for (int i = 16; i < 64; i++) {
+            const uint32_t s0 = ror(w[i - 15], 7) ^ ror(w[i - 15], 18) ^ (w[i - 15] >> 3);
+            const uint32_t s1 = ror(w[i - 2], 17) ^ ror(w[i - 2], 19) ^ (w[i - 2] >> 10);
+            w[i] = w[i - 16] + s0 + w[i - 7] + s1;
+        }
+        uint32_t a = h[0], bb = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], hh = h[7];
+        for (int i = 0; i < 64; i++) {
+            const uint32_t t1 = hh + (ror(e, 6) ^ ror(e, 11) ^ ror(e, 25)) + ((e & f) ^ (~e & g)) + K[i] + w[i];
+            const uint32_t t2 = (ror(a, 2) ^ ror(a, 13) ^ ror(a, 22)) + ((a & bb) ^ (a & c) ^ (bb & c));
+            hh = g;
+            g = f;
+            f = e;
+            e = d + t1;
+            d = c;
+            c = bb;
+            bb = a;
+            a = t1 + t2;
+        }
+        h[0] += a;
+        h[1] += bb;
+        h[2] += c;
+        h[3] += d;
+        h[4] += e;
+        h[5] += f;
+        h[6] += g;
+        h[7] += hh;
+    }
+    uint32_t h_[8];
+    uint8_t buf_[64];
+    size_t len_;
+    uint64_t total_;
+};
+
+}  // namespace host
+}  // namespace b64

@@ -1,0 +1,375 @@
+// gfx950 kernels for the B128 encode / commit / fold path and their launchers.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gf128_dev.hpp"
+#include "internal.hpp"
+#include "sha256_dev.hpp"
+
+namespace b64 {
+
+constexpr int BLOCK = 256;
+
+static inline unsigned grid_for(uint64_t n, int per_block = BLOCK) {
+    return (unsigned)((n + per_block - 1) / per_block);
+}
+
+// ---------------------------------------------------------------------------
+// Element-wise products (device multiply parity surface)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_gf_mul(const uint4 *__restrict__ a, const uint4 *__restrict__ b,
+                                                  uint4 *__restrict__ out, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) out[i] = gf_mul(a[i], b[i]);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_gf_mul_const(const uint4 *__restrict__ a, uint4 c,
+                                                        uint4 *__restrict__ out, uint64_t n) {
+    __shared__ uint4 tab[WIN_ENTRIES];
+    if (threadIdx.x < WIN_ROWS) win_build_row(tab, c, threadIdx.x);
+    __syncthreads();
+    for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK)
+        out[i] = win_mul(tab, a[i]);
+}
+
+hipError_t launch_gf_mul(const uint4 *a, const uint4 *b, uint4 *out, uint64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    k_gf_mul<<<grid_for(n), BLOCK, 0, s>>>(a, b, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_gf_mul_const(const uint4 *a, uint4 c, uint4 *out, uint64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    unsigned g = grid_for(n);
+    if (g > 4096) g = 4096;
+    k_gf_mul_const<<<g, BLOCK, 0, s>>>(a, c, out, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// encode_batch front end (reed_solomon.rs:152-187): out[r*2^m + i] = msg[rev_m(i)]
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_bitrev_replicate(const uint4 *__restrict__ msg,
+                                                            uint4 *__restrict__ out, int m, int lir) {
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t n = 1ull << m;
+    if (i >= n) return;
+    const uint64_t src = m == 0 ? 0 : (__brevll(i) >> (64 - m));
+    const uint4 v = msg[src];
+    for (int r = 0; r < (1 << lir); r++) out[((uint64_t)r << m) | i] = v;
+}
+
+hipError_t launch_bitrev_replicate(const uint4 *msg, uint4 *out, int m, int lir, hipStream_t s) {
+    k_bitrev_replicate<<<grid_for(1ull << m), BLOCK, 0, s>>>(msg, out, m, lir);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Additive NTT (ntt/reference.rs:17-76): LDS-tiled passes of up to NTT_MAX_K layers.
+//
+// A pass over layers [layer0, layer0+k) flips index bits lb+k-1 .. lb with
+// lb = log_d - layer0 - k.  A tile holds every combination of those k bits and
+// of the c lowest index bits (contiguous in HBM, so loads coalesce); the bits in
+// between and above come from the tile id.  Twiddles only depend on bits above
+// the flipped bit, so a tile needs 2^j distinct twiddles in its j-th layer.
+// ---------------------------------------------------------------------------
+constexpr int NTT_TILE_LOG = 11;
+constexpr int NTT_MAX_K = 7;
+
+struct NttPassArgs {
+    uint4 *data;
+    const uint4 *W;
+    int log_d, layer0, k, c;
+};
+
+template <bool INVERSE>
+__global__ __launch_bounds__(BLOCK) void k_ntt_pass(NttPassArgs a) {
+    __shared__ uint4 s[1 << NTT_TILE_LOG];
+    __shared__ uint4 tw[1 << (NTT_MAX_K - 1)];
+    const int k = a.k, c = a.c;
+    const int lb = a.log_d - a.layer0 - k;
+    const int mid_bits = lb - c;
+    const uint64_t tile = blockIdx.x;
+    const uint64_t mid = tile & ((1ull << mid_bits) - 1);
+    const uint64_t hi = tile >> mid_bits;
+    const uint64_t base = (hi << (lb + k)) | (mid << c);
+    const int n_el = 1 << (k + c);
+    const int cmask = (1 << c) - 1;
+
+    for (int L = threadIdx.x; L < n_el; L += BLOCK)
+        s[L] = a.data[base | ((uint64_t)(L >> c) << lb) | (uint64_t)(L & cmask)];
+    __syncthreads();
+
+    for (int jj = 0; jj < k; jj++) {
+        const int j = INVERSE ? (k - 1 - jj) : jj;
+        const int layer = a.layer0 + j;
+        const uint4 *Wl = a.W + layer * W_STRIDE;
+        // twiddle(layer, (hi << j) | loc) for the 2^j local blocks
+        for (int loc = threadIdx.x; loc < (1 << j); loc += BLOCK) {
+            uint4 t = make_uint4(0, 0, 0, 0);
+            for (int i = 0; i < j; i++)
+                if ((loc >> i) & 1) t = gxor(t, Wl[i]);
+            for (int i = 0; i < a.layer0; i++)
+                if ((hi >> i) & 1) t = gxor(t, Wl[j + i]);
+            tw[loc] = t;
+        }
+        __syncthreads();
+        const int pos = c + (k - 1 - j);
+        const int lowmask = (1 << pos) - 1;
+        for (int P = threadIdx.x; P < (n_el >> 1); P += BLOCK) {
+            const int L0 = ((P >> pos) << (pos + 1)) | (P & lowmask);
+            const int L1 = L0 | (1 << pos);
+            const uint4 t = tw[(L0 >> c) >> (k - j)];
+            uint4 u = s[L0], v = s[L1];
+            if (!INVERSE) {
+                u = gxor(u, gf_mul(v, t));
+                v = gxor(v, u);
+            } else {
+                v = gxor(v, u);
+                u = gxor(u, gf_mul(v, t));
+            }
+            s[L0] = u;
+            s[L1] = v;
+        }
+        __syncthreads();
+    }
+
+    for (int L = threadIdx.x; L < n_el; L += BLOCK)
+        a.data[base | ((uint64_t)(L >> c) << lb) | (uint64_t)(L & cmask)] = s[L];
+}
+
+hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_early, int skip_late,
+                      bool inverse, hipStream_t s) {
+    const int first = skip_early, end = log_d - skip_late;
+    const int n_layers = end - first;
+    if (n_layers <= 0) return hipSuccess;
+    const int n_passes = (n_layers + NTT_MAX_K - 1) / NTT_MAX_K;
+    // balanced split of the layer range into passes
+    int starts[64], ks[64];
+    int at = first;
+    for (int p = 0; p < n_passes; p++) {
+        const int left = end - at, passes_left = n_passes - p;
+        const int k = (left + passes_left - 1) / passes_left;
+        starts[p] = at;
+        ks[p] = k;
+        at += k;
+    }
+    for (int q = 0; q < n_passes; q++) {
+        const int p = inverse ? (n_passes - 1 - q) : q;
+        NttPassArgs a;
+        a.data = data;
+        a.W = dd.W;
+        a.log_d = log_d;
+        a.layer0 = starts[p];
+        a.k = ks[p];
+        const int lb = log_d - a.layer0 - a.k;
+        a.c = lb < (NTT_TILE_LOG - a.k) ? lb : (NTT_TILE_LOG - a.k);
+        const uint64_t n_tiles = 1ull << (log_d - a.k - a.c);
+        if (inverse)
+            k_ntt_pass<true><<<(unsigned)n_tiles, BLOCK, 0, s>>>(a);
+        else
+            k_ntt_pass<false><<<(unsigned)n_tiles, BLOCK, 0, s>>>(a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// Merkle leaves: SHA-256 over the LE16 bytes of `batch` consecutive elements
+// (hash/parallel_digest.rs:156-172).  Digest bytes are stored as 8 LE words.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_leaf_hash(const uint4 *__restrict__ elems, uint64_t n_leaves,
+                                                     uint64_t batch, uint4 *__restrict__ digests) {
+    const uint64_t leaf = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (leaf >= n_leaves) return;
+    const uint4 *p = elems + leaf * batch;
+    uint32_t st[8];
+    sha256_init(st);
+    uint32_t w[16];
+    const uint64_t n_full = batch >> 2;
+    for (uint64_t blk = 0; blk < n_full; blk++) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint4 e = p[blk * 4 + q];
+            w[4 * q + 0] = bswap32(e.x);
+            w[4 * q + 1] = bswap32(e.y);
+            w[4 * q + 2] = bswap32(e.z);
+            w[4 * q + 3] = bswap32(e.w);
+        }
+        sha256_compress(st, w);
+    }
+    const int rem = (int)(batch & 3);
+#pragma unroll
+    for (int q = 0; q < 16; q++) w[q] = 0;
+    for (int q = 0; q < rem; q++) {
+        const uint4 e = p[n_full * 4 + q];
+        w[4 * q + 0] = bswap32(e.x);
+        w[4 * q + 1] = bswap32(e.y);
+        w[4 * q + 2] = bswap32(e.z);
+        w[4 * q + 3] = bswap32(e.w);
+    }
+    w[4 * rem] = 0x80000000u;
+    const uint64_t bits = batch * 128ull;
+    w[14] = (uint32_t)(bits >> 32);
+    w[15] = (uint32_t)bits;
+    sha256_compress(st, w);
+    digests[2 * leaf] = make_uint4(bswap32(st[0]), bswap32(st[1]), bswap32(st[2]), bswap32(st[3]));
+    digests[2 * leaf + 1] = make_uint4(bswap32(st[4]), bswap32(st[5]), bswap32(st[6]), bswap32(st[7]));
+}
+
+hipError_t launch_leaf_hash(const uint4 *elems, uint64_t n_leaves, int, uint64_t batch, uint4 *digests,
+                            hipStream_t s) {
+    k_leaf_hash<<<grid_for(n_leaves), BLOCK, 0, s>>>(elems, n_leaves, batch, digests);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Inner nodes: Sha256Compression (hash/compress.rs:46-56): one compress256 of
+// left||right from the custom IV, output state written little-endian.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_merkle_level(const uint4 *__restrict__ prev, uint4 *__restrict__ next,
+                                                        uint64_t n_out, CompressIV iv) {
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n_out) return;
+    const uint4 l0 = prev[4 * i], l1 = prev[4 * i + 1], r0 = prev[4 * i + 2], r1 = prev[4 * i + 3];
+    uint32_t w[16] = {bswap32(l0.x), bswap32(l0.y), bswap32(l0.z), bswap32(l0.w),
+                      bswap32(l1.x), bswap32(l1.y), bswap32(l1.z), bswap32(l1.w),
+                      bswap32(r0.x), bswap32(r0.y), bswap32(r0.z), bswap32(r0.w),
+                      bswap32(r1.x), bswap32(r1.y), bswap32(r1.z), bswap32(r1.w)};
+    uint32_t st[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) st[q] = iv.w[q];
+    sha256_compress(st, w);
+    next[2 * i] = make_uint4(st[0], st[1], st[2], st[3]);
+    next[2 * i + 1] = make_uint4(st[4], st[5], st[6], st[7]);
+}
+
+hipError_t launch_merkle_levels(uint4 *tree, uint64_t n_leaves, const CompressIV &iv, hipStream_t s) {
+    uint4 *prev = tree;
+    uint64_t len = n_leaves;
+    while (len > 1) {
+        uint4 *next = prev + 2 * len;
+        k_merkle_level<<<grid_for(len / 2), BLOCK, 0, s>>>(prev, next, len / 2, iv);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        prev = next;
+        len >>= 1;
+    }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// FRI folds
+// ---------------------------------------------------------------------------
+constexpr int MAX_FOLD_CHAL = 12;
+struct FoldChallenges {
+    uint4 r[MAX_FOLD_CHAL];
+};
+
+// fold_interleaved (prover fri/fold.rs:260-282): out[i] = <cw[i*B..(i+1)*B], eq(r)>.
+// Computed as log_batch successive line folds (bit t of the in-row index pairs
+// with r_t, the eq tensor's bit order, multilinear/eq.rs:39-71); the result is
+// the same field element as the inner product.
+template <int MAXB>
+__global__ __launch_bounds__(BLOCK) void k_fold_interleaved(const uint4 *__restrict__ cw, uint4 *__restrict__ out,
+                                                            uint64_t n_out, int log_batch, FoldChallenges ch) {
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n_out) return;
+    uint4 v[MAXB];
+    const int B = 1 << log_batch;
+    for (int j = 0; j < B; j++) v[j] = cw[i * B + j];
+    int len = B;
+    for (int t = 0; t < log_batch; t++) {
+        len >>= 1;
+        for (int q = 0; q < len; q++) {
+            const uint4 x0 = v[2 * q], x1 = v[2 * q + 1];
+            v[q] = gxor(x0, gf_mul(gxor(x0, x1), ch.r[t]));
+        }
+    }
+    out[i] = v[0];
+}
+
+hipError_t launch_fold_interleaved(const uint4 *cw, uint4 *out, uint64_t n_out, int log_batch,
+                                   const uint4 *chal, hipStream_t s) {
+    FoldChallenges ch;
+    for (int t = 0; t < log_batch; t++) ch.r[t] = chal[t];
+    if (log_batch <= 4)
+        k_fold_interleaved<16><<<grid_for(n_out), BLOCK, 0, s>>>(cw, out, n_out, log_batch, ch);
+    else if (log_batch <= 6)
+        k_fold_interleaved<64><<<grid_for(n_out), BLOCK, 0, s>>>(cw, out, n_out, log_batch, ch);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// twiddle(layer, block) from the layer's F2 basis.
+__device__ __forceinline__ uint4 twiddle_of(const uint4 *__restrict__ W, int layer, uint64_t block) {
+    const uint4 *Wl = W + layer * W_STRIDE;
+    uint4 t = make_uint4(0, 0, 0, 0);
+    for (int b = 0; b < layer; b++)
+        if ((block >> b) & 1) t = gxor(t, Wl[b]);
+    return t;
+}
+
+// fold_codeword (prover fri/fold.rs:297-318) with fold_chunk / fold_pair
+// (verifier fri/fold.rs:21-99): per coset, n_chal rounds of inverse butterfly
+// and line extrapolation x0 + (x1 - x0) * r.
+template <int MAXV>
+__global__ __launch_bounds__(BLOCK) void k_fold_codeword(const uint4 *__restrict__ cw, uint4 *__restrict__ out,
+                                                         uint64_t n_out, int log_len, int n_chal,
+                                                         const uint4 *__restrict__ W, FoldChallenges ch) {
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n_out) return;
+    uint4 v[MAXV];
+    const int cs = 1 << n_chal;
+    for (int j = 0; j < cs; j++) v[j] = cw[(i << n_chal) + j];
+    int ll = log_len, log_size = n_chal;
+    for (int r = 0; r < n_chal; r++) {
+        for (int off = 0; off < (1 << (log_size - 1)); off++) {
+            const uint64_t idx = (i << (log_size - 1)) | (uint64_t)off;
+            const uint4 t = twiddle_of(W, ll - 1, idx);
+            uint4 u = v[off << 1], w = v[(off << 1) | 1];
+            w = gxor(w, u);
+            u = gxor(u, gf_mul(w, t));
+            v[off] = gxor(u, gf_mul(gxor(w, u), ch.r[r]));
+        }
+        ll--;
+        log_size--;
+    }
+    out[i] = v[0];
+}
+
+hipError_t launch_fold_codeword(const DeviceDomain &dd, const uint4 *cw, uint4 *out, int log_len,
+                                const uint4 *chal, int n_chal, hipStream_t s) {
+    FoldChallenges ch;
+    for (int t = 0; t < n_chal; t++) ch.r[t] = chal[t];
+    const uint64_t n_out = 1ull << (log_len - n_chal);
+    if (n_chal <= 4)
+        k_fold_codeword<16><<<grid_for(n_out), BLOCK, 0, s>>>(cw, out, n_out, log_len, n_chal, dd.W, ch);
+    else if (n_chal <= 8)
+        k_fold_codeword<256><<<grid_for(n_out), BLOCK, 0, s>>>(cw, out, n_out, log_len, n_chal, dd.W, ch);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// pack_witness (prover/prover/src/prove.rs:286-327), zero padded.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_pack_witness(const uint64_t *__restrict__ words, uint64_t n_words,
+                                                        uint4 *__restrict__ out, uint64_t n_elems) {
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n_elems) return;
+    const uint64_t w0 = 2 * i < n_words ? words[2 * i] : 0;
+    const uint64_t w1 = 2 * i + 1 < n_words ? words[2 * i + 1] : 0;
+    out[i] = make_uint4((uint32_t)w0, (uint32_t)(w0 >> 32), (uint32_t)w1, (uint32_t)(w1 >> 32));
+}
+
+hipError_t launch_pack_witness(const uint64_t *words, uint64_t n_words, uint4 *out, uint64_t n_elems,
+                               hipStream_t s) {
+    k_pack_witness<<<grid_for(n_elems), BLOCK, 0, s>>>(words, n_words, out, n_elems);
+    return hipGetLastError();
+}
+
+}  // namespace b64

@@ -1,0 +1,200 @@
+/*
+ * libb64hip: MI355X (gfx950) implementation of Binius64's polynomial-commitment
+ * hot path -- B128 additive NTT / Reed-Solomon encode, SHA-256 Merkle commit,
+ * FRI interleaved and coset folds, and the FRI fold/query prover -- behind a
+ * C ABI.
+ *
+ * Conventions
+ *  - Field elements are BinaryField128bGhash values, 16 bytes each, stored as the
+ *    little-endian u128 (verifier/utils/src/serialization.rs:66-72); as uint64_t
+ *    pairs they are (lo, hi).
+ *  - Pointers named d_* are device (HBM) pointers; every other pointer is host.
+ *  - `stream` is a hipStream_t (NULL = the null stream).  Calls are stream-ordered;
+ *    calls that return host data (roots, layers, branches) synchronize the stream.
+ *  - Every function returns a b64 status code; no exception crosses the ABI.
+ *    Error codes mirror the reference's error variants where one exists.
+ *  - Digests are the raw 32 bytes of Output<Sha256>.
+ *
+ * Each declaration cites the reference interface it replaces (paths relative to
+ * the reference repository root).
+ */
+#ifndef B64HIP_H
+#define B64HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define B64_OK 0
+#define B64_ERR_INVALID_ARGS 1                /* fri::Error::InvalidArgs, assert! preconditions */
+#define B64_ERR_ENCODER_SUBSPACE_MISMATCH 2   /* reed_solomon.rs:206 */
+#define B64_ERR_SUBSPACE_DIMENSION_MISMATCH 3 /* reed_solomon.rs:208 */
+#define B64_ERR_INCORRECT_BATCH_SIZE 4        /* merkle_tree::Error::IncorrectBatchSize */
+#define B64_ERR_POWER_OF_TWO_LENGTH_REQUIRED 5 /* merkle_tree::Error::PowerOfTwoLengthRequired */
+#define B64_ERR_EARLY_PROVER_FINISH 6         /* fri::Error::EarlyProverFinish */
+#define B64_ERR_INCORRECT_LAYER_DEPTH 7       /* merkle_tree::Error::IncorrectLayerDepth */
+#define B64_ERR_INDEX_OUT_OF_RANGE 8          /* merkle_tree::Error::IndexOutOfRange */
+#define B64_ERR_INVALID_FOLD_ARITY_SEQUENCE 9 /* fri::Error::InvalidFoldAritySequence */
+#define B64_ERR_BUFFER_TOO_SMALL 10
+#define B64_ERR_NOT_COMMIT_ROUND 11
+#define B64_ERR_NO_DEVICE 20
+#define B64_ERR_HIP 21
+#define B64_ERR_OUT_OF_MEMORY 22
+#define B64_ERR_ADMISSION_REJECTED 23
+
+typedef struct b64_domain b64_domain;
+typedef struct b64_transcript b64_transcript;
+typedef struct b64_fri_prover b64_fri_prover;
+
+/* FRIParams (verifier/verifier/src/fri/common.rs:15-31). */
+#define B64_MAX_FOLD_ARITIES 32
+typedef struct b64_fri_params {
+    uint32_t log_dim;        /* rs_code.log_dim() */
+    uint32_t log_inv_rate;   /* rs_code.log_inv_rate() */
+    uint32_t log_batch_size;
+    uint32_t n_test_queries;
+    uint32_t n_fold_arities;
+    uint32_t fold_arities[B64_MAX_FOLD_ARITIES];
+    uint32_t log_terminal_dim; /* n_final_challenges() */
+} b64_fri_params;
+
+/* ---- library / device ----------------------------------------------------- */
+const char *b64_version(void);
+const char *b64_status_string(int status);
+/* Number of visible HIP devices (0 when none). */
+int b64_device_count(void);
+int b64_device_alloc(size_t bytes, void **d_ptr);
+int b64_device_free(void *d_ptr);
+int b64_memcpy_htod(void *d_dst, const void *src, size_t bytes, void *stream);
+int b64_memcpy_dtoh(void *dst, const void *d_src, size_t bytes, void *stream);
+int b64_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes, void *stream);
+int b64_stream_synchronize(void *stream);
+
+/* ---- field ------------------------------------------------------------------ */
+/* BinaryField128bGhash Mul (verifier/field/src/ghash.rs:188-195), host side. */
+void b64_gf_mul(const uint64_t a[2], const uint64_t b[2], uint64_t out[2]);
+/* invert_or_zero (arch/portable/packed_ghash_128.rs:117-128), host side. */
+void b64_gf_inv(const uint64_t a[2], uint64_t out[2]);
+/* Element-wise d_out[i] = d_a[i] * d_b[i] on the device (the device multiply
+ * every kernel uses; exposed for parity testing). */
+int b64_gf_mul_device(const void *d_a, const void *d_b, void *d_out, uint64_t n, void *stream);
+/* d_out[i] = d_a[i] * c for a constant c through the LDS window-table path. */
+int b64_gf_mul_const_device(const void *d_a, const uint64_t c[2], void *d_out, uint64_t n, void *stream);
+
+/* ---- DomainContext (verifier/math/src/ntt/mod.rs:118-179) --------------------- */
+/* GenericPreExpanded::generate_from_subspace (ntt/domain_context.rs:99-132) over
+ * an explicit basis of `dim` elements. */
+int b64_domain_create(const uint64_t *basis, uint32_t dim, b64_domain **out);
+/* ... over BinarySubspace::with_dim(dim) (verifier/math/src/binary_subspace.rs:81-86). */
+int b64_domain_create_standard(uint32_t dim, b64_domain **out);
+void b64_domain_destroy(b64_domain *d);
+/* DomainContext::log_domain_size (ntt/mod.rs:130) */
+uint32_t b64_domain_log_size(const b64_domain *d);
+/* DomainContext::subspace(i) basis, i elements (ntt/mod.rs:143) */
+int b64_domain_subspace(const b64_domain *d, uint32_t i, uint64_t *basis_out);
+/* DomainContext::twiddle(layer, block) (ntt/mod.rs:162) */
+int b64_domain_twiddle(const b64_domain *d, uint32_t layer, uint64_t block, uint64_t out[2]);
+
+/* ---- AdditiveNTT (verifier/math/src/ntt/mod.rs:58-113) ------------------------ */
+/* forward_transform(data, skip_early, skip_late) in place on 2^log_len elements;
+ * preconditions of input_check (ntt/reference.rs:89-100) return INVALID_ARGS. */
+int b64_ntt_forward(const b64_domain *d, void *d_data, uint32_t log_len, uint32_t skip_early,
+                    uint32_t skip_late, void *stream);
+/* inverse_transform (ntt/reference.rs:49-76) */
+int b64_ntt_inverse(const b64_domain *d, void *d_data, uint32_t log_len, uint32_t skip_early,
+                    uint32_t skip_late, void *stream);
+
+/* ---- ReedSolomonCode::encode_batch (verifier/math/src/reed_solomon.rs:128-201) -- */
+/* The code is ReedSolomonCode::with_domain_context_subspace(d, log_dim, log_inv_rate)
+ * (reed_solomon.rs:54-65): SUBSPACE_DIMENSION_MISMATCH if log_dim+log_inv_rate
+ * exceeds the domain.  d_msg holds 2^(log_dim+log_batch) elements, d_out
+ * 2^(log_dim+log_batch+log_inv_rate). */
+int b64_rs_encode_batch(const b64_domain *d, uint32_t log_dim, uint32_t log_inv_rate, const void *d_msg,
+                        uint32_t log_batch, void *d_out, void *stream);
+
+/* ---- MerkleTreeProver (prover/prover/src/merkle_tree/mod.rs:16-70) ----------- */
+/* BinaryMerkleTreeProver::commit (mod.rs:26-38, prover.rs:93-118) with
+ * Sha256 leaves and Sha256Compression nodes.  d_tree receives the flattened
+ * inner_nodes [leaves | ... | root], (2*n_leaves - 1) * 32 bytes. */
+int b64_merkle_commit(const void *d_elems, uint64_t n_elems, uint64_t batch_size, void *d_tree,
+                      uint8_t root_out[32], void *stream);
+/* BinaryMerkleTree::layer (binary_merkle_tree.rs:142-149): 2^depth digests. */
+int b64_merkle_layer(const void *d_tree, uint32_t log_len, uint32_t depth, uint8_t *out, void *stream);
+/* BinaryMerkleTree::branch (binary_merkle_tree.rs:154-169): log_len-depth digests. */
+int b64_merkle_branch(const void *d_tree, uint32_t log_len, uint64_t index, uint32_t depth, uint8_t *out,
+                      void *stream);
+
+/* ---- FRI folds (prover/prover/src/fri/fold.rs:260-318) ------------------------ */
+/* fold_interleaved: d_cw has 2^(log_len+log_batch) elements, challenges log_batch
+ * host elements, d_out 2^log_len elements. */
+int b64_fri_fold_interleaved(const void *d_cw, uint32_t log_len, uint32_t log_batch,
+                             const uint64_t *challenges, void *d_out, void *stream);
+/* fold_codeword: d_cw has 2^log_len elements, n_chal host challenges,
+ * d_out 2^(log_len-n_chal) elements. */
+int b64_fri_fold_codeword(const b64_domain *d, const void *d_cw, uint32_t log_len,
+                          const uint64_t *challenges, uint32_t n_chal, void *d_out, void *stream);
+
+/* ---- FRIParams (verifier/verifier/src/fri/common.rs) -------------------------- */
+uint32_t b64_calculate_n_test_queries(uint32_t security_bits, uint32_t log_inv_rate); /* :182-186 */
+uint32_t b64_estimate_optimal_arity(uint32_t log_block_length, uint32_t digest_size,
+                                    uint32_t field_size); /* :191-219 */
+/* FRIParams::new (:35-54) */
+int b64_fri_params_new(uint32_t log_dim, uint32_t log_inv_rate, uint32_t log_batch_size,
+                       const uint32_t *fold_arities, uint32_t n_fold_arities, uint32_t n_test_queries,
+                       b64_fri_params *out);
+/* FRIParams::choose_with_constant_fold_arity (:64-115) for an NTT over `d`. */
+int b64_fri_params_choose(const b64_domain *d, uint32_t log_msg_len, uint32_t security_bits,
+                          uint32_t log_inv_rate, uint32_t arity, b64_fri_params *out);
+/* Verifier::setup + Prover::setup parameter chain (verifier/verifier/src/verify.rs:61-106,
+ * prover/prover/src/prove.rs:93-115): FRI params for 2^log_witness_elems B128
+ * elements, and the prover's NTT domain (GenericPreExpanded over the RS code's
+ * subspace).  The caller destroys *domain_out. */
+int b64_setup_commit_params(uint32_t log_witness_elems, uint32_t log_inv_rate, uint32_t security_bits,
+                            b64_fri_params *params_out, b64_domain **domain_out);
+
+/* ---- commit_interleaved (prover/prover/src/fri/commit.rs:25-75) --------------- */
+/* d_codeword: 2^(rs_log_len+log_batch) elements; d_tree: (2^(rs_log_len+1)-1)*32 bytes. */
+int b64_fri_commit_interleaved(const b64_fri_params *p, const b64_domain *d, const void *d_msg,
+                               void *d_codeword, void *d_tree, uint8_t root_out[32], void *stream);
+
+/* ---- ProverTranscript<HasherChallenger<Sha256>> (verifier/transcript/src) ----- */
+int b64_transcript_create(b64_transcript **out);
+void b64_transcript_destroy(b64_transcript *t);
+void b64_transcript_message(b64_transcript *t, const uint8_t *bytes, size_t n);   /* transcript.rs:318 */
+void b64_transcript_observe(b64_transcript *t, const uint8_t *bytes, size_t n);   /* transcript.rs:283 */
+void b64_transcript_decommit(b64_transcript *t, const uint8_t *bytes, size_t n);  /* transcript.rs:308 */
+void b64_transcript_sample(b64_transcript *t, uint8_t *out, size_t n);            /* CanSample, :403 */
+uint32_t b64_transcript_sample_bits(b64_transcript *t, uint32_t bits);            /* :413 */
+size_t b64_transcript_tape(const b64_transcript *t, uint8_t *out, size_t cap);    /* finalize, :245 */
+
+/* ---- FRIFoldProver / FRIQueryProver (prover/prover/src/fri/fold.rs, query.rs) - */
+/* FRIFoldProver::new (fold.rs:57-82).  The prover keeps pointers to the committed
+ * codeword and tree (device) and owns the folded oracles it commits. */
+int b64_fri_prover_create(const b64_fri_params *p, const b64_domain *d, const void *d_codeword,
+                          const void *d_tree, void *stream, b64_fri_prover **out);
+void b64_fri_prover_destroy(b64_fri_prover *fp);
+uint32_t b64_fri_prover_n_rounds(const b64_fri_prover *fp);              /* fold.rs:85-87 */
+uint32_t b64_fri_prover_curr_round(const b64_fri_prover *fp);            /* fold.rs:90-92 */
+int b64_fri_prover_receive_challenge(b64_fri_prover *fp, const uint64_t challenge[2]); /* :107-110 */
+/* execute_fold_round (fold.rs:117-175): *committed = 1 and root_out set when the
+ * round commits a folded oracle. */
+int b64_fri_prover_execute_fold_round(b64_fri_prover *fp, uint8_t root_out[32], int *committed);
+/* finish_proof (fold.rs:221-244): terminal codeword, optimal layers and the
+ * n_test_queries coset openings written to the transcript's tape. */
+int b64_fri_prover_finish_proof(b64_fri_prover *fp, b64_transcript *t);
+/* Committed folded oracle k (0-based): device pointer and log length (for tests). */
+int b64_fri_prover_oracle(const b64_fri_prover *fp, uint32_t k, const void **d_codeword,
+                          uint32_t *log_len, const void **d_tree);
+
+/* ---- witness packing (prover/prover/src/prove.rs:286-327) --------------------- */
+/* B128 element i = word[2i+1] << 64 | word[2i]; zero-padded to 2^log_elems. */
+int b64_pack_witness(const void *d_words, uint64_t n_words, uint32_t log_elems, void *d_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

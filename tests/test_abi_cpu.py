@@ -1,0 +1,122 @@
+"""CPU-side checks of libb64hip.so: the library loads, exports every symbol the
+header declares, and its host logic (domain twiddles, FRI parameters, the
+Fiat-Shamir transcript, host field arithmetic) agrees with the oracle.  No
+device call is made here."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+@pytest.fixture(scope="module")
+def b64():
+    import binius64_amd as b
+    b.lib()
+    return b
+
+
+def declared_functions():
+    src = (ROOT / "include" / "b64hip.h").read_text()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(b64_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol(b64):
+    names = declared_functions()
+    assert len(names) > 40
+    L = b64.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    from binius64_amd._abi import SIGNATURES
+    assert set(SIGNATURES) == set(names), set(SIGNATURES) ^ set(names)
+
+
+def test_version_and_status_strings(b64):
+    assert b64.lib().b64_version().decode().count(".") == 2
+    assert b"subspace" in b64.lib().b64_status_string(2)
+
+
+def test_host_field_matches_oracle(b64, orc):
+    a = orc.random_elems(1, 64)
+    b = orc.random_elems(2, 64)
+    want = orc.to_ints(orc.gf_mul(a, b))
+    got = [b64.gf_mul(x, y) for x, y in zip(orc.to_ints(a), orc.to_ints(b))]
+    assert got == want
+    inv = [b64.gf_inv(x) for x in orc.to_ints(a)]
+    assert inv == orc.to_ints(orc.gf_inv(a))
+
+
+@pytest.mark.parametrize("basis", [None, [5, 7, 22, 95, 0x1234, 0xABCDEF0123]])
+def test_domain_matches_oracle(b64, orc, basis):
+    if basis is None:
+        dom, ref = b64.DomainContext.standard(12), orc.Domain(dim=12)
+    else:
+        dom, ref = b64.DomainContext(orc.to_elems(basis)), orc.Domain(orc.to_elems(basis))
+    d = dom.log_domain_size()
+    assert d == ref.dim
+    for i in range(d + 1):
+        assert np.array_equal(dom.subspace(i), ref.subspace(i))
+    for layer in range(d):
+        for block in list(range(min(1 << layer, 64))) + [(1 << layer) - 1]:
+            assert dom.twiddle(layer, block) == ref.twiddle(layer, block)
+
+
+@pytest.mark.parametrize("n", [8, 12, 16, 20, 24, 26])
+def test_setup_commit_params_match_oracle(b64, orc, n):
+    p, dom = b64.setup_commit_params(n, 1, 96)
+    ref = orc.fri_params_setup(n, 1, 96)
+    assert p.log_batch_size == ref["log_batch_size"]
+    assert p.log_dim == ref["log_dim"]
+    assert p.rs_log_len() == ref["rs_log_len"]
+    assert p.fold_arities == ref["fold_arities"]
+    assert p.n_test_queries == 232
+    # the prover domain is GenericPreExpanded over the Verifier::setup RS subspace
+    std = orc.Domain(dim=n + 1)
+    rs_basis = std.subspace(p.rs_log_len())
+    assert np.array_equal(dom.subspace(p.rs_log_len()), rs_basis)
+    ref_dom = orc.Domain(rs_basis)
+    for layer in range(dom.log_domain_size()):
+        assert dom.twiddle(layer, (1 << layer) - 1) == ref_dom.twiddle(layer, (1 << layer) - 1)
+
+
+def test_fri_param_kats(b64):
+    assert b64.calculate_n_test_queries(96, 1) == 232
+    assert b64.calculate_n_test_queries(96, 2) == 142
+    for L in range(22, 35):
+        assert b64.estimate_optimal_arity(L, 256, 128) == 4
+    with pytest.raises(b64.B64Error) as e:
+        b64.FRIParams.new(4, 1, 0, [3, 2], 10)
+    assert e.value.code == 9  # InvalidFoldAritySequence
+
+
+def test_transcript_matches_oracle(b64, orc):
+    rng = np.random.default_rng(5)
+    t, r = b64.ProverTranscript(), orc.Transcript()
+    for step in range(60):
+        op = step % 5
+        data = rng.integers(0, 256, int(rng.integers(0, 200)), dtype=np.uint8).tobytes()
+        if op == 0:
+            t.message(data), r.message(data)
+        elif op == 1:
+            t.observe(data), r.observe(data)
+        elif op == 2:
+            t.decommit(data), r.decommit(data)
+        elif op == 3:
+            n = int(rng.integers(1, 70))
+            assert t.sample_bytes(n) == r.sample(n)
+        else:
+            bits = int(rng.integers(0, 40))
+            assert t.sample_bits(bits) == r.sample_bits(bits)
+    assert t.finalize() == r.tape()
+
+
+def test_device_calls_report_no_device_or_run(b64):
+    """Without a GPU, device entry points must fail loudly with NO_DEVICE."""
+    if b64.device_count() > 0:
+        pytest.skip("a device is present")
+    with pytest.raises(b64.B64Error) as e:
+        b64.DeviceBuffer(64)
+    assert e.value.code == 20

@@ -1,0 +1,223 @@
+"""GPU parity: every kernel path through the C ABI against the oracle, bit-exact.
+
+Sizes are ones the oracle finishes in seconds; the full-size configurations are
+covered in test_gpu_large.py through the oracle's multithreaded restatement and
+size-independent properties.
+"""
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.loads((Path(__file__).parent / "golden" / "reference_vectors.json").read_text())
+
+
+@pytest.fixture(scope="module")
+def b64():
+    import binius64_amd as b
+    if b.device_count() == 0:
+        pytest.fail("no HIP device visible")
+    return b
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_device_mul_kats_and_random(b64, orc):
+    kats = GOLD["ghash_mul_kats"]
+    a = orc.to_elems(int(x[0], 16) for x in kats)
+    b = orc.to_elems(int(x[1], 16) for x in kats)
+    assert orc.to_ints(b64.gf_mul_device(a, b)) == [int(x[2], 16) for x in kats]
+    a = orc.random_elems(1, 1 << 16)
+    b = orc.random_elems(2, 1 << 16)
+    assert np.array_equal(b64.gf_mul_device(a, b), orc.gf_mul(a, b))
+
+
+def test_device_mul_aes_embedding(b64, orc):
+    phi = orc.to_elems(int(x, 16) for x in GOLD["aes_to_ghash"])
+    A, B = np.meshgrid(np.arange(256), np.arange(256), indexing="ij")
+    got = b64.gf_mul_device(phi[A.ravel()], phi[B.ravel()])
+    assert np.array_equal(got, orc.gf_mul(phi[A.ravel()], phi[B.ravel()]))
+
+
+@pytest.mark.parametrize("c", [0, 1, 2, 0x87, (1 << 127), 0x494EF99794D5244F9152DF59D87A9186])
+def test_device_mul_const_table(b64, orc, c):
+    a = orc.random_elems(3, 5000)
+    got = b64.gf_mul_const_device(a, c)
+    assert np.array_equal(got, orc.gf_mul(a, orc.to_elems([c] * a.shape[0])))
+
+
+def ntt_dev(b64, dom, data, se, sl, inverse=False):
+    buf = b64.DeviceBuffer.from_host(data)
+    ntt = b64.AdditiveNTT(dom)
+    n = data.shape[0]
+    log_d = n.bit_length() - 1
+    if inverse:
+        ntt.inverse_transform(buf, log_d, se, sl)
+    else:
+        ntt.forward_transform(buf, log_d, se, sl)
+    return buf.to_host(n)
+
+
+@pytest.mark.parametrize("log_d", [0, 1, 2, 5, 8, 11, 12, 14, 17])
+def test_ntt_forward_windows(b64, orc, log_d):
+    dom, ref = b64.DomainContext.standard(log_d + 1), orc.Domain(dim=log_d + 1)
+    data = orc.random_elems(100 + log_d, 1 << log_d)
+    h = log_d // 2
+    for se, sl in sorted({(0, 0), (1, 0), (0, 1), (h, 0), (0, log_d - h), (1, 4), (3, 2)}):
+        if se + sl > log_d:
+            continue
+        say("ntt", log_d, se, sl)
+        assert np.array_equal(ntt_dev(b64, dom, data, se, sl), orc.ntt_forward(ref, data, se, sl)), (se, sl)
+
+
+@pytest.mark.parametrize("basis", [[5, 7, 22, 95], [0x1234, 0x98765, 0xABCDEF, 0x1111111111, 0x3 << 100, 7, 9, 1 << 77]])
+def test_ntt_nonstandard_basis(b64, orc, basis):
+    dom, ref = b64.DomainContext(orc.to_elems(basis)), orc.Domain(orc.to_elems(basis))
+    d = len(basis)
+    data = orc.random_elems(9, 1 << d)
+    assert np.array_equal(ntt_dev(b64, dom, data, 0, 0), orc.ntt_forward(ref, data, 0, 0))
+
+
+@pytest.mark.parametrize("log_d", [1, 6, 13])
+def test_ntt_inverse(b64, orc, log_d):
+    dom, ref = b64.DomainContext.standard(log_d), orc.Domain(dim=log_d)
+    data = orc.random_elems(4, 1 << log_d)
+    for se, sl in ((0, 0), (1, 0), (0, 1), (1, 1)):
+        if se + sl > log_d:
+            continue
+        got = ntt_dev(b64, dom, data, se, sl, inverse=True)
+        assert np.array_equal(got, orc.ntt_inverse(ref, data, se, sl))
+        assert np.array_equal(ntt_dev(b64, dom, got, se, sl), data)
+
+
+def test_ntt_precondition_errors(b64):
+    dom = b64.DomainContext.standard(4)
+    buf = b64.DeviceBuffer(16 << 6)
+    with pytest.raises(b64.B64Error):
+        b64.AdditiveNTT(dom).forward_transform(buf, 6, 0, 0)
+    with pytest.raises(b64.B64Error):
+        b64.AdditiveNTT(dom).forward_transform(buf, 3, 2, 2)
+
+
+@pytest.mark.parametrize("log_dim,lir,lbs", [(4, 2, 0), (6, 2, 1), (8, 3, 2), (1, 2, 0), (16, 1, 0), (12, 1, 4)])
+def test_encode_batch(b64, orc, log_dim, lir, lbs):
+    """Config 1 is (16, 1, 0): 2^16 message elements, standard 17-dim subspace."""
+    say("encode", log_dim, lir, lbs)
+    dom, ref = b64.DomainContext.standard(log_dim + lir), orc.Domain(dim=log_dim + lir)
+    msg = orc.random_elems(50, 1 << (log_dim + lbs))
+    code = b64.ReedSolomonCode(dom, log_dim, lir)
+    out = code.encode_batch(b64.DeviceBuffer.from_host(msg), lbs)
+    got = out.to_host(1 << (log_dim + lir + lbs))
+    assert np.array_equal(got, orc.encode_batch(ref, msg, log_dim, lir, lbs, threads=8))
+
+
+def test_encode_dimension_mismatch(b64):
+    dom = b64.DomainContext.standard(5)
+    with pytest.raises(b64.B64Error) as e:
+        b64.ReedSolomonCode(dom, 5, 1).encode_batch(b64.DeviceBuffer(16 << 5), 0)
+    assert e.value.code == 3
+
+
+@pytest.mark.parametrize("n_elems,batch", [(1, 1), (2, 1), (16, 16), (64, 4), (1 << 12, 16), (256, 2),
+                                           (1 << 10, 128), (8, 8), (12, 3)])
+def test_merkle_commit(b64, orc, n_elems, batch):
+    elems = orc.random_elems(60 + n_elems, n_elems)
+    root, tree = b64.BinaryMerkleTreeProver.commit(b64.DeviceBuffer.from_host(elems), n_elems, batch)
+    ref = orc.merkle_build(elems, batch)
+    n_leaves = n_elems // batch
+    got = np.frombuffer(tree.to_bytes(32 * (2 * n_leaves - 1)), dtype=np.uint8).reshape(-1, 32)
+    assert np.array_equal(got, ref)
+    assert root == ref[-1].tobytes()
+    log_len = n_leaves.bit_length() - 1
+    for depth in range(log_len + 1):
+        layer = b64.BinaryMerkleTreeProver.layer(tree, log_len, depth)
+        start = (2 * n_leaves - 1) + 1 - (2 << depth)
+        assert layer == [ref[start + i].tobytes() for i in range(1 << depth)]
+    for idx in (0, n_leaves - 1, n_leaves // 3):
+        br = b64.BinaryMerkleTreeProver.branch(tree, log_len, idx, 0)
+        assert br == [x.tobytes() for x in orc.merkle_branch(ref, log_len, idx, 0)]
+
+
+def test_merkle_errors(b64):
+    buf = b64.DeviceBuffer(16 * 12)
+    with pytest.raises(b64.B64Error) as e:
+        b64.BinaryMerkleTreeProver.commit(buf, 12, 5)
+    assert e.value.code == 4
+    with pytest.raises(b64.B64Error) as e:
+        b64.BinaryMerkleTreeProver.commit(buf, 12, 4)
+    assert e.value.code == 5
+
+
+@pytest.mark.parametrize("log_len,lbs", [(0, 0), (3, 1), (10, 4), (14, 4), (9, 6)])
+def test_fold_interleaved(b64, orc, log_len, lbs):
+    cw = orc.random_elems(70, 1 << (log_len + lbs))
+    chal = orc.random_elems(71, lbs)
+    out = b64.DeviceBuffer(16 << log_len)
+    d_cw = b64.DeviceBuffer.from_host(cw)
+    ch = np.ascontiguousarray(chal, dtype=np.uint64)
+    b64.check(b64.lib().b64_fri_fold_interleaved(d_cw.ptr, log_len, lbs, b64._p64(ch) if lbs else None, out.ptr, None))
+    assert np.array_equal(out.to_host(1 << log_len), orc.fold_interleaved(cw, log_len, lbs, chal))
+
+
+@pytest.mark.parametrize("log_len,n_chal", [(1, 1), (5, 3), (13, 4), (17, 4), (9, 1), (12, 6)])
+def test_fold_codeword(b64, orc, log_len, n_chal):
+    dom, ref = b64.DomainContext.standard(log_len + 2), orc.Domain(dim=log_len + 2)
+    cw = orc.random_elems(80, 1 << log_len)
+    chal = orc.random_elems(81, n_chal)
+    d_cw = b64.DeviceBuffer.from_host(cw)
+    out = b64.DeviceBuffer(16 << (log_len - n_chal))
+    ch = np.ascontiguousarray(chal, dtype=np.uint64)
+    b64.check(b64.lib().b64_fri_fold_codeword(dom.ptr, d_cw.ptr, log_len, b64._p64(ch), n_chal, out.ptr, None))
+    assert np.array_equal(out.to_host(1 << (log_len - n_chal)), orc.fold_codeword(ref, cw, log_len, chal))
+
+
+def test_pack_witness(b64, orc):
+    words = orc.random_words(3, 1000)
+    d = b64.DeviceBuffer.from_host(words)
+    out = b64.pack_witness(d, 1000, 10)
+    got = out.to_host(1024)
+    want = np.zeros((1024, 2), dtype=np.uint64)
+    want[:500] = orc.pack_witness(words)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("log_dim,lir,lbs,arities,nq", [
+    (8, 2, 0, [1] * 7, 3), (8, 2, 0, [3, 2, 1], 3), (6, 2, 2, [3, 2, 1], 3), (4, 2, 2, [], 3),
+    (8, 1, 4, [4, 1], 232), (10, 1, 4, [4, 4, 1], 232),
+])
+def test_fri_proof_tape_matches_oracle(b64, orc, log_dim, lir, lbs, arities, nq):
+    say("fri", log_dim, lir, lbs, arities)
+    dom, ref = b64.DomainContext.standard(log_dim + lir), orc.Domain(dim=log_dim + lir)
+    msg = orc.random_elems(90, 1 << (log_dim + lbs))
+    params = b64.FRIParams.new(log_dim, lir, lbs, arities, nq)
+    tape, roots, chal = b64.fri_prove(params, dom, msg)
+    want = orc.fri_prove(ref, msg, log_dim, lir, lbs, arities, nq, threads=8)
+    assert roots == want["roots"]
+    assert orc.to_ints(want["challenges"]) == chal
+    assert tape == want["tape"]
+
+
+def test_fri_commit_config2_setup_chain(b64, orc):
+    """Config 2: 2^21 random u64 witness words -> pack_witness -> 2^20 B128,
+    Verifier::setup parameters (rs (16,1), batch 2^4, arities [4,4,1], 232 queries);
+    every root, every folded oracle and the proof tape are byte-identical."""
+    n = 20
+    params, dom = b64.setup_commit_params(n, 1, 96)
+    assert (params.log_dim, params.log_batch_size, params.fold_arities) == (16, 4, [4, 4, 1])
+    words = orc.random_words(1, 1 << (n + 1))
+    d_words = b64.DeviceBuffer.from_host(words)
+    msg_dev = b64.pack_witness(d_words, 1 << (n + 1), n)
+    msg = orc.pack_witness(words)
+    assert np.array_equal(msg_dev.to_host(1 << n), msg)
+    tape, roots, chal = b64.fri_prove(params, dom, msg)
+    ref = orc.Domain(dom.subspace(params.rs_log_len()))
+    want = orc.fri_prove(ref, msg, params.log_dim, 1, params.log_batch_size, params.fold_arities, 232, threads=8)
+    assert roots == want["roots"]
+    assert tape == want["tape"]
