@@ -71,10 +71,13 @@ def main():
     chal = np.random.default_rng(4).integers(0, 2 ** 63, size=(lbs, 2), dtype=np.uint64)
     chal_p = chal.ctypes.data_as(C.POINTER(C.c_uint64))
 
+    pc = params.to_c()
+
     def step():
-        b64.check(L.b64_rs_encode_batch(dom.ptr, log_dim, lir, msg.data_ptr(), lbs, cw.data_ptr(), sp), "encode")
-        b64.check(L.b64_merkle_commit(cw.data_ptr(), 1 << log_cw, 1 << lbs, tree.data_ptr(), None, sp), "merkle")
-        b64.check(L.b64_fri_fold_interleaved(cw.data_ptr(), rs_log_len, lbs, chal_p, folded.data_ptr(), sp), "fold")
+        # encode, then one pass over the codeword for leaf digests + the folded
+        # oracle, then the inner tree levels (b64_fri_commit_fold)
+        b64.check(L.b64_fri_commit_fold(C.byref(pc), dom.ptr, msg.data_ptr(), cw.data_ptr(), tree.data_ptr(),
+                                        chal_p, folded.data_ptr(), None, sp), "commit_fold")
 
     for _ in range(args.warmup):
         step()

@@ -27,7 +27,7 @@ __all__ = [
     "B64Error", "DeviceBuffer", "DomainContext", "AdditiveNTT", "ReedSolomonCode",
     "BinaryMerkleTreeProver", "FRIParams", "CommitOutput", "commit_interleaved", "setup_commit_params",
     "ProverTranscript", "FRIFoldProver", "gf_mul", "gf_inv", "gf_mul_device", "gf_mul_const_device",
-    "pack_witness", "device_count", "LIB_PATH", "lib",
+    "pack_witness", "device_count", "config_set", "config_get", "LIB_PATH", "lib",
 ]
 
 
@@ -51,6 +51,17 @@ def _stream(stream) -> Optional[int]:
 
 def device_count() -> int:
     return lib().b64_device_count()
+
+
+def config_set(key: str, value: int) -> None:
+    """Set a kernel tuning knob (see b64_config_set in include/b64hip.h)."""
+    check(lib().b64_config_set(key.encode(), int(value)), f"config_set {key}")
+
+
+def config_get(key: str) -> int:
+    v = C.c_int64()
+    check(lib().b64_config_get(key.encode(), C.byref(v)), f"config_get {key}")
+    return v.value
 
 
 def _elems(x) -> np.ndarray:

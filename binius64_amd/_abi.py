@@ -42,6 +42,8 @@ SIGNATURES = {
     "b64_memcpy_dtoh": (C.c_int, [vp, vp, C.c_size_t, vp]),
     "b64_memcpy_dtod": (C.c_int, [vp, vp, C.c_size_t, vp]),
     "b64_stream_synchronize": (C.c_int, [vp]),
+    "b64_config_set": (C.c_int, [C.c_char_p, C.c_int64]),
+    "b64_config_get": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64)]),
     "b64_gf_mul": (None, [u64p, u64p, u64p]),
     "b64_gf_inv": (None, [u64p, u64p]),
     "b64_gf_mul_device": (C.c_int, [vp, vp, vp, C.c_uint64, vp]),
@@ -69,6 +71,7 @@ SIGNATURES = {
     "b64_setup_commit_params": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(FriParamsC),
                                           C.POINTER(vp)]),
     "b64_fri_commit_interleaved": (C.c_int, [C.POINTER(FriParamsC), vp, vp, vp, vp, u8p, vp]),
+    "b64_fri_commit_fold": (C.c_int, [C.POINTER(FriParamsC), vp, vp, vp, vp, u64p, vp, u8p, vp]),
     "b64_transcript_create": (C.c_int, [C.POINTER(vp)]),
     "b64_transcript_destroy": (None, [vp]),
     "b64_transcript_message": (None, [vp, u8p, C.c_size_t]),

@@ -172,6 +172,32 @@ int b64_stream_synchronize(void *stream) {
     return B64_OK;
 }
 
+int b64_config_set(const char *key, int64_t value) {
+    if (!key) return B64_ERR_INVALID_ARGS;
+    if (!strcmp(key, "ntt.variant")) {
+        if (value < 0 || value > 1) return B64_ERR_INVALID_ARGS;
+        b64::g_ntt_tuning.variant = (int)value;
+    } else if (!strcmp(key, "ntt.max_k")) {
+        if (value < 1 || value > 7) return B64_ERR_INVALID_ARGS;
+        b64::g_ntt_tuning.max_k = (int)value;
+    } else if (!strcmp(key, "ntt.tile_log")) {
+        if (value < 1 || value > 11) return B64_ERR_INVALID_ARGS;
+        b64::g_ntt_tuning.tile_log = (int)value;
+    } else {
+        return B64_ERR_INVALID_ARGS;
+    }
+    return B64_OK;
+}
+
+int b64_config_get(const char *key, int64_t *value) {
+    if (!key || !value) return B64_ERR_INVALID_ARGS;
+    if (!strcmp(key, "ntt.variant")) *value = b64::g_ntt_tuning.variant;
+    else if (!strcmp(key, "ntt.max_k")) *value = b64::g_ntt_tuning.max_k;
+    else if (!strcmp(key, "ntt.tile_log")) *value = b64::g_ntt_tuning.tile_log;
+    else return B64_ERR_INVALID_ARGS;
+    return B64_OK;
+}
+
 void b64_gf_mul(const uint64_t a[2], const uint64_t b[2], uint64_t out[2]) {
     st128(out, b64::host::gf_mul(ld128(a), ld128(b)));
 }
@@ -458,6 +484,33 @@ int b64_fri_commit_interleaved(const b64_fri_params *p, const b64_domain *d, con
     B64_TRY(b64_rs_encode_batch(d, p->log_dim, p->log_inv_rate, d_msg, p->log_batch_size, d_codeword, stream));
     const uint64_t n = 1ull << (p->log_dim + p->log_inv_rate + p->log_batch_size);
     return b64_merkle_commit(d_codeword, n, 1ull << p->log_batch_size, d_tree, root_out, stream);
+}
+
+int b64_fri_commit_fold(const b64_fri_params *p, const b64_domain *d, const void *d_msg, void *d_codeword,
+                        void *d_tree, const uint64_t *challenges, void *d_folded, uint8_t root_out[32],
+                        void *stream) {
+    if (!p || !d || !d_msg || !d_codeword || !d_tree || !d_folded || (p->log_batch_size && !challenges))
+        return B64_ERR_INVALID_ARGS;
+    const uint32_t lbs = p->log_batch_size, rs_log_len = p->log_dim + p->log_inv_rate;
+    if (lbs > 6) return B64_ERR_INVALID_ARGS;
+    hipStream_t s = as_stream(stream);
+    B64_TRY(b64_rs_encode_batch(d, p->log_dim, p->log_inv_rate, d_msg, lbs, d_codeword, stream));
+    uint4 ch[8];
+    for (uint32_t t = 0; t < lbs; t++) ch[t] = to_u4(ld128(challenges + 2 * t));
+    const uint64_t n_leaves = 1ull << rs_log_len;
+    if (lbs == 4) {
+        // one pass over the codeword: leaf digests and the folded oracle
+        HIP_TRY(b64::launch_leaf16_fold((const uint4 *)d_codeword, n_leaves, (uint4 *)d_tree, (uint4 *)d_folded, ch, s));
+        HIP_TRY(b64::launch_merkle_levels((uint4 *)d_tree, n_leaves, compress_iv(), s));
+    } else {
+        B64_TRY(merkle_commit_impl((const uint4 *)d_codeword, n_leaves << lbs, 1ull << lbs, (uint4 *)d_tree, s));
+        HIP_TRY(b64::launch_fold_interleaved((const uint4 *)d_codeword, (uint4 *)d_folded, n_leaves, (int)lbs, ch, s));
+    }
+    if (root_out) {
+        HIP_TRY(hipMemcpyAsync(root_out, (const uint8_t *)d_tree + 32 * (2 * n_leaves - 2), 32, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    return B64_OK;
 }
 
 }  // extern "C"

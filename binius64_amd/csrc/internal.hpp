@@ -29,6 +29,13 @@ struct CompressIV {
     uint32_t w[8];
 };
 
+struct NttTuning {
+    int variant;   // 0 = general product, 1 = LDS window tables
+    int max_k;     // layers per pass
+    int tile_log;  // log2 elements per LDS tile
+};
+extern NttTuning g_ntt_tuning;
+
 // Kernel launchers.  All return hipError_t of the launch.
 hipError_t launch_gf_mul(const uint4 *a, const uint4 *b, uint4 *out, uint64_t n, hipStream_t s);
 hipError_t launch_gf_mul_const(const uint4 *a, uint4 c, uint4 *out, uint64_t n, hipStream_t s);
@@ -38,6 +45,9 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
 hipError_t launch_leaf_hash(const uint4 *elems, uint64_t n_leaves, int log_batch, uint64_t batch,
                             uint4 *digests, hipStream_t s);
 hipError_t launch_merkle_levels(uint4 *tree, uint64_t n_leaves, const CompressIV &iv, hipStream_t s);
+// 16-element leaves; with folded != nullptr also fold_interleaved with 4 challenges.
+hipError_t launch_leaf16_fold(const uint4 *cw, uint64_t n_leaves, uint4 *digests, uint4 *folded, const uint4 *chal,
+                              hipStream_t s);
 hipError_t launch_fold_interleaved(const uint4 *cw, uint4 *out, uint64_t n_out, int log_batch,
                                    const uint4 *eq_host, hipStream_t s);
 hipError_t launch_fold_codeword(const DeviceDomain &dd, const uint4 *cw, uint4 *out, int log_len,

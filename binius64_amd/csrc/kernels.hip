@@ -75,6 +75,12 @@ hipError_t launch_bitrev_replicate(const uint4 *msg, uint4 *out, int m, int lir,
 // ---------------------------------------------------------------------------
 constexpr int NTT_TILE_LOG = 11;
 constexpr int NTT_MAX_K = 7;
+constexpr int NTT_TAB_GROUP = 4;  // window tables resident in LDS at once (8 KiB each)
+
+// Runtime knobs (set through b64_config_set): the multiply used by the NTT
+// passes (0 = general product, 1 = LDS window tables per twiddle), the
+// largest number of layers per pass and the tile size.
+NttTuning g_ntt_tuning = {1, NTT_MAX_K, NTT_TILE_LOG};
 
 struct NttPassArgs {
     uint4 *data;
@@ -138,12 +144,86 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass(NttPassArgs a) {
         a.data[base | ((uint64_t)(L >> c) << lb) | (uint64_t)(L & cmask)] = s[L];
 }
 
+// Same pass structure, but each butterfly multiplies by its block's twiddle
+// through an LDS window table.  The 2^j blocks of layer j are contiguous ranges
+// of the butterfly enumeration; NTT_TAB_GROUP tables are built at a time (32
+// lanes per table) and consumed by the butterflies of those blocks.
+template <bool INVERSE>
+__global__ __launch_bounds__(BLOCK) void k_ntt_pass_tab(NttPassArgs a) {
+    __shared__ uint4 s[1 << NTT_TILE_LOG];
+    __shared__ uint4 tabs[NTT_TAB_GROUP * WIN_ENTRIES];
+    __shared__ uint4 tw[1 << (NTT_MAX_K - 1)];
+    const int k = a.k, c = a.c;
+    const int lb = a.log_d - a.layer0 - k;
+    const int mid_bits = lb - c;
+    const uint64_t tile = blockIdx.x;
+    const uint64_t mid = tile & ((1ull << mid_bits) - 1);
+    const uint64_t hi = tile >> mid_bits;
+    const uint64_t base = (hi << (lb + k)) | (mid << c);
+    const int n_el = 1 << (k + c);
+    const int cmask = (1 << c) - 1;
+
+    for (int L = threadIdx.x; L < n_el; L += BLOCK)
+        s[L] = a.data[base | ((uint64_t)(L >> c) << lb) | (uint64_t)(L & cmask)];
+
+    for (int jj = 0; jj < k; jj++) {
+        const int j = INVERSE ? (k - 1 - jj) : jj;
+        const int layer = a.layer0 + j;
+        const uint4 *Wl = a.W + layer * W_STRIDE;
+        for (int loc = threadIdx.x; loc < (1 << j); loc += BLOCK) {
+            uint4 t = make_uint4(0, 0, 0, 0);
+            for (int i = 0; i < j; i++)
+                if ((loc >> i) & 1) t = gxor(t, Wl[i]);
+            for (int i = 0; i < a.layer0; i++)
+                if ((hi >> i) & 1) t = gxor(t, Wl[j + i]);
+            tw[loc] = t;
+        }
+        __syncthreads();
+        const int pos = c + (k - 1 - j);
+        const int per_blk = 1 << pos;
+        const int lowmask = per_blk - 1;
+        const int nblk = 1 << j;
+        for (int g0 = 0; g0 < nblk; g0 += NTT_TAB_GROUP) {
+            const int ng = nblk - g0 < NTT_TAB_GROUP ? nblk - g0 : NTT_TAB_GROUP;
+            if (threadIdx.x < 32 * ng)
+                win_build_row(tabs + (threadIdx.x >> 5) * WIN_ENTRIES, tw[g0 + (threadIdx.x >> 5)],
+                              threadIdx.x & 31);
+            __syncthreads();
+            const int n_bf = ng * per_blk;
+            for (int q = threadIdx.x; q < n_bf; q += BLOCK) {
+                const int P = (g0 << pos) + q;
+                const int L0 = ((P >> pos) << (pos + 1)) | (P & lowmask);
+                const int L1 = L0 | per_blk;
+                const uint4 *tab = tabs + (q >> pos) * WIN_ENTRIES;
+                uint4 u = s[L0], v = s[L1];
+                if (!INVERSE) {
+                    u = gxor(u, win_mul(tab, v));
+                    v = gxor(v, u);
+                } else {
+                    v = gxor(v, u);
+                    u = gxor(u, win_mul(tab, v));
+                }
+                s[L0] = u;
+                s[L1] = v;
+            }
+            __syncthreads();
+        }
+    }
+
+    for (int L = threadIdx.x; L < n_el; L += BLOCK)
+        a.data[base | ((uint64_t)(L >> c) << lb) | (uint64_t)(L & cmask)] = s[L];
+}
+
 hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_early, int skip_late,
                       bool inverse, hipStream_t s) {
     const int first = skip_early, end = log_d - skip_late;
     const int n_layers = end - first;
     if (n_layers <= 0) return hipSuccess;
-    const int n_passes = (n_layers + NTT_MAX_K - 1) / NTT_MAX_K;
+    int max_k = g_ntt_tuning.max_k;
+    if (max_k < 1 || max_k > NTT_MAX_K) max_k = NTT_MAX_K;
+    int tile_log = g_ntt_tuning.tile_log;
+    if (tile_log < max_k || tile_log > NTT_TILE_LOG) tile_log = NTT_TILE_LOG;
+    const int n_passes = (n_layers + max_k - 1) / max_k;
     // balanced split of the layer range into passes
     int starts[64], ks[64];
     int at = first;
@@ -163,12 +243,19 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
         a.layer0 = starts[p];
         a.k = ks[p];
         const int lb = log_d - a.layer0 - a.k;
-        a.c = lb < (NTT_TILE_LOG - a.k) ? lb : (NTT_TILE_LOG - a.k);
+        a.c = lb < (tile_log - a.k) ? lb : (tile_log - a.k);
         const uint64_t n_tiles = 1ull << (log_d - a.k - a.c);
-        if (inverse)
-            k_ntt_pass<true><<<(unsigned)n_tiles, BLOCK, 0, s>>>(a);
-        else
-            k_ntt_pass<false><<<(unsigned)n_tiles, BLOCK, 0, s>>>(a);
+        if (g_ntt_tuning.variant == 1) {
+            if (inverse)
+                k_ntt_pass_tab<true><<<(unsigned)n_tiles, BLOCK, 0, s>>>(a);
+            else
+                k_ntt_pass_tab<false><<<(unsigned)n_tiles, BLOCK, 0, s>>>(a);
+        } else {
+            if (inverse)
+                k_ntt_pass<true><<<(unsigned)n_tiles, BLOCK, 0, s>>>(a);
+            else
+                k_ntt_pass<false><<<(unsigned)n_tiles, BLOCK, 0, s>>>(a);
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -218,8 +305,12 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_hash(const uint4 *__restrict__ e
     digests[2 * leaf + 1] = make_uint4(bswap32(st[4]), bswap32(st[5]), bswap32(st[6]), bswap32(st[7]));
 }
 
+hipError_t launch_leaf16_fold(const uint4 *cw, uint64_t n_leaves, uint4 *digests, uint4 *folded, const uint4 *chal,
+                              hipStream_t s);
+
 hipError_t launch_leaf_hash(const uint4 *elems, uint64_t n_leaves, int, uint64_t batch, uint4 *digests,
                             hipStream_t s) {
+    if (batch == 16) return launch_leaf16_fold(elems, n_leaves, digests, nullptr, nullptr, s);
     k_leaf_hash<<<grid_for(n_leaves), BLOCK, 0, s>>>(elems, n_leaves, batch, digests);
     return hipGetLastError();
 }
@@ -228,11 +319,14 @@ hipError_t launch_leaf_hash(const uint4 *elems, uint64_t n_leaves, int, uint64_t
 // Inner nodes: Sha256Compression (hash/compress.rs:46-56): one compress256 of
 // left||right from the custom IV, output state written little-endian.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_merkle_level(const uint4 *__restrict__ prev, uint4 *__restrict__ next,
-                                                        uint64_t n_out, CompressIV iv) {
-    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n_out) return;
-    const uint4 l0 = prev[4 * i], l1 = prev[4 * i + 1], r0 = prev[4 * i + 2], r1 = prev[4 * i + 3];
+// Subtree reduction: a block takes 2^levels consecutive nodes of one tree level
+// (levels <= MERKLE_RED_LOG) and writes the next `levels` levels, keeping the
+// working level in LDS.  The tree is stored level after level, so level l+1
+// starts right after level l.
+constexpr int MERKLE_RED_LOG = 9;
+
+__device__ __forceinline__ void node_compress(uint4 l0, uint4 l1, uint4 r0, uint4 r1, const CompressIV &iv,
+                                              uint4 &o0, uint4 &o1) {
     uint32_t w[16] = {bswap32(l0.x), bswap32(l0.y), bswap32(l0.z), bswap32(l0.w),
                       bswap32(l1.x), bswap32(l1.y), bswap32(l1.z), bswap32(l1.w),
                       bswap32(r0.x), bswap32(r0.y), bswap32(r0.z), bswap32(r0.w),
@@ -241,20 +335,53 @@ __global__ __launch_bounds__(BLOCK) void k_merkle_level(const uint4 *__restrict_
 #pragma unroll
     for (int q = 0; q < 8; q++) st[q] = iv.w[q];
     sha256_compress(st, w);
-    next[2 * i] = make_uint4(st[0], st[1], st[2], st[3]);
-    next[2 * i + 1] = make_uint4(st[4], st[5], st[6], st[7]);
+    o0 = make_uint4(st[0], st[1], st[2], st[3]);
+    o1 = make_uint4(st[4], st[5], st[6], st[7]);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_merkle_subtree(uint4 *__restrict__ tree, uint64_t in_off, uint64_t n_in,
+                                                          int levels, CompressIV iv) {
+    __shared__ uint4 dig[2][2 << MERKLE_RED_LOG];
+    const int blk_nodes = 1 << levels;
+    const uint64_t b = blockIdx.x;
+    const uint4 *src = tree + 2 * (in_off + b * (uint64_t)blk_nodes);
+    for (int t = threadIdx.x; t < 2 * blk_nodes; t += BLOCK) dig[0][t] = src[t];
+    __syncthreads();
+    uint64_t off = in_off, n = n_in;
+    int cur = 0;
+    for (int l = 1; l <= levels; l++) {
+        const uint64_t out_off = off + n;
+        const int cnt = blk_nodes >> l;
+        uint4 *dst = tree + 2 * (out_off + b * (uint64_t)cnt);
+        for (int t = threadIdx.x; t < cnt; t += BLOCK) {
+            uint4 o0, o1;
+            node_compress(dig[cur][4 * t], dig[cur][4 * t + 1], dig[cur][4 * t + 2], dig[cur][4 * t + 3], iv, o0, o1);
+            dig[cur ^ 1][2 * t] = o0;
+            dig[cur ^ 1][2 * t + 1] = o1;
+            dst[2 * t] = o0;
+            dst[2 * t + 1] = o1;
+        }
+        __syncthreads();
+        cur ^= 1;
+        off = out_off;
+        n >>= 1;
+    }
 }
 
 hipError_t launch_merkle_levels(uint4 *tree, uint64_t n_leaves, const CompressIV &iv, hipStream_t s) {
-    uint4 *prev = tree;
-    uint64_t len = n_leaves;
-    while (len > 1) {
-        uint4 *next = prev + 2 * len;
-        k_merkle_level<<<grid_for(len / 2), BLOCK, 0, s>>>(prev, next, len / 2, iv);
+    uint64_t off = 0, n = n_leaves;
+    while (n > 1) {
+        int log_n = 0;
+        while ((1ull << log_n) < n) log_n++;
+        const int levels = log_n < MERKLE_RED_LOG ? log_n : MERKLE_RED_LOG;
+        const uint64_t blocks = n >> levels;
+        k_merkle_subtree<<<(unsigned)blocks, BLOCK, 0, s>>>(tree, off, n, levels, iv);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        prev = next;
-        len >>= 1;
+        for (int l = 0; l < levels; l++) {
+            off += n;
+            n >>= 1;
+        }
     }
     return hipSuccess;
 }
@@ -267,25 +394,89 @@ struct FoldChallenges {
     uint4 r[MAX_FOLD_CHAL];
 };
 
+// Leaves of 16 elements (the interleaved batch 2^4 of the commit) hashed and,
+// optionally, folded by fold_interleaved in the same pass over the codeword:
+// the 16 elements of leaf i are exactly the 16 inputs of folded element i.
+// The fold multiplies by the four challenges through LDS window tables.
+__global__ __launch_bounds__(BLOCK) void k_leaf16_fold(const uint4 *__restrict__ cw, uint64_t n_leaves,
+                                                       uint4 *__restrict__ digests, uint4 *__restrict__ folded,
+                                                       FoldChallenges ch) {
+    __shared__ uint4 tabs[4 * WIN_ENTRIES];
+    const bool do_fold = folded != nullptr;
+    if (do_fold && threadIdx.x < 4 * WIN_ROWS)
+        win_build_row(tabs + (threadIdx.x >> 5) * WIN_ENTRIES, ch.r[threadIdx.x >> 5], threadIdx.x & 31);
+    __syncthreads();
+    const uint64_t leaf = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (leaf >= n_leaves) return;
+    const uint4 *p = cw + leaf * 16;
+    uint4 v[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) v[j] = p[j];
+    uint32_t st[8];
+    sha256_init(st);
+    uint32_t w[16];
+#pragma unroll
+    for (int blk = 0; blk < 4; blk++) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint4 e = v[blk * 4 + q];
+            w[4 * q + 0] = bswap32(e.x);
+            w[4 * q + 1] = bswap32(e.y);
+            w[4 * q + 2] = bswap32(e.z);
+            w[4 * q + 3] = bswap32(e.w);
+        }
+        sha256_compress(st, w);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) w[q] = 0;
+    w[0] = 0x80000000u;
+    w[15] = 16 * 128;
+    sha256_compress(st, w);
+    digests[2 * leaf] = make_uint4(bswap32(st[0]), bswap32(st[1]), bswap32(st[2]), bswap32(st[3]));
+    digests[2 * leaf + 1] = make_uint4(bswap32(st[4]), bswap32(st[5]), bswap32(st[6]), bswap32(st[7]));
+    if (do_fold) {
+        int len = 16;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            len >>= 1;
+#pragma unroll
+            for (int q = 0; q < len; q++) v[q] = gxor(v[2 * q], win_mul(tabs + t * WIN_ENTRIES, gxor(v[2 * q], v[2 * q + 1])));
+        }
+        folded[leaf] = v[0];
+    }
+}
+
+hipError_t launch_leaf16_fold(const uint4 *cw, uint64_t n_leaves, uint4 *digests, uint4 *folded, const uint4 *chal,
+                              hipStream_t s) {
+    FoldChallenges ch;
+    for (int t = 0; t < 4; t++) ch.r[t] = folded ? chal[t] : make_uint4(0, 0, 0, 0);
+    k_leaf16_fold<<<grid_for(n_leaves), BLOCK, 0, s>>>(cw, n_leaves, digests, folded, ch);
+    return hipGetLastError();
+}
+
 // fold_interleaved (prover fri/fold.rs:260-282): out[i] = <cw[i*B..(i+1)*B], eq(r)>.
 // Computed as log_batch successive line folds (bit t of the in-row index pairs
 // with r_t, the eq tensor's bit order, multilinear/eq.rs:39-71); the result is
 // the same field element as the inner product.
-template <int MAXB>
+template <int LOGB>
 __global__ __launch_bounds__(BLOCK) void k_fold_interleaved(const uint4 *__restrict__ cw, uint4 *__restrict__ out,
-                                                            uint64_t n_out, int log_batch, FoldChallenges ch) {
+                                                            uint64_t n_out, FoldChallenges ch) {
+    __shared__ uint4 tabs[(LOGB > 0 ? LOGB : 1) * WIN_ENTRIES];
+    for (int r = threadIdx.x; r < LOGB * WIN_ROWS; r += BLOCK)
+        win_build_row(tabs + (r >> 5) * WIN_ENTRIES, ch.r[r >> 5], r & 31);
+    __syncthreads();
     const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n_out) return;
-    uint4 v[MAXB];
-    const int B = 1 << log_batch;
+    constexpr int B = 1 << LOGB;
+    uint4 v[B];
+#pragma unroll
     for (int j = 0; j < B; j++) v[j] = cw[i * B + j];
     int len = B;
-    for (int t = 0; t < log_batch; t++) {
+#pragma unroll
+    for (int t = 0; t < LOGB; t++) {
         len >>= 1;
-        for (int q = 0; q < len; q++) {
-            const uint4 x0 = v[2 * q], x1 = v[2 * q + 1];
-            v[q] = gxor(x0, gf_mul(gxor(x0, x1), ch.r[t]));
-        }
+#pragma unroll
+        for (int q = 0; q < len; q++) v[q] = gxor(v[2 * q], win_mul(tabs + t * WIN_ENTRIES, gxor(v[2 * q], v[2 * q + 1])));
     }
     out[i] = v[0];
 }
@@ -294,12 +485,17 @@ hipError_t launch_fold_interleaved(const uint4 *cw, uint4 *out, uint64_t n_out, 
                                    const uint4 *chal, hipStream_t s) {
     FoldChallenges ch;
     for (int t = 0; t < log_batch; t++) ch.r[t] = chal[t];
-    if (log_batch <= 4)
-        k_fold_interleaved<16><<<grid_for(n_out), BLOCK, 0, s>>>(cw, out, n_out, log_batch, ch);
-    else if (log_batch <= 6)
-        k_fold_interleaved<64><<<grid_for(n_out), BLOCK, 0, s>>>(cw, out, n_out, log_batch, ch);
-    else
-        return hipErrorInvalidValue;
+    const unsigned g = grid_for(n_out);
+    switch (log_batch) {
+        case 0: k_fold_interleaved<0><<<g, BLOCK, 0, s>>>(cw, out, n_out, ch); break;
+        case 1: k_fold_interleaved<1><<<g, BLOCK, 0, s>>>(cw, out, n_out, ch); break;
+        case 2: k_fold_interleaved<2><<<g, BLOCK, 0, s>>>(cw, out, n_out, ch); break;
+        case 3: k_fold_interleaved<3><<<g, BLOCK, 0, s>>>(cw, out, n_out, ch); break;
+        case 4: k_fold_interleaved<4><<<g, BLOCK, 0, s>>>(cw, out, n_out, ch); break;
+        case 5: k_fold_interleaved<5><<<g, BLOCK, 0, s>>>(cw, out, n_out, ch); break;
+        case 6: k_fold_interleaved<6><<<g, BLOCK, 0, s>>>(cw, out, n_out, ch); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

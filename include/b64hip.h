@@ -74,6 +74,13 @@ int b64_memcpy_dtoh(void *dst, const void *d_src, size_t bytes, void *stream);
 int b64_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes, void *stream);
 int b64_stream_synchronize(void *stream);
 
+/* Kernel tuning knobs (no counterpart in the reference; the autotuner's
+ * interface).  Keys: "ntt.variant" (0 = general multiply, 1 = LDS window
+ * tables), "ntt.max_k" (layers per NTT pass, 1..7), "ntt.tile_log" (log2
+ * elements per LDS tile).  Returns INVALID_ARGS for an unknown key. */
+int b64_config_set(const char *key, int64_t value);
+int b64_config_get(const char *key, int64_t *value);
+
 /* ---- field ------------------------------------------------------------------ */
 /* BinaryField128bGhash Mul (verifier/field/src/ghash.rs:188-195), host side. */
 void b64_gf_mul(const uint64_t a[2], const uint64_t b[2], uint64_t out[2]);
@@ -160,6 +167,16 @@ int b64_setup_commit_params(uint32_t log_witness_elems, uint32_t log_inv_rate, u
 /* d_codeword: 2^(rs_log_len+log_batch) elements; d_tree: (2^(rs_log_len+1)-1)*32 bytes. */
 int b64_fri_commit_interleaved(const b64_fri_params *p, const b64_domain *d, const void *d_msg,
                                void *d_codeword, void *d_tree, uint8_t root_out[32], void *stream);
+
+/* commit_interleaved followed by the first fold_interleaved (prover fri/fold.rs:260-282)
+ * with log_batch_size challenges supplied up front, fused so the codeword is read
+ * once for leaf hashing and folding.  In the transcript-driven protocol the
+ * challenges are sampled after the root is written (protocols/basefold.rs:139-147),
+ * so this form serves benchmark / pre-seeded-challenge callers.  d_folded receives
+ * 2^rs_log_len elements. */
+int b64_fri_commit_fold(const b64_fri_params *p, const b64_domain *d, const void *d_msg, void *d_codeword,
+                        void *d_tree, const uint64_t *challenges, void *d_folded, uint8_t root_out[32],
+                        void *stream);
 
 /* ---- ProverTranscript<HasherChallenger<Sha256>> (verifier/transcript/src) ----- */
 int b64_transcript_create(b64_transcript **out);

@@ -77,6 +77,26 @@ def test_ntt_forward_windows(b64, orc, log_d):
         assert np.array_equal(ntt_dev(b64, dom, data, se, sl), orc.ntt_forward(ref, data, se, sl)), (se, sl)
 
 
+@pytest.mark.parametrize("variant,max_k,tile_log", [(0, 7, 11), (1, 7, 11), (1, 3, 6), (1, 1, 1), (1, 5, 9),
+                                                    (0, 2, 4)])
+def test_ntt_tuning_variants(b64, orc, variant, max_k, tile_log):
+    """Every multiply variant and pass/tile shape gives the same transform."""
+    old = {k: b64.config_get(k) for k in ("ntt.variant", "ntt.max_k", "ntt.tile_log")}
+    try:
+        b64.config_set("ntt.variant", variant)
+        b64.config_set("ntt.max_k", max_k)
+        b64.config_set("ntt.tile_log", tile_log)
+        for log_d, se, sl in ((13, 1, 4), (12, 0, 0), (9, 2, 1), (3, 0, 0)):
+            dom, ref = b64.DomainContext.standard(log_d), orc.Domain(dim=log_d)
+            data = orc.random_elems(7 * log_d, 1 << log_d)
+            say("ntt-variant", variant, max_k, tile_log, log_d, se, sl)
+            assert np.array_equal(ntt_dev(b64, dom, data, se, sl), orc.ntt_forward(ref, data, se, sl))
+            assert np.array_equal(ntt_dev(b64, dom, data, se, sl, inverse=True), orc.ntt_inverse(ref, data, se, sl))
+    finally:
+        for k, v in old.items():
+            b64.config_set(k, v)
+
+
 @pytest.mark.parametrize("basis", [[5, 7, 22, 95], [0x1234, 0x98765, 0xABCDEF, 0x1111111111, 0x3 << 100, 7, 9, 1 << 77]])
 def test_ntt_nonstandard_basis(b64, orc, basis):
     dom, ref = b64.DomainContext(orc.to_elems(basis)), orc.Domain(orc.to_elems(basis))
@@ -176,6 +196,33 @@ def test_fold_codeword(b64, orc, log_len, n_chal):
     ch = np.ascontiguousarray(chal, dtype=np.uint64)
     b64.check(b64.lib().b64_fri_fold_codeword(dom.ptr, d_cw.ptr, log_len, b64._p64(ch), n_chal, out.ptr, None))
     assert np.array_equal(out.to_host(1 << (log_len - n_chal)), orc.fold_codeword(ref, cw, log_len, chal))
+
+
+@pytest.mark.parametrize("n", [8, 12, 16])
+def test_commit_fold_fused(b64, orc, n):
+    """b64_fri_commit_fold (encode + leaf hash + first interleaved fold in one
+    pass over the codeword) = oracle encode, Merkle root and fold_interleaved."""
+    import ctypes as C
+    params, dom = b64.setup_commit_params(n, 1, 96)
+    lbs, rs = params.log_batch_size, params.rs_log_len()
+    msg = orc.random_elems(300 + n, 1 << n)
+    chal = np.ascontiguousarray(orc.random_elems(301, lbs))
+    d_msg = b64.DeviceBuffer.from_host(msg)
+    cw = b64.DeviceBuffer(16 << params.log_len())
+    tree = b64.DeviceBuffer(32 * ((2 << rs) - 1))
+    folded = b64.DeviceBuffer(16 << rs)
+    root = np.zeros(32, dtype=np.uint8)
+    pc = params.to_c()
+    b64.check(b64.lib().b64_fri_commit_fold(C.byref(pc), dom.ptr, d_msg.ptr, cw.ptr, tree.ptr, b64._p64(chal),
+                                            folded.ptr, b64._p8(root), None))
+    ref_dom = orc.Domain(dom.subspace(rs))
+    cw_ref = orc.encode_batch(ref_dom, msg, params.log_dim, 1, lbs, threads=8)
+    assert np.array_equal(cw.to_host(1 << params.log_len()), cw_ref)
+    tree_ref = orc.merkle_build(cw_ref, 1 << lbs, threads=8)
+    assert root.tobytes() == tree_ref[-1].tobytes()
+    got_tree = np.frombuffer(tree.to_bytes(32 * ((2 << rs) - 1)), dtype=np.uint8).reshape(-1, 32)
+    assert np.array_equal(got_tree, tree_ref)
+    assert np.array_equal(folded.to_host(1 << rs), orc.fold_interleaved(cw_ref, rs, lbs, chal, threads=8))
 
 
 def test_pack_witness(b64, orc):
