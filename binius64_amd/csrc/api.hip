@@ -304,8 +304,14 @@ int b64_rs_encode_batch(const b64_domain *dc, uint32_t log_dim, uint32_t lir, co
     b64::DeviceDomain dd;
     B64_TRY(d->device_view(dd));
     hipStream_t s = as_stream(stream);
-    HIP_TRY(b64::launch_bitrev_replicate((const uint4 *)d_msg, (uint4 *)d_out, (int)m, (int)lir, s));
-    HIP_TRY(b64::launch_ntt(dd, (uint4 *)d_out, (int)(m + lir), (int)lir, (int)log_batch, false, s));
+    if (log_dim == 0) {
+        // no butterfly layer: the codeword is the replicated bit-reversed message
+        HIP_TRY(b64::launch_bitrev_replicate((const uint4 *)d_msg, (uint4 *)d_out, (int)m, (int)lir, s));
+        return B64_OK;
+    }
+    // the first NTT pass gathers the bit-reversed, replicated message itself
+    HIP_TRY(b64::launch_ntt(dd, (uint4 *)d_out, (int)(m + lir), (int)lir, (int)log_batch, false, s,
+                            (const uint4 *)d_msg, (int)m));
     return B64_OK;
 }
 

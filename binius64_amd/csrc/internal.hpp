@@ -41,7 +41,7 @@ hipError_t launch_gf_mul(const uint4 *a, const uint4 *b, uint4 *out, uint64_t n,
 hipError_t launch_gf_mul_const(const uint4 *a, uint4 c, uint4 *out, uint64_t n, hipStream_t s);
 hipError_t launch_bitrev_replicate(const uint4 *msg, uint4 *out, int m, int lir, hipStream_t s);
 hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_early, int skip_late,
-                      bool inverse, hipStream_t s);
+                      bool inverse, hipStream_t s, const uint4 *msg = nullptr, int msg_log = 0);
 hipError_t launch_leaf_hash(const uint4 *elems, uint64_t n_leaves, int log_batch, uint64_t batch,
                             uint4 *digests, hipStream_t s);
 hipError_t launch_merkle_levels(uint4 *tree, uint64_t n_leaves, const CompressIV &iv, hipStream_t s);

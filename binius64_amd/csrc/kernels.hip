@@ -85,8 +85,35 @@ NttTuning g_ntt_tuning = {1, NTT_MAX_K, NTT_TILE_LOG};
 struct NttPassArgs {
     uint4 *data;
     const uint4 *W;
-    int log_d, layer0, k, c;
+    // First encode pass only: read the tile straight from the message,
+    // out[r*2^m + i] = msg[rev_m(i)] (reed_solomon.rs:164-187), instead of data.
+    const uint4 *msg;
+    int log_d, layer0, k, c, m;
 };
+
+__device__ __forceinline__ uint32_t rev_bits(uint32_t x, int b) { return b ? (__brev(x) >> (32 - b)) : 0u; }
+
+// Loads the tile: element L = (bf << c) | low holds index
+// base | bf << lb | low.  In message mode the loop runs in message order so the
+// reads are runs of 2^k consecutive message elements: with
+// i = bf << (m-k) | mid << c | low, rev_m(i) = rev_k(bf) | rev(mid) << k | rev_c(low) << (m-c).
+__device__ __forceinline__ void ntt_load_tile(uint4 *__restrict__ s, const NttPassArgs &a, uint64_t base,
+                                              uint64_t mid, int lb, int n_el) {
+    const int k = a.k, c = a.c;
+    if (a.msg == nullptr) {
+        const int cmask = (1 << c) - 1;
+        for (int L = threadIdx.x; L < n_el; L += BLOCK)
+            s[L] = a.data[base | ((uint64_t)(L >> c) << lb) | (uint64_t)(L & cmask)];
+    } else {
+        const int mid_bits = lb - c;
+        const uint64_t mbase = mid_bits ? ((__brevll(mid) >> (64 - mid_bits)) << k) : 0;
+        const int kmask = (1 << k) - 1;
+        for (int t = threadIdx.x; t < n_el; t += BLOCK) {
+            const int tk = t & kmask, tc = t >> k;
+            s[(rev_bits(tk, k) << c) | rev_bits(tc, c)] = a.msg[mbase + tk + ((uint64_t)tc << (a.m - c))];
+        }
+    }
+}
 
 template <bool INVERSE>
 __global__ __launch_bounds__(BLOCK) void k_ntt_pass(NttPassArgs a) {
@@ -102,8 +129,7 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass(NttPassArgs a) {
     const int n_el = 1 << (k + c);
     const int cmask = (1 << c) - 1;
 
-    for (int L = threadIdx.x; L < n_el; L += BLOCK)
-        s[L] = a.data[base | ((uint64_t)(L >> c) << lb) | (uint64_t)(L & cmask)];
+    ntt_load_tile(s, a, base, mid, lb, n_el);
     __syncthreads();
 
     for (int jj = 0; jj < k; jj++) {
@@ -163,8 +189,7 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass_tab(NttPassArgs a) {
     const int n_el = 1 << (k + c);
     const int cmask = (1 << c) - 1;
 
-    for (int L = threadIdx.x; L < n_el; L += BLOCK)
-        s[L] = a.data[base | ((uint64_t)(L >> c) << lb) | (uint64_t)(L & cmask)];
+    ntt_load_tile(s, a, base, mid, lb, n_el);
 
     for (int jj = 0; jj < k; jj++) {
         const int j = INVERSE ? (k - 1 - jj) : jj;
@@ -215,10 +240,12 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass_tab(NttPassArgs a) {
 }
 
 hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_early, int skip_late,
-                      bool inverse, hipStream_t s) {
+                      bool inverse, hipStream_t s, const uint4 *msg, int msg_log) {
     const int first = skip_early, end = log_d - skip_late;
     const int n_layers = end - first;
     if (n_layers <= 0) return hipSuccess;
+    // message mode needs the first pass to start at layer lir = log_d - msg_log
+    if (msg && (inverse || skip_early != log_d - msg_log)) return hipErrorInvalidValue;
     int max_k = g_ntt_tuning.max_k;
     if (max_k < 1 || max_k > NTT_MAX_K) max_k = NTT_MAX_K;
     int tile_log = g_ntt_tuning.tile_log;
@@ -239,6 +266,8 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
         NttPassArgs a;
         a.data = data;
         a.W = dd.W;
+        a.msg = (q == 0 && !inverse) ? msg : nullptr;
+        a.m = msg_log;
         a.log_d = log_d;
         a.layer0 = starts[p];
         a.k = ks[p];
