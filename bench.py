@@ -35,6 +35,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--log-n", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs leg")
     ap.add_argument("--cpu-log-n", type=int, default=24, help="CPU baseline sample size")
     return ap.parse_args()
 
@@ -131,6 +132,8 @@ def main():
     step_bytes = 58 * (1 << n)  # algorithmic HBM bytes of the whole step
     path_achieved = step_bytes / (ms * 1e-3) / 1e9
 
+    batch = None if args.no_batch else batch_bench(b64, torch, stream, sp)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_log_n)
@@ -157,8 +160,48 @@ def main():
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "wall_s_timed_region": round(wall, 4),
         "cpu_baseline": cpu,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
+
+
+def batch_bench(b64, torch, stream, sp, n_jobs: int = 64, reps: int = 5):
+    """64 independent commit jobs, log_n ~ U{12..20} (seed 4), challenges seeded
+    by job id: per-job plain launches vs one batched hipGraph replay."""
+    import numpy as np
+
+    rng = np.random.default_rng(4)
+    sizes = [int(x) for x in rng.integers(12, 21, size=n_jobs)]
+    jobs = []
+    for jid, n in enumerate(sizes):
+        r = np.random.default_rng(1000 + jid)
+        msg = r.integers(0, 2 ** 63, size=(1 << n, 2), dtype=np.uint64)
+        chal = np.random.default_rng(jid).integers(0, 2 ** 63, size=(n, 2), dtype=np.uint64)
+        jobs.append(b64.CommitJob(n, msg, chal))
+
+    def plain():
+        for j in jobs:
+            j.run(sp.value)
+
+    plain()  # warm: domain setup and twiddle uploads happen here
+    torch.cuda.synchronize()
+    want = [j.roots() for j in jobs]
+    t = time.perf_counter()
+    for _ in range(reps):
+        plain()
+    t_plain = (time.perf_counter() - t) / reps
+    plan = b64.BatchPlan(jobs, use_graph=True)
+    plan.run(sp.value)
+    same = [j.roots() for j in jobs] == want
+    t = time.perf_counter()
+    for _ in range(reps):
+        plan.run(sp.value)
+    t_batch = (time.perf_counter() - t) / reps
+    same = same and [j.roots() for j in jobs] == want
+    return {"jobs": n_jobs, "log_n_range": [12, 20], "sizes_seed": 4, "n_launches_batched": plan.n_launches(),
+            "plain_jobs_per_s": round(n_jobs / t_plain, 1), "batched_jobs_per_s": round(n_jobs / t_batch, 1),
+            "speedup": round(t_plain / t_batch, 3), "plain_ms": round(1e3 * t_plain, 3),
+            "batched_ms": round(1e3 * t_batch, 3), "roots_identical": bool(same)}
 
 
 def cpu_baseline(log_n: int):

@@ -445,6 +445,72 @@ def pack_witness(words: DeviceBuffer, n_words: int, log_elems: int, out: Optiona
     return out
 
 
+class CommitJob:
+    """One FRI commit phase with pre-seeded challenges (b64_job_desc): encode,
+    Merkle commit, every fold round and its commitment."""
+
+    def __init__(self, log_msg_len: int, message, challenges: np.ndarray):
+        from ._abi import JobDescC
+        self.log_msg_len = log_msg_len
+        if isinstance(message, DeviceBuffer):
+            self.msg = message
+        else:
+            self.msg = DeviceBuffer.from_host(_elems(message))
+        self.challenges = np.ascontiguousarray(_elems(challenges))
+        assert self.challenges.shape[0] >= log_msg_len
+        ws = lib().b64_job_workspace_size(log_msg_len)
+        if ws == 0:
+            raise B64Error(1, "job_workspace_size")
+        self.workspace = DeviceBuffer(ws)
+        self.n_roots = lib().b64_job_n_roots(log_msg_len)
+        self.roots_buf = np.zeros(32 * self.n_roots, dtype=np.uint8)
+        self.desc = JobDescC(log_msg_len, self.msg.ptr, _p64(self.challenges), self.workspace.ptr,
+                             _p8(self.roots_buf))
+
+    @staticmethod
+    def workspace_size(log_msg_len: int) -> int:
+        return lib().b64_job_workspace_size(log_msg_len)
+
+    def run(self, stream=None):
+        check(lib().b64_job_run(C.byref(self.desc), _stream(stream)), "job_run")
+
+    def roots(self) -> List[bytes]:
+        b = self.roots_buf.tobytes()
+        return [b[32 * i:32 * i + 32] for i in range(self.n_roots)]
+
+    def oracle(self, k: int) -> np.ndarray:
+        ptr, log_len = C.c_void_p(), C.c_uint32()
+        check(lib().b64_job_oracle(C.byref(self.desc), k, C.byref(ptr), C.byref(log_len)), "job_oracle")
+        n = 1 << log_len.value
+        out = np.zeros((n, 2), dtype=np.uint64)
+        check(lib().b64_memcpy_dtoh(out.ctypes.data, ptr.value, 16 * n, None), "dtoh")
+        return out
+
+
+class BatchPlan:
+    """Many CommitJobs per launch, optionally replayed from a hipGraph."""
+
+    def __init__(self, jobs: List[CommitJob], use_graph: bool = True, stream=None):
+        from ._abi import JobDescC
+        self.jobs = list(jobs)
+        arr = (JobDescC * len(self.jobs))(*[j.desc for j in self.jobs])
+        out = C.c_void_p()
+        check(lib().b64_batch_plan_create(arr, len(self.jobs), 1 if use_graph else 0, _stream(stream),
+                                          C.byref(out)), "batch_plan_create")
+        self.ptr = out.value
+
+    def n_launches(self) -> int:
+        return lib().b64_batch_plan_n_launches(self.ptr)
+
+    def run(self, stream=None):
+        check(lib().b64_batch_plan_run(self.ptr, _stream(stream)), "batch_plan_run")
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().b64_batch_plan_destroy(self.ptr)
+            self.ptr = None
+
+
 def fri_prove(params: FRIParams, domain: DomainContext, message: np.ndarray, challenges=None):
     """FRI-only prover flow (prover/prover/src/fri/tests.rs:58-89): commit, fold
     rounds with transcript challenges (or the given ones), finish_proof.

@@ -31,8 +31,26 @@ class FriParamsC(C.Structure):
     ]
 
 
+class JobDescC(C.Structure):
+    _fields_ = [
+        ("log_msg_len", C.c_uint32),
+        ("d_msg", C.c_void_p),
+        ("challenges", C.POINTER(C.c_uint64)),
+        ("d_workspace", C.c_void_p),
+        ("roots_out", C.POINTER(C.c_uint8)),
+    ]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
+    "b64_job_workspace_size": (C.c_size_t, [C.c_uint32]),
+    "b64_job_n_roots": (C.c_uint32, [C.c_uint32]),
+    "b64_job_oracle": (C.c_int, [C.POINTER(JobDescC), C.c_uint32, C.POINTER(vp), u32p]),
+    "b64_job_run": (C.c_int, [C.POINTER(JobDescC), vp]),
+    "b64_batch_plan_create": (C.c_int, [C.POINTER(JobDescC), C.c_uint32, C.c_int, vp, C.POINTER(vp)]),
+    "b64_batch_plan_run": (C.c_int, [vp, vp]),
+    "b64_batch_plan_n_launches": (C.c_uint32, [vp]),
+    "b64_batch_plan_destroy": (None, [vp]),
     "b64_version": (C.c_char_p, []),
     "b64_status_string": (C.c_char_p, [C.c_int]),
     "b64_device_count": (C.c_int, []),

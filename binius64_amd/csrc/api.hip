@@ -102,6 +102,11 @@ struct b64_domain {
     }
 };
 
+int b64_domain_device_view_internal(b64_domain *d, b64::DeviceDomain *dd) {
+    if (!d || !dd) return B64_ERR_INVALID_ARGS;
+    return d->device_view(*dd);
+}
+
 extern "C" {
 
 const char *b64_version(void) { return B64_VERSION; }

@@ -1,0 +1,505 @@
+// Commit jobs: one FRI commit phase (encode, Merkle commit, every fold round and
+// its commitment) with pre-seeded challenges, run either alone (plain per-job
+// launches) or many at once (batched: each launch covers the same stage of every
+// job, and the whole launch sequence is captured in a hipGraph).
+//
+// A job of 2^n message elements uses the Verifier::setup parameters
+// (verifier/verifier/src/verify.rs:61-106): log_inv_rate 1, batch 2^4, arity 4.
+// Its stages follow FRIFoldProver::execute_fold_round (prover fri/fold.rs:117-175):
+//   NTT passes -> 16-element leaves + interleaved fold -> tree 0
+//   -> for each folded oracle k: leaves (coset 2^arity_k or 2^n_final) -> tree
+//   -> fold_codeword with the next arity's challenges -> next oracle.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "host_util.hpp"
+#include "internal.hpp"
+
+using b64::host::u128;
+
+namespace {
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+inline uint4 to_u4(const uint64_t *p) {
+    return make_uint4((uint32_t)p[0], (uint32_t)(p[0] >> 32), (uint32_t)p[1], (uint32_t)(p[1] >> 32));
+}
+constexpr size_t ALIGN = 256;
+inline size_t align_up(size_t x) { return (x + ALIGN - 1) & ~(ALIGN - 1); }
+
+#define HIP_TRY(expr)                                                                               \
+    do {                                                                                            \
+        hipError_t _e = (expr);                                                                     \
+        if (_e != hipSuccess) return _e == hipErrorOutOfMemory ? B64_ERR_OUT_OF_MEMORY : B64_ERR_HIP; \
+    } while (0)
+#define B64_TRY(expr)                  \
+    do {                               \
+        int _rc = (expr);              \
+        if (_rc != B64_OK) return _rc; \
+    } while (0)
+
+b64::CompressIV job_iv() {
+    static b64::CompressIV iv;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        static const char tag[] = "BINIUS SHA-256 COMPRESS";
+        uint8_t d[32];
+        b64::host::Sha256::digest((const uint8_t *)tag, sizeof(tag) - 1, d);
+        for (int k = 0; k < 8; k++)
+            iv.w[k] = (uint32_t)d[4 * k] | ((uint32_t)d[4 * k + 1] << 8) | ((uint32_t)d[4 * k + 2] << 16) |
+                      ((uint32_t)d[4 * k + 3] << 24);
+    });
+    return iv;
+}
+
+// Parameters and prover domain per message size, shared by every job of that size.
+struct SizeInfo {
+    b64_fri_params p;
+    b64_domain *dom = nullptr;
+    b64::DeviceDomain dd = {0, nullptr};
+};
+
+std::mutex g_size_mu;
+std::map<uint32_t, SizeInfo *> g_sizes;
+
+int size_info(uint32_t n, SizeInfo **out) {
+    if (n < 5 || n > 40) return B64_ERR_INVALID_ARGS;  // at least one NTT layer (log_dim >= 1)
+    std::lock_guard<std::mutex> g(g_size_mu);
+    auto it = g_sizes.find(n);
+    if (it != g_sizes.end()) {
+        *out = it->second;
+        return B64_OK;
+    }
+    SizeInfo *si = new (std::nothrow) SizeInfo();
+    if (!si) return B64_ERR_OUT_OF_MEMORY;
+    int rc = b64_setup_commit_params(n, 1, 96, &si->p, &si->dom);
+    if (rc == B64_OK && si->p.log_batch_size != 4) rc = B64_ERR_INVALID_ARGS;
+    if (rc != B64_OK) {
+        if (si->dom) b64_domain_destroy(si->dom);
+        delete si;
+        return rc;
+    }
+    g_sizes[n] = si;
+    *out = si;
+    return B64_OK;
+}
+
+// The size's twiddle basis on the device (uploaded on first use).
+int size_device(SizeInfo *si) {
+    if (si->dd.W) return B64_OK;
+    return b64_domain_device_view_internal(si->dom, &si->dd);
+}
+
+// Device workspace layout of one job.
+struct Layout {
+    const SizeInfo *si;
+    int rs, lbs, n_oracles;  // folded oracles = n_fold_arities + 1
+    size_t cw, tree0;
+    size_t oracle[B64_MAX_FOLD_ARITIES + 1], tree[B64_MAX_FOLD_ARITIES + 1];
+    int olen[B64_MAX_FOLD_ARITIES + 1], ocoset[B64_MAX_FOLD_ARITIES + 1];
+    size_t total;
+};
+
+void layout_of(const SizeInfo *si, Layout &L) {
+    const b64_fri_params &p = si->p;
+    L.si = si;
+    L.rs = (int)(p.log_dim + p.log_inv_rate);
+    L.lbs = (int)p.log_batch_size;
+    L.n_oracles = (int)p.n_fold_arities + 1;
+    size_t at = 0;
+    L.cw = at;
+    at = align_up(at + (16ull << (L.rs + L.lbs)));
+    L.tree0 = at;
+    at = align_up(at + 32ull * ((2ull << L.rs) - 1));
+    int len = L.rs;
+    for (int k = 0; k < L.n_oracles; k++) {
+        if (k > 0) len -= (int)p.fold_arities[k - 1];
+        const int coset = k < (int)p.n_fold_arities ? (int)p.fold_arities[k] : (int)p.log_terminal_dim;
+        L.olen[k] = len;
+        L.ocoset[k] = coset;
+        L.oracle[k] = at;
+        at = align_up(at + (16ull << len));
+        L.tree[k] = at;
+        at = align_up(at + 32ull * ((2ull << (len - coset)) - 1));
+    }
+    L.total = at;
+}
+
+// A stage of a job, tagged with its slot in the common launch order.
+struct Stage {
+    int slot;
+    b64::MultiKind kind;
+    union {
+        b64::NttPassArgs ntt;
+        b64::Leaf16Args leaf16;
+        b64::LeafArgs leaf;
+        b64::SubtreeArgs sub;
+        b64::FoldCwArgs fold;
+    };
+    uint32_t blocks;
+    Stage() {}
+};
+
+// Slot numbering: 0..15 NTT passes, 16 leaves+fold, 17..24 tree-0 steps, then
+// per folded oracle k (base 32 + 16k): 0 fold_codeword, 1 16-element leaves,
+// 2 other leaves, 3.. tree steps.
+constexpr int SLOT_LEAF16 = 16, SLOT_TREE0 = 17, SLOT_ORACLE = 32;
+
+inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + 255) / 256); }
+
+int job_stages(const b64_job_desc &jd, const Layout &L, std::vector<Stage> &out, std::vector<const uint4 *> &roots) {
+    const SizeInfo *si = L.si;
+    const b64_fri_params &p = si->p;
+    uint8_t *ws = (uint8_t *)jd.d_workspace;
+    uint4 *cw = (uint4 *)(ws + L.cw);
+    uint4 *tree0 = (uint4 *)(ws + L.tree0);
+    const int log_d = L.rs + L.lbs;
+    const int m = (int)jd.log_msg_len;
+    // NTT passes, the first gathering the bit-reversed message
+    int starts[64], ks[64], cs[64];
+    const int np = b64::ntt_pass_plan(log_d, 1, L.lbs, starts, ks, cs);
+    if (np > 16) return B64_ERR_INVALID_ARGS;
+    for (int q = 0; q < np; q++) {
+        Stage st;
+        st.slot = q;
+        st.kind = b64::MultiKind::NttPass;
+        st.ntt.data = cw;
+        st.ntt.W = si->dd.W;
+        st.ntt.msg = q == 0 ? (const uint4 *)jd.d_msg : nullptr;
+        st.ntt.m = m;
+        st.ntt.log_d = log_d;
+        st.ntt.layer0 = starts[q];
+        st.ntt.k = ks[q];
+        st.ntt.c = cs[q];
+        st.blocks = (uint32_t)(1ull << (log_d - ks[q] - cs[q]));
+        out.push_back(st);
+    }
+    // leaves of the codeword + interleaved fold into oracle 0
+    uint4 *o0 = (uint4 *)(ws + L.oracle[0]);
+    {
+        Stage st;
+        st.slot = SLOT_LEAF16;
+        st.kind = b64::MultiKind::Leaf16Fold;
+        st.leaf16.cw = cw;
+        st.leaf16.n_leaves = 1ull << L.rs;
+        st.leaf16.digests = tree0;
+        st.leaf16.folded = o0;
+        for (int t = 0; t < 4; t++) st.leaf16.chal[t] = to_u4(jd.challenges + 2 * t);
+        st.blocks = blocks_for(st.leaf16.n_leaves);
+        out.push_back(st);
+    }
+    auto tree_steps = [&](uint4 *tree, uint64_t n_leaves, int slot0) {
+        uint64_t offs[16], ns[16];
+        int lv[16];
+        const int steps = b64::merkle_reduce_steps(n_leaves, offs, ns, lv);
+        for (int i = 0; i < steps; i++) {
+            Stage st;
+            st.slot = slot0 + i;
+            st.kind = b64::MultiKind::Subtree;
+            st.sub.tree = tree;
+            st.sub.in_off = offs[i];
+            st.sub.n_in = ns[i];
+            st.sub.levels = lv[i];
+            st.sub.iv = job_iv();
+            st.blocks = (uint32_t)(ns[i] >> lv[i]);
+            out.push_back(st);
+        }
+        roots.push_back(tree + 2 * (2 * n_leaves - 2));
+    };
+    tree_steps(tree0, 1ull << L.rs, SLOT_TREE0);
+    // folded oracles: fold (k > 0), leaves, tree
+    int chal_at = L.lbs;
+    for (int k = 0; k < L.n_oracles; k++) {
+        const int base = SLOT_ORACLE + 16 * k;
+        uint4 *ok = (uint4 *)(ws + L.oracle[k]);
+        if (k > 0) {
+            const int a = (int)p.fold_arities[k - 1];
+            if (a > 4) return B64_ERR_INVALID_ARGS;
+            Stage st;
+            st.slot = base;
+            st.kind = b64::MultiKind::FoldCodeword;
+            st.fold.cw = (const uint4 *)(ws + L.oracle[k - 1]);
+            st.fold.out = ok;
+            st.fold.n_out = 1ull << L.olen[k];
+            st.fold.log_len = L.olen[k - 1];
+            st.fold.n_chal = a;
+            st.fold.W = si->dd.W;
+            for (int t = 0; t < a; t++) st.fold.chal[t] = to_u4(jd.challenges + 2 * (chal_at + t));
+            chal_at += a;
+            st.blocks = blocks_for(st.fold.n_out);
+            out.push_back(st);
+        }
+        uint4 *tk = (uint4 *)(ws + L.tree[k]);
+        const uint64_t n_leaves = 1ull << (L.olen[k] - L.ocoset[k]);
+        Stage st;
+        st.slot = L.ocoset[k] == 4 ? base + 1 : base + 2;  // one kernel kind per slot
+        if (L.ocoset[k] == 4) {
+            st.kind = b64::MultiKind::Leaf16Fold;
+            st.leaf16.cw = ok;
+            st.leaf16.n_leaves = n_leaves;
+            st.leaf16.digests = tk;
+            st.leaf16.folded = nullptr;
+            for (int t = 0; t < 4; t++) st.leaf16.chal[t] = make_uint4(0, 0, 0, 0);
+        } else {
+            st.kind = b64::MultiKind::LeafHash;
+            st.leaf.elems = ok;
+            st.leaf.n_leaves = n_leaves;
+            st.leaf.batch = 1ull << L.ocoset[k];
+            st.leaf.digests = tk;
+        }
+        st.blocks = blocks_for(n_leaves);
+        out.push_back(st);
+        tree_steps(tk, n_leaves, base + 3);
+    }
+    return B64_OK;
+}
+
+__global__ void k_gather_roots(const uint4 *const *__restrict__ src, uint4 *__restrict__ dst, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        dst[2 * i] = src[i][0];
+        dst[2 * i + 1] = src[i][1];
+    }
+}
+
+}  // namespace
+
+struct b64_batch_plan {
+    struct Launch {
+        b64::MultiKind kind;
+        size_t args_off, start_off;  // into d_tables
+        int n_jobs;
+        uint32_t blocks;
+    };
+    std::vector<Launch> launches;
+    std::vector<b64_job_desc> jobs;
+    std::vector<int> roots_per_job;
+    int n_roots = 0;
+    uint8_t *d_tables = nullptr;
+    const uint4 **d_root_src = nullptr;
+    uint4 *d_roots = nullptr;
+    uint8_t *h_roots = nullptr;  // pinned
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool use_graph = false;
+
+    ~b64_batch_plan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (d_tables) (void)hipFree(d_tables);
+        if (d_roots) (void)hipFree(d_roots);
+        if (h_roots) (void)hipHostFree(h_roots);
+    }
+
+    int enqueue(hipStream_t s) const {
+        for (const Launch &l : launches)
+            HIP_TRY(b64::launch_multi(l.kind, d_tables + l.args_off, (const uint32_t *)(d_tables + l.start_off),
+                                      l.n_jobs, l.blocks, s));
+        if (n_roots) {
+            k_gather_roots<<<(n_roots + 63) / 64, 64, 0, s>>>(d_root_src, d_roots, n_roots);
+            HIP_TRY(hipGetLastError());
+        }
+        return B64_OK;
+    }
+};
+
+extern "C" {
+
+size_t b64_job_workspace_size(uint32_t log_msg_len) {
+    SizeInfo *si = nullptr;
+    if (log_msg_len < 5 || size_info(log_msg_len, &si) != B64_OK) return 0;
+    Layout L;
+    layout_of(si, L);
+    return L.total;
+}
+
+uint32_t b64_job_n_roots(uint32_t log_msg_len) {
+    SizeInfo *si = nullptr;
+    if (log_msg_len < 5 || size_info(log_msg_len, &si) != B64_OK) return 0;
+    return si->p.n_fold_arities + 2;
+}
+
+int b64_job_oracle(const b64_job_desc *job, uint32_t k, const void **d_codeword, uint32_t *log_len) {
+    if (!job) return B64_ERR_INVALID_ARGS;
+    SizeInfo *si = nullptr;
+    B64_TRY(size_info(job->log_msg_len, &si));
+    Layout L;
+    layout_of(si, L);
+    if (k > (uint32_t)L.n_oracles) return B64_ERR_INDEX_OUT_OF_RANGE;
+    const uint8_t *ws = (const uint8_t *)job->d_workspace;
+    if (k == 0) {
+        if (d_codeword) *d_codeword = ws + L.cw;
+        if (log_len) *log_len = (uint32_t)(L.rs + L.lbs);
+    } else {
+        if (d_codeword) *d_codeword = ws + L.oracle[k - 1];
+        if (log_len) *log_len = (uint32_t)L.olen[k - 1];
+    }
+    return B64_OK;
+}
+
+// Plain path: the job's stages as single-job launches on `stream`, then its
+// roots read back.
+int b64_job_run(const b64_job_desc *job, void *stream) {
+    if (!job || !job->d_msg || !job->d_workspace || !job->challenges || !job->roots_out) return B64_ERR_INVALID_ARGS;
+    SizeInfo *si = nullptr;
+    B64_TRY(size_info(job->log_msg_len, &si));
+    B64_TRY(size_device(si));
+    Layout L;
+    layout_of(si, L);
+    std::vector<Stage> stages;
+    std::vector<const uint4 *> roots;
+    B64_TRY(job_stages(*job, L, stages, roots));
+    hipStream_t s = as_stream(stream);
+    const b64_fri_params &p = si->p;
+    uint8_t *ws = (uint8_t *)job->d_workspace;
+    // the same computation through the single-job launchers
+    HIP_TRY(b64::launch_ntt(si->dd, (uint4 *)(ws + L.cw), L.rs + L.lbs, 1, L.lbs, false, s,
+                            (const uint4 *)job->d_msg, (int)job->log_msg_len));
+    uint4 ch[4];
+    for (int t = 0; t < 4; t++) ch[t] = to_u4(job->challenges + 2 * t);
+    HIP_TRY(b64::launch_leaf16_fold((const uint4 *)(ws + L.cw), 1ull << L.rs, (uint4 *)(ws + L.tree0),
+                                    (uint4 *)(ws + L.oracle[0]), ch, s));
+    HIP_TRY(b64::launch_merkle_levels((uint4 *)(ws + L.tree0), 1ull << L.rs, job_iv(), s));
+    int chal_at = L.lbs;
+    for (int k = 0; k < L.n_oracles; k++) {
+        if (k > 0) {
+            const int a = (int)p.fold_arities[k - 1];
+            uint4 c[b64::MAX_FOLD_CHAL];
+            for (int t = 0; t < a; t++) c[t] = to_u4(job->challenges + 2 * (chal_at + t));
+            chal_at += a;
+            HIP_TRY(b64::launch_fold_codeword(si->dd, (const uint4 *)(ws + L.oracle[k - 1]), (uint4 *)(ws + L.oracle[k]),
+                                              L.olen[k - 1], c, a, s));
+        }
+        const uint64_t n_leaves = 1ull << (L.olen[k] - L.ocoset[k]);
+        HIP_TRY(b64::launch_leaf_hash((const uint4 *)(ws + L.oracle[k]), n_leaves, 0, 1ull << L.ocoset[k],
+                                      (uint4 *)(ws + L.tree[k]), s));
+        HIP_TRY(b64::launch_merkle_levels((uint4 *)(ws + L.tree[k]), n_leaves, job_iv(), s));
+    }
+    for (size_t r = 0; r < roots.size(); r++)
+        HIP_TRY(hipMemcpyAsync(job->roots_out + 32 * r, roots[r], 32, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return B64_OK;
+}
+
+int b64_batch_plan_create(const b64_job_desc *jobs, uint32_t n_jobs, int use_graph, void *stream,
+                          b64_batch_plan **out) {
+    if (!jobs || !n_jobs || !out) return B64_ERR_INVALID_ARGS;
+    b64_batch_plan *plan = new (std::nothrow) b64_batch_plan();
+    if (!plan) return B64_ERR_OUT_OF_MEMORY;
+    std::unique_ptr<b64_batch_plan> guard(plan);
+    plan->jobs.assign(jobs, jobs + n_jobs);
+    // per-slot job lists
+    std::map<int, std::vector<Stage>> slots;
+    std::vector<const uint4 *> root_src;
+    for (uint32_t j = 0; j < n_jobs; j++) {
+        const b64_job_desc &jd = jobs[j];
+        if (!jd.d_msg || !jd.d_workspace || !jd.challenges || !jd.roots_out) return B64_ERR_INVALID_ARGS;
+        SizeInfo *si = nullptr;
+        B64_TRY(size_info(jd.log_msg_len, &si));
+        B64_TRY(size_device(si));
+        Layout L;
+        layout_of(si, L);
+        std::vector<Stage> st;
+        std::vector<const uint4 *> roots;
+        B64_TRY(job_stages(jd, L, st, roots));
+        for (const Stage &x : st) slots[x.slot].push_back(x);
+        plan->roots_per_job.push_back((int)roots.size());
+        root_src.insert(root_src.end(), roots.begin(), roots.end());
+    }
+    plan->n_roots = (int)root_src.size();
+    // device tables: per launch an args array and a block-start array
+    std::vector<uint8_t> host;
+    auto put = [&](const void *p, size_t n) {
+        const size_t off = align_up(host.size());
+        host.resize(off + n);
+        memcpy(host.data() + off, p, n);
+        return off;
+    };
+    for (auto &kv : slots) {
+        const std::vector<Stage> &v = kv.second;
+        // stages in one slot share a kind
+        b64_batch_plan::Launch l;
+        l.kind = v[0].kind;
+        l.n_jobs = (int)v.size();
+        std::vector<uint32_t> start;
+        uint32_t total = 0;
+        std::vector<uint8_t> args;
+        for (const Stage &x : v) {
+            if (x.kind != l.kind) return B64_ERR_INVALID_ARGS;
+            start.push_back(total);
+            total += x.blocks;
+            const void *a = nullptr;
+            size_t sz = 0;
+            switch (x.kind) {
+                case b64::MultiKind::NttPass: a = &x.ntt; sz = sizeof(x.ntt); break;
+                case b64::MultiKind::Leaf16Fold: a = &x.leaf16; sz = sizeof(x.leaf16); break;
+                case b64::MultiKind::LeafHash: a = &x.leaf; sz = sizeof(x.leaf); break;
+                case b64::MultiKind::Subtree: a = &x.sub; sz = sizeof(x.sub); break;
+                case b64::MultiKind::FoldCodeword: a = &x.fold; sz = sizeof(x.fold); break;
+            }
+            const size_t o = args.size();
+            args.resize(o + sz);
+            memcpy(args.data() + o, a, sz);
+        }
+        l.blocks = total;
+        l.args_off = put(args.data(), args.size());
+        l.start_off = put(start.data(), start.size() * sizeof(uint32_t));
+        plan->launches.push_back(l);
+    }
+    const size_t root_src_off = put(root_src.data(), root_src.size() * sizeof(void *));
+    HIP_TRY(hipMalloc(&plan->d_tables, host.size() ? host.size() : 16));
+    HIP_TRY(hipMemcpy(plan->d_tables, host.data(), host.size(), hipMemcpyHostToDevice));
+    plan->d_root_src = (const uint4 **)(plan->d_tables + root_src_off);
+    HIP_TRY(hipMalloc(&plan->d_roots, 32ull * (plan->n_roots ? plan->n_roots : 1)));
+    HIP_TRY(hipHostMalloc((void **)&plan->h_roots, 32ull * (plan->n_roots ? plan->n_roots : 1), hipHostMallocDefault));
+    if (use_graph) {
+        hipStream_t cap = nullptr;
+        HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        int rc = B64_OK;
+        if (hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = B64_ERR_HIP;
+        if (rc == B64_OK) {
+            rc = plan->enqueue(cap);
+            hipGraph_t g = nullptr;
+            if (hipStreamEndCapture(cap, &g) != hipSuccess) rc = B64_ERR_HIP;
+            plan->graph = g;
+        }
+        if (rc == B64_OK && hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0) != hipSuccess)
+            rc = B64_ERR_HIP;
+        (void)hipStreamDestroy(cap);
+        B64_TRY(rc);
+        plan->use_graph = true;
+    }
+    (void)stream;
+    *out = guard.release();
+    return B64_OK;
+}
+
+int b64_batch_plan_run(b64_batch_plan *plan, void *stream) {
+    if (!plan) return B64_ERR_INVALID_ARGS;
+    hipStream_t s = as_stream(stream);
+    if (plan->use_graph)
+        HIP_TRY(hipGraphLaunch(plan->exec, s));
+    else
+        B64_TRY(plan->enqueue(s));
+    if (plan->n_roots) {
+        HIP_TRY(hipMemcpyAsync(plan->h_roots, plan->d_roots, 32ull * plan->n_roots, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        size_t at = 0;
+        for (size_t j = 0; j < plan->jobs.size(); j++) {
+            memcpy(plan->jobs[j].roots_out, plan->h_roots + 32 * at, 32ull * plan->roots_per_job[j]);
+            at += plan->roots_per_job[j];
+        }
+    }
+    return B64_OK;
+}
+
+uint32_t b64_batch_plan_n_launches(const b64_batch_plan *plan) { return plan ? (uint32_t)plan->launches.size() : 0; }
+
+void b64_batch_plan_destroy(b64_batch_plan *plan) { delete plan; }
+
+}  // extern "C"

@@ -29,6 +29,47 @@ struct CompressIV {
     uint32_t w[8];
 };
 
+constexpr int MAX_FOLD_CHAL = 12;
+constexpr int MERKLE_RED_LOG = 9;  // tree levels reduced per subtree launch
+
+// Per-stage kernel arguments.  Single-job launches pass them by value; the
+// batched path keeps one table per launch in device memory.
+struct NttPassArgs {
+    uint4 *data;
+    const uint4 *W;
+    // First encode pass only: read the tile straight from the message,
+    // out[r*2^m + i] = msg[rev_m(i)] (reed_solomon.rs:164-187), instead of data.
+    const uint4 *msg;
+    int log_d, layer0, k, c, m;
+};
+struct LeafArgs {
+    const uint4 *elems;
+    uint64_t n_leaves, batch;
+    uint4 *digests;
+};
+struct Leaf16Args {
+    const uint4 *cw;
+    uint64_t n_leaves;
+    uint4 *digests;
+    uint4 *folded;  // nullptr: hash only
+    uint4 chal[4];
+};
+struct SubtreeArgs {
+    uint4 *tree;
+    uint64_t in_off, n_in;
+    int levels;
+    CompressIV iv;
+};
+struct FoldCwArgs {
+    const uint4 *cw;
+    uint4 *out;
+    uint64_t n_out;
+    int log_len, n_chal;
+    const uint4 *W;
+    uint4 chal[MAX_FOLD_CHAL];
+};
+enum class MultiKind { NttPass, Leaf16Fold, LeafHash, Subtree, FoldCodeword };
+
 struct NttTuning {
     int variant;   // 0 = general product, 1 = LDS window tables
     int max_k;     // layers per pass
@@ -52,7 +93,19 @@ hipError_t launch_fold_interleaved(const uint4 *cw, uint4 *out, uint64_t n_out, 
                                    const uint4 *eq_host, hipStream_t s);
 hipError_t launch_fold_codeword(const DeviceDomain &dd, const uint4 *cw, uint4 *out, int log_len,
                                 const uint4 *chal_host, int n_chal, hipStream_t s);
+// Subtree-reduction schedule of a tree with n_leaves leaves: per launch the
+// input level offset (in digests), its node count and the levels reduced.
+int merkle_reduce_steps(uint64_t n_leaves, uint64_t *offs, uint64_t *ns, int *levels);
+// Pass schedule of a forward NTT (layer0, k, c per pass); returns the count.
+int ntt_pass_plan(int log_d, int skip_early, int skip_late, int *layer0, int *k, int *c);
+// One launch of `kind` over n_jobs jobs: d_jobs is a device array of the
+// kind's args, d_start[j] the first block of job j, `blocks` the total.
+hipError_t launch_multi(MultiKind kind, const void *d_jobs, const uint32_t *d_start, int n_jobs, uint32_t blocks,
+                        hipStream_t s);
 hipError_t launch_pack_witness(const uint64_t *words, uint64_t n_words, uint4 *out, uint64_t n_elems,
                                hipStream_t s);
 
 }  // namespace b64
+
+// Device view of a domain (uploads its twiddle basis on first use).
+int b64_domain_device_view_internal(b64_domain *d, b64::DeviceDomain *dd);

@@ -82,14 +82,6 @@ constexpr int NTT_TAB_GROUP = 4;  // window tables resident in LDS at once (8 Ki
 // largest number of layers per pass and the tile size.
 NttTuning g_ntt_tuning = {1, NTT_MAX_K, NTT_TILE_LOG};
 
-struct NttPassArgs {
-    uint4 *data;
-    const uint4 *W;
-    // First encode pass only: read the tile straight from the message,
-    // out[r*2^m + i] = msg[rev_m(i)] (reed_solomon.rs:164-187), instead of data.
-    const uint4 *msg;
-    int log_d, layer0, k, c, m;
-};
 
 __device__ __forceinline__ uint32_t rev_bits(uint32_t x, int b) { return b ? (__brev(x) >> (32 - b)) : 0u; }
 
@@ -174,15 +166,20 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass(NttPassArgs a) {
 // through an LDS window table.  The 2^j blocks of layer j are contiguous ranges
 // of the butterfly enumeration; NTT_TAB_GROUP tables are built at a time (32
 // lanes per table) and consumed by the butterflies of those blocks.
+struct NttTabSmem {
+    uint4 s[1 << NTT_TILE_LOG];
+    uint4 tabs[NTT_TAB_GROUP * WIN_ENTRIES];
+    uint4 tw[1 << (NTT_MAX_K - 1)];
+};
+
 template <bool INVERSE>
-__global__ __launch_bounds__(BLOCK) void k_ntt_pass_tab(NttPassArgs a) {
-    __shared__ uint4 s[1 << NTT_TILE_LOG];
-    __shared__ uint4 tabs[NTT_TAB_GROUP * WIN_ENTRIES];
-    __shared__ uint4 tw[1 << (NTT_MAX_K - 1)];
+__device__ __forceinline__ void ntt_tab_tile(const NttPassArgs &a, uint64_t tile, NttTabSmem &sm) {
+    uint4 *__restrict__ s = sm.s;
+    uint4 *__restrict__ tabs = sm.tabs;
+    uint4 *__restrict__ tw = sm.tw;
     const int k = a.k, c = a.c;
     const int lb = a.log_d - a.layer0 - k;
     const int mid_bits = lb - c;
-    const uint64_t tile = blockIdx.x;
     const uint64_t mid = tile & ((1ull << mid_bits) - 1);
     const uint64_t hi = tile >> mid_bits;
     const uint64_t base = (hi << (lb + k)) | (mid << c);
@@ -239,28 +236,42 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass_tab(NttPassArgs a) {
         a.data[base | ((uint64_t)(L >> c) << lb) | (uint64_t)(L & cmask)] = s[L];
 }
 
-hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_early, int skip_late,
-                      bool inverse, hipStream_t s, const uint4 *msg, int msg_log) {
+template <bool INVERSE>
+__global__ __launch_bounds__(BLOCK) void k_ntt_pass_tab(NttPassArgs a) {
+    __shared__ NttTabSmem sm;
+    ntt_tab_tile<INVERSE>(a, blockIdx.x, sm);
+}
+
+
+int ntt_pass_plan(int log_d, int skip_early, int skip_late, int *layer0, int *kk, int *cc) {
     const int first = skip_early, end = log_d - skip_late;
     const int n_layers = end - first;
-    if (n_layers <= 0) return hipSuccess;
-    // message mode needs the first pass to start at layer lir = log_d - msg_log
-    if (msg && (inverse || skip_early != log_d - msg_log)) return hipErrorInvalidValue;
+    if (n_layers <= 0) return 0;
     int max_k = g_ntt_tuning.max_k;
     if (max_k < 1 || max_k > NTT_MAX_K) max_k = NTT_MAX_K;
     int tile_log = g_ntt_tuning.tile_log;
     if (tile_log < max_k || tile_log > NTT_TILE_LOG) tile_log = NTT_TILE_LOG;
     const int n_passes = (n_layers + max_k - 1) / max_k;
     // balanced split of the layer range into passes
-    int starts[64], ks[64];
     int at = first;
     for (int p = 0; p < n_passes; p++) {
         const int left = end - at, passes_left = n_passes - p;
         const int k = (left + passes_left - 1) / passes_left;
-        starts[p] = at;
-        ks[p] = k;
+        const int lb = log_d - at - k;
+        layer0[p] = at;
+        kk[p] = k;
+        cc[p] = lb < (tile_log - k) ? lb : (tile_log - k);
         at += k;
     }
+    return n_passes;
+}
+
+hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_early, int skip_late,
+                      bool inverse, hipStream_t s, const uint4 *msg, int msg_log) {
+    // message mode needs the first pass to start at layer lir = log_d - msg_log
+    if (msg && (inverse || skip_early != log_d - msg_log)) return hipErrorInvalidValue;
+    int starts[64], ks[64], cs[64];
+    const int n_passes = ntt_pass_plan(log_d, skip_early, skip_late, starts, ks, cs);
     for (int q = 0; q < n_passes; q++) {
         const int p = inverse ? (n_passes - 1 - q) : q;
         NttPassArgs a;
@@ -271,8 +282,7 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
         a.log_d = log_d;
         a.layer0 = starts[p];
         a.k = ks[p];
-        const int lb = log_d - a.layer0 - a.k;
-        a.c = lb < (tile_log - a.k) ? lb : (tile_log - a.k);
+        a.c = cs[p];
         const uint64_t n_tiles = 1ull << (log_d - a.k - a.c);
         if (g_ntt_tuning.variant == 1) {
             if (inverse)
@@ -294,25 +304,36 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
 // ---------------------------------------------------------------------------
 // Merkle leaves: SHA-256 over the LE16 bytes of `batch` consecutive elements
 // (hash/parallel_digest.rs:156-172).  Digest bytes are stored as 8 LE words.
+// Every stage below has a per-block core, a single-job kernel and a multi-job
+// kernel (block -> job through a prefix table) used by the batched path.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_leaf_hash(const uint4 *__restrict__ elems, uint64_t n_leaves,
-                                                     uint64_t batch, uint4 *__restrict__ digests) {
-    const uint64_t leaf = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (leaf >= n_leaves) return;
-    const uint4 *p = elems + leaf * batch;
+__device__ __forceinline__ void load_block_words(uint32_t w[16], const uint4 *p) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint4 e = p[q];
+        w[4 * q + 0] = bswap32(e.x);
+        w[4 * q + 1] = bswap32(e.y);
+        w[4 * q + 2] = bswap32(e.z);
+        w[4 * q + 3] = bswap32(e.w);
+    }
+}
+
+__device__ __forceinline__ void store_digest_be(uint4 *d, const uint32_t st[8]) {
+    d[0] = make_uint4(bswap32(st[0]), bswap32(st[1]), bswap32(st[2]), bswap32(st[3]));
+    d[1] = make_uint4(bswap32(st[4]), bswap32(st[5]), bswap32(st[6]), bswap32(st[7]));
+}
+
+__device__ __forceinline__ void leaf_hash_core(const LeafArgs &a, uint64_t blk) {
+    const uint64_t leaf = blk * BLOCK + threadIdx.x;
+    if (leaf >= a.n_leaves) return;
+    const uint64_t batch = a.batch;
+    const uint4 *p = a.elems + leaf * batch;
     uint32_t st[8];
     sha256_init(st);
     uint32_t w[16];
     const uint64_t n_full = batch >> 2;
-    for (uint64_t blk = 0; blk < n_full; blk++) {
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const uint4 e = p[blk * 4 + q];
-            w[4 * q + 0] = bswap32(e.x);
-            w[4 * q + 1] = bswap32(e.y);
-            w[4 * q + 2] = bswap32(e.z);
-            w[4 * q + 3] = bswap32(e.w);
-        }
+    for (uint64_t b = 0; b < n_full; b++) {
+        load_block_words(w, p + 4 * b);
         sha256_compress(st, w);
     }
     const int rem = (int)(batch & 3);
@@ -330,30 +351,115 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_hash(const uint4 *__restrict__ e
     w[14] = (uint32_t)(bits >> 32);
     w[15] = (uint32_t)bits;
     sha256_compress(st, w);
-    digests[2 * leaf] = make_uint4(bswap32(st[0]), bswap32(st[1]), bswap32(st[2]), bswap32(st[3]));
-    digests[2 * leaf + 1] = make_uint4(bswap32(st[4]), bswap32(st[5]), bswap32(st[6]), bswap32(st[7]));
+    store_digest_be(a.digests + 2 * leaf, st);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_leaf_hash(LeafArgs a) { leaf_hash_core(a, blockIdx.x); }
+
+// Many jobs in one launch: block b belongs to the job j with
+// start[j] <= b < start[j+1] and works on that job's block b - start[j].
+__device__ __forceinline__ int job_of_block2(const uint32_t *__restrict__ start, int n_jobs, uint32_t b) {
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_leaf_hash_multi(const LeafArgs *__restrict__ jobs,
+                                                           const uint32_t *__restrict__ start, int n_jobs) {
+    const int j = job_of_block2(start, n_jobs, blockIdx.x);
+    leaf_hash_core(jobs[j], blockIdx.x - start[j]);
+}
+
+// Leaves of 16 elements (the interleaved batch 2^4 of the commit) hashed and,
+// optionally, folded by fold_interleaved in the same pass over the codeword:
+// the 16 elements of leaf i are exactly the 16 inputs of folded element i.
+// The fold multiplies by the four challenges through LDS window tables.
+__device__ __forceinline__ void leaf16_fold_core(const Leaf16Args &a, uint64_t blk, uint4 *__restrict__ tabs) {
+    const bool do_fold = a.folded != nullptr;
+    if (do_fold && threadIdx.x < 4 * WIN_ROWS)
+        win_build_row(tabs + (threadIdx.x >> 5) * WIN_ENTRIES, a.chal[threadIdx.x >> 5], threadIdx.x & 31);
+    __syncthreads();
+    const uint64_t leaf = blk * BLOCK + threadIdx.x;
+    if (leaf >= a.n_leaves) return;
+    const uint4 *p = a.cw + leaf * 16;
+    uint4 v[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) v[j] = p[j];
+    uint32_t st[8];
+    sha256_init(st);
+    uint32_t w[16];
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        load_block_words(w, v + 4 * b);
+        sha256_compress(st, w);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) w[q] = 0;
+    w[0] = 0x80000000u;
+    w[15] = 16 * 128;
+    sha256_compress(st, w);
+    store_digest_be(a.digests + 2 * leaf, st);
+    if (do_fold) {
+        int len = 16;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            len >>= 1;
+#pragma unroll
+            for (int q = 0; q < len; q++)
+                v[q] = gxor(v[2 * q], win_mul(tabs + t * WIN_ENTRIES, gxor(v[2 * q], v[2 * q + 1])));
+        }
+        a.folded[leaf] = v[0];
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_leaf16_fold(Leaf16Args a) {
+    __shared__ uint4 tabs[4 * WIN_ENTRIES];
+    leaf16_fold_core(a, blockIdx.x, tabs);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_leaf16_fold_multi(const Leaf16Args *__restrict__ jobs,
+                                                             const uint32_t *__restrict__ start, int n_jobs) {
+    __shared__ uint4 tabs[4 * WIN_ENTRIES];
+    const int j = job_of_block2(start, n_jobs, blockIdx.x);
+    leaf16_fold_core(jobs[j], blockIdx.x - start[j], tabs);
 }
 
 hipError_t launch_leaf16_fold(const uint4 *cw, uint64_t n_leaves, uint4 *digests, uint4 *folded, const uint4 *chal,
-                              hipStream_t s);
+                              hipStream_t s) {
+    Leaf16Args a;
+    a.cw = cw;
+    a.n_leaves = n_leaves;
+    a.digests = digests;
+    a.folded = folded;
+    for (int t = 0; t < 4; t++) a.chal[t] = folded ? chal[t] : make_uint4(0, 0, 0, 0);
+    k_leaf16_fold<<<grid_for(n_leaves), BLOCK, 0, s>>>(a);
+    return hipGetLastError();
+}
 
 hipError_t launch_leaf_hash(const uint4 *elems, uint64_t n_leaves, int, uint64_t batch, uint4 *digests,
                             hipStream_t s) {
     if (batch == 16) return launch_leaf16_fold(elems, n_leaves, digests, nullptr, nullptr, s);
-    k_leaf_hash<<<grid_for(n_leaves), BLOCK, 0, s>>>(elems, n_leaves, batch, digests);
+    LeafArgs a;
+    a.elems = elems;
+    a.n_leaves = n_leaves;
+    a.batch = batch;
+    a.digests = digests;
+    k_leaf_hash<<<grid_for(n_leaves), BLOCK, 0, s>>>(a);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
 // Inner nodes: Sha256Compression (hash/compress.rs:46-56): one compress256 of
 // left||right from the custom IV, output state written little-endian.
-// ---------------------------------------------------------------------------
 // Subtree reduction: a block takes 2^levels consecutive nodes of one tree level
 // (levels <= MERKLE_RED_LOG) and writes the next `levels` levels, keeping the
 // working level in LDS.  The tree is stored level after level, so level l+1
 // starts right after level l.
-constexpr int MERKLE_RED_LOG = 9;
-
+// ---------------------------------------------------------------------------
 __device__ __forceinline__ void node_compress(uint4 l0, uint4 l1, uint4 r0, uint4 r1, const CompressIV &iv,
                                               uint4 &o0, uint4 &o1) {
     uint32_t w[16] = {bswap32(l0.x), bswap32(l0.y), bswap32(l0.z), bswap32(l0.w),
@@ -368,23 +474,21 @@ __device__ __forceinline__ void node_compress(uint4 l0, uint4 l1, uint4 r0, uint
     o1 = make_uint4(st[4], st[5], st[6], st[7]);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_merkle_subtree(uint4 *__restrict__ tree, uint64_t in_off, uint64_t n_in,
-                                                          int levels, CompressIV iv) {
-    __shared__ uint4 dig[2][2 << MERKLE_RED_LOG];
-    const int blk_nodes = 1 << levels;
-    const uint64_t b = blockIdx.x;
-    const uint4 *src = tree + 2 * (in_off + b * (uint64_t)blk_nodes);
+__device__ __forceinline__ void subtree_core(const SubtreeArgs &a, uint64_t b, uint4 (*dig)[2 << MERKLE_RED_LOG]) {
+    const int blk_nodes = 1 << a.levels;
+    const uint4 *src = a.tree + 2 * (a.in_off + b * (uint64_t)blk_nodes);
     for (int t = threadIdx.x; t < 2 * blk_nodes; t += BLOCK) dig[0][t] = src[t];
     __syncthreads();
-    uint64_t off = in_off, n = n_in;
+    uint64_t off = a.in_off, n = a.n_in;
     int cur = 0;
-    for (int l = 1; l <= levels; l++) {
+    for (int l = 1; l <= a.levels; l++) {
         const uint64_t out_off = off + n;
         const int cnt = blk_nodes >> l;
-        uint4 *dst = tree + 2 * (out_off + b * (uint64_t)cnt);
+        uint4 *dst = a.tree + 2 * (out_off + b * (uint64_t)cnt);
         for (int t = threadIdx.x; t < cnt; t += BLOCK) {
             uint4 o0, o1;
-            node_compress(dig[cur][4 * t], dig[cur][4 * t + 1], dig[cur][4 * t + 2], dig[cur][4 * t + 3], iv, o0, o1);
+            node_compress(dig[cur][4 * t], dig[cur][4 * t + 1], dig[cur][4 * t + 2], dig[cur][4 * t + 3], a.iv, o0,
+                          o1);
             dig[cur ^ 1][2 * t] = o0;
             dig[cur ^ 1][2 * t + 1] = o1;
             dst[2 * t] = o0;
@@ -397,20 +501,53 @@ __global__ __launch_bounds__(BLOCK) void k_merkle_subtree(uint4 *__restrict__ tr
     }
 }
 
-hipError_t launch_merkle_levels(uint4 *tree, uint64_t n_leaves, const CompressIV &iv, hipStream_t s) {
+__global__ __launch_bounds__(BLOCK) void k_merkle_subtree(SubtreeArgs a) {
+    __shared__ uint4 dig[2][2 << MERKLE_RED_LOG];
+    subtree_core(a, blockIdx.x, dig);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_merkle_subtree_multi(const SubtreeArgs *__restrict__ jobs,
+                                                                const uint32_t *__restrict__ start, int n_jobs) {
+    __shared__ uint4 dig[2][2 << MERKLE_RED_LOG];
+    const int j = job_of_block2(start, n_jobs, blockIdx.x);
+    subtree_core(jobs[j], blockIdx.x - start[j], dig);
+}
+
+int merkle_reduce_steps(uint64_t n_leaves, uint64_t *offs, uint64_t *ns, int *levels_out) {
     uint64_t off = 0, n = n_leaves;
+    int steps = 0;
     while (n > 1) {
         int log_n = 0;
         while ((1ull << log_n) < n) log_n++;
         const int levels = log_n < MERKLE_RED_LOG ? log_n : MERKLE_RED_LOG;
-        const uint64_t blocks = n >> levels;
-        k_merkle_subtree<<<(unsigned)blocks, BLOCK, 0, s>>>(tree, off, n, levels, iv);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
+        if (offs) {
+            offs[steps] = off;
+            ns[steps] = n;
+            levels_out[steps] = levels;
+        }
+        steps++;
         for (int l = 0; l < levels; l++) {
             off += n;
             n >>= 1;
         }
+    }
+    return steps;
+}
+
+hipError_t launch_merkle_levels(uint4 *tree, uint64_t n_leaves, const CompressIV &iv, hipStream_t s) {
+    uint64_t offs[16], ns[16];
+    int lv[16];
+    const int steps = merkle_reduce_steps(n_leaves, offs, ns, lv);
+    for (int i = 0; i < steps; i++) {
+        SubtreeArgs a;
+        a.tree = tree;
+        a.in_off = offs[i];
+        a.n_in = ns[i];
+        a.levels = lv[i];
+        a.iv = iv;
+        k_merkle_subtree<<<(unsigned)(ns[i] >> lv[i]), BLOCK, 0, s>>>(a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
@@ -418,70 +555,9 @@ hipError_t launch_merkle_levels(uint4 *tree, uint64_t n_leaves, const CompressIV
 // ---------------------------------------------------------------------------
 // FRI folds
 // ---------------------------------------------------------------------------
-constexpr int MAX_FOLD_CHAL = 12;
 struct FoldChallenges {
     uint4 r[MAX_FOLD_CHAL];
 };
-
-// Leaves of 16 elements (the interleaved batch 2^4 of the commit) hashed and,
-// optionally, folded by fold_interleaved in the same pass over the codeword:
-// the 16 elements of leaf i are exactly the 16 inputs of folded element i.
-// The fold multiplies by the four challenges through LDS window tables.
-__global__ __launch_bounds__(BLOCK) void k_leaf16_fold(const uint4 *__restrict__ cw, uint64_t n_leaves,
-                                                       uint4 *__restrict__ digests, uint4 *__restrict__ folded,
-                                                       FoldChallenges ch) {
-    __shared__ uint4 tabs[4 * WIN_ENTRIES];
-    const bool do_fold = folded != nullptr;
-    if (do_fold && threadIdx.x < 4 * WIN_ROWS)
-        win_build_row(tabs + (threadIdx.x >> 5) * WIN_ENTRIES, ch.r[threadIdx.x >> 5], threadIdx.x & 31);
-    __syncthreads();
-    const uint64_t leaf = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (leaf >= n_leaves) return;
-    const uint4 *p = cw + leaf * 16;
-    uint4 v[16];
-#pragma unroll
-    for (int j = 0; j < 16; j++) v[j] = p[j];
-    uint32_t st[8];
-    sha256_init(st);
-    uint32_t w[16];
-#pragma unroll
-    for (int blk = 0; blk < 4; blk++) {
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const uint4 e = v[blk * 4 + q];
-            w[4 * q + 0] = bswap32(e.x);
-            w[4 * q + 1] = bswap32(e.y);
-            w[4 * q + 2] = bswap32(e.z);
-            w[4 * q + 3] = bswap32(e.w);
-        }
-        sha256_compress(st, w);
-    }
-#pragma unroll
-    for (int q = 0; q < 16; q++) w[q] = 0;
-    w[0] = 0x80000000u;
-    w[15] = 16 * 128;
-    sha256_compress(st, w);
-    digests[2 * leaf] = make_uint4(bswap32(st[0]), bswap32(st[1]), bswap32(st[2]), bswap32(st[3]));
-    digests[2 * leaf + 1] = make_uint4(bswap32(st[4]), bswap32(st[5]), bswap32(st[6]), bswap32(st[7]));
-    if (do_fold) {
-        int len = 16;
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            len >>= 1;
-#pragma unroll
-            for (int q = 0; q < len; q++) v[q] = gxor(v[2 * q], win_mul(tabs + t * WIN_ENTRIES, gxor(v[2 * q], v[2 * q + 1])));
-        }
-        folded[leaf] = v[0];
-    }
-}
-
-hipError_t launch_leaf16_fold(const uint4 *cw, uint64_t n_leaves, uint4 *digests, uint4 *folded, const uint4 *chal,
-                              hipStream_t s) {
-    FoldChallenges ch;
-    for (int t = 0; t < 4; t++) ch.r[t] = folded ? chal[t] : make_uint4(0, 0, 0, 0);
-    k_leaf16_fold<<<grid_for(n_leaves), BLOCK, 0, s>>>(cw, n_leaves, digests, folded, ch);
-    return hipGetLastError();
-}
 
 // fold_interleaved (prover fri/fold.rs:260-282): out[i] = <cw[i*B..(i+1)*B], eq(r)>.
 // Computed as log_batch successive line folds (bit t of the in-row index pairs
@@ -541,41 +617,92 @@ __device__ __forceinline__ uint4 twiddle_of(const uint4 *__restrict__ W, int lay
 // (verifier fri/fold.rs:21-99): per coset, n_chal rounds of inverse butterfly
 // and line extrapolation x0 + (x1 - x0) * r.
 template <int MAXV>
-__global__ __launch_bounds__(BLOCK) void k_fold_codeword(const uint4 *__restrict__ cw, uint4 *__restrict__ out,
-                                                         uint64_t n_out, int log_len, int n_chal,
-                                                         const uint4 *__restrict__ W, FoldChallenges ch) {
-    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n_out) return;
+__device__ __forceinline__ void fold_codeword_core(const FoldCwArgs &a, uint64_t blk) {
+    const uint64_t i = blk * BLOCK + threadIdx.x;
+    if (i >= a.n_out) return;
+    const int n_chal = a.n_chal;
     uint4 v[MAXV];
     const int cs = 1 << n_chal;
-    for (int j = 0; j < cs; j++) v[j] = cw[(i << n_chal) + j];
-    int ll = log_len, log_size = n_chal;
+    for (int j = 0; j < cs; j++) v[j] = a.cw[(i << n_chal) + j];
+    int ll = a.log_len, log_size = n_chal;
     for (int r = 0; r < n_chal; r++) {
         for (int off = 0; off < (1 << (log_size - 1)); off++) {
             const uint64_t idx = (i << (log_size - 1)) | (uint64_t)off;
-            const uint4 t = twiddle_of(W, ll - 1, idx);
+            const uint4 t = twiddle_of(a.W, ll - 1, idx);
             uint4 u = v[off << 1], w = v[(off << 1) | 1];
             w = gxor(w, u);
             u = gxor(u, gf_mul(w, t));
-            v[off] = gxor(u, gf_mul(gxor(w, u), ch.r[r]));
+            v[off] = gxor(u, gf_mul(gxor(w, u), a.chal[r]));
         }
         ll--;
         log_size--;
     }
-    out[i] = v[0];
+    a.out[i] = v[0];
+}
+
+template <int MAXV>
+__global__ __launch_bounds__(BLOCK) void k_fold_codeword(FoldCwArgs a) {
+    fold_codeword_core<MAXV>(a, blockIdx.x);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fold_codeword_multi(const FoldCwArgs *__restrict__ jobs,
+                                                               const uint32_t *__restrict__ start, int n_jobs) {
+    const int j = job_of_block2(start, n_jobs, blockIdx.x);
+    fold_codeword_core<16>(jobs[j], blockIdx.x - start[j]);
 }
 
 hipError_t launch_fold_codeword(const DeviceDomain &dd, const uint4 *cw, uint4 *out, int log_len,
                                 const uint4 *chal, int n_chal, hipStream_t s) {
-    FoldChallenges ch;
-    for (int t = 0; t < n_chal; t++) ch.r[t] = chal[t];
-    const uint64_t n_out = 1ull << (log_len - n_chal);
+    FoldCwArgs a;
+    a.cw = cw;
+    a.out = out;
+    a.n_out = 1ull << (log_len - n_chal);
+    a.log_len = log_len;
+    a.n_chal = n_chal;
+    a.W = dd.W;
+    if (n_chal > MAX_FOLD_CHAL) return hipErrorInvalidValue;
+    for (int t = 0; t < n_chal; t++) a.chal[t] = chal[t];
     if (n_chal <= 4)
-        k_fold_codeword<16><<<grid_for(n_out), BLOCK, 0, s>>>(cw, out, n_out, log_len, n_chal, dd.W, ch);
+        k_fold_codeword<16><<<grid_for(a.n_out), BLOCK, 0, s>>>(a);
     else if (n_chal <= 8)
-        k_fold_codeword<256><<<grid_for(n_out), BLOCK, 0, s>>>(cw, out, n_out, log_len, n_chal, dd.W, ch);
+        k_fold_codeword<256><<<grid_for(a.n_out), BLOCK, 0, s>>>(a);
     else
         return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ntt_pass_tab_multi(const NttPassArgs *__restrict__ jobs,
+                                                              const uint32_t *__restrict__ start, int n_jobs) {
+    __shared__ NttTabSmem sm;
+    const int j = job_of_block2(start, n_jobs, blockIdx.x);
+    const NttPassArgs a = jobs[j];
+    ntt_tab_tile<false>(a, blockIdx.x - start[j], sm);
+}
+
+// ---------------------------------------------------------------------------
+// Multi-job launchers (batched path): one launch covers the same stage of many
+// jobs; tables live in device memory prepared by the caller.
+// ---------------------------------------------------------------------------
+hipError_t launch_multi(MultiKind kind, const void *d_jobs, const uint32_t *d_start, int n_jobs, uint32_t blocks,
+                        hipStream_t s) {
+    if (blocks == 0 || n_jobs == 0) return hipSuccess;
+    switch (kind) {
+        case MultiKind::NttPass:
+            k_ntt_pass_tab_multi<<<blocks, BLOCK, 0, s>>>((const NttPassArgs *)d_jobs, d_start, n_jobs);
+            break;
+        case MultiKind::Leaf16Fold:
+            k_leaf16_fold_multi<<<blocks, BLOCK, 0, s>>>((const Leaf16Args *)d_jobs, d_start, n_jobs);
+            break;
+        case MultiKind::LeafHash:
+            k_leaf_hash_multi<<<blocks, BLOCK, 0, s>>>((const LeafArgs *)d_jobs, d_start, n_jobs);
+            break;
+        case MultiKind::Subtree:
+            k_merkle_subtree_multi<<<blocks, BLOCK, 0, s>>>((const SubtreeArgs *)d_jobs, d_start, n_jobs);
+            break;
+        case MultiKind::FoldCodeword:
+            k_fold_codeword_multi<<<blocks, BLOCK, 0, s>>>((const FoldCwArgs *)d_jobs, d_start, n_jobs);
+            break;
+    }
     return hipGetLastError();
 }
 

@@ -207,6 +207,36 @@ int b64_fri_prover_finish_proof(b64_fri_prover *fp, b64_transcript *t);
 int b64_fri_prover_oracle(const b64_fri_prover *fp, uint32_t k, const void **d_codeword,
                           uint32_t *log_len, const void **d_tree);
 
+/* ---- commit jobs: plain and batched (no single counterpart in the reference) -- */
+/* A job is the FRI commit phase of one message of 2^log_msg_len B128 elements
+ * with the Verifier::setup parameters (verify.rs:61-106: rate 1/2, batch 2^4,
+ * arity 4) and pre-seeded challenges: commit_interleaved (fri/commit.rs:25-75),
+ * then every commit round of FRIFoldProver::execute_fold_round (fri/fold.rs:117-175).
+ * roots_out receives the codeword root and each folded oracle's root, the same
+ * digests the transcript-driven prover writes with these challenges. */
+typedef struct b64_job_desc {
+    uint32_t log_msg_len;       /* 5 .. 40 */
+    const void *d_msg;          /* device, 2^log_msg_len elements */
+    const uint64_t *challenges; /* host, log_msg_len elements (read when the job is run / planned) */
+    void *d_workspace;          /* device, b64_job_workspace_size(log_msg_len) bytes */
+    uint8_t *roots_out;         /* host, b64_job_n_roots(log_msg_len) * 32 bytes */
+} b64_job_desc;
+typedef struct b64_batch_plan b64_batch_plan;
+
+size_t b64_job_workspace_size(uint32_t log_msg_len);
+uint32_t b64_job_n_roots(uint32_t log_msg_len);
+/* Oracle k of a job's workspace: k = 0 the interleaved codeword, k >= 1 folded oracle k-1. */
+int b64_job_oracle(const b64_job_desc *job, uint32_t k, const void **d_codeword, uint32_t *log_len);
+/* Plain path: the job's kernels launched one stage at a time, then its roots read back. */
+int b64_job_run(const b64_job_desc *job, void *stream);
+/* Batched path: each launch runs one stage for every job that has it; with
+ * use_graph the launch sequence is captured once in a hipGraph and replayed. */
+int b64_batch_plan_create(const b64_job_desc *jobs, uint32_t n_jobs, int use_graph, void *stream,
+                          b64_batch_plan **out);
+int b64_batch_plan_run(b64_batch_plan *plan, void *stream);
+uint32_t b64_batch_plan_n_launches(const b64_batch_plan *plan);
+void b64_batch_plan_destroy(b64_batch_plan *plan);
+
 /* ---- witness packing (prover/prover/src/prove.rs:286-327) --------------------- */
 /* B128 element i = word[2i+1] << 64 | word[2i]; zero-padded to 2^log_elems. */
 int b64_pack_witness(const void *d_words, uint64_t n_words, uint32_t log_elems, void *d_out, void *stream);

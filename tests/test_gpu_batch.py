@@ -1,0 +1,78 @@
+"""GPU parity of commit jobs: the plain per-job path and the batched (hipGraph)
+path give the oracle's roots and oracles, for uneven job sizes."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def b64():
+    import binius64_amd as b
+    if b.device_count() == 0:
+        pytest.fail("no HIP device visible")
+    return b
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_job(orc, n, msg, chal):
+    p = orc.fri_params_setup(n, 1, 96)
+    std = orc.Domain(dim=n + 1)
+    dom = orc.Domain(std.subspace(p["rs_log_len"]))
+    res = orc.fri_prove(dom, msg, p["log_dim"], 1, p["log_batch_size"], p["fold_arities"], 1,
+                        challenges=chal, threads=8)
+    return res
+
+
+def make_job(b64, orc, n, seed):
+    msg = orc.random_elems(1000 + seed, 1 << n)
+    chal = orc.random_elems(2000 + seed, n)
+    return b64.CommitJob(n, msg, chal), msg, chal
+
+
+@pytest.mark.parametrize("n", [5, 8, 12, 13, 16])
+def test_plain_job_matches_oracle(b64, orc, n):
+    say("plain job", n)
+    job, msg, chal = make_job(b64, orc, n, n)
+    job.run()
+    want = oracle_job(orc, n, msg, chal)
+    assert job.roots() == want["roots"]
+    # the terminal oracle equals the oracle's terminal codeword
+    assert np.array_equal(job.oracle(job.n_roots - 1), want["terminal"])
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_equals_plain_uneven_sizes(b64, orc, use_graph):
+    rng = np.random.default_rng(4)
+    sizes = [int(x) for x in rng.integers(12, 17, size=12)] + [5, 20]
+    say("batch sizes", sizes, "graph", use_graph)
+    jobs = [make_job(b64, orc, n, i)[0] for i, n in enumerate(sizes)]
+    plain = []
+    for j in jobs:
+        j.run()
+        plain.append(j.roots())
+    # clear the outputs, then run the same jobs batched
+    for j in jobs:
+        j.roots_buf[:] = 0
+    plan = b64.BatchPlan(jobs, use_graph=use_graph)
+    assert plan.n_launches() < 40
+    for rep in range(3):  # replays give identical results
+        for j in jobs:
+            j.roots_buf[:] = 0
+        plan.run()
+        assert [j.roots() for j in jobs] == plain, rep
+
+
+def test_batch_matches_oracle_small(b64, orc):
+    sizes = [12, 14, 12, 15, 13]
+    made = [make_job(b64, orc, n, 50 + i) for i, n in enumerate(sizes)]
+    plan = b64.BatchPlan([m[0] for m in made], use_graph=True)
+    plan.run()
+    for (job, msg, chal), n in zip(made, sizes):
+        assert job.roots() == oracle_job(orc, n, msg, chal)["roots"]
