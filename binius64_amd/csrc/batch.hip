@@ -68,7 +68,7 @@ std::mutex g_size_mu;
 std::map<uint32_t, SizeInfo *> g_sizes;
 
 int size_info(uint32_t n, SizeInfo **out) {
-    if (n < 5 || n > 40) return B64_ERR_INVALID_ARGS;  // at least one NTT layer (log_dim >= 1)
+    if (n < 8 || n > 40) return B64_ERR_INVALID_ARGS;  // batch 2^4 and at least 4 NTT layers
     std::lock_guard<std::mutex> g(g_size_mu);
     auto it = g_sizes.find(n);
     if (it != g_sizes.end()) {
@@ -312,7 +312,7 @@ extern "C" {
 
 size_t b64_job_workspace_size(uint32_t log_msg_len) {
     SizeInfo *si = nullptr;
-    if (log_msg_len < 5 || size_info(log_msg_len, &si) != B64_OK) return 0;
+    if (log_msg_len < 8 || size_info(log_msg_len, &si) != B64_OK) return 0;
     Layout L;
     layout_of(si, L);
     return L.total;
@@ -320,7 +320,7 @@ size_t b64_job_workspace_size(uint32_t log_msg_len) {
 
 uint32_t b64_job_n_roots(uint32_t log_msg_len) {
     SizeInfo *si = nullptr;
-    if (log_msg_len < 5 || size_info(log_msg_len, &si) != B64_OK) return 0;
+    if (log_msg_len < 8 || size_info(log_msg_len, &si) != B64_OK) return 0;
     return si->p.n_fold_arities + 2;
 }
 

@@ -215,7 +215,7 @@ int b64_fri_prover_oracle(const b64_fri_prover *fp, uint32_t k, const void **d_c
  * roots_out receives the codeword root and each folded oracle's root, the same
  * digests the transcript-driven prover writes with these challenges. */
 typedef struct b64_job_desc {
-    uint32_t log_msg_len;       /* 5 .. 40 */
+    uint32_t log_msg_len;       /* 8 .. 40 (batch 2^4) */
     const void *d_msg;          /* device, 2^log_msg_len elements */
     const uint64_t *challenges; /* host, log_msg_len elements (read when the job is run / planned) */
     void *d_workspace;          /* device, b64_job_workspace_size(log_msg_len) bytes */

@@ -36,7 +36,7 @@ def make_job(b64, orc, n, seed):
     return b64.CommitJob(n, msg, chal), msg, chal
 
 
-@pytest.mark.parametrize("n", [5, 8, 12, 13, 16])
+@pytest.mark.parametrize("n", [8, 9, 12, 13, 14, 16, 17])
 def test_plain_job_matches_oracle(b64, orc, n):
     say("plain job", n)
     job, msg, chal = make_job(b64, orc, n, n)
@@ -50,7 +50,7 @@ def test_plain_job_matches_oracle(b64, orc, n):
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_batch_equals_plain_uneven_sizes(b64, orc, use_graph):
     rng = np.random.default_rng(4)
-    sizes = [int(x) for x in rng.integers(12, 17, size=12)] + [5, 20]
+    sizes = [int(x) for x in rng.integers(12, 17, size=12)] + [8, 20]
     say("batch sizes", sizes, "graph", use_graph)
     jobs = [make_job(b64, orc, n, i)[0] for i, n in enumerate(sizes)]
     plain = []
