@@ -176,6 +176,7 @@ int job_stages(const b64_job_desc &jd, const Layout &L, std::vector<Stage> &out,
         st.ntt.layer0 = starts[q];
         st.ntt.k = ks[q];
         st.ntt.c = cs[q];
+        st.ntt.tab_min = b64::g_ntt_tuning.tab_min;
         st.blocks = (uint32_t)(1ull << (log_d - ks[q] - cs[q]));
         out.push_back(st);
     }

@@ -140,6 +140,49 @@ __device__ __forceinline__ uint4 win_word(const uint4 *__restrict__ tab, uint32_
     return acc;
 }
 
+// Same product with the lookup addresses built by byte permutes: the even and
+// odd nibbles of a word, pre-scaled by 16 (the entry size), sit in the bytes of
+// two masked words; v_perm_b32 drops one of those bytes into the low byte of the
+// 256-byte-aligned table address, and the row offset (row * 256) rides in the
+// ds_read immediate.  One VALU op per lookup instead of two.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) const u32x4 lds_cu32x4;
+typedef __attribute__((address_space(3))) const uint4 lds_cuint4;
+
+__device__ __forceinline__ uint4 gxor4(uint4 a, u32x4 b) {
+    return make_uint4(a.x ^ b.x, a.y ^ b.y, a.z ^ b.z, a.w ^ b.w);
+}
+
+__device__ __forceinline__ uint4 win_word_perm(uint32_t base, uint32_t w, int row0, uint4 acc) {
+    const uint32_t lo = (w << 4) & 0xF0F0F0F0u;  // nibbles 0,2,4,6 * 16
+    const uint32_t hi = w & 0xF0F0F0F0u;         // nibbles 1,3,5,7 * 16
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t sel = 0x07060500u | (uint32_t)k;  // byte k of src1, bytes 1..3 of base
+        const uint32_t a0 = __builtin_amdgcn_perm(base, lo, sel);
+        const uint32_t a1 = __builtin_amdgcn_perm(base, hi, sel);
+        const lds_cu32x4 *p0 = (const lds_cu32x4 *)(uintptr_t)a0;
+        const lds_cu32x4 *p1 = (const lds_cu32x4 *)(uintptr_t)a1;
+        acc = gxor4(acc, p0[(row0 + 2 * k) * 16]);
+        acc = gxor4(acc, p1[(row0 + 2 * k + 1) * 16]);
+    }
+    return acc;
+}
+
+// tab must be 256-byte aligned in LDS.
+__device__ __forceinline__ uint32_t lds_addr(const uint4 *p) {
+    return (uint32_t)(uintptr_t)(const lds_cuint4 *)p;
+}
+
+__device__ __forceinline__ uint4 win_mul_perm(uint32_t tab_base, uint4 v) {
+    uint4 acc = make_uint4(0, 0, 0, 0);
+    acc = win_word_perm(tab_base, v.x, 0, acc);
+    acc = win_word_perm(tab_base, v.y, 8, acc);
+    acc = win_word_perm(tab_base, v.z, 16, acc);
+    acc = win_word_perm(tab_base, v.w, 24, acc);
+    return acc;
+}
+
 // v * c using c's window table.
 __device__ __forceinline__ uint4 win_mul(const uint4 *__restrict__ tab, uint4 v) {
     uint4 acc = make_uint4(0, 0, 0, 0);

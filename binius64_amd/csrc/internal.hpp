@@ -41,6 +41,7 @@ struct NttPassArgs {
     // out[r*2^m + i] = msg[rev_m(i)] (reed_solomon.rs:164-187), instead of data.
     const uint4 *msg;
     int log_d, layer0, k, c, m;
+    int tab_min;  // layers whose twiddle covers >= tab_min butterflies use window tables
 };
 struct LeafArgs {
     const uint4 *elems;
@@ -71,9 +72,10 @@ struct FoldCwArgs {
 enum class MultiKind { NttPass, Leaf16Fold, LeafHash, Subtree, FoldCodeword };
 
 struct NttTuning {
-    int variant;   // 0 = general product, 1 = LDS window tables
+    int variant;   // 0 = general product, 1 = per-layer choice (window tables where amortised)
     int max_k;     // layers per pass
     int tile_log;  // log2 elements per LDS tile
+    int tab_min;   // variant 1: butterflies per twiddle (in a tile) needed to build a table
 };
 extern NttTuning g_ntt_tuning;
 

@@ -75,12 +75,12 @@ hipError_t launch_bitrev_replicate(const uint4 *msg, uint4 *out, int m, int lir,
 // ---------------------------------------------------------------------------
 constexpr int NTT_TILE_LOG = 11;
 constexpr int NTT_MAX_K = 7;
-constexpr int NTT_TAB_GROUP = 4;  // window tables resident in LDS at once (8 KiB each)
+constexpr int NTT_TAB_GROUP = 2;  // window tables resident in LDS at once (8 KiB each)
 
 // Runtime knobs (set through b64_config_set): the multiply used by the NTT
 // passes (0 = general product, 1 = LDS window tables per twiddle), the
 // largest number of layers per pass and the tile size.
-NttTuning g_ntt_tuning = {1, NTT_MAX_K, NTT_TILE_LOG};
+NttTuning g_ntt_tuning = {1, NTT_MAX_K, NTT_TILE_LOG, 128};
 
 
 __device__ __forceinline__ uint32_t rev_bits(uint32_t x, int b) { return b ? (__brev(x) >> (32 - b)) : 0u; }
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass(NttPassArgs a) {
 // through an LDS window table.  The 2^j blocks of layer j are contiguous ranges
 // of the butterfly enumeration; NTT_TAB_GROUP tables are built at a time (32
 // lanes per table) and consumed by the butterflies of those blocks.
-struct NttTabSmem {
+struct alignas(256) NttTabSmem {
     uint4 s[1 << NTT_TILE_LOG];
     uint4 tabs[NTT_TAB_GROUP * WIN_ENTRIES];
     uint4 tw[1 << (NTT_MAX_K - 1)];
@@ -205,6 +205,26 @@ __device__ __forceinline__ void ntt_tab_tile(const NttPassArgs &a, uint64_t tile
         const int per_blk = 1 << pos;
         const int lowmask = per_blk - 1;
         const int nblk = 1 << j;
+        if (per_blk < a.tab_min) {
+            // few butterflies per twiddle: a table would not pay for itself
+            for (int P = threadIdx.x; P < (n_el >> 1); P += BLOCK) {
+                const int L0 = ((P >> pos) << (pos + 1)) | (P & lowmask);
+                const int L1 = L0 | per_blk;
+                const uint4 t = tw[P >> pos];
+                uint4 u = s[L0], v = s[L1];
+                if (!INVERSE) {
+                    u = gxor(u, gf_mul(v, t));
+                    v = gxor(v, u);
+                } else {
+                    v = gxor(v, u);
+                    u = gxor(u, gf_mul(v, t));
+                }
+                s[L0] = u;
+                s[L1] = v;
+            }
+            __syncthreads();
+            continue;
+        }
         for (int g0 = 0; g0 < nblk; g0 += NTT_TAB_GROUP) {
             const int ng = nblk - g0 < NTT_TAB_GROUP ? nblk - g0 : NTT_TAB_GROUP;
             if (threadIdx.x < 32 * ng)
@@ -219,11 +239,11 @@ __device__ __forceinline__ void ntt_tab_tile(const NttPassArgs &a, uint64_t tile
                 const uint4 *tab = tabs + (q >> pos) * WIN_ENTRIES;
                 uint4 u = s[L0], v = s[L1];
                 if (!INVERSE) {
-                    u = gxor(u, win_mul(tab, v));
+                    u = gxor(u, win_mul_perm(lds_addr(tab), v));
                     v = gxor(v, u);
                 } else {
                     v = gxor(v, u);
-                    u = gxor(u, win_mul(tab, v));
+                    u = gxor(u, win_mul_perm(lds_addr(tab), v));
                 }
                 s[L0] = u;
                 s[L1] = v;
@@ -283,6 +303,7 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
         a.layer0 = starts[p];
         a.k = ks[p];
         a.c = cs[p];
+        a.tab_min = g_ntt_tuning.tab_min;
         const uint64_t n_tiles = 1ull << (log_d - a.k - a.c);
         if (g_ntt_tuning.variant == 1) {
             if (inverse)

@@ -75,9 +75,10 @@ int b64_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes, void *stream);
 int b64_stream_synchronize(void *stream);
 
 /* Kernel tuning knobs (no counterpart in the reference; the autotuner's
- * interface).  Keys: "ntt.variant" (0 = general multiply, 1 = LDS window
- * tables), "ntt.max_k" (layers per NTT pass, 1..7), "ntt.tile_log" (log2
- * elements per LDS tile).  Returns INVALID_ARGS for an unknown key. */
+ * interface).  Keys: "ntt.variant" (0 = general multiply, 1 = per layer, LDS
+ * window tables where a twiddle covers at least "ntt.tab_min" butterflies of a
+ * tile), "ntt.max_k" (layers per NTT pass, 1..7), "ntt.tile_log" (log2 elements
+ * per LDS tile).  Returns INVALID_ARGS for an unknown key or value. */
 int b64_config_set(const char *key, int64_t value);
 int b64_config_get(const char *key, int64_t *value);
 
