@@ -26,6 +26,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the NTT pass kernel (FETCH_SIZE + WRITE_SIZE from
+# rocprofv3 --pmc, scaled by the same counters on a copy of known size), keyed by
+# (kernel, log codeword length).  Filled from the committed counter summary.
+TRAFFIC = {}
 
 
 def parse():
@@ -36,6 +40,7 @@ def parse():
     ap.add_argument("--log-n", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--cpu-log-n", type=int, default=24, help="CPU baseline sample size")
     return ap.parse_args()
 
@@ -111,8 +116,9 @@ def main():
 
     scratch = cw.clone()
     n_layers = rs_log_len - lir  # layers lir .. log_cw - lbs
-    max_k = 7
-    n_passes = (n_layers + max_k - 1) // max_k
+    max_k = b64.config_get("ntt.max_k")
+    n_passes = (n_layers + max_k - 1) // max_k  # the library's balanced pass split
+    kernel = "k_ntt_pass_tab" if b64.config_get("ntt.variant") == 1 else "k_ntt_pass"
     stages = {
         "encode_ms": timed(lambda: b64.check(L.b64_rs_encode_batch(dom.ptr, log_dim, lir, msg.data_ptr(), lbs,
                                                                    cw.data_ptr(), sp))),
@@ -125,14 +131,17 @@ def main():
     ntt_pass_ms = stages["ntt_ms"] / n_passes
     pass_bytes = 2 * (16 << log_cw)  # each pass reads and writes the codeword once
     achieved = pass_bytes / (ntt_pass_ms * 1e-3) / 1e9
+    muls = n_layers * (1 << (log_cw - 1))  # one multiply per butterfly
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_ntt_pass", "avg_launch_ms": round(ntt_pass_ms, 4),
-                "algorithmic_bytes_per_launch": pass_bytes}
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": TRAFFIC.get((kernel, log_cw)),
+                "kernel": kernel, "launches_per_ntt": n_passes, "avg_launch_ms": round(ntt_pass_ms, 4),
+                "algorithmic_bytes_per_launch": pass_bytes,
+                "gf_mul_per_s": float(f"{muls / (stages['ntt_ms'] * 1e-3):.4g}")}
     step_bytes = 58 * (1 << n)  # algorithmic HBM bytes of the whole step
     path_achieved = step_bytes / (ms * 1e-3) / 1e9
 
     batch = None if args.no_batch else batch_bench(b64, torch, stream, sp)
+    serve = None if args.no_serve else serve_bench(b64, torch, sp)
 
     cpu = None
     if not args.no_cpu:
@@ -161,6 +170,7 @@ def main():
         "wall_s_timed_region": round(wall, 4),
         "cpu_baseline": cpu,
         "batch": batch,
+        "serve": serve,
     }
     print(json.dumps(out), flush=True)
 
@@ -202,6 +212,120 @@ def batch_bench(b64, torch, stream, sp, n_jobs: int = 64, reps: int = 5):
             "plain_jobs_per_s": round(n_jobs / t_plain, 1), "batched_jobs_per_s": round(n_jobs / t_batch, 1),
             "speedup": round(t_plain / t_batch, 3), "plain_ms": round(1e3 * t_plain, 3),
             "batched_ms": round(1e3 * t_batch, 3), "roots_identical": bool(same)}
+
+
+def serve_bench(b64, torch, sp, n_jobs: int = 400, rate: float = 200.0):
+    """Commit requests arriving as a Poisson stream (mean `rate`/s, seed 5) with
+    witness sizes 2^14..2^22 64-bit words, log-uniform (n = 13..21 B128
+    elements after packing two words per element), uniform random contents.
+
+    Served: the continuous-batching server (b64_server_*), messages uploaded from
+    pinned host memory, p50/p99 submit-to-roots latency.  Plain: the same jobs one
+    at a time (upload + b64_job_run), timed back to back; its latency under the
+    same arrivals follows from those service times (FIFO, one job at a time).
+    Saturated throughput: every job submitted at once, vs the plain loop."""
+    import threading
+
+    import numpy as np
+
+    from binius64_amd._abi import JobDescC
+
+    L = b64.lib()
+    rng = np.random.default_rng(5)
+    sizes = [int(w) - 1 for w in rng.integers(14, 23, size=n_jobs)]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_jobs))
+    pool_elems = 1 << 22
+    g = torch.Generator().manual_seed(5)
+    pool_t = torch.randint(-(2 ** 63), 2 ** 63 - 1, (2 * pool_elems,), dtype=torch.int64, generator=g)
+    pool_t = pool_t.pin_memory()
+    pool = pool_t.numpy().view(np.uint64).reshape(-1, 2)
+    offs = [int(rng.integers(0, pool_elems - (1 << n) + 1)) for n in sizes]
+    msgs = [pool[o:o + (1 << n)] for o, n in zip(offs, sizes)]
+    chals = [np.random.default_rng(10_000 + i).integers(0, 2 ** 63, size=(n, 2), dtype=np.uint64)
+             for i, n in enumerate(sizes)]
+    n_roots = [b64.CommitJob.n_roots_for(n) for n in sizes]
+
+    # ---- plain: one job at a time ------------------------------------------
+    nmax = max(sizes)
+    d_msg = b64.DeviceBuffer(16 << nmax)
+    d_ws = b64.DeviceBuffer(b64.CommitJob.workspace_size(nmax))
+    roots_plain = [np.zeros(32 * r, dtype=np.uint8) for r in n_roots]
+    descs = [JobDescC(n, d_msg.ptr, chals[i].ctypes.data_as(C.POINTER(C.c_uint64)), d_ws.ptr,
+                      roots_plain[i].ctypes.data_as(C.POINTER(C.c_uint8))) for i, n in enumerate(sizes)]
+
+    def plain_one(i):
+        b64.check(L.b64_memcpy_htod(d_msg.ptr, msgs[i].ctypes.data, msgs[i].nbytes, sp), "htod")
+        b64.check(L.b64_job_run(C.byref(descs[i]), sp), "job_run")
+
+    for n in sorted(set(sizes)):  # warm every size (domain and plan tables)
+        plain_one(sizes.index(n))
+    service = np.zeros(n_jobs)
+    t_all = time.perf_counter()
+    for i in range(n_jobs):
+        t = time.perf_counter()
+        plain_one(i)
+        service[i] = time.perf_counter() - t
+    t_plain = time.perf_counter() - t_all
+    done, lat_plain = 0.0, np.zeros(n_jobs)
+    for i in range(n_jobs):
+        done = max(done, arrivals[i]) + service[i]
+        lat_plain[i] = done - arrivals[i]
+    want = [r.tobytes() for r in roots_plain]
+
+    def joined(rs):
+        return b"".join(rs)
+
+    # ---- served: Poisson arrivals --------------------------------------------
+    srv = b64.Server(hbm_budget_bytes=8 << 30, max_batch=64, n_streams=3)
+    try:
+        tickets = [0] * n_jobs
+
+        def submitter():
+            t0 = time.perf_counter()
+            for i in range(n_jobs):
+                dt = t0 + arrivals[i] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                tickets[i] = srv.submit(sizes[i], msgs[i], chals[i])
+                ready[i].set()
+
+        ready = [threading.Event() for _ in range(n_jobs)]
+        t0 = time.perf_counter()
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat, same = np.zeros(n_jobs), True
+        for i in range(n_jobs):
+            ready[i].wait()
+            roots, lat[i] = srv.wait(tickets[i], n_roots[i])
+            same = same and joined(roots) == want[i]
+        t_poisson = time.perf_counter() - t0
+        th.join()
+        st_poisson = srv.stats()
+
+        # ---- served: saturated (all jobs queued at once) ---------------------
+        t0 = time.perf_counter()
+        tk = [srv.submit(sizes[i], msgs[i], chals[i]) for i in range(n_jobs)]
+        for i in range(n_jobs):
+            roots, _ = srv.wait(tk[i], n_roots[i])
+            same = same and joined(roots) == want[i]
+        t_sat = time.perf_counter() - t0
+        st = srv.stats()
+    finally:
+        srv.close()
+
+    def pct(a, q):
+        return round(1e3 * float(np.percentile(a, q)), 3)
+
+    return {"jobs": n_jobs, "arrival_rate_per_s": rate, "seed": 5, "log_n_range": [13, 21],
+            "hbm_budget_bytes": 8 << 30, "max_batch": 64, "n_streams": 3,
+            "served_p50_ms": pct(lat, 50), "served_p99_ms": pct(lat, 99),
+            "served_jobs_per_s_at_rate": round(n_jobs / t_poisson, 1),
+            "plain_p50_ms": pct(lat_plain, 50), "plain_p99_ms": pct(lat_plain, 99),
+            "saturated_served_jobs_per_s": round(n_jobs / t_sat, 1),
+            "plain_jobs_per_s": round(n_jobs / t_plain, 1),
+            "saturated_speedup": round(t_plain / t_sat, 3),
+            "batches_at_rate": st_poisson["batches"], "batches_total": st["batches"],
+            "peak_pool_bytes": st["peak_pool_bytes"], "roots_identical": bool(same)}
 
 
 def cpu_baseline(log_n: int):

@@ -28,6 +28,7 @@ __all__ = [
     "BinaryMerkleTreeProver", "FRIParams", "CommitOutput", "commit_interleaved", "setup_commit_params",
     "ProverTranscript", "FRIFoldProver", "gf_mul", "gf_inv", "gf_mul_device", "gf_mul_const_device",
     "pack_witness", "device_count", "config_set", "config_get", "LIB_PATH", "lib",
+    "CommitJob", "BatchPlan", "Server", "fri_prove",
 ]
 
 
@@ -471,6 +472,10 @@ class CommitJob:
     def workspace_size(log_msg_len: int) -> int:
         return lib().b64_job_workspace_size(log_msg_len)
 
+    @staticmethod
+    def n_roots_for(log_msg_len: int) -> int:
+        return lib().b64_job_n_roots(log_msg_len)
+
     def run(self, stream=None):
         check(lib().b64_job_run(C.byref(self.desc), _stream(stream)), "job_run")
 
@@ -509,6 +514,63 @@ class BatchPlan:
         if getattr(self, "ptr", None):
             lib().b64_batch_plan_destroy(self.ptr)
             self.ptr = None
+
+
+class Server:
+    """Continuous-batching commit server (b64_server_*): jobs are admitted by HBM
+    footprint and batched onto up to n_streams streams."""
+
+    def __init__(self, hbm_budget_bytes: int, max_batch: int = 64, n_streams: int = 2):
+        from ._abi import ServerConfigC
+        cfg = ServerConfigC(hbm_budget_bytes, max_batch, n_streams)
+        out = C.c_void_p()
+        check(lib().b64_server_create(C.byref(cfg), C.byref(out)), "server_create")
+        self.ptr = out.value
+        self._held = {}
+
+    def submit(self, log_msg_len: int, message: np.ndarray, challenges: np.ndarray) -> int:
+        """Queue a job; the message array is held until poll/wait reports it done
+        (the server reads it when the job is admitted)."""
+        m = np.ascontiguousarray(_elems(message))
+        ch = np.ascontiguousarray(_elems(challenges))
+        assert m.shape[0] == 1 << log_msg_len and ch.shape[0] >= log_msg_len
+        t = C.c_uint64()
+        check(lib().b64_server_submit(self.ptr, log_msg_len, _p64(m), _p64(ch), C.byref(t)), "server_submit")
+        self._held[t.value] = m
+        return t.value
+
+    def poll(self, ticket: int, n_roots: int):
+        roots = np.zeros(32 * n_roots, dtype=np.uint8)
+        done, lat = C.c_int(0), C.c_double(0)
+        check(lib().b64_server_poll(self.ptr, ticket, _p8(roots), C.byref(done), C.byref(lat)), "server_poll")
+        if not done.value:
+            return None
+        self._held.pop(ticket, None)
+        b = roots.tobytes()
+        return [b[32 * i:32 * i + 32] for i in range(n_roots)], lat.value
+
+    def wait(self, ticket: int, n_roots: int):
+        roots = np.zeros(32 * n_roots, dtype=np.uint8)
+        lat = C.c_double(0)
+        check(lib().b64_server_wait(self.ptr, ticket, _p8(roots), C.byref(lat)), "server_wait")
+        self._held.pop(ticket, None)
+        b = roots.tobytes()
+        return [b[32 * i:32 * i + 32] for i in range(n_roots)], lat.value
+
+    def stats(self):
+        d, b, p = C.c_uint64(), C.c_uint64(), C.c_size_t()
+        check(lib().b64_server_stats(self.ptr, C.byref(d), C.byref(b), C.byref(p)), "server_stats")
+        return {"jobs_done": d.value, "batches": b.value, "peak_pool_bytes": p.value}
+
+    def close(self):
+        # destroy drains every queued and running job before it returns
+        if getattr(self, "ptr", None):
+            lib().b64_server_destroy(self.ptr)
+            self.ptr = None
+        self._held = {}
+
+    def __del__(self):
+        self.close()
 
 
 def fri_prove(params: FRIParams, domain: DomainContext, message: np.ndarray, challenges=None):

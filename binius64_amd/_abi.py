@@ -31,6 +31,14 @@ class FriParamsC(C.Structure):
     ]
 
 
+class ServerConfigC(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_size_t),
+        ("max_batch", C.c_uint32),
+        ("n_streams", C.c_uint32),
+    ]
+
+
 class JobDescC(C.Structure):
     _fields_ = [
         ("log_msg_len", C.c_uint32),
@@ -49,6 +57,14 @@ SIGNATURES = {
     "b64_job_run": (C.c_int, [C.POINTER(JobDescC), vp]),
     "b64_batch_plan_create": (C.c_int, [C.POINTER(JobDescC), C.c_uint32, C.c_int, vp, C.POINTER(vp)]),
     "b64_batch_plan_run": (C.c_int, [vp, vp]),
+    "b64_batch_plan_enqueue": (C.c_int, [vp, vp]),
+    "b64_batch_plan_collect": (C.c_int, [vp]),
+    "b64_server_create": (C.c_int, [C.POINTER(ServerConfigC), C.POINTER(vp)]),
+    "b64_server_submit": (C.c_int, [vp, C.c_uint32, u64p, u64p, C.POINTER(C.c_uint64)]),
+    "b64_server_poll": (C.c_int, [vp, C.c_uint64, u8p, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "b64_server_wait": (C.c_int, [vp, C.c_uint64, u8p, C.POINTER(C.c_double)]),
+    "b64_server_stats": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_size_t)]),
+    "b64_server_destroy": (None, [vp]),
     "b64_batch_plan_n_launches": (C.c_uint32, [vp]),
     "b64_batch_plan_destroy": (None, [vp]),
     "b64_version": (C.c_char_p, []),

@@ -288,13 +288,16 @@ struct b64_batch_plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool use_graph = false;
+    bool owns_buffers = true;
 
     ~b64_batch_plan() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        if (d_tables) (void)hipFree(d_tables);
-        if (d_roots) (void)hipFree(d_roots);
-        if (h_roots) (void)hipHostFree(h_roots);
+        if (owns_buffers) {
+            if (d_tables) (void)hipFree(d_tables);
+            if (d_roots) (void)hipFree(d_roots);
+            if (h_roots) (void)hipHostFree(h_roots);
+        }
     }
 
     int enqueue(hipStream_t s) const {
@@ -389,6 +392,13 @@ int b64_job_run(const b64_job_desc *job, void *stream) {
 
 int b64_batch_plan_create(const b64_job_desc *jobs, uint32_t n_jobs, int use_graph, void *stream,
                           b64_batch_plan **out) {
+    return b64_batch_plan_build(jobs, n_jobs, use_graph, nullptr, as_stream(stream), out);
+}
+
+}  // extern "C"
+
+int b64_batch_plan_build(const b64_job_desc *jobs, uint32_t n_jobs, int use_graph, b64::BatchStorage *st,
+                         hipStream_t upload_stream, b64_batch_plan **out) {
     if (!jobs || !n_jobs || !out) return B64_ERR_INVALID_ARGS;
     b64_batch_plan *plan = new (std::nothrow) b64_batch_plan();
     if (!plan) return B64_ERR_OUT_OF_MEMORY;
@@ -453,11 +463,23 @@ int b64_batch_plan_create(const b64_job_desc *jobs, uint32_t n_jobs, int use_gra
         plan->launches.push_back(l);
     }
     const size_t root_src_off = put(root_src.data(), root_src.size() * sizeof(void *));
-    HIP_TRY(hipMalloc(&plan->d_tables, host.size() ? host.size() : 16));
-    HIP_TRY(hipMemcpy(plan->d_tables, host.data(), host.size(), hipMemcpyHostToDevice));
+    if (st) {
+        // caller-owned buffers, uploaded in stream order (no allocation, no device sync)
+        if (host.size() > st->tables_cap || (size_t)plan->n_roots > st->roots_cap) return B64_ERR_BUFFER_TOO_SMALL;
+        plan->owns_buffers = false;
+        plan->d_tables = st->d_tables;
+        plan->d_roots = st->d_roots;
+        plan->h_roots = st->h_roots;
+        memcpy(st->h_stage, host.data(), host.size());
+        HIP_TRY(hipMemcpyAsync(plan->d_tables, st->h_stage, host.size(), hipMemcpyHostToDevice, upload_stream));
+    } else {
+        HIP_TRY(hipMalloc(&plan->d_tables, host.size() ? host.size() : 16));
+        HIP_TRY(hipMemcpy(plan->d_tables, host.data(), host.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMalloc(&plan->d_roots, 32ull * (plan->n_roots ? plan->n_roots : 1)));
+        HIP_TRY(hipHostMalloc((void **)&plan->h_roots, 32ull * (plan->n_roots ? plan->n_roots : 1),
+                              hipHostMallocDefault));
+    }
     plan->d_root_src = (const uint4 **)(plan->d_tables + root_src_off);
-    HIP_TRY(hipMalloc(&plan->d_roots, 32ull * (plan->n_roots ? plan->n_roots : 1)));
-    HIP_TRY(hipHostMalloc((void **)&plan->h_roots, 32ull * (plan->n_roots ? plan->n_roots : 1), hipHostMallocDefault));
     if (use_graph) {
         hipStream_t cap = nullptr;
         HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
@@ -475,28 +497,38 @@ int b64_batch_plan_create(const b64_job_desc *jobs, uint32_t n_jobs, int use_gra
         B64_TRY(rc);
         plan->use_graph = true;
     }
-    (void)stream;
     *out = guard.release();
     return B64_OK;
 }
 
-int b64_batch_plan_run(b64_batch_plan *plan, void *stream) {
+extern "C" {
+
+int b64_batch_plan_enqueue(b64_batch_plan *plan, void *stream) {
     if (!plan) return B64_ERR_INVALID_ARGS;
     hipStream_t s = as_stream(stream);
     if (plan->use_graph)
         HIP_TRY(hipGraphLaunch(plan->exec, s));
     else
         B64_TRY(plan->enqueue(s));
-    if (plan->n_roots) {
+    if (plan->n_roots)
         HIP_TRY(hipMemcpyAsync(plan->h_roots, plan->d_roots, 32ull * plan->n_roots, hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        size_t at = 0;
-        for (size_t j = 0; j < plan->jobs.size(); j++) {
-            memcpy(plan->jobs[j].roots_out, plan->h_roots + 32 * at, 32ull * plan->roots_per_job[j]);
-            at += plan->roots_per_job[j];
-        }
+    return B64_OK;
+}
+
+int b64_batch_plan_collect(b64_batch_plan *plan) {
+    if (!plan) return B64_ERR_INVALID_ARGS;
+    size_t at = 0;
+    for (size_t j = 0; j < plan->jobs.size(); j++) {
+        memcpy(plan->jobs[j].roots_out, plan->h_roots + 32 * at, 32ull * plan->roots_per_job[j]);
+        at += plan->roots_per_job[j];
     }
     return B64_OK;
+}
+
+int b64_batch_plan_run(b64_batch_plan *plan, void *stream) {
+    B64_TRY(b64_batch_plan_enqueue(plan, stream));
+    HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    return b64_batch_plan_collect(plan);
 }
 
 uint32_t b64_batch_plan_n_launches(const b64_batch_plan *plan) { return plan ? (uint32_t)plan->launches.size() : 0; }

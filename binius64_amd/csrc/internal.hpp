@@ -107,7 +107,23 @@ hipError_t launch_multi(MultiKind kind, const void *d_jobs, const uint32_t *d_st
 hipError_t launch_pack_witness(const uint64_t *words, uint64_t n_words, uint4 *out, uint64_t n_elems,
                                hipStream_t s);
 
+// Caller-owned buffers for a batch plan (the serving loop reuses one per stream
+// slot, so building a plan neither allocates nor synchronizes the device).
+struct BatchStorage {
+    uint8_t *d_tables;  // device, tables_cap bytes
+    uint8_t *h_stage;   // pinned host staging for the table upload, tables_cap bytes
+    size_t tables_cap;
+    uint4 *d_roots;     // device, roots_cap digests
+    uint8_t *h_roots;   // pinned host, roots_cap digests
+    size_t roots_cap;
+};
+
 }  // namespace b64
+
+// Batch plan over caller storage (st may be null: the plan allocates its own);
+// tables are uploaded on upload_stream.
+int b64_batch_plan_build(const b64_job_desc *jobs, uint32_t n_jobs, int use_graph, b64::BatchStorage *st,
+                         hipStream_t upload_stream, b64_batch_plan **out);
 
 // Device view of a domain (uploads its twiddle basis on first use).
 int b64_domain_device_view_internal(b64_domain *d, b64::DeviceDomain *dd);

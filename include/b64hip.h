@@ -234,9 +234,41 @@ int b64_job_run(const b64_job_desc *job, void *stream);
  * use_graph the launch sequence is captured once in a hipGraph and replayed. */
 int b64_batch_plan_create(const b64_job_desc *jobs, uint32_t n_jobs, int use_graph, void *stream,
                           b64_batch_plan **out);
+/* run = enqueue (kernels + roots copied to pinned memory, asynchronous) +
+ * stream synchronize + collect (roots delivered to each job's roots_out). */
 int b64_batch_plan_run(b64_batch_plan *plan, void *stream);
+int b64_batch_plan_enqueue(b64_batch_plan *plan, void *stream);
+int b64_batch_plan_collect(b64_batch_plan *plan);
 uint32_t b64_batch_plan_n_launches(const b64_batch_plan *plan);
 void b64_batch_plan_destroy(b64_batch_plan *plan);
+
+/* ---- serving: continuous batching of commit jobs (no counterpart in the reference) */
+/* A worker thread admits submitted jobs in arrival order whenever their HBM
+ * footprint (message + job workspace) fits in the server's budget, groups the
+ * admitted jobs into batches of up to max_batch, and keeps up to n_streams
+ * batches in flight on separate streams, so jobs that arrive while others run
+ * are admitted as memory and stream slots free up.  Results equal b64_job_run's. */
+typedef struct b64_server b64_server;
+typedef struct b64_server_config {
+    size_t hbm_budget_bytes; /* device memory pool for admitted jobs */
+    uint32_t max_batch;      /* jobs per batch launch */
+    uint32_t n_streams;      /* batches in flight */
+} b64_server_config;
+int b64_server_create(const b64_server_config *cfg, b64_server **out);
+/* Queues a job and returns a ticket.  The host message (2^log_msg_len B128
+ * elements; pinned memory makes its upload asynchronous) is read when the job is
+ * admitted and must stay valid until poll/wait reports the job done; the
+ * challenges are copied.  ADMISSION_REJECTED if the job alone exceeds the budget. */
+int b64_server_submit(b64_server *srv, uint32_t log_msg_len, const uint64_t *msg, const uint64_t *challenges,
+                      uint64_t *ticket);
+/* *done = 1 and roots (b64_job_n_roots * 32 bytes) written once the job has
+ * finished; latency_s = submit-to-completion time measured by the server. */
+int b64_server_poll(b64_server *srv, uint64_t ticket, uint8_t *roots_out, int *done, double *latency_s);
+/* Blocks until the job has finished. */
+int b64_server_wait(b64_server *srv, uint64_t ticket, uint8_t *roots_out, double *latency_s);
+/* jobs completed, batches launched, peak bytes of the pool in use */
+int b64_server_stats(b64_server *srv, uint64_t *jobs_done, uint64_t *batches, size_t *peak_bytes);
+void b64_server_destroy(b64_server *srv);
 
 /* ---- witness packing (prover/prover/src/prove.rs:286-327) --------------------- */
 /* B128 element i = word[2i+1] << 64 | word[2i]; zero-padded to 2^log_elems. */

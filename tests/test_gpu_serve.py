@@ -1,0 +1,96 @@
+"""GPU tests of the serving loop: every result equals the job run alone, in any
+mix, under a tight HBM budget (jobs wait for admission) and with several streams."""
+import threading
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def b64():
+    import binius64_amd as b
+    if b.device_count() == 0:
+        pytest.fail("no HIP device visible")
+    return b
+
+
+def job_inputs(orc, n, seed):
+    return orc.random_elems(5000 + seed, 1 << n), orc.random_elems(6000 + seed, n)
+
+
+def alone(b64, n, msg, chal):
+    job = b64.CommitJob(n, msg, chal)
+    job.run()
+    return job.roots()
+
+
+def test_mixed_submissions_equal_alone(b64, orc):
+    rng = np.random.default_rng(5)
+    sizes = [int(x) for x in rng.integers(9, 17, size=40)]
+    inputs = [job_inputs(orc, n, i) for i, n in enumerate(sizes)]
+    want = [alone(b64, n, m, c) for n, (m, c) in zip(sizes, inputs)]
+    srv = b64.Server(hbm_budget_bytes=256 << 20, max_batch=16, n_streams=3)
+    try:
+        tickets = [srv.submit(n, m, c) for n, (m, c) in zip(sizes, inputs)]
+        got = [srv.wait(t, b64.CommitJob.n_roots_for(n))[0] for t, n in zip(tickets, sizes)]
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert got == want
+    assert st["jobs_done"] == len(sizes) and st["batches"] >= 1
+
+
+def test_tight_budget_admits_as_memory_frees(b64, orc):
+    sizes = [16, 14, 16, 12, 15, 16, 13]
+    ws = max(b64.CommitJob.workspace_size(n) + (16 << n) for n in sizes)
+    srv = b64.Server(hbm_budget_bytes=int(ws * 1.5) + (1 << 20), max_batch=8, n_streams=2)
+    inputs = [job_inputs(orc, n, 100 + i) for i, n in enumerate(sizes)]
+    try:
+        tickets = [srv.submit(n, m, c) for n, (m, c) in zip(sizes, inputs)]
+        got = [srv.wait(t, b64.CommitJob.n_roots_for(n))[0] for t, n in zip(tickets, sizes)]
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert got == [alone(b64, n, m, c) for n, (m, c) in zip(sizes, inputs)]
+    assert st["peak_pool_bytes"] <= int(ws * 1.5) + (1 << 20)
+    assert st["batches"] >= 3  # two 2^16 jobs never fit together
+
+
+def test_oversized_job_rejected(b64, orc):
+    srv = b64.Server(hbm_budget_bytes=1 << 20, max_batch=4, n_streams=1)
+    try:
+        m, c = job_inputs(orc, 16, 1)
+        with pytest.raises(b64.B64Error) as e:
+            srv.submit(16, m, c)
+        assert e.value.code == 23
+        # a small job still runs
+        m, c = job_inputs(orc, 9, 2)
+        t = srv.submit(9, m, c)
+        assert srv.wait(t, b64.CommitJob.n_roots_for(9))[0] == alone(b64, 9, m, c)
+    finally:
+        srv.close()
+
+
+def test_concurrent_submitters(b64, orc):
+    sizes = [10, 12, 14, 11, 13, 12, 10, 15]
+    inputs = [job_inputs(orc, n, 200 + i) for i, n in enumerate(sizes)]
+    srv = b64.Server(hbm_budget_bytes=128 << 20, max_batch=4, n_streams=2)
+    results = {}
+
+    def client(i):
+        n = sizes[i]
+        t = srv.submit(n, *inputs[i])
+        results[i] = srv.wait(t, b64.CommitJob.n_roots_for(n))[0]
+
+    try:
+        th = [threading.Thread(target=client, args=(i,)) for i in range(len(sizes))]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+    finally:
+        srv.close()
+    for i, n in enumerate(sizes):
+        assert results[i] == alone(b64, n, *inputs[i])
