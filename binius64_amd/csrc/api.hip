@@ -191,6 +191,9 @@ int b64_config_set(const char *key, int64_t value) {
     } else if (!strcmp(key, "ntt.tab_min")) {
         if (value < 1 || value > (1 << 30)) return B64_ERR_INVALID_ARGS;
         b64::g_ntt_tuning.tab_min = (int)value;
+    } else if (!strcmp(key, "ntt.tab_group")) {
+        if (value != 2 && value != 4 && value != 8) return B64_ERR_INVALID_ARGS;
+        b64::g_ntt_tuning.tab_group = (int)value;
     } else {
         return B64_ERR_INVALID_ARGS;
     }
@@ -203,6 +206,7 @@ int b64_config_get(const char *key, int64_t *value) {
     else if (!strcmp(key, "ntt.max_k")) *value = b64::g_ntt_tuning.max_k;
     else if (!strcmp(key, "ntt.tile_log")) *value = b64::g_ntt_tuning.tile_log;
     else if (!strcmp(key, "ntt.tab_min")) *value = b64::g_ntt_tuning.tab_min;
+    else if (!strcmp(key, "ntt.tab_group")) *value = b64::g_ntt_tuning.tab_group;
     else return B64_ERR_INVALID_ARGS;
     return B64_OK;
 }

@@ -25,19 +25,43 @@ __device__ __forceinline__ uint4 gxor(uint4 a, uint4 b) {
 }
 __device__ __forceinline__ bool gzero(uint4 a) { return (a.x | a.y | a.z | a.w) == 0u; }
 
+// Three-input bitwise ops of gfx950 (v_bitop3_b32).  The immediate is the
+// truth table f(0xF0, 0xCC, 0xAA) of the operands in order.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+// m ? a : b, bit by bit
+__device__ __forceinline__ uint32_t bsel(uint32_t a, uint32_t b, uint32_t m) {
+    return __builtin_amdgcn_bitop3_b32(a, b, m, 0xE4);
+}
+__device__ __forceinline__ uint64_t xor3_64(uint64_t a, uint64_t b, uint64_t c) {
+    const uint32_t lo = xor3((uint32_t)a, (uint32_t)b, (uint32_t)c);
+    const uint32_t hi = xor3((uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(c >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t xor4_64(uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
+    const uint32_t lo = xor3((uint32_t)a, (uint32_t)b, (uint32_t)c) ^ (uint32_t)d;
+    const uint32_t hi = xor3((uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(c >> 32)) ^ (uint32_t)(d >> 32);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // 32x32 -> 64 carry-less product: operands split into four masks with 3-bit
 // holes; each integer product sums at most 8 one-bit terms per kept position,
-// so carries stay inside the holes and are masked away.
+// so carries stay inside the holes.  The four partial sums z_t are exact at the
+// positions = t (mod 4) and are merged with two levels of bit selects.
 __device__ __forceinline__ uint64_t clmul32(uint32_t a, uint32_t b) {
     const uint32_t m0 = 0x11111111u, m1 = 0x22222222u, m2 = 0x44444444u, m3 = 0x88888888u;
     const uint32_t a0 = a & m0, a1 = a & m1, a2 = a & m2, a3 = a & m3;
     const uint32_t b0 = b & m0, b1 = b & m1, b2 = b & m2, b3 = b & m3;
-    uint64_t z0 = ((uint64_t)a0 * b0) ^ ((uint64_t)a1 * b3) ^ ((uint64_t)a2 * b2) ^ ((uint64_t)a3 * b1);
-    uint64_t z1 = ((uint64_t)a0 * b1) ^ ((uint64_t)a1 * b0) ^ ((uint64_t)a2 * b3) ^ ((uint64_t)a3 * b2);
-    uint64_t z2 = ((uint64_t)a0 * b2) ^ ((uint64_t)a1 * b1) ^ ((uint64_t)a2 * b0) ^ ((uint64_t)a3 * b3);
-    uint64_t z3 = ((uint64_t)a0 * b3) ^ ((uint64_t)a1 * b2) ^ ((uint64_t)a2 * b1) ^ ((uint64_t)a3 * b0);
-    return (z0 & 0x1111111111111111ull) | (z1 & 0x2222222222222222ull) |
-           (z2 & 0x4444444444444444ull) | (z3 & 0x8888888888888888ull);
+    const uint64_t z0 = xor4_64((uint64_t)a0 * b0, (uint64_t)a1 * b3, (uint64_t)a2 * b2, (uint64_t)a3 * b1);
+    const uint64_t z1 = xor4_64((uint64_t)a0 * b1, (uint64_t)a1 * b0, (uint64_t)a2 * b3, (uint64_t)a3 * b2);
+    const uint64_t z2 = xor4_64((uint64_t)a0 * b2, (uint64_t)a1 * b1, (uint64_t)a2 * b0, (uint64_t)a3 * b3);
+    const uint64_t z3 = xor4_64((uint64_t)a0 * b3, (uint64_t)a1 * b2, (uint64_t)a2 * b1, (uint64_t)a3 * b0);
+    // positions 0,1 (mod 4) from z0/z1, positions 2,3 from z2/z3
+    const uint32_t lo = bsel(bsel((uint32_t)z0, (uint32_t)z1, m0), bsel((uint32_t)z2, (uint32_t)z3, m2), m0 | m1);
+    const uint32_t hi = bsel(bsel((uint32_t)(z0 >> 32), (uint32_t)(z1 >> 32), m0),
+                             bsel((uint32_t)(z2 >> 32), (uint32_t)(z3 >> 32), m2), m0 | m1);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // 64x64 -> 128 carry-less product (one Karatsuba level over clmul32).
@@ -46,7 +70,7 @@ __device__ __forceinline__ void clmul64(uint64_t a, uint64_t b, uint64_t &lo, ui
     const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
     const uint64_t p0 = clmul32(a0, b0);
     const uint64_t p2 = clmul32(a1, b1);
-    const uint64_t p1 = clmul32(a0 ^ a1, b0 ^ b1) ^ p0 ^ p2;
+    const uint64_t p1 = xor3_64(clmul32(a0 ^ a1, b0 ^ b1), p0, p2);
     lo = p0 ^ (p1 << 32);
     hi = p2 ^ (p1 >> 32);
 }
@@ -54,10 +78,10 @@ __device__ __forceinline__ void clmul64(uint64_t a, uint64_t b, uint64_t &lo, ui
 // Reduce the 256-bit product (v0..v3, 64-bit words, v0 lowest) modulo
 // x^128 + x^7 + x^2 + x + 1.
 __device__ __forceinline__ uint4 gf_reduce(uint64_t v0, uint64_t v1, uint64_t v2, uint64_t v3) {
-    v1 ^= v3 ^ (v3 << 1) ^ (v3 << 2) ^ (v3 << 7);
-    v2 ^= (v3 >> 63) ^ (v3 >> 62) ^ (v3 >> 57);
-    v0 ^= v2 ^ (v2 << 1) ^ (v2 << 2) ^ (v2 << 7);
-    v1 ^= (v2 >> 63) ^ (v2 >> 62) ^ (v2 >> 57);
+    v1 = xor3_64(xor3_64(v1, v3, v3 << 1), v3 << 2, v3 << 7);
+    v2 = xor4_64(v2, v3 >> 63, v3 >> 62, v3 >> 57);
+    v0 = xor3_64(xor3_64(v0, v2, v2 << 1), v2 << 2, v2 << 7);
+    v1 = xor4_64(v1, v2 >> 63, v2 >> 62, v2 >> 57);
     return make_uint4((uint32_t)v0, (uint32_t)(v0 >> 32), (uint32_t)v1, (uint32_t)(v1 >> 32));
 }
 
@@ -68,9 +92,8 @@ __device__ __forceinline__ uint4 gf_mul(uint4 a, uint4 b) {
     clmul64(a0, b0, l0, h0);
     clmul64(a1, b1, l2, h2);
     clmul64(a0 ^ a1, b0 ^ b1, l1, h1);
-    l1 ^= l0 ^ l2;
-    h1 ^= h0 ^ h2;
-    return gf_reduce(l0, h0 ^ l1, l2 ^ h1, h2);
+    // middle term l1 ^ l0 ^ l2 lands on words 1 and 2
+    return gf_reduce(l0, xor4_64(h0, l1, l0, l2), xor4_64(l2, h1, h0, h2), h2);
 }
 
 // v * x (ghash.rs:69-80)
@@ -80,58 +103,62 @@ __device__ __forceinline__ uint4 gf_mul_x(uint4 v) {
                       (v.w << 1) | (v.z >> 31));
 }
 
-// v * x^s for 0 <= s < 128 by shifting and folding the overflow through
-// x^128 = x^7 + x^2 + x + 1 twice.
-__device__ inline uint4 gf_mul_xpow(uint4 v, int s) {
-    unsigned __int128 t = ((unsigned __int128)(((uint64_t)v.w << 32) | v.z) << 64) |
-                          (((uint64_t)v.y << 32) | v.x);
-    if (s == 0) return v;
-    unsigned __int128 lo = t << s;
-    unsigned __int128 of = t >> (128 - s);  // < 2^s
-    // of * (1 + x + x^2 + x^7): low part and the part that overflows again
-    unsigned __int128 r = of ^ (of << 1) ^ (of << 2) ^ (of << 7);
-    unsigned __int128 of2 = (of >> 127) ^ (of >> 126) ^ (of >> 121);  // bits shifted past 2^128
-    r ^= of2 ^ (of2 << 1) ^ (of2 << 2) ^ (of2 << 7);
-    unsigned __int128 out = lo ^ r;
-    const uint64_t o0 = (uint64_t)out, o1 = (uint64_t)(out >> 64);
-    return make_uint4((uint32_t)o0, (uint32_t)(o0 >> 32), (uint32_t)o1, (uint32_t)(o1 >> 32));
-}
-
 // ---- 4-bit window tables ---------------------------------------------------
 constexpr int WIN_ROWS = 32;
 constexpr int WIN_ENTRIES = WIN_ROWS * 16;  // uint4 entries per table (8 KiB)
 
+// Entry n of row r is stored at slot n ^ (r & 15) of the row.  Row builders
+// (one lane per row) then write 16 distinct 16-byte slots across any 16
+// consecutive rows at each step, so the stores are free of LDS bank conflicts;
+// lookups fold the same XOR into the nibble (win_swz).
+__device__ __forceinline__ constexpr uint32_t win_swz(int row0) {
+    // nibble j of a word whose rows are row0 .. row0+7 (row0 a multiple of 8)
+    return (row0 & 8) ? 0xFEDCBA98u : 0x76543210u;
+}
+
+// v * x^(4r) for 0 <= r < 32: the 256-bit shift, then the general reduction.
+__device__ __forceinline__ uint4 gf_mul_xpow4(uint4 v, uint32_t r) {
+    const uint64_t lo = ((uint64_t)v.y << 32) | v.x, hi = ((uint64_t)v.w << 32) | v.z;
+    const uint32_t s = 4 * r, t = s & 63;
+    const bool big = s >= 64;
+    const uint64_t a0 = big ? 0 : lo, a1 = big ? lo : hi, a2 = big ? hi : 0;
+    const uint64_t w0 = a0 << t;
+    const uint64_t w1 = (a1 << t) | (t ? a0 >> (64 - t) : 0);
+    const uint64_t w2 = (a2 << t) | (t ? a1 >> (64 - t) : 0);
+    const uint64_t w3 = t ? a2 >> (64 - t) : 0;
+    return gf_reduce(w0, w1, w2, w3);
+}
+
 // Builds row `row` (0..31) of the window table of constant c: entries
 // c * n * x^(4*row) for n = 0..15.  32 lanes build a whole table.
 __device__ inline void win_build_row(uint4 *__restrict__ tab, uint4 c, int row) {
-    const uint4 p0 = gf_mul_xpow(c, 4 * row);
+    const uint4 p0 = gf_mul_xpow4(c, (uint32_t)row);
     const uint4 p1 = gf_mul_x(p0);
     const uint4 p2 = gf_mul_x(p1);
     const uint4 p3 = gf_mul_x(p2);
     uint4 *r = tab + row * 16;
-    const uint4 z = make_uint4(0, 0, 0, 0);
-    r[0] = z;
-    r[1] = p0;
-    r[2] = p1;
-    const uint4 p01 = gxor(p0, p1);
-    r[3] = p01;
-    r[4] = p2;
-    r[5] = gxor(p2, p0);
-    r[6] = gxor(p2, p1);
-    r[7] = gxor(p2, p01);
-    const uint4 p3_01 = gxor(p3, p01);
-    r[8] = p3;
-    r[9] = gxor(p3, p0);
-    r[10] = gxor(p3, p1);
-    r[11] = p3_01;
-    const uint4 p32 = gxor(p3, p2);
-    r[12] = p32;
-    r[13] = gxor(p32, p0);
-    r[14] = gxor(p32, p1);
-    r[15] = gxor(p32, p01);
+    const int sw = row & 15;
+    const uint4 p01 = gxor(p0, p1), p32 = gxor(p3, p2);
+    r[0 ^ sw] = make_uint4(0, 0, 0, 0);
+    r[1 ^ sw] = p0;
+    r[2 ^ sw] = p1;
+    r[3 ^ sw] = p01;
+    r[4 ^ sw] = p2;
+    r[5 ^ sw] = gxor(p2, p0);
+    r[6 ^ sw] = gxor(p2, p1);
+    r[7 ^ sw] = gxor(p2, p01);
+    r[8 ^ sw] = p3;
+    r[9 ^ sw] = gxor(p3, p0);
+    r[10 ^ sw] = gxor(p3, p1);
+    r[11 ^ sw] = gxor(p3, p01);
+    r[12 ^ sw] = p32;
+    r[13 ^ sw] = gxor(p32, p0);
+    r[14 ^ sw] = gxor(p32, p1);
+    r[15 ^ sw] = gxor(p32, p01);
 }
 
 __device__ __forceinline__ uint4 win_word(const uint4 *__restrict__ tab, uint32_t w, int row0, uint4 acc) {
+    w ^= win_swz(row0);
 #pragma unroll
     for (int j = 0; j < 8; j++) {
         const uint4 e = tab[(row0 + j) * 16 + ((w >> (4 * j)) & 15u)];
@@ -149,11 +176,12 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) const u32x4 lds_cu32x4;
 typedef __attribute__((address_space(3))) const uint4 lds_cuint4;
 
-__device__ __forceinline__ uint4 gxor4(uint4 a, u32x4 b) {
-    return make_uint4(a.x ^ b.x, a.y ^ b.y, a.z ^ b.z, a.w ^ b.w);
+__device__ __forceinline__ uint4 gxor3v(uint4 a, u32x4 b, u32x4 c) {
+    return make_uint4(xor3(a.x, b.x, c.x), xor3(a.y, b.y, c.y), xor3(a.z, b.z, c.z), xor3(a.w, b.w, c.w));
 }
 
 __device__ __forceinline__ uint4 win_word_perm(uint32_t base, uint32_t w, int row0, uint4 acc) {
+    w ^= win_swz(row0);
     const uint32_t lo = (w << 4) & 0xF0F0F0F0u;  // nibbles 0,2,4,6 * 16
     const uint32_t hi = w & 0xF0F0F0F0u;         // nibbles 1,3,5,7 * 16
 #pragma unroll
@@ -163,8 +191,7 @@ __device__ __forceinline__ uint4 win_word_perm(uint32_t base, uint32_t w, int ro
         const uint32_t a1 = __builtin_amdgcn_perm(base, hi, sel);
         const lds_cu32x4 *p0 = (const lds_cu32x4 *)(uintptr_t)a0;
         const lds_cu32x4 *p1 = (const lds_cu32x4 *)(uintptr_t)a1;
-        acc = gxor4(acc, p0[(row0 + 2 * k) * 16]);
-        acc = gxor4(acc, p1[(row0 + 2 * k + 1) * 16]);
+        acc = gxor3v(acc, p0[(row0 + 2 * k) * 16], p1[(row0 + 2 * k + 1) * 16]);
     }
     return acc;
 }

@@ -76,6 +76,7 @@ struct NttTuning {
     int max_k;     // layers per pass
     int tile_log;  // log2 elements per LDS tile
     int tab_min;   // variant 1: butterflies per twiddle (in a tile) needed to build a table
+    int tab_group; // variant 1: window tables resident in LDS at once (2, 4 or 8; 8 KiB each)
 };
 extern NttTuning g_ntt_tuning;
 

@@ -75,42 +75,122 @@ hipError_t launch_bitrev_replicate(const uint4 *msg, uint4 *out, int m, int lir,
 // ---------------------------------------------------------------------------
 constexpr int NTT_TILE_LOG = 11;
 constexpr int NTT_MAX_K = 7;
-constexpr int NTT_TAB_GROUP = 2;  // window tables resident in LDS at once (8 KiB each)
 
 // Runtime knobs (set through b64_config_set): the multiply used by the NTT
-// passes (0 = general product, 1 = LDS window tables per twiddle), the
-// largest number of layers per pass and the tile size.
-NttTuning g_ntt_tuning = {1, NTT_MAX_K, NTT_TILE_LOG, 128};
+// passes (0 = general product, 1 = LDS window tables where they pay), the
+// largest number of layers per pass, the tile size, the butterflies per
+// twiddle that justify a table, and the tables resident in LDS at once.
+NttTuning g_ntt_tuning = {1, NTT_MAX_K, NTT_TILE_LOG, 128, 2};
 
 
 __device__ __forceinline__ uint32_t rev_bits(uint32_t x, int b) { return b ? (__brev(x) >> (32 - b)) : 0u; }
+
+constexpr int NTT_PER_THREAD = (1 << NTT_TILE_LOG) / BLOCK;
 
 // Loads the tile: element L = (bf << c) | low holds index
 // base | bf << lb | low.  In message mode the loop runs in message order so the
 // reads are runs of 2^k consecutive message elements: with
 // i = bf << (m-k) | mid << c | low, rev_m(i) = rev_k(bf) | rev(mid) << k | rev_c(low) << (m-c).
+// Every thread issues all of its loads before the first LDS store, so a block
+// keeps 2^(k+c) loads in flight.
 __device__ __forceinline__ void ntt_load_tile(uint4 *__restrict__ s, const NttPassArgs &a, uint64_t base,
                                               uint64_t mid, int lb, int n_el) {
     const int k = a.k, c = a.c;
     if (a.msg == nullptr) {
         const int cmask = (1 << c) - 1;
-        for (int L = threadIdx.x; L < n_el; L += BLOCK)
-            s[L] = a.data[base | ((uint64_t)(L >> c) << lb) | (uint64_t)(L & cmask)];
+        uint4 r[NTT_PER_THREAD];
+#pragma unroll
+        for (int i = 0; i < NTT_PER_THREAD; i++) {
+            const int L = threadIdx.x + i * BLOCK;
+            r[i] = L < n_el ? a.data[base | ((uint64_t)(L >> c) << lb) | (uint64_t)(L & cmask)] : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < NTT_PER_THREAD; i++) {
+            const int L = threadIdx.x + i * BLOCK;
+            if (L < n_el) s[L] = r[i];
+        }
     } else {
         const int mid_bits = lb - c;
         const uint64_t mbase = mid_bits ? ((__brevll(mid) >> (64 - mid_bits)) << k) : 0;
         const int kmask = (1 << k) - 1;
-        for (int t = threadIdx.x; t < n_el; t += BLOCK) {
-            const int tk = t & kmask, tc = t >> k;
-            s[(rev_bits(tk, k) << c) | rev_bits(tc, c)] = a.msg[mbase + tk + ((uint64_t)tc << (a.m - c))];
+        uint4 r[NTT_PER_THREAD];
+#pragma unroll
+        for (int i = 0; i < NTT_PER_THREAD; i++) {
+            const int t = threadIdx.x + i * BLOCK;
+            r[i] = t < n_el ? a.msg[mbase + (t & kmask) + ((uint64_t)(t >> k) << (a.m - c))] : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < NTT_PER_THREAD; i++) {
+            const int t = threadIdx.x + i * BLOCK;
+            if (t < n_el) s[(rev_bits(t & kmask, k) << c) | rev_bits(t >> k, c)] = r[i];
         }
     }
+}
+
+__device__ __forceinline__ void ntt_store_tile(const uint4 *__restrict__ s, const NttPassArgs &a, uint64_t base,
+                                               int lb, int n_el) {
+    const int cmask = (1 << a.c) - 1;
+    uint4 r[NTT_PER_THREAD];
+#pragma unroll
+    for (int i = 0; i < NTT_PER_THREAD; i++) {
+        const int L = threadIdx.x + i * BLOCK;
+        r[i] = L < n_el ? s[L] : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < NTT_PER_THREAD; i++) {
+        const int L = threadIdx.x + i * BLOCK;
+        if (L < n_el) a.data[base | ((uint64_t)(L >> a.c) << lb) | (uint64_t)(L & cmask)] = r[i];
+    }
+}
+
+// Twiddles of every layer of the pass for this tile:
+// tw[2^j - 1 + loc] = twiddle(layer0 + j, hi << j | loc) = XOR of W[layer][i]
+// over the set bits i of the block index.  The part from hi (bits above the
+// tile) is shared by a layer's 2^j twiddles and is summed once per layer, with
+// the loads of a layer issued together.  Ends with a barrier.
+constexpr int NTT_TW = (1 << NTT_MAX_K) - 1;
+
+__device__ __forceinline__ void ntt_tile_twiddles(const NttPassArgs &a, uint64_t hi, uint4 *__restrict__ tw,
+                                                  uint4 *__restrict__ thi) {
+    const int k = a.k;
+    if ((int)threadIdx.x < k) {
+        const int j = threadIdx.x;
+        const uint4 *Wl = a.W + (a.layer0 + j) * W_STRIDE + j;
+        uint4 acc = make_uint4(0, 0, 0, 0);
+        for (int i0 = 0; i0 < a.layer0; i0 += 8) {
+            uint4 w[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int i = i0 + u;
+                w[u] = (i < a.layer0 && ((hi >> i) & 1)) ? Wl[i] : make_uint4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc = gxor(acc, w[u]);
+        }
+        thi[j] = acc;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < (1 << k) - 1; t += BLOCK) {
+        const int j = 31 - __clz(t + 1);  // t + 1 in [2^j, 2^(j+1))
+        const int loc = t + 1 - (1 << j);
+        const uint4 *Wl = a.W + (a.layer0 + j) * W_STRIDE;
+        uint4 w[NTT_MAX_K - 1];
+#pragma unroll
+        for (int i = 0; i < NTT_MAX_K - 1; i++)
+            w[i] = (i < j && ((loc >> i) & 1)) ? Wl[i] : make_uint4(0, 0, 0, 0);
+        uint4 acc = thi[j];
+#pragma unroll
+        for (int i = 0; i < NTT_MAX_K - 1; i++) acc = gxor(acc, w[i]);
+        tw[t] = acc;
+    }
+    __syncthreads();
 }
 
 template <bool INVERSE>
 __global__ __launch_bounds__(BLOCK) void k_ntt_pass(NttPassArgs a) {
     __shared__ uint4 s[1 << NTT_TILE_LOG];
-    __shared__ uint4 tw[1 << (NTT_MAX_K - 1)];
+    __shared__ uint4 tw[NTT_TW];
+    __shared__ uint4 thi[NTT_MAX_K];
     const int k = a.k, c = a.c;
     const int lb = a.log_d - a.layer0 - k;
     const int mid_bits = lb - c;
@@ -119,31 +199,19 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass(NttPassArgs a) {
     const uint64_t hi = tile >> mid_bits;
     const uint64_t base = (hi << (lb + k)) | (mid << c);
     const int n_el = 1 << (k + c);
-    const int cmask = (1 << c) - 1;
 
     ntt_load_tile(s, a, base, mid, lb, n_el);
-    __syncthreads();
+    ntt_tile_twiddles(a, hi, tw, thi);
 
     for (int jj = 0; jj < k; jj++) {
         const int j = INVERSE ? (k - 1 - jj) : jj;
-        const int layer = a.layer0 + j;
-        const uint4 *Wl = a.W + layer * W_STRIDE;
-        // twiddle(layer, (hi << j) | loc) for the 2^j local blocks
-        for (int loc = threadIdx.x; loc < (1 << j); loc += BLOCK) {
-            uint4 t = make_uint4(0, 0, 0, 0);
-            for (int i = 0; i < j; i++)
-                if ((loc >> i) & 1) t = gxor(t, Wl[i]);
-            for (int i = 0; i < a.layer0; i++)
-                if ((hi >> i) & 1) t = gxor(t, Wl[j + i]);
-            tw[loc] = t;
-        }
-        __syncthreads();
+        const uint4 *twl = tw + (1 << j) - 1;
         const int pos = c + (k - 1 - j);
         const int lowmask = (1 << pos) - 1;
         for (int P = threadIdx.x; P < (n_el >> 1); P += BLOCK) {
             const int L0 = ((P >> pos) << (pos + 1)) | (P & lowmask);
             const int L1 = L0 | (1 << pos);
-            const uint4 t = tw[(L0 >> c) >> (k - j)];
+            const uint4 t = twl[P >> pos];
             uint4 u = s[L0], v = s[L1];
             if (!INVERSE) {
                 u = gxor(u, gf_mul(v, t));
@@ -158,25 +226,27 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass(NttPassArgs a) {
         __syncthreads();
     }
 
-    for (int L = threadIdx.x; L < n_el; L += BLOCK)
-        a.data[base | ((uint64_t)(L >> c) << lb) | (uint64_t)(L & cmask)] = s[L];
+    ntt_store_tile(s, a, base, lb, n_el);
 }
 
 // Same pass structure, but each butterfly multiplies by its block's twiddle
 // through an LDS window table.  The 2^j blocks of layer j are contiguous ranges
-// of the butterfly enumeration; NTT_TAB_GROUP tables are built at a time (32
-// lanes per table) and consumed by the butterflies of those blocks.
+// of the butterfly enumeration; up to G tables are built at a time (one lane
+// per table row, all lanes of the block) and consumed by the butterflies of
+// those blocks.  Layers with fewer than tab_min butterflies per twiddle use the
+// general product instead.
+template <int G>
 struct alignas(256) NttTabSmem {
     uint4 s[1 << NTT_TILE_LOG];
-    uint4 tabs[NTT_TAB_GROUP * WIN_ENTRIES];
-    uint4 tw[1 << (NTT_MAX_K - 1)];
+    uint4 tabs[G * WIN_ENTRIES];
+    uint4 tw[NTT_TW];
+    uint4 thi[NTT_MAX_K];
 };
 
-template <bool INVERSE>
-__device__ __forceinline__ void ntt_tab_tile(const NttPassArgs &a, uint64_t tile, NttTabSmem &sm) {
+template <bool INVERSE, int G>
+__device__ __forceinline__ void ntt_tab_tile(const NttPassArgs &a, uint64_t tile, NttTabSmem<G> &sm) {
     uint4 *__restrict__ s = sm.s;
     uint4 *__restrict__ tabs = sm.tabs;
-    uint4 *__restrict__ tw = sm.tw;
     const int k = a.k, c = a.c;
     const int lb = a.log_d - a.layer0 - k;
     const int mid_bits = lb - c;
@@ -184,23 +254,13 @@ __device__ __forceinline__ void ntt_tab_tile(const NttPassArgs &a, uint64_t tile
     const uint64_t hi = tile >> mid_bits;
     const uint64_t base = (hi << (lb + k)) | (mid << c);
     const int n_el = 1 << (k + c);
-    const int cmask = (1 << c) - 1;
 
     ntt_load_tile(s, a, base, mid, lb, n_el);
+    ntt_tile_twiddles(a, hi, sm.tw, sm.thi);
 
     for (int jj = 0; jj < k; jj++) {
         const int j = INVERSE ? (k - 1 - jj) : jj;
-        const int layer = a.layer0 + j;
-        const uint4 *Wl = a.W + layer * W_STRIDE;
-        for (int loc = threadIdx.x; loc < (1 << j); loc += BLOCK) {
-            uint4 t = make_uint4(0, 0, 0, 0);
-            for (int i = 0; i < j; i++)
-                if ((loc >> i) & 1) t = gxor(t, Wl[i]);
-            for (int i = 0; i < a.layer0; i++)
-                if ((hi >> i) & 1) t = gxor(t, Wl[j + i]);
-            tw[loc] = t;
-        }
-        __syncthreads();
+        const uint4 *__restrict__ tw = sm.tw + (1 << j) - 1;
         const int pos = c + (k - 1 - j);
         const int per_blk = 1 << pos;
         const int lowmask = per_blk - 1;
@@ -225,11 +285,10 @@ __device__ __forceinline__ void ntt_tab_tile(const NttPassArgs &a, uint64_t tile
             __syncthreads();
             continue;
         }
-        for (int g0 = 0; g0 < nblk; g0 += NTT_TAB_GROUP) {
-            const int ng = nblk - g0 < NTT_TAB_GROUP ? nblk - g0 : NTT_TAB_GROUP;
-            if (threadIdx.x < 32 * ng)
-                win_build_row(tabs + (threadIdx.x >> 5) * WIN_ENTRIES, tw[g0 + (threadIdx.x >> 5)],
-                              threadIdx.x & 31);
+        for (int g0 = 0; g0 < nblk; g0 += G) {
+            const int ng = nblk - g0 < G ? nblk - g0 : G;
+            for (int r = threadIdx.x; r < WIN_ROWS * ng; r += BLOCK)
+                win_build_row(tabs + (r >> 5) * WIN_ENTRIES, tw[g0 + (r >> 5)], r & 31);
             __syncthreads();
             const int n_bf = ng * per_blk;
             for (int q = threadIdx.x; q < n_bf; q += BLOCK) {
@@ -252,14 +311,22 @@ __device__ __forceinline__ void ntt_tab_tile(const NttPassArgs &a, uint64_t tile
         }
     }
 
-    for (int L = threadIdx.x; L < n_el; L += BLOCK)
-        a.data[base | ((uint64_t)(L >> c) << lb) | (uint64_t)(L & cmask)] = s[L];
+    ntt_store_tile(s, a, base, lb, n_el);
+}
+
+template <bool INVERSE, int G>
+__global__ __launch_bounds__(BLOCK) void k_ntt_pass_tab(NttPassArgs a) {
+    __shared__ NttTabSmem<G> sm;
+    ntt_tab_tile<INVERSE, G>(a, blockIdx.x, sm);
 }
 
 template <bool INVERSE>
-__global__ __launch_bounds__(BLOCK) void k_ntt_pass_tab(NttPassArgs a) {
-    __shared__ NttTabSmem sm;
-    ntt_tab_tile<INVERSE>(a, blockIdx.x, sm);
+static void launch_ntt_tab(const NttPassArgs &a, unsigned blocks, hipStream_t s) {
+    switch (g_ntt_tuning.tab_group) {
+        case 8: k_ntt_pass_tab<INVERSE, 8><<<blocks, BLOCK, 0, s>>>(a); break;
+        case 4: k_ntt_pass_tab<INVERSE, 4><<<blocks, BLOCK, 0, s>>>(a); break;
+        default: k_ntt_pass_tab<INVERSE, 2><<<blocks, BLOCK, 0, s>>>(a); break;
+    }
 }
 
 
@@ -307,9 +374,9 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
         const uint64_t n_tiles = 1ull << (log_d - a.k - a.c);
         if (g_ntt_tuning.variant == 1) {
             if (inverse)
-                k_ntt_pass_tab<true><<<(unsigned)n_tiles, BLOCK, 0, s>>>(a);
+                launch_ntt_tab<true>(a, (unsigned)n_tiles, s);
             else
-                k_ntt_pass_tab<false><<<(unsigned)n_tiles, BLOCK, 0, s>>>(a);
+                launch_ntt_tab<false>(a, (unsigned)n_tiles, s);
         } else {
             if (inverse)
                 k_ntt_pass<true><<<(unsigned)n_tiles, BLOCK, 0, s>>>(a);
@@ -692,12 +759,13 @@ hipError_t launch_fold_codeword(const DeviceDomain &dd, const uint4 *cw, uint4 *
     return hipGetLastError();
 }
 
+template <int G>
 __global__ __launch_bounds__(BLOCK) void k_ntt_pass_tab_multi(const NttPassArgs *__restrict__ jobs,
                                                               const uint32_t *__restrict__ start, int n_jobs) {
-    __shared__ NttTabSmem sm;
+    __shared__ NttTabSmem<G> sm;
     const int j = job_of_block2(start, n_jobs, blockIdx.x);
     const NttPassArgs a = jobs[j];
-    ntt_tab_tile<false>(a, blockIdx.x - start[j], sm);
+    ntt_tab_tile<false, G>(a, blockIdx.x - start[j], sm);
 }
 
 // ---------------------------------------------------------------------------
@@ -709,7 +777,17 @@ hipError_t launch_multi(MultiKind kind, const void *d_jobs, const uint32_t *d_st
     if (blocks == 0 || n_jobs == 0) return hipSuccess;
     switch (kind) {
         case MultiKind::NttPass:
-            k_ntt_pass_tab_multi<<<blocks, BLOCK, 0, s>>>((const NttPassArgs *)d_jobs, d_start, n_jobs);
+            switch (g_ntt_tuning.tab_group) {
+                case 8:
+                    k_ntt_pass_tab_multi<8><<<blocks, BLOCK, 0, s>>>((const NttPassArgs *)d_jobs, d_start, n_jobs);
+                    break;
+                case 4:
+                    k_ntt_pass_tab_multi<4><<<blocks, BLOCK, 0, s>>>((const NttPassArgs *)d_jobs, d_start, n_jobs);
+                    break;
+                default:
+                    k_ntt_pass_tab_multi<2><<<blocks, BLOCK, 0, s>>>((const NttPassArgs *)d_jobs, d_start, n_jobs);
+                    break;
+            }
             break;
         case MultiKind::Leaf16Fold:
             k_leaf16_fold_multi<<<blocks, BLOCK, 0, s>>>((const Leaf16Args *)d_jobs, d_start, n_jobs);

@@ -78,7 +78,9 @@ int b64_stream_synchronize(void *stream);
  * interface).  Keys: "ntt.variant" (0 = general multiply, 1 = per layer, LDS
  * window tables where a twiddle covers at least "ntt.tab_min" butterflies of a
  * tile), "ntt.max_k" (layers per NTT pass, 1..7), "ntt.tile_log" (log2 elements
- * per LDS tile).  Returns INVALID_ARGS for an unknown key or value. */
+ * per LDS tile), "ntt.tab_group" (window tables resident in LDS at once: 2, 4
+ * or 8).  Returns INVALID_ARGS for an unknown key or value.  Settings are
+ * process-wide; change them only while no call is running. */
 int b64_config_set(const char *key, int64_t value);
 int b64_config_get(const char *key, int64_t *value);
 
