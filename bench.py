@@ -42,6 +42,11 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--cpu-log-n", type=int, default=24, help="CPU baseline sample size")
+    ap.add_argument("--ntt", default=None, metavar="V,K,T,M,G",
+                    help="NTT tuning for this run: variant, max_k, tile_log, tab_min, tab_group")
+    ap.add_argument("--counters", action="store_true",
+                    help="hardware-counter workload: one encode, one forward NTT of the codeword and a "
+                         "512 MiB device copy (the byte-counter calibration), then exit")
     return ap.parse_args()
 
 
@@ -59,6 +64,10 @@ def main():
     L = b64.lib()
     if b64.device_count() == 0:
         raise SystemExit("no HIP device")
+    if args.ntt:
+        for key, v in zip(("ntt.variant", "ntt.max_k", "ntt.tile_log", "ntt.tab_min", "ntt.tab_group"),
+                          args.ntt.split(",")):
+            b64.config_set(key, int(v))
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
     sp = C.c_void_p(stream.cuda_stream)
@@ -78,6 +87,18 @@ def main():
     chal_p = chal.ctypes.data_as(C.POINTER(C.c_uint64))
 
     pc = params.to_c()
+
+    if args.counters:
+        b64.check(L.b64_rs_encode_batch(dom.ptr, log_dim, lir, msg.data_ptr(), lbs, cw.data_ptr(), sp), "encode")
+        torch.cuda.synchronize()
+        b64.check(L.b64_ntt_forward(dom.ptr, cw.data_ptr(), log_cw, lir, lbs, sp), "ntt_forward")
+        torch.cuda.synchronize()
+        cal = torch.empty((1 << 26,), dtype=torch.int64, device=dev)
+        cal.copy_(cw[: 1 << 26])
+        torch.cuda.synchronize()
+        print(json.dumps({"counters_workload": {"log_n": n, "ntt_launches": "encode then ntt_forward",
+                                                "copy_bytes": 8 << 26}}), flush=True)
+        return
 
     def step():
         # encode, then one pass over the codeword for leaf digests + the folded
@@ -116,9 +137,10 @@ def main():
 
     scratch = cw.clone()
     n_layers = rs_log_len - lir  # layers lir .. log_cw - lbs
-    max_k = b64.config_get("ntt.max_k")
+    tuning = {k: b64.config_get(f"ntt.{k}:{log_cw}") for k in ("variant", "max_k", "tile_log", "tab_min", "tab_group")}
+    max_k = tuning["max_k"]
     n_passes = (n_layers + max_k - 1) // max_k  # the library's balanced pass split
-    kernel = "k_ntt_pass_tab" if b64.config_get("ntt.variant") == 1 else "k_ntt_pass"
+    kernel = "k_ntt_pass_tab" if tuning["variant"] == 1 else "k_ntt_pass"
     stages = {
         "encode_ms": timed(lambda: b64.check(L.b64_rs_encode_batch(dom.ptr, log_dim, lir, msg.data_ptr(), lbs,
                                                                    cw.data_ptr(), sp))),
@@ -134,7 +156,8 @@ def main():
     muls = n_layers * (1 << (log_cw - 1))  # one multiply per butterfly
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": TRAFFIC.get((kernel, log_cw)),
-                "kernel": kernel, "launches_per_ntt": n_passes, "avg_launch_ms": round(ntt_pass_ms, 4),
+                "kernel": kernel, "ntt_tuning": tuning, "launches_per_ntt": n_passes,
+                "avg_launch_ms": round(ntt_pass_ms, 4),
                 "algorithmic_bytes_per_launch": pass_bytes,
                 "gf_mul_per_s": float(f"{muls / (stages['ntt_ms'] * 1e-3):.4g}")}
     step_bytes = 58 * (1 << n)  # algorithmic HBM bytes of the whole step
@@ -208,13 +231,15 @@ def batch_bench(b64, torch, stream, sp, n_jobs: int = 64, reps: int = 5):
         plan.run(sp.value)
     t_batch = (time.perf_counter() - t) / reps
     same = same and [j.roots() for j in jobs] == want
+    from binius64_amd._abi import TUNING_PATH
     return {"jobs": n_jobs, "log_n_range": [12, 20], "sizes_seed": 4, "n_launches_batched": plan.n_launches(),
+            "ntt_schedule": "per-size autotuned table" if TUNING_PATH.exists() else "library defaults",
             "plain_jobs_per_s": round(n_jobs / t_plain, 1), "batched_jobs_per_s": round(n_jobs / t_batch, 1),
             "speedup": round(t_plain / t_batch, 3), "plain_ms": round(1e3 * t_plain, 3),
             "batched_ms": round(1e3 * t_batch, 3), "roots_identical": bool(same)}
 
 
-def serve_bench(b64, torch, sp, n_jobs: int = 400, rate: float = 200.0):
+def serve_bench(b64, torch, sp, n_jobs: int = 400, rate: float = 200.0, detail: bool = False):
     """Commit requests arriving as a Poisson stream (mean `rate`/s, seed 5) with
     witness sizes 2^14..2^22 64-bit words, log-uniform (n = 13..21 B128
     elements after packing two words per element), uniform random contents.
@@ -278,6 +303,12 @@ def serve_bench(b64, torch, sp, n_jobs: int = 400, rate: float = 200.0):
     # ---- served: Poisson arrivals --------------------------------------------
     srv = b64.Server(hbm_budget_bytes=8 << 30, max_batch=64, n_streams=3)
     try:
+        # warm-up (untimed): three jobs of each size, so first-launch costs
+        # (code-object loads, first use of a stream slot) stay out of the latencies
+        for n in sorted(set(sizes)):
+            i = sizes.index(n)
+            for t in [srv.submit(n, msgs[i], chals[i]) for _ in range(3)]:
+                srv.wait(t, n_roots[i])
         tickets = [0] * n_jobs
 
         def submitter():
@@ -316,7 +347,12 @@ def serve_bench(b64, torch, sp, n_jobs: int = 400, rate: float = 200.0):
     def pct(a, q):
         return round(1e3 * float(np.percentile(a, q)), 3)
 
-    return {"jobs": n_jobs, "arrival_rate_per_s": rate, "seed": 5, "log_n_range": [13, 21],
+    extra = {}
+    if detail:
+        worst = np.argsort(lat)[-8:]
+        extra = {"worst_served": [(sizes[i], round(1e3 * lat[i], 3), round(1e3 * lat_plain[i], 3),
+                                   round(float(arrivals[i]), 4)) for i in worst]}
+    return {**extra, "jobs": n_jobs, "arrival_rate_per_s": rate, "seed": 5, "log_n_range": [13, 21],
             "hbm_budget_bytes": 8 << 30, "max_batch": 64, "n_streams": 3,
             "served_p50_ms": pct(lat, 50), "served_p99_ms": pct(lat, 99),
             "served_jobs_per_s_at_rate": round(n_jobs / t_poisson, 1),
@@ -343,17 +379,17 @@ def cpu_baseline(log_n: int):
     msg = orc.random_elems(22, 1 << log_n)
     chal = orc.random_elems(23, p["log_batch_size"])
     lbs = p["log_batch_size"]
-    reps, times = 2, []
+    reps, times = 5, []
     for _ in range(reps):
         t = time.perf_counter()
         cw = orc.encode_batch(dom, msg, p["log_dim"], 1, lbs, threads=cores)
         orc.merkle_build(cw, 1 << lbs, threads=cores)
         orc.fold_interleaved(cw, p["rs_log_len"], lbs, chal, threads=cores)
         times.append(time.perf_counter() - t)
-    best = min(times)
-    return {"value": round((16 << log_n) / best / 1e9, 4), "unit": "GB/s", "cores": cores, "kind": "port",
-            "sample": f"log_n={log_n} (2^{log_n} B128 message, same encode+merkle+fold step), best of {reps}",
-            "seconds": round(best, 3)}
+    med = sorted(times)[reps // 2]
+    return {"value": round((16 << log_n) / med / 1e9, 4), "unit": "GB/s", "cores": cores, "kind": "port",
+            "sample": f"log_n={log_n} (2^{log_n} B128 message, same encode+merkle+fold step), median of {reps}",
+            "seconds": round(med, 3)}
 
 
 if __name__ == "__main__":

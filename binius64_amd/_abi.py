@@ -141,7 +141,18 @@ def lib():
             fn.restype = res
             fn.argtypes = args
         _lib = L
+        _apply_tuning(L)
     return _lib
+
+
+TUNING_PATH = Path(__file__).resolve().parent / "ntt_tuning.json"
+
+
+def _apply_tuning(L) -> None:
+    """Per-size NTT schedules measured by binius64_amd.autotune, if present."""
+    if TUNING_PATH.exists():
+        from .autotune import apply_table, load_table
+        apply_table(load_table(TUNING_PATH), L)
 
 
 class B64Error(RuntimeError):

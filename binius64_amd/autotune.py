@@ -1,0 +1,131 @@
+"""Autotuner for the NTT pass grouping.
+
+The forward NTT of a commit job runs as a few LDS-tiled passes; how many
+layers a pass takes (ntt.max_k), the tile size (ntt.tile_log), which layers
+multiply through window tables (ntt.variant, ntt.tab_min) and how many tables
+stay resident (ntt.tab_group) trade HBM passes against table builds and
+occupancy, and the best choice changes with the transform size.  This module
+times every candidate on the device, per transform size, and stores the
+winners as per-size overrides ("ntt.<knob>:<log_d>", see b64_config_set).
+
+The table is saved as JSON next to the library (ntt_tuning.json) and applied
+whenever the library is loaded, so the plain, batched and served paths all use
+it.  Every candidate computes the same transform (the parity tests run each
+knob combination against the oracle); only the schedule changes.
+
+Usage on the GPU:  python -m binius64_amd.autotune --out binius64_amd/ntt_tuning.json
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+from pathlib import Path
+from typing import Dict, Iterable, List, Optional, Sequence, Tuple
+
+KNOBS = ("ntt.variant", "ntt.max_k", "ntt.tile_log", "ntt.tab_min", "ntt.tab_group")
+DEFAULT_PATH = Path(__file__).resolve().parent / "ntt_tuning.json"
+
+Candidate = Tuple[int, int, int, int, int]
+
+
+def default_candidates() -> List[Candidate]:
+    c: List[Candidate] = [(0, k, 11, 1, 2) for k in (5, 6, 7)]
+    for k in (5, 6, 7):
+        for tile in (10, 11):
+            for tab_min in (32, 64, 128):
+                c.append((1, k, tile, tab_min, 2))
+        c.append((1, k, 11, 64, 4))
+    c += [(1, 4, 11, 32, 2), (1, 4, 11, 64, 2)]
+    return c
+
+
+def apply_table(table: Dict[int, Candidate], lib=None) -> None:
+    """Install per-size overrides (log_d -> candidate) in the loaded library."""
+    from ._abi import check
+    from ._abi import lib as _lib
+    L = lib if lib is not None else _lib()
+    check(L.b64_config_set(b"ntt.clear_sizes", 0), "config_set ntt.clear_sizes")
+    for log_d, cand in table.items():
+        for key, v in zip(KNOBS, cand):
+            check(L.b64_config_set(f"{key}:{int(log_d)}".encode(), int(v)), f"config_set {key}:{log_d}")
+
+
+def load_table(path: Path = DEFAULT_PATH) -> Dict[int, Candidate]:
+    data = json.loads(Path(path).read_text())
+    return {int(k): tuple(int(x) for x in v["config"]) for k, v in data["sizes"].items()}
+
+
+class _EncodeBench:
+    """A commit job's encode at message size 2^log_n on uniform random data."""
+
+    def __init__(self, b64, log_n: int):
+        import numpy as np
+        self.b64, self.L = b64, b64.lib()
+        self.params, self.dom = b64.setup_commit_params(log_n, 1, 96)
+        words = np.random.default_rng(log_n).integers(0, 2 ** 63, size=2 << log_n, dtype=np.uint64)
+        self.msg = b64.DeviceBuffer.from_host(words)
+        self.cw = b64.DeviceBuffer(32 << log_n)
+
+    def run(self):
+        p = self.params
+        self.b64.check(self.L.b64_rs_encode_batch(self.dom.ptr, p.log_dim, 1, self.msg.ptr, p.log_batch_size,
+                                                  self.cw.ptr, None), "encode")
+
+    def seconds(self, reps: int) -> float:
+        self.run()
+        self.b64.check(self.L.b64_stream_synchronize(None), "sync")
+        t = time.perf_counter()
+        for _ in range(reps):
+            self.run()
+        self.b64.check(self.L.b64_stream_synchronize(None), "sync")
+        return (time.perf_counter() - t) / reps
+
+
+def tune(log_ns: Iterable[int], candidates: Optional[Sequence[Candidate]] = None, budget_s: float = 0.02,
+         verbose: bool = False) -> Dict[int, dict]:
+    """Time every candidate for commit jobs of each message size log_n (the NTT
+    has log_d = log_n + 1) and return {log_d: {"config": best, "ms": t, "all": {...}}}."""
+    import binius64_amd as b64
+    from ._abi import check, lib
+
+    L = lib()
+    cands = list(candidates or default_candidates())
+    out: Dict[int, dict] = {}
+    for log_n in log_ns:
+        log_d = log_n + 1
+        bench = _EncodeBench(b64, log_n)
+        est = bench.seconds(1)
+        reps = max(3, min(200, int(budget_s / max(est, 1e-6))))
+        times = {}
+        for cand in cands:
+            for key, v in zip(KNOBS, cand):
+                check(L.b64_config_set(f"{key}:{log_d}".encode(), int(v)), key)
+            times[cand] = min(bench.seconds(reps) for _ in range(2))
+        best = min(times, key=times.get)
+        for key, v in zip(KNOBS, best):
+            check(L.b64_config_set(f"{key}:{log_d}".encode(), int(v)), key)
+        out[log_d] = {"config": list(best), "ms": round(1e3 * times[best], 5),
+                      "all": {",".join(map(str, c)): round(1e3 * t, 5) for c, t in times.items()}}
+        if verbose:
+            print(json.dumps({"log_n": log_n, "log_d": log_d, "best": best, "ms": out[log_d]["ms"]}), flush=True)
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--log-n", type=int, nargs="+", default=list(range(8, 25)))
+    ap.add_argument("--out", default=str(DEFAULT_PATH))
+    args = ap.parse_args(argv)
+    from ._abi import lib
+    lib().b64_config_set(b"ntt.clear_sizes", 0)
+    res = tune(args.log_n, verbose=True)
+    doc = {"device": "MI355X (gfx950)", "knobs": list(KNOBS),
+           "what": "per transform size log_d = log_n + 1 of a commit job's encode: best NTT schedule",
+           "sizes": {str(k): v for k, v in sorted(res.items())}}
+    Path(args.out).write_text(json.dumps(doc, indent=1) + "\n")
+    print(json.dumps({"written": args.out, "sizes": len(res)}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

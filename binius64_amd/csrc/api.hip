@@ -177,36 +177,71 @@ int b64_stream_synchronize(void *stream) {
     return B64_OK;
 }
 
+// "ntt.<field>" addresses the defaults, "ntt.<field>:<log_d>" the tuning of
+// transforms of size 2^log_d (first set copies the defaults).
+static bool parse_tuning_key(const char *key, char *field, size_t cap, int *log_d) {
+    const char *colon = strchr(key, ':');
+    const size_t n = colon ? (size_t)(colon - key) : strlen(key);
+    if (n + 1 > cap) return false;
+    memcpy(field, key, n);
+    field[n] = 0;
+    *log_d = -1;
+    if (colon) {
+        char *end = nullptr;
+        const long v = strtol(colon + 1, &end, 10);
+        if (end == colon + 1 || *end != 0 || v < 1 || v > b64::MAX_DIM) return false;
+        *log_d = (int)v;
+    }
+    return true;
+}
+
 int b64_config_set(const char *key, int64_t value) {
     if (!key) return B64_ERR_INVALID_ARGS;
-    if (!strcmp(key, "ntt.variant")) {
+    if (!strcmp(key, "ntt.clear_sizes")) {
+        (void)b64::ntt_tuning_for(0);  // initialises the table
+        for (auto &t : b64::g_ntt_tuning_size) t.variant = -1;
+        return B64_OK;
+    }
+    char field[32];
+    int log_d;
+    if (!parse_tuning_key(key, field, sizeof field, &log_d)) return B64_ERR_INVALID_ARGS;
+    b64::NttTuning t = b64::ntt_tuning_for(log_d);
+    if (!strcmp(field, "ntt.variant")) {
         if (value < 0 || value > 1) return B64_ERR_INVALID_ARGS;
-        b64::g_ntt_tuning.variant = (int)value;
-    } else if (!strcmp(key, "ntt.max_k")) {
+        t.variant = (int)value;
+    } else if (!strcmp(field, "ntt.max_k")) {
         if (value < 1 || value > 7) return B64_ERR_INVALID_ARGS;
-        b64::g_ntt_tuning.max_k = (int)value;
-    } else if (!strcmp(key, "ntt.tile_log")) {
+        t.max_k = (int)value;
+    } else if (!strcmp(field, "ntt.tile_log")) {
         if (value < 1 || value > 11) return B64_ERR_INVALID_ARGS;
-        b64::g_ntt_tuning.tile_log = (int)value;
-    } else if (!strcmp(key, "ntt.tab_min")) {
+        t.tile_log = (int)value;
+    } else if (!strcmp(field, "ntt.tab_min")) {
         if (value < 1 || value > (1 << 30)) return B64_ERR_INVALID_ARGS;
-        b64::g_ntt_tuning.tab_min = (int)value;
-    } else if (!strcmp(key, "ntt.tab_group")) {
+        t.tab_min = (int)value;
+    } else if (!strcmp(field, "ntt.tab_group")) {
         if (value != 2 && value != 4 && value != 8) return B64_ERR_INVALID_ARGS;
-        b64::g_ntt_tuning.tab_group = (int)value;
+        t.tab_group = (int)value;
     } else {
         return B64_ERR_INVALID_ARGS;
     }
+    if (log_d < 0)
+        b64::g_ntt_tuning = t;
+    else
+        b64::g_ntt_tuning_size[log_d] = t;
     return B64_OK;
 }
 
 int b64_config_get(const char *key, int64_t *value) {
     if (!key || !value) return B64_ERR_INVALID_ARGS;
-    if (!strcmp(key, "ntt.variant")) *value = b64::g_ntt_tuning.variant;
-    else if (!strcmp(key, "ntt.max_k")) *value = b64::g_ntt_tuning.max_k;
-    else if (!strcmp(key, "ntt.tile_log")) *value = b64::g_ntt_tuning.tile_log;
-    else if (!strcmp(key, "ntt.tab_min")) *value = b64::g_ntt_tuning.tab_min;
-    else if (!strcmp(key, "ntt.tab_group")) *value = b64::g_ntt_tuning.tab_group;
+    char field[32];
+    int log_d;
+    if (!parse_tuning_key(key, field, sizeof field, &log_d)) return B64_ERR_INVALID_ARGS;
+    const b64::NttTuning &t = b64::ntt_tuning_for(log_d);
+    if (!strcmp(field, "ntt.variant")) *value = t.variant;
+    else if (!strcmp(field, "ntt.max_k")) *value = t.max_k;
+    else if (!strcmp(field, "ntt.tile_log")) *value = t.tile_log;
+    else if (!strcmp(field, "ntt.tab_min")) *value = t.tab_min;
+    else if (!strcmp(field, "ntt.tab_group")) *value = t.tab_group;
     else return B64_ERR_INVALID_ARGS;
     return B64_OK;
 }

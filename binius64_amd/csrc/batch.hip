@@ -176,7 +176,10 @@ int job_stages(const b64_job_desc &jd, const Layout &L, std::vector<Stage> &out,
         st.ntt.layer0 = starts[q];
         st.ntt.k = ks[q];
         st.ntt.c = cs[q];
-        st.ntt.tab_min = b64::g_ntt_tuning.tab_min;
+        // the batched pass kernel is the table kernel; variant 0 maps to
+        // "no layer qualifies for a table"
+        const b64::NttTuning &tu = b64::ntt_tuning_for(log_d);
+        st.ntt.tab_min = tu.variant == 0 ? (1 << 30) : tu.tab_min;
         st.blocks = (uint32_t)(1ull << (log_d - ks[q] - cs[q]));
         out.push_back(st);
     }
@@ -207,7 +210,7 @@ int job_stages(const b64_job_desc &jd, const Layout &L, std::vector<Stage> &out,
             st.sub.n_in = ns[i];
             st.sub.levels = lv[i];
             st.sub.iv = job_iv();
-            st.blocks = (uint32_t)(ns[i] >> lv[i]);
+            st.blocks = (uint32_t)(ns[i] / b64::merkle_chunk(ns[i]));
             out.push_back(st);
         }
         roots.push_back(tree + 2 * (2 * n_leaves - 2));

@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "bitop_dev.hpp"
+
 namespace b64 {
 
 __device__ __forceinline__ uint4 gxor(uint4 a, uint4 b) {
@@ -25,15 +27,6 @@ __device__ __forceinline__ uint4 gxor(uint4 a, uint4 b) {
 }
 __device__ __forceinline__ bool gzero(uint4 a) { return (a.x | a.y | a.z | a.w) == 0u; }
 
-// Three-input bitwise ops of gfx950 (v_bitop3_b32).  The immediate is the
-// truth table f(0xF0, 0xCC, 0xAA) of the operands in order.
-__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
-    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
-}
-// m ? a : b, bit by bit
-__device__ __forceinline__ uint32_t bsel(uint32_t a, uint32_t b, uint32_t m) {
-    return __builtin_amdgcn_bitop3_b32(a, b, m, 0xE4);
-}
 __device__ __forceinline__ uint64_t xor3_64(uint64_t a, uint64_t b, uint64_t c) {
     const uint32_t lo = xor3((uint32_t)a, (uint32_t)b, (uint32_t)c);
     const uint32_t hi = xor3((uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(c >> 32));
@@ -160,9 +153,11 @@ __device__ inline void win_build_row(uint4 *__restrict__ tab, uint4 c, int row) 
 __device__ __forceinline__ uint4 win_word(const uint4 *__restrict__ tab, uint32_t w, int row0, uint4 acc) {
     w ^= win_swz(row0);
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-        const uint4 e = tab[(row0 + j) * 16 + ((w >> (4 * j)) & 15u)];
-        acc = gxor(acc, e);
+    for (int j = 0; j < 8; j += 2) {
+        const uint4 e0 = tab[(row0 + j) * 16 + ((w >> (4 * j)) & 15u)];
+        const uint4 e1 = tab[(row0 + j + 1) * 16 + ((w >> (4 * j + 4)) & 15u)];
+        acc = make_uint4(xor3(acc.x, e0.x, e1.x), xor3(acc.y, e0.y, e1.y), xor3(acc.z, e0.z, e1.z),
+                         xor3(acc.w, e0.w, e1.w));
     }
     return acc;
 }

@@ -30,7 +30,15 @@ struct CompressIV {
 };
 
 constexpr int MAX_FOLD_CHAL = 12;
-constexpr int MERKLE_RED_LOG = 9;  // tree levels reduced per subtree launch
+constexpr int MERKLE_RED_LOG = 9;  // a subtree block reduces a chunk of up to 2^9 nodes
+// Levels wider than 2^MERKLE_WIDE_LOG nodes are reduced MERKLE_WIDE_LEVELS at a
+// time, so most compressions run in full wavefronts.
+constexpr int MERKLE_WIDE_LOG = 13;
+constexpr int MERKLE_WIDE_LEVELS = 4;
+// Nodes per block of a subtree launch over a level of n_in nodes.
+inline uint64_t merkle_chunk(uint64_t n_in) {
+    return n_in < (1ull << MERKLE_RED_LOG) ? n_in : (1ull << MERKLE_RED_LOG);
+}
 
 // Per-stage kernel arguments.  Single-job launches pass them by value; the
 // batched path keeps one table per launch in device memory.
@@ -78,7 +86,10 @@ struct NttTuning {
     int tab_min;   // variant 1: butterflies per twiddle (in a tile) needed to build a table
     int tab_group; // variant 1: window tables resident in LDS at once (2, 4 or 8; 8 KiB each)
 };
-extern NttTuning g_ntt_tuning;
+extern NttTuning g_ntt_tuning;  // defaults
+// Per transform size (log_d) overrides; variant < 0 marks an unset entry.
+extern NttTuning g_ntt_tuning_size[MAX_DIM + 1];
+const NttTuning &ntt_tuning_for(int log_d);
 
 // Kernel launchers.  All return hipError_t of the launch.
 hipError_t launch_gf_mul(const uint4 *a, const uint4 *b, uint4 *out, uint64_t n, hipStream_t s);

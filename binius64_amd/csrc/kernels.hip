@@ -81,65 +81,129 @@ constexpr int NTT_MAX_K = 7;
 // largest number of layers per pass, the tile size, the butterflies per
 // twiddle that justify a table, and the tables resident in LDS at once.
 NttTuning g_ntt_tuning = {1, NTT_MAX_K, NTT_TILE_LOG, 128, 2};
+NttTuning g_ntt_tuning_size[MAX_DIM + 1] = {};
+
+const NttTuning &ntt_tuning_for(int log_d) {
+    static bool init = false;
+    if (!init) {
+        for (auto &t : g_ntt_tuning_size) t.variant = -1;
+        init = true;
+    }
+    if (log_d >= 0 && log_d <= MAX_DIM && g_ntt_tuning_size[log_d].variant >= 0) return g_ntt_tuning_size[log_d];
+    return g_ntt_tuning;
+}
 
 
 __device__ __forceinline__ uint32_t rev_bits(uint32_t x, int b) { return b ? (__brev(x) >> (32 - b)) : 0u; }
 
 constexpr int NTT_PER_THREAD = (1 << NTT_TILE_LOG) / BLOCK;
 
-// Loads the tile: element L = (bf << c) | low holds index
-// base | bf << lb | low.  In message mode the loop runs in message order so the
-// reads are runs of 2^k consecutive message elements: with
+// Block barrier that orders LDS only.  __syncthreads() also waits for every
+// outstanding global load, which would stall on the next tile's prefetch; the
+// NTT blocks share data through LDS alone.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Position of a tile: it covers index bits lb .. lb+k-1 (flipped) and 0 .. c-1;
+// the bits in between (mid) and above (hi) come from the tile id.
+struct NttTile {
+    uint64_t base, mid, hi;
+    int lb, n_el;
+};
+__device__ __forceinline__ NttTile ntt_tile_at(const NttPassArgs &a, uint64_t tile) {
+    NttTile t;
+    t.lb = a.log_d - a.layer0 - a.k;
+    const int mid_bits = t.lb - a.c;
+    t.mid = tile & ((1ull << mid_bits) - 1);
+    t.hi = tile >> mid_bits;
+    t.base = (t.hi << (t.lb + a.k)) | (t.mid << a.c);
+    t.n_el = 1 << (a.k + a.c);
+    return t;
+}
+
+// A tile in flight: thread x holds elements x, x + BLOCK, ... of the load order,
+// as one register vector (the compiler keeps it in VGPRs where a uint4 array
+// passed between functions can end up in scratch memory).
+typedef uint32_t u32x32 __attribute__((ext_vector_type(4 * NTT_PER_THREAD)));
+struct NttTileRegs {
+    u32x32 v;
+    __device__ __forceinline__ void set(int i, uint4 e) {
+        v[4 * i] = e.x;
+        v[4 * i + 1] = e.y;
+        v[4 * i + 2] = e.z;
+        v[4 * i + 3] = e.w;
+    }
+    __device__ __forceinline__ uint4 get(int i) const {
+        return make_uint4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+    }
+};
+
+// Issues the tile's loads.  Element L = (bf << c) | low of the tile holds
+// index base | bf << lb | low.  In message mode the loads run in message order
+// so they are runs of 2^k consecutive message elements: with
 // i = bf << (m-k) | mid << c | low, rev_m(i) = rev_k(bf) | rev(mid) << k | rev_c(low) << (m-c).
-// Every thread issues all of its loads before the first LDS store, so a block
-// keeps 2^(k+c) loads in flight.
-__device__ __forceinline__ void ntt_load_tile(uint4 *__restrict__ s, const NttPassArgs &a, uint64_t base,
-                                              uint64_t mid, int lb, int n_el) {
+__device__ __forceinline__ void ntt_fetch_tile(NttTileRegs &R, const NttPassArgs &a, const NttTile &t) {
     const int k = a.k, c = a.c;
     if (a.msg == nullptr) {
         const int cmask = (1 << c) - 1;
-        uint4 r[NTT_PER_THREAD];
 #pragma unroll
         for (int i = 0; i < NTT_PER_THREAD; i++) {
-            const int L = threadIdx.x + i * BLOCK;
-            r[i] = L < n_el ? a.data[base | ((uint64_t)(L >> c) << lb) | (uint64_t)(L & cmask)] : make_uint4(0, 0, 0, 0);
-        }
-#pragma unroll
-        for (int i = 0; i < NTT_PER_THREAD; i++) {
-            const int L = threadIdx.x + i * BLOCK;
-            if (L < n_el) s[L] = r[i];
+            // lanes past a small tile re-read a valid element (not stored)
+            const int L = (threadIdx.x + i * BLOCK) & (t.n_el - 1);
+            R.set(i, a.data[t.base | ((uint64_t)(L >> c) << t.lb) | (uint64_t)(L & cmask)]);
         }
     } else {
-        const int mid_bits = lb - c;
-        const uint64_t mbase = mid_bits ? ((__brevll(mid) >> (64 - mid_bits)) << k) : 0;
+        const int mid_bits = t.lb - c;
+        const uint64_t mbase = mid_bits ? ((__brevll(t.mid) >> (64 - mid_bits)) << k) : 0;
         const int kmask = (1 << k) - 1;
-        uint4 r[NTT_PER_THREAD];
 #pragma unroll
         for (int i = 0; i < NTT_PER_THREAD; i++) {
-            const int t = threadIdx.x + i * BLOCK;
-            r[i] = t < n_el ? a.msg[mbase + (t & kmask) + ((uint64_t)(t >> k) << (a.m - c))] : make_uint4(0, 0, 0, 0);
-        }
-#pragma unroll
-        for (int i = 0; i < NTT_PER_THREAD; i++) {
-            const int t = threadIdx.x + i * BLOCK;
-            if (t < n_el) s[(rev_bits(t & kmask, k) << c) | rev_bits(t >> k, c)] = r[i];
+            const int q = (threadIdx.x + i * BLOCK) & (t.n_el - 1);
+            R.set(i, a.msg[mbase + (q & kmask) + ((uint64_t)(q >> k) << (a.m - c))]);
         }
     }
 }
 
-__device__ __forceinline__ void ntt_store_tile(const uint4 *__restrict__ s, const NttPassArgs &a, uint64_t base,
-                                               int lb, int n_el) {
+// Writes a fetched tile into LDS in tile order.
+__device__ __forceinline__ void ntt_put_tile(uint4 *__restrict__ s, const NttTileRegs &R, const NttPassArgs &a,
+                                             int n_el) {
+    const int k = a.k, c = a.c;
+    if (a.msg == nullptr) {
+#pragma unroll
+        for (int i = 0; i < NTT_PER_THREAD; i++) {
+            const int L = threadIdx.x + i * BLOCK;
+            if (L < n_el) s[L] = R.get(i);
+        }
+    } else {
+        const int kmask = (1 << k) - 1;
+#pragma unroll
+        for (int i = 0; i < NTT_PER_THREAD; i++) {
+            const int q = threadIdx.x + i * BLOCK;
+            if (q < n_el) s[(rev_bits(q & kmask, k) << c) | rev_bits(q >> k, c)] = R.get(i);
+        }
+    }
+}
+
+// Every thread issues all of its loads before the first LDS store, so a block
+// keeps 2^(k+c) loads in flight.
+__device__ __forceinline__ void ntt_load_tile(uint4 *__restrict__ s, const NttPassArgs &a, const NttTile &t) {
+    NttTileRegs R;
+    ntt_fetch_tile(R, a, t);
+    ntt_put_tile(s, R, a, t.n_el);
+}
+
+__device__ __forceinline__ void ntt_store_tile(const uint4 *__restrict__ s, const NttPassArgs &a, const NttTile &t) {
     const int cmask = (1 << a.c) - 1;
     uint4 r[NTT_PER_THREAD];
 #pragma unroll
-    for (int i = 0; i < NTT_PER_THREAD; i++) {
-        const int L = threadIdx.x + i * BLOCK;
-        r[i] = L < n_el ? s[L] : make_uint4(0, 0, 0, 0);
-    }
+    for (int i = 0; i < NTT_PER_THREAD; i++) r[i] = s[(threadIdx.x + i * BLOCK) & (t.n_el - 1)];
 #pragma unroll
     for (int i = 0; i < NTT_PER_THREAD; i++) {
         const int L = threadIdx.x + i * BLOCK;
-        if (L < n_el) a.data[base | ((uint64_t)(L >> a.c) << lb) | (uint64_t)(L & cmask)] = r[i];
+        if (L < t.n_el) a.data[t.base | ((uint64_t)(L >> a.c) << t.lb) | (uint64_t)(L & cmask)] = r[i];
     }
 }
 
@@ -169,7 +233,7 @@ __device__ __forceinline__ void ntt_tile_twiddles(const NttPassArgs &a, uint64_t
         }
         thi[j] = acc;
     }
-    __syncthreads();
+    lds_barrier();
     for (int t = threadIdx.x; t < (1 << k) - 1; t += BLOCK) {
         const int j = 31 - __clz(t + 1);  // t + 1 in [2^j, 2^(j+1))
         const int loc = t + 1 - (1 << j);
@@ -183,7 +247,7 @@ __device__ __forceinline__ void ntt_tile_twiddles(const NttPassArgs &a, uint64_t
         for (int i = 0; i < NTT_MAX_K - 1; i++) acc = gxor(acc, w[i]);
         tw[t] = acc;
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 template <bool INVERSE>
@@ -192,16 +256,11 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass(NttPassArgs a) {
     __shared__ uint4 tw[NTT_TW];
     __shared__ uint4 thi[NTT_MAX_K];
     const int k = a.k, c = a.c;
-    const int lb = a.log_d - a.layer0 - k;
-    const int mid_bits = lb - c;
-    const uint64_t tile = blockIdx.x;
-    const uint64_t mid = tile & ((1ull << mid_bits) - 1);
-    const uint64_t hi = tile >> mid_bits;
-    const uint64_t base = (hi << (lb + k)) | (mid << c);
-    const int n_el = 1 << (k + c);
+    const NttTile T = ntt_tile_at(a, blockIdx.x);
+    const int n_el = T.n_el;
 
-    ntt_load_tile(s, a, base, mid, lb, n_el);
-    ntt_tile_twiddles(a, hi, tw, thi);
+    ntt_load_tile(s, a, T);
+    ntt_tile_twiddles(a, T.hi, tw, thi);
 
     for (int jj = 0; jj < k; jj++) {
         const int j = INVERSE ? (k - 1 - jj) : jj;
@@ -223,10 +282,10 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass(NttPassArgs a) {
             s[L0] = u;
             s[L1] = v;
         }
-        __syncthreads();
+        lds_barrier();
     }
 
-    ntt_store_tile(s, a, base, lb, n_el);
+    ntt_store_tile(s, a, T);
 }
 
 // Same pass structure, but each butterfly multiplies by its block's twiddle
@@ -243,21 +302,15 @@ struct alignas(256) NttTabSmem {
     uint4 thi[NTT_MAX_K];
 };
 
-template <bool INVERSE, int G>
-__device__ __forceinline__ void ntt_tab_tile(const NttPassArgs &a, uint64_t tile, NttTabSmem<G> &sm) {
+// The k layers of a pass on a tile resident in sm.s, twiddles in sm.tw.
+// HYBRID = false compiles the table path alone (the launcher picks it when every
+// layer of the pass qualifies), which keeps the general multiply's registers out
+// of the kernel.
+template <bool INVERSE, int G, bool HYBRID>
+__device__ __forceinline__ void ntt_tab_layers(const NttPassArgs &a, int n_el, NttTabSmem<G> &sm) {
     uint4 *__restrict__ s = sm.s;
     uint4 *__restrict__ tabs = sm.tabs;
     const int k = a.k, c = a.c;
-    const int lb = a.log_d - a.layer0 - k;
-    const int mid_bits = lb - c;
-    const uint64_t mid = tile & ((1ull << mid_bits) - 1);
-    const uint64_t hi = tile >> mid_bits;
-    const uint64_t base = (hi << (lb + k)) | (mid << c);
-    const int n_el = 1 << (k + c);
-
-    ntt_load_tile(s, a, base, mid, lb, n_el);
-    ntt_tile_twiddles(a, hi, sm.tw, sm.thi);
-
     for (int jj = 0; jj < k; jj++) {
         const int j = INVERSE ? (k - 1 - jj) : jj;
         const uint4 *__restrict__ tw = sm.tw + (1 << j) - 1;
@@ -265,7 +318,7 @@ __device__ __forceinline__ void ntt_tab_tile(const NttPassArgs &a, uint64_t tile
         const int per_blk = 1 << pos;
         const int lowmask = per_blk - 1;
         const int nblk = 1 << j;
-        if (per_blk < a.tab_min) {
+        if (HYBRID && per_blk < a.tab_min) {
             // few butterflies per twiddle: a table would not pay for itself
             for (int P = threadIdx.x; P < (n_el >> 1); P += BLOCK) {
                 const int L0 = ((P >> pos) << (pos + 1)) | (P & lowmask);
@@ -282,14 +335,14 @@ __device__ __forceinline__ void ntt_tab_tile(const NttPassArgs &a, uint64_t tile
                 s[L0] = u;
                 s[L1] = v;
             }
-            __syncthreads();
+            lds_barrier();
             continue;
         }
         for (int g0 = 0; g0 < nblk; g0 += G) {
             const int ng = nblk - g0 < G ? nblk - g0 : G;
             for (int r = threadIdx.x; r < WIN_ROWS * ng; r += BLOCK)
                 win_build_row(tabs + (r >> 5) * WIN_ENTRIES, tw[g0 + (r >> 5)], r & 31);
-            __syncthreads();
+            lds_barrier();
             const int n_bf = ng * per_blk;
             for (int q = threadIdx.x; q < n_bf; q += BLOCK) {
                 const int P = (g0 << pos) + q;
@@ -307,26 +360,43 @@ __device__ __forceinline__ void ntt_tab_tile(const NttPassArgs &a, uint64_t tile
                 s[L0] = u;
                 s[L1] = v;
             }
-            __syncthreads();
+            lds_barrier();
         }
     }
 
-    ntt_store_tile(s, a, base, lb, n_el);
 }
 
-template <bool INVERSE, int G>
+template <bool INVERSE, int G, bool HYBRID>
+__device__ __forceinline__ void ntt_tab_tile(const NttPassArgs &a, uint64_t tile, NttTabSmem<G> &sm) {
+    const NttTile T = ntt_tile_at(a, tile);
+    ntt_load_tile(sm.s, a, T);
+    ntt_tile_twiddles(a, T.hi, sm.tw, sm.thi);
+    ntt_tab_layers<INVERSE, G, HYBRID>(a, T.n_el, sm);
+    ntt_store_tile(sm.s, a, T);
+}
+
+template <bool INVERSE, int G, bool HYBRID>
 __global__ __launch_bounds__(BLOCK) void k_ntt_pass_tab(NttPassArgs a) {
     __shared__ NttTabSmem<G> sm;
-    ntt_tab_tile<INVERSE, G>(a, blockIdx.x, sm);
+    ntt_tab_tile<INVERSE, G, HYBRID>(a, blockIdx.x, sm);
 }
 
-template <bool INVERSE>
-static void launch_ntt_tab(const NttPassArgs &a, unsigned blocks, hipStream_t s) {
-    switch (g_ntt_tuning.tab_group) {
-        case 8: k_ntt_pass_tab<INVERSE, 8><<<blocks, BLOCK, 0, s>>>(a); break;
-        case 4: k_ntt_pass_tab<INVERSE, 4><<<blocks, BLOCK, 0, s>>>(a); break;
-        default: k_ntt_pass_tab<INVERSE, 2><<<blocks, BLOCK, 0, s>>>(a); break;
+template <bool INVERSE, bool HYBRID>
+static void launch_ntt_tab_h(const NttPassArgs &a, uint32_t n_tiles, int tab_group, hipStream_t s) {
+    switch (tab_group) {
+        case 8: k_ntt_pass_tab<INVERSE, 8, HYBRID><<<n_tiles, BLOCK, 0, s>>>(a); break;
+        case 4: k_ntt_pass_tab<INVERSE, 4, HYBRID><<<n_tiles, BLOCK, 0, s>>>(a); break;
+        default: k_ntt_pass_tab<INVERSE, 2, HYBRID><<<n_tiles, BLOCK, 0, s>>>(a); break;
     }
+}
+
+// The deepest layer of a pass has the fewest butterflies per twiddle, 2^c.
+template <bool INVERSE>
+static void launch_ntt_tab(const NttPassArgs &a, uint32_t n_tiles, int tab_group, hipStream_t s) {
+    if ((1 << a.c) >= a.tab_min)
+        launch_ntt_tab_h<INVERSE, false>(a, n_tiles, tab_group, s);
+    else
+        launch_ntt_tab_h<INVERSE, true>(a, n_tiles, tab_group, s);
 }
 
 
@@ -334,9 +404,10 @@ int ntt_pass_plan(int log_d, int skip_early, int skip_late, int *layer0, int *kk
     const int first = skip_early, end = log_d - skip_late;
     const int n_layers = end - first;
     if (n_layers <= 0) return 0;
-    int max_k = g_ntt_tuning.max_k;
+    const NttTuning &tu = ntt_tuning_for(log_d);
+    int max_k = tu.max_k;
     if (max_k < 1 || max_k > NTT_MAX_K) max_k = NTT_MAX_K;
-    int tile_log = g_ntt_tuning.tile_log;
+    int tile_log = tu.tile_log;
     if (tile_log < max_k || tile_log > NTT_TILE_LOG) tile_log = NTT_TILE_LOG;
     const int n_passes = (n_layers + max_k - 1) / max_k;
     // balanced split of the layer range into passes
@@ -358,6 +429,7 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
     // message mode needs the first pass to start at layer lir = log_d - msg_log
     if (msg && (inverse || skip_early != log_d - msg_log)) return hipErrorInvalidValue;
     int starts[64], ks[64], cs[64];
+    const NttTuning &tu = ntt_tuning_for(log_d);
     const int n_passes = ntt_pass_plan(log_d, skip_early, skip_late, starts, ks, cs);
     for (int q = 0; q < n_passes; q++) {
         const int p = inverse ? (n_passes - 1 - q) : q;
@@ -370,13 +442,13 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
         a.layer0 = starts[p];
         a.k = ks[p];
         a.c = cs[p];
-        a.tab_min = g_ntt_tuning.tab_min;
+        a.tab_min = tu.tab_min;
         const uint64_t n_tiles = 1ull << (log_d - a.k - a.c);
-        if (g_ntt_tuning.variant == 1) {
+        if (tu.variant == 1) {
             if (inverse)
-                launch_ntt_tab<true>(a, (unsigned)n_tiles, s);
+                launch_ntt_tab<true>(a, (unsigned)n_tiles, tu.tab_group, s);
             else
-                launch_ntt_tab<false>(a, (unsigned)n_tiles, s);
+                launch_ntt_tab<false>(a, (unsigned)n_tiles, tu.tab_group, s);
         } else {
             if (inverse)
                 k_ntt_pass<true><<<(unsigned)n_tiles, BLOCK, 0, s>>>(a);
@@ -562,10 +634,23 @@ __device__ __forceinline__ void node_compress(uint4 l0, uint4 l1, uint4 r0, uint
     o1 = make_uint4(st[4], st[5], st[6], st[7]);
 }
 
+// Block b reduces nodes [b*chunk, (b+1)*chunk) of the input level by
+// a.levels levels, writing every intermediate level.
 __device__ __forceinline__ void subtree_core(const SubtreeArgs &a, uint64_t b, uint4 (*dig)[2 << MERKLE_RED_LOG]) {
-    const int blk_nodes = 1 << a.levels;
+    const int blk_nodes = (int)(a.n_in < (1ull << MERKLE_RED_LOG) ? a.n_in : (1ull << MERKLE_RED_LOG));
     const uint4 *src = a.tree + 2 * (a.in_off + b * (uint64_t)blk_nodes);
-    for (int t = threadIdx.x; t < 2 * blk_nodes; t += BLOCK) dig[0][t] = src[t];
+    constexpr int PER = (2 << MERKLE_RED_LOG) / BLOCK;
+    uint4 r[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int t = threadIdx.x + i * BLOCK;
+        r[i] = t < 2 * blk_nodes ? src[t] : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int t = threadIdx.x + i * BLOCK;
+        if (t < 2 * blk_nodes) dig[0][t] = r[i];
+    }
     __syncthreads();
     uint64_t off = a.in_off, n = a.n_in;
     int cur = 0;
@@ -607,7 +692,8 @@ int merkle_reduce_steps(uint64_t n_leaves, uint64_t *offs, uint64_t *ns, int *le
     while (n > 1) {
         int log_n = 0;
         while ((1ull << log_n) < n) log_n++;
-        const int levels = log_n < MERKLE_RED_LOG ? log_n : MERKLE_RED_LOG;
+        const int levels = log_n > MERKLE_WIDE_LOG ? MERKLE_WIDE_LEVELS
+                           : (log_n < MERKLE_RED_LOG ? log_n : MERKLE_RED_LOG);
         if (offs) {
             offs[steps] = off;
             ns[steps] = n;
@@ -633,7 +719,7 @@ hipError_t launch_merkle_levels(uint4 *tree, uint64_t n_leaves, const CompressIV
         a.n_in = ns[i];
         a.levels = lv[i];
         a.iv = iv;
-        k_merkle_subtree<<<(unsigned)(ns[i] >> lv[i]), BLOCK, 0, s>>>(a);
+        k_merkle_subtree<<<(unsigned)(ns[i] / merkle_chunk(ns[i])), BLOCK, 0, s>>>(a);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -765,7 +851,7 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass_tab_multi(const NttPassArgs 
     __shared__ NttTabSmem<G> sm;
     const int j = job_of_block2(start, n_jobs, blockIdx.x);
     const NttPassArgs a = jobs[j];
-    ntt_tab_tile<false, G>(a, blockIdx.x - start[j], sm);
+    ntt_tab_tile<false, G, true>(a, blockIdx.x - start[j], sm);
 }
 
 // ---------------------------------------------------------------------------

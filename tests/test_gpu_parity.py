@@ -84,6 +84,7 @@ def test_ntt_forward_windows(b64, orc, log_d):
 def test_ntt_tuning_variants(b64, orc, variant, max_k, tile_log, tab_min, tab_group):
     """Every multiply variant and pass/tile shape gives the same transform."""
     old = {k: b64.config_get(k) for k in ("ntt.variant", "ntt.max_k", "ntt.tile_log", "ntt.tab_min", "ntt.tab_group")}
+    b64.config_set("ntt.clear_sizes", 0)  # per-size schedules would override the defaults set here
     try:
         b64.config_set("ntt.variant", variant)
         b64.config_set("ntt.max_k", max_k)
@@ -99,6 +100,27 @@ def test_ntt_tuning_variants(b64, orc, variant, max_k, tile_log, tab_min, tab_gr
     finally:
         for k, v in old.items():
             b64.config_set(k, v)
+        from binius64_amd._abi import _apply_tuning
+        _apply_tuning(b64.lib())
+
+
+def test_per_size_tuning_override(b64, orc):
+    """A per-size schedule applies to its transform size only, and any schedule
+    gives the oracle's transform."""
+    log_d = 14
+    dom, ref = b64.DomainContext.standard(log_d), orc.Domain(dim=log_d)
+    data = orc.random_elems(77, 1 << log_d)
+    want = orc.ntt_forward(ref, data, 1, 4)
+    try:
+        for cand in ((1, 3, 8, 16, 4), (0, 6, 11, 1, 2), (1, 7, 11, 1, 8)):
+            for key, v in zip(("ntt.variant", "ntt.max_k", "ntt.tile_log", "ntt.tab_min", "ntt.tab_group"), cand):
+                b64.config_set(f"{key}:{log_d}", v)
+            assert b64.config_get(f"ntt.max_k:{log_d}") == cand[1]
+            assert np.array_equal(ntt_dev(b64, dom, data, 1, 4), want), cand
+    finally:
+        from binius64_amd._abi import _apply_tuning
+        b64.config_set("ntt.clear_sizes", 0)
+        _apply_tuning(b64.lib())
 
 
 @pytest.mark.parametrize("basis", [[5, 7, 22, 95], [0x1234, 0x98765, 0xABCDEF, 0x1111111111, 0x3 << 100, 7, 9, 1 << 77]])
