@@ -42,8 +42,9 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--cpu-log-n", type=int, default=24, help="CPU baseline sample size")
-    ap.add_argument("--ntt", default=None, metavar="V,K,T,M,G",
-                    help="NTT tuning for this run: variant, max_k, tile_log, tab_min, tab_group")
+    ap.add_argument("--ntt", default=None, metavar="V,K,T,M,G[,W]",
+                    help="NTT schedule for every size in this run (replaces the autotuned table): "
+                         "variant, max_k, tile_log, tab_min, tab_group[, waves]")
     ap.add_argument("--counters", action="store_true",
                     help="hardware-counter workload: one encode, one forward NTT of the codeword and a "
                          "512 MiB device copy (the byte-counter calibration), then exit")
@@ -65,7 +66,8 @@ def main():
     if b64.device_count() == 0:
         raise SystemExit("no HIP device")
     if args.ntt:
-        for key, v in zip(("ntt.variant", "ntt.max_k", "ntt.tile_log", "ntt.tab_min", "ntt.tab_group"),
+        b64.config_set("ntt.clear_sizes", 0)
+        for key, v in zip(("ntt.variant", "ntt.max_k", "ntt.tile_log", "ntt.tab_min", "ntt.tab_group", "ntt.waves"),
                           args.ntt.split(",")):
             b64.config_set(key, int(v))
     dev = torch.device("cuda:0")
@@ -137,7 +139,8 @@ def main():
 
     scratch = cw.clone()
     n_layers = rs_log_len - lir  # layers lir .. log_cw - lbs
-    tuning = {k: b64.config_get(f"ntt.{k}:{log_cw}") for k in ("variant", "max_k", "tile_log", "tab_min", "tab_group")}
+    tuning = {k: b64.config_get(f"ntt.{k}:{log_cw}")
+              for k in ("variant", "max_k", "tile_log", "tab_min", "tab_group", "waves")}
     max_k = tuning["max_k"]
     n_passes = (n_layers + max_k - 1) // max_k  # the library's balanced pass split
     kernel = "k_ntt_pass_tab" if tuning["variant"] == 1 else "k_ntt_pass"

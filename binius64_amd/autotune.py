@@ -23,20 +23,21 @@ import time
 from pathlib import Path
 from typing import Dict, Iterable, List, Optional, Sequence, Tuple
 
-KNOBS = ("ntt.variant", "ntt.max_k", "ntt.tile_log", "ntt.tab_min", "ntt.tab_group")
+KNOBS = ("ntt.variant", "ntt.max_k", "ntt.tile_log", "ntt.tab_min", "ntt.tab_group", "ntt.waves")
 DEFAULT_PATH = Path(__file__).resolve().parent / "ntt_tuning.json"
 
-Candidate = Tuple[int, int, int, int, int]
+Candidate = Tuple[int, int, int, int, int, int]
 
 
 def default_candidates() -> List[Candidate]:
-    c: List[Candidate] = [(0, k, 11, 1, 2) for k in (5, 6, 7)]
+    c: List[Candidate] = [(0, k, 11, 1, 2, 0) for k in (5, 6, 7)]
     for k in (5, 6, 7):
         for tile in (10, 11):
             for tab_min in (32, 64, 128):
-                c.append((1, k, tile, tab_min, 2))
-        c.append((1, k, 11, 64, 4))
-    c += [(1, 4, 11, 32, 2), (1, 4, 11, 64, 2)]
+                c.append((1, k, tile, tab_min, 2, 0))
+            c.append((1, k, tile, 64, 2, 4))
+        c.append((1, k, 11, 64, 4, 0))
+    c += [(1, 4, 11, 32, 2, 0), (1, 4, 11, 64, 2, 0), (1, 4, 10, 32, 2, 4)]
     return c
 
 
@@ -53,7 +54,12 @@ def apply_table(table: Dict[int, Candidate], lib=None) -> None:
 
 def load_table(path: Path = DEFAULT_PATH) -> Dict[int, Candidate]:
     data = json.loads(Path(path).read_text())
-    return {int(k): tuple(int(x) for x in v["config"]) for k, v in data["sizes"].items()}
+    knobs = data.get("knobs", list(KNOBS))
+    out = {}
+    for k, v in data["sizes"].items():
+        cfg = dict(zip(knobs, v["config"]))
+        out[int(k)] = tuple(int(cfg.get(knob, 0)) for knob in KNOBS)
+    return out
 
 
 class _EncodeBench:

@@ -221,6 +221,9 @@ int b64_config_set(const char *key, int64_t value) {
     } else if (!strcmp(field, "ntt.tab_group")) {
         if (value != 2 && value != 4 && value != 8) return B64_ERR_INVALID_ARGS;
         t.tab_group = (int)value;
+    } else if (!strcmp(field, "ntt.waves")) {
+        if (value != 0 && value != 4) return B64_ERR_INVALID_ARGS;
+        t.waves = (int)value;
     } else {
         return B64_ERR_INVALID_ARGS;
     }
@@ -242,6 +245,7 @@ int b64_config_get(const char *key, int64_t *value) {
     else if (!strcmp(field, "ntt.tile_log")) *value = t.tile_log;
     else if (!strcmp(field, "ntt.tab_min")) *value = t.tab_min;
     else if (!strcmp(field, "ntt.tab_group")) *value = t.tab_group;
+    else if (!strcmp(field, "ntt.waves")) *value = t.waves;
     else return B64_ERR_INVALID_ARGS;
     return B64_OK;
 }

@@ -85,6 +85,7 @@ struct NttTuning {
     int tile_log;  // log2 elements per LDS tile
     int tab_min;   // variant 1: butterflies per twiddle (in a tile) needed to build a table
     int tab_group; // variant 1: window tables resident in LDS at once (2, 4 or 8; 8 KiB each)
+    int waves;     // tables-only passes: 4 = kernel limited to 128 VGPRs (4 waves per SIMD), 0 = default
 };
 extern NttTuning g_ntt_tuning;  // defaults
 // Per transform size (log_d) overrides; variant < 0 marks an unset entry.

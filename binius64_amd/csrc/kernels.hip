@@ -80,7 +80,7 @@ constexpr int NTT_MAX_K = 7;
 // passes (0 = general product, 1 = LDS window tables where they pay), the
 // largest number of layers per pass, the tile size, the butterflies per
 // twiddle that justify a table, and the tables resident in LDS at once.
-NttTuning g_ntt_tuning = {1, NTT_MAX_K, NTT_TILE_LOG, 128, 2};
+NttTuning g_ntt_tuning = {1, NTT_MAX_K, NTT_TILE_LOG, 128, 2, 0};
 NttTuning g_ntt_tuning_size[MAX_DIM + 1] = {};
 
 const NttTuning &ntt_tuning_for(int log_d) {
@@ -151,9 +151,10 @@ __device__ __forceinline__ void ntt_fetch_tile(NttTileRegs &R, const NttPassArgs
         const int cmask = (1 << c) - 1;
 #pragma unroll
         for (int i = 0; i < NTT_PER_THREAD; i++) {
-            // lanes past a small tile re-read a valid element (not stored)
+            // the test is uniform across the block; in tiles smaller than a
+            // block, lanes past the end re-read a valid element (not stored)
             const int L = (threadIdx.x + i * BLOCK) & (t.n_el - 1);
-            R.set(i, a.data[t.base | ((uint64_t)(L >> c) << t.lb) | (uint64_t)(L & cmask)]);
+            if (i * BLOCK < t.n_el) R.set(i, a.data[t.base | ((uint64_t)(L >> c) << t.lb) | (uint64_t)(L & cmask)]);
         }
     } else {
         const int mid_bits = t.lb - c;
@@ -162,7 +163,7 @@ __device__ __forceinline__ void ntt_fetch_tile(NttTileRegs &R, const NttPassArgs
 #pragma unroll
         for (int i = 0; i < NTT_PER_THREAD; i++) {
             const int q = (threadIdx.x + i * BLOCK) & (t.n_el - 1);
-            R.set(i, a.msg[mbase + (q & kmask) + ((uint64_t)(q >> k) << (a.m - c))]);
+            if (i * BLOCK < t.n_el) R.set(i, a.msg[mbase + (q & kmask) + ((uint64_t)(q >> k) << (a.m - c))]);
         }
     }
 }
@@ -197,13 +198,14 @@ __device__ __forceinline__ void ntt_load_tile(uint4 *__restrict__ s, const NttPa
 
 __device__ __forceinline__ void ntt_store_tile(const uint4 *__restrict__ s, const NttPassArgs &a, const NttTile &t) {
     const int cmask = (1 << a.c) - 1;
-    uint4 r[NTT_PER_THREAD];
+    NttTileRegs R;
 #pragma unroll
-    for (int i = 0; i < NTT_PER_THREAD; i++) r[i] = s[(threadIdx.x + i * BLOCK) & (t.n_el - 1)];
+    for (int i = 0; i < NTT_PER_THREAD; i++)
+        if (i * BLOCK < t.n_el) R.set(i, s[(threadIdx.x + i * BLOCK) & (t.n_el - 1)]);
 #pragma unroll
     for (int i = 0; i < NTT_PER_THREAD; i++) {
         const int L = threadIdx.x + i * BLOCK;
-        if (L < t.n_el) a.data[t.base | ((uint64_t)(L >> a.c) << t.lb) | (uint64_t)(L & cmask)] = r[i];
+        if (L < t.n_el) a.data[t.base | ((uint64_t)(L >> a.c) << t.lb) | (uint64_t)(L & cmask)] = R.get(i);
     }
 }
 
@@ -294,9 +296,9 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass(NttPassArgs a) {
 // per table row, all lanes of the block) and consumed by the butterflies of
 // those blocks.  Layers with fewer than tab_min butterflies per twiddle use the
 // general product instead.
-template <int G>
+template <int G, int TL = NTT_TILE_LOG>
 struct alignas(256) NttTabSmem {
-    uint4 s[1 << NTT_TILE_LOG];
+    uint4 s[1 << TL];
     uint4 tabs[G * WIN_ENTRIES];
     uint4 tw[NTT_TW];
     uint4 thi[NTT_MAX_K];
@@ -306,8 +308,8 @@ struct alignas(256) NttTabSmem {
 // HYBRID = false compiles the table path alone (the launcher picks it when every
 // layer of the pass qualifies), which keeps the general multiply's registers out
 // of the kernel.
-template <bool INVERSE, int G, bool HYBRID>
-__device__ __forceinline__ void ntt_tab_layers(const NttPassArgs &a, int n_el, NttTabSmem<G> &sm) {
+template <bool INVERSE, int G, bool HYBRID, class SM>
+__device__ __forceinline__ void ntt_tab_layers(const NttPassArgs &a, int n_el, SM &sm) {
     uint4 *__restrict__ s = sm.s;
     uint4 *__restrict__ tabs = sm.tabs;
     const int k = a.k, c = a.c;
@@ -366,8 +368,8 @@ __device__ __forceinline__ void ntt_tab_layers(const NttPassArgs &a, int n_el, N
 
 }
 
-template <bool INVERSE, int G, bool HYBRID>
-__device__ __forceinline__ void ntt_tab_tile(const NttPassArgs &a, uint64_t tile, NttTabSmem<G> &sm) {
+template <bool INVERSE, int G, bool HYBRID, class SM>
+__device__ __forceinline__ void ntt_tab_tile(const NttPassArgs &a, uint64_t tile, SM &sm) {
     const NttTile T = ntt_tile_at(a, tile);
     ntt_load_tile(sm.s, a, T);
     ntt_tile_twiddles(a, T.hi, sm.tw, sm.thi);
@@ -381,6 +383,16 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass_tab(NttPassArgs a) {
     ntt_tab_tile<INVERSE, G, HYBRID>(a, blockIdx.x, sm);
 }
 
+// Tables-only passes are latency bound on LDS lookups.  This form takes tiles of
+// at most 2^10 elements, so its LDS fits four blocks per CU, and asks for four
+// waves per SIMD (at most 128 VGPRs).
+constexpr int NTT_W4_TILE_LOG = 10;
+template <bool INVERSE, int G>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_ntt_pass_tab_w4(NttPassArgs a) {
+    __shared__ NttTabSmem<G, NTT_W4_TILE_LOG> sm;
+    ntt_tab_tile<INVERSE, G, false>(a, blockIdx.x, sm);
+}
+
 template <bool INVERSE, bool HYBRID>
 static void launch_ntt_tab_h(const NttPassArgs &a, uint32_t n_tiles, int tab_group, hipStream_t s) {
     switch (tab_group) {
@@ -390,13 +402,26 @@ static void launch_ntt_tab_h(const NttPassArgs &a, uint32_t n_tiles, int tab_gro
     }
 }
 
+template <bool INVERSE>
+static void launch_ntt_tab_w4(const NttPassArgs &a, uint32_t n_tiles, int tab_group, hipStream_t s) {
+    switch (tab_group) {
+        case 8: k_ntt_pass_tab_w4<INVERSE, 8><<<n_tiles, BLOCK, 0, s>>>(a); break;
+        case 4: k_ntt_pass_tab_w4<INVERSE, 4><<<n_tiles, BLOCK, 0, s>>>(a); break;
+        default: k_ntt_pass_tab_w4<INVERSE, 2><<<n_tiles, BLOCK, 0, s>>>(a); break;
+    }
+}
+
 // The deepest layer of a pass has the fewest butterflies per twiddle, 2^c.
 template <bool INVERSE>
-static void launch_ntt_tab(const NttPassArgs &a, uint32_t n_tiles, int tab_group, hipStream_t s) {
-    if ((1 << a.c) >= a.tab_min)
-        launch_ntt_tab_h<INVERSE, false>(a, n_tiles, tab_group, s);
-    else
-        launch_ntt_tab_h<INVERSE, true>(a, n_tiles, tab_group, s);
+static void launch_ntt_tab(const NttPassArgs &a, uint32_t n_tiles, const NttTuning &tu, hipStream_t s) {
+    if ((1 << a.c) >= a.tab_min) {
+        if (tu.waves == 4 && a.k + a.c <= NTT_W4_TILE_LOG)
+            launch_ntt_tab_w4<INVERSE>(a, n_tiles, tu.tab_group, s);
+        else
+            launch_ntt_tab_h<INVERSE, false>(a, n_tiles, tu.tab_group, s);
+    } else {
+        launch_ntt_tab_h<INVERSE, true>(a, n_tiles, tu.tab_group, s);
+    }
 }
 
 
@@ -446,9 +471,9 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
         const uint64_t n_tiles = 1ull << (log_d - a.k - a.c);
         if (tu.variant == 1) {
             if (inverse)
-                launch_ntt_tab<true>(a, (unsigned)n_tiles, tu.tab_group, s);
+                launch_ntt_tab<true>(a, (unsigned)n_tiles, tu, s);
             else
-                launch_ntt_tab<false>(a, (unsigned)n_tiles, tu.tab_group, s);
+                launch_ntt_tab<false>(a, (unsigned)n_tiles, tu, s);
         } else {
             if (inverse)
                 k_ntt_pass<true><<<(unsigned)n_tiles, BLOCK, 0, s>>>(a);

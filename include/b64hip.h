@@ -79,8 +79,11 @@ int b64_stream_synchronize(void *stream);
  * window tables where a twiddle covers at least "ntt.tab_min" butterflies of a
  * tile), "ntt.max_k" (layers per NTT pass, 1..7), "ntt.tile_log" (log2 elements
  * per LDS tile), "ntt.tab_group" (window tables resident in LDS at once: 2, 4
- * or 8).  Returns INVALID_ARGS for an unknown key or value.  Settings are
- * process-wide; change them only while no call is running. */
+ * or 8), "ntt.waves" (4 = tables-only passes use a kernel limited to 128 VGPRs,
+ * 0 = default).  "ntt.<knob>:<log_d>" sets the knob for transforms of size
+ * 2^log_d only (the autotuner's per-size table); "ntt.clear_sizes" drops every
+ * per-size setting.  Returns INVALID_ARGS for an unknown key or value.  Settings
+ * are process-wide; change them only while no call is running. */
 int b64_config_set(const char *key, int64_t value);
 int b64_config_get(const char *key, int64_t *value);
 

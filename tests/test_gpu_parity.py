@@ -77,13 +77,15 @@ def test_ntt_forward_windows(b64, orc, log_d):
         assert np.array_equal(ntt_dev(b64, dom, data, se, sl), orc.ntt_forward(ref, data, se, sl)), (se, sl)
 
 
-@pytest.mark.parametrize("variant,max_k,tile_log,tab_min,tab_group", [
-    (0, 7, 11, 1, 2), (1, 7, 11, 1, 2), (1, 7, 11, 128, 2), (1, 7, 11, 1 << 20, 2), (1, 3, 6, 4, 2),
-    (1, 1, 1, 1, 2), (1, 5, 9, 64, 2), (0, 2, 4, 1, 2), (1, 4, 11, 32, 2), (1, 7, 11, 1, 4), (1, 6, 11, 16, 8),
-    (1, 5, 10, 8, 8), (1, 3, 11, 1, 4)])
-def test_ntt_tuning_variants(b64, orc, variant, max_k, tile_log, tab_min, tab_group):
+@pytest.mark.parametrize("variant,max_k,tile_log,tab_min,tab_group,waves", [
+    (0, 7, 11, 1, 2, 0), (1, 7, 11, 1, 2, 0), (1, 7, 11, 128, 2, 0), (1, 7, 11, 1 << 20, 2, 0), (1, 3, 6, 4, 2, 0),
+    (1, 1, 1, 1, 2, 0), (1, 5, 9, 64, 2, 0), (0, 2, 4, 1, 2, 0), (1, 4, 11, 32, 2, 0), (1, 7, 11, 1, 4, 0),
+    (1, 6, 11, 16, 8, 0), (1, 5, 10, 8, 8, 0), (1, 3, 11, 1, 4, 0), (1, 5, 10, 16, 2, 4), (1, 4, 9, 1, 4, 4),
+    (1, 7, 11, 1, 2, 4), (1, 3, 10, 1, 8, 4)])
+def test_ntt_tuning_variants(b64, orc, variant, max_k, tile_log, tab_min, tab_group, waves):
     """Every multiply variant and pass/tile shape gives the same transform."""
-    old = {k: b64.config_get(k) for k in ("ntt.variant", "ntt.max_k", "ntt.tile_log", "ntt.tab_min", "ntt.tab_group")}
+    old = {k: b64.config_get(k) for k in ("ntt.variant", "ntt.max_k", "ntt.tile_log", "ntt.tab_min", "ntt.tab_group",
+                                          "ntt.waves")}
     b64.config_set("ntt.clear_sizes", 0)  # per-size schedules would override the defaults set here
     try:
         b64.config_set("ntt.variant", variant)
@@ -91,6 +93,7 @@ def test_ntt_tuning_variants(b64, orc, variant, max_k, tile_log, tab_min, tab_gr
         b64.config_set("ntt.tile_log", tile_log)
         b64.config_set("ntt.tab_min", tab_min)
         b64.config_set("ntt.tab_group", tab_group)
+        b64.config_set("ntt.waves", waves)
         for log_d, se, sl in ((13, 1, 4), (12, 0, 0), (9, 2, 1), (3, 0, 0)):
             dom, ref = b64.DomainContext.standard(log_d), orc.Domain(dim=log_d)
             data = orc.random_elems(7 * log_d, 1 << log_d)
