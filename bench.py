@@ -26,10 +26,13 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the NTT pass kernel (FETCH_SIZE + WRITE_SIZE from
-# rocprofv3 --pmc, scaled by the same counters on a copy of known size), keyed by
-# (kernel, log codeword length).  Filled from the committed counter summary.
-TRAFFIC = {}
+# Per launch of the forward-NTT pass kernel, from rocprofv3 --pmc runs of
+# `bench.py --counters` (profiles/ntt_pass_log_n24_counters.md): HBM bytes
+# (FETCH_SIZE + WRITE_SIZE, each scaled by the same counter on a 512 MiB device
+# copy) and VALU issue rate (SQ_INSTS_VALU per CU per clock).  Keyed by kernel,
+# log2 codeword length, layers per pass and tile size; the counters cannot be
+# read from inside the timed run, so other schedules report null.
+TRAFFIC = {("k_ntt_pass_tab", 25, 7, 11): {"bytes": 1073929374, "valu_issue_per_cu_clk": 0.91}}
 
 
 def parse():
@@ -158,7 +161,10 @@ def main():
     achieved = pass_bytes / (ntt_pass_ms * 1e-3) / 1e9
     muls = n_layers * (1 << (log_cw - 1))  # one multiply per butterfly
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": TRAFFIC.get((kernel, log_cw)),
+                "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": (TRAFFIC.get((kernel, log_cw, tuning["max_k"], tuning["tile_log"])) or {}).get("bytes"),
+                "valu_issue_per_cu_clk": (TRAFFIC.get((kernel, log_cw, tuning["max_k"], tuning["tile_log"]))
+                                          or {}).get("valu_issue_per_cu_clk"),
                 "kernel": kernel, "ntt_tuning": tuning, "launches_per_ntt": n_passes,
                 "avg_launch_ms": round(ntt_pass_ms, 4),
                 "algorithmic_bytes_per_launch": pass_bytes,
