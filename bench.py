@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-ring-switch", action="store_true", help="skip the ring-switch folds leg")
     ap.add_argument("--cpu-log-n", type=int, default=24, help="CPU baseline sample size")
     ap.add_argument("--ntt", default=None, metavar="V,K,T,M,G[,W]",
                     help="NTT schedule for every size in this run (replaces the autotuned table): "
@@ -174,6 +175,7 @@ def main():
 
     batch = None if args.no_batch else batch_bench(b64, torch, stream, sp)
     serve = None if args.no_serve else serve_bench(b64, torch, sp)
+    ring = None if args.no_ring_switch else ring_switch_bench(b64, torch, stream, sp, with_cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
@@ -203,6 +205,7 @@ def main():
         "cpu_baseline": cpu,
         "batch": batch,
         "serve": serve,
+        "ring_switch": ring,
     }
     print(json.dumps(out), flush=True)
 
@@ -371,6 +374,64 @@ def serve_bench(b64, torch, sp, n_jobs: int = 400, rate: float = 200.0, detail: 
             "saturated_speedup": round(t_plain / t_sat, 3),
             "batches_at_rate": st_poisson["batches"], "batches_total": st["batches"],
             "peak_pool_bytes": st["peak_pool_bytes"], "roots_identical": bool(same)}
+
+
+def ring_switch_bench(b64, torch, stream, sp, log_len: int = 24, reps: int = 10, with_cpu: bool = True):
+    """The ring-switching step of the PCS opening (prover/prover/src/pcs.rs:127-153) at
+    2^24 rows: fold_1b_rows_for_b128(packed witness, suffix tensor) and
+    fold_b128_elems_inplace(suffix tensor, 128-element eq tensor).  Throughput is
+    the reference bench's convention (benches/ring_switch.rs): 16 * 2^log_len bytes."""
+    import numpy as np
+
+    L = b64.lib()
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+    n = 1 << log_len
+    mat = torch.randint(-(2 ** 63), 2 ** 63 - 1, (2 * n,), dtype=torch.int64, device=dev, generator=g)
+    vec = torch.randint(-(2 ** 63), 2 ** 63 - 1, (2 * n,), dtype=torch.int64, device=dev, generator=g)
+    out = torch.empty((256,), dtype=torch.int64, device=dev)
+    w = np.random.default_rng(8).integers(0, 2 ** 63, size=(128, 2), dtype=np.uint64)
+    wp = w.ctypes.data_as(C.POINTER(C.c_uint64))
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    t1 = timed(lambda: b64.check(L.b64_ring_switch_fold_1b_rows(mat.data_ptr(), vec.data_ptr(), log_len,
+                                                                out.data_ptr(), sp), "fold_1b_rows"))
+    t2 = timed(lambda: b64.check(L.b64_ring_switch_fold_elems(vec.data_ptr(), log_len, wp, sp), "fold_elems"))
+    res = {"log_len": log_len,
+           "fold_1b_rows_ms": round(t1, 4), "fold_1b_rows_GBs": round((16 << log_len) / (t1 * 1e-3) / 1e9, 2),
+           "fold_1b_rows_hbm_frac": round((32 << log_len) / (t1 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "fold_elems_ms": round(t2, 4), "fold_elems_GBs": round((16 << log_len) / (t2 * 1e-3) / 1e9, 2),
+           "fold_elems_hbm_frac": round((32 << log_len) / (t2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    if with_cpu:
+        from oracle import oracle as orc
+        cores = min(16, os.cpu_count() or 1)
+        m_h = mat.cpu().numpy().view(np.uint64).reshape(-1, 2)
+        v_h = vec.cpu().numpy().view(np.uint64).reshape(-1, 2)
+        ts = []
+        for _ in range(3):
+            t = time.perf_counter()
+            orc.rs_fold_1b_rows(m_h, v_h, 0, threads=cores)
+            ts.append(time.perf_counter() - t)
+        ts2 = []
+        for _ in range(3):
+            t = time.perf_counter()
+            orc.rs_fold_elems(v_h, w, 0, threads=cores)
+            ts2.append(time.perf_counter() - t)
+        res["cpu_baseline"] = {"cores": cores, "kind": "port", "sample": f"same 2^{log_len} inputs, median of 3",
+                               "fold_1b_rows_GBs": round((16 << log_len) / sorted(ts)[1] / 1e9, 3),
+                               "fold_elems_GBs": round((16 << log_len) / sorted(ts2)[1] / 1e9, 3)}
+    return res
 
 
 def cpu_baseline(log_n: int):

@@ -28,7 +28,7 @@ __all__ = [
     "BinaryMerkleTreeProver", "FRIParams", "CommitOutput", "commit_interleaved", "setup_commit_params",
     "ProverTranscript", "FRIFoldProver", "gf_mul", "gf_inv", "gf_mul_device", "gf_mul_const_device",
     "pack_witness", "device_count", "config_set", "config_get", "LIB_PATH", "lib",
-    "CommitJob", "BatchPlan", "Server", "fri_prove",
+    "CommitJob", "BatchPlan", "Server", "fri_prove", "ring_switch_fold_1b_rows", "ring_switch_fold_elems",
 ]
 
 
@@ -571,6 +571,25 @@ class Server:
 
     def __del__(self):
         self.close()
+
+
+def ring_switch_fold_1b_rows(mat: DeviceBuffer, vec: DeviceBuffer, log_len: int,
+                             out: Optional[DeviceBuffer] = None, stream=None) -> DeviceBuffer:
+    """ring_switch::fold_1b_rows_for_b128 (prover/prover/src/ring_switch.rs:197-265):
+    128 elements, out[b] = sum_r vec[r] * bit_b(mat[r])."""
+    out = out or DeviceBuffer(16 * 128)
+    check(lib().b64_ring_switch_fold_1b_rows(mat.ptr, vec.ptr, log_len, out.ptr, _stream(stream)),
+          "ring_switch_fold_1b_rows")
+    return out
+
+
+def ring_switch_fold_elems(elems: DeviceBuffer, log_len: int, vec: np.ndarray, stream=None) -> DeviceBuffer:
+    """ring_switch::fold_b128_elems_inplace (ring_switch.rs:130-176), in place:
+    x <- sum_b bit_b(x) * vec[b] for 128 host elements vec."""
+    v = np.ascontiguousarray(_elems(vec))
+    assert v.shape[0] == 128
+    check(lib().b64_ring_switch_fold_elems(elems.ptr, log_len, _p64(v), _stream(stream)), "ring_switch_fold_elems")
+    return elems
 
 
 def fri_prove(params: FRIParams, domain: DomainContext, message: np.ndarray, challenges=None):

@@ -123,6 +123,8 @@ SIGNATURES = {
     "b64_fri_prover_finish_proof": (C.c_int, [vp, vp]),
     "b64_fri_prover_oracle": (C.c_int, [vp, C.c_uint32, C.POINTER(vp), u32p, C.POINTER(vp)]),
     "b64_pack_witness": (C.c_int, [vp, C.c_uint64, C.c_uint32, vp, vp]),
+    "b64_ring_switch_fold_1b_rows": (C.c_int, [vp, vp, C.c_uint32, vp, vp]),
+    "b64_ring_switch_fold_elems": (C.c_int, [vp, C.c_uint32, u64p, vp]),
 }
 
 _lib = None

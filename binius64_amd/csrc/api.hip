@@ -446,6 +446,35 @@ int b64_fri_fold_codeword(const b64_domain *dc, const void *d_cw, uint32_t log_l
     return B64_OK;
 }
 
+// ===========================================================================
+// Ring switch (prover/prover/src/ring_switch.rs)
+// ===========================================================================
+int b64_ring_switch_fold_1b_rows(const void *d_mat, const void *d_vec, uint32_t log_len, void *d_out,
+                                 void *stream) {
+    if (!d_mat || !d_vec || !d_out || log_len > 40) return B64_ERR_INVALID_ARGS;
+    if (!device_ok()) return B64_ERR_NO_DEVICE;
+    const uint64_t n = 1ull << log_len;
+    hipStream_t s = as_stream(stream);
+    // per-call partial sums from the stream-ordered pool, so concurrent calls on
+    // different streams never share them
+    void *partial = nullptr;
+    HIP_TRY(hipMallocAsync(&partial, (size_t)b64::rs_fold_1b_rows_blocks(n) * 128 * sizeof(uint4), s));
+    const hipError_t e = b64::launch_rs_fold_1b_rows((const uint4 *)d_mat, (const uint4 *)d_vec, n,
+                                                     (uint4 *)partial, (uint4 *)d_out, s);
+    HIP_TRY(hipFreeAsync(partial, s));
+    HIP_TRY(e);
+    return B64_OK;
+}
+
+int b64_ring_switch_fold_elems(void *d_elems, uint32_t log_len, const uint64_t *vec, void *stream) {
+    if (!d_elems || !vec || log_len > 40) return B64_ERR_INVALID_ARGS;
+    if (!device_ok()) return B64_ERR_NO_DEVICE;
+    b64::RsVec v;
+    for (int b = 0; b < 128; b++) v.v[b] = to_u4(ld128(vec + 2 * b));
+    HIP_TRY(b64::launch_rs_fold_elems((uint4 *)d_elems, 1ull << log_len, v, as_stream(stream)));
+    return B64_OK;
+}
+
 int b64_pack_witness(const void *d_words, uint64_t n_words, uint32_t log_elems, void *d_out, void *stream) {
     if (!d_words || !d_out || log_elems > 40 || n_words > (2ull << log_elems)) return B64_ERR_INVALID_ARGS;
     if (!device_ok()) return B64_ERR_NO_DEVICE;

@@ -117,6 +117,15 @@ int ntt_pass_plan(int log_d, int skip_early, int skip_late, int *layer0, int *k,
 // kind's args, d_start[j] the first block of job j, `blocks` the total.
 hipError_t launch_multi(MultiKind kind, const void *d_jobs, const uint32_t *d_start, int n_jobs, uint32_t blocks,
                         hipStream_t s);
+// 128 B128 elements passed by value to a kernel (2 KiB of arguments).
+struct RsVec {
+    uint4 v[128];
+};
+hipError_t launch_rs_fold_elems(uint4 *elems, uint64_t n, const RsVec &v, hipStream_t s);
+// Blocks (and 128-element partial sums) of a fold_1b_rows launch over n rows.
+int rs_fold_1b_rows_blocks(uint64_t n);
+hipError_t launch_rs_fold_1b_rows(const uint4 *mat, const uint4 *vec, uint64_t n, uint4 *partial, uint4 *out,
+                                  hipStream_t s);
 hipError_t launch_pack_witness(const uint64_t *words, uint64_t n_words, uint4 *out, uint64_t n_elems,
                                hipStream_t s);
 

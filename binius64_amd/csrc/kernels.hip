@@ -934,4 +934,148 @@ hipError_t launch_pack_witness(const uint64_t *words, uint64_t n_words, uint4 *o
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Ring switch (prover/prover/src/ring_switch.rs)
+// ---------------------------------------------------------------------------
+
+// fold_b128_elems_inplace (ring_switch.rs:130-176): x <- sum_b bit_b(x) * vec[b].
+// The map is F2-linear in x, so it is a window-table product: row k, entry n of
+// the table holds sum_i n_i * vec[4k + i] (same swizzled layout as the twiddle
+// tables), and win_mul_perm sums the 32 rows selected by x's nibbles.
+__global__ __launch_bounds__(BLOCK) void k_rs_fold_elems(uint4 *__restrict__ elems, uint64_t n, RsVec v) {
+    __shared__ __attribute__((aligned(256))) uint4 tab[WIN_ENTRIES];
+    for (int e = threadIdx.x; e < WIN_ENTRIES; e += BLOCK) {
+        const int k = e >> 4, nib = e & 15;
+        uint4 acc = make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if ((nib >> i) & 1) acc = gxor(acc, v.v[4 * k + i]);
+        tab[k * 16 + (nib ^ (k & 15))] = acc;
+    }
+    __syncthreads();
+    const uint32_t tb = lds_addr(tab);
+    for (uint64_t x = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; x < n; x += (uint64_t)gridDim.x * BLOCK)
+        elems[x] = win_mul_perm(tb, elems[x]);
+}
+
+hipError_t launch_rs_fold_elems(uint4 *elems, uint64_t n, const RsVec &v, hipStream_t s) {
+    const uint64_t blocks = (n + BLOCK - 1) / BLOCK;
+    k_rs_fold_elems<<<(unsigned)(blocks < 8192 ? blocks : 8192), BLOCK, 0, s>>>(elems, n, v);
+    return hipGetLastError();
+}
+
+// fold_1b_rows_for_b128 (ring_switch.rs:197-265): out[b] = sum_r vec[r] * bit_b(mat[r]).
+// Method of four Russians as in the reference: per chunk of 4 rows the 16 subset
+// sums of the vec rows are shared by all 128 output bits, and output bit b adds
+// the sum indexed by bit b of the 4 mat rows.  As in the reference, each chunk's
+// mat rows are first transposed in 4x4 bit blocks (square_transpose_const_size,
+// ring_switch.rs:267-320), so the index for bit b = 4q + j is nibble q of
+// transposed row j: one LDS read per chunk and bit.
+// A block takes batches of 256 rows (one mat and one vec row per lane; the next
+// batch's loads are issued before the current batch is processed and stay in
+// flight, since the block synchronises through LDS only).  Lanes transpose one
+// 32-bit word of one chunk and build 4 subset sums each; lane pairs share an
+// output bit and take alternate chunks.  Each block writes 128 partial sums;
+// k_rs_reduce XORs them.
+constexpr int RS_BATCH = BLOCK;
+
+// In-place 4x4 transpose of bit blocks within every nibble of four words:
+// afterwards bit 4q + r of word j is bit 4q + j of the original word r.
+__device__ __forceinline__ void transpose4_nibbles(uint32_t &e0, uint32_t &e1, uint32_t &e2, uint32_t &e3) {
+    uint32_t t;
+    t = ((e0 >> 1) ^ e1) & 0x55555555u;
+    e1 ^= t;
+    e0 ^= t << 1;
+    t = ((e2 >> 1) ^ e3) & 0x55555555u;
+    e3 ^= t;
+    e2 ^= t << 1;
+    t = ((e0 >> 2) ^ e2) & 0x33333333u;
+    e2 ^= t;
+    e0 ^= t << 2;
+    t = ((e1 >> 2) ^ e3) & 0x33333333u;
+    e3 ^= t;
+    e1 ^= t << 2;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rs_fold_1b_rows(const uint4 *__restrict__ mat,
+                                                           const uint4 *__restrict__ vec, uint64_t n,
+                                                           uint4 *__restrict__ partial) {
+    __shared__ uint4 m_rows[RS_BATCH], v_rows[RS_BATCH];
+    __shared__ uint4 sums[RS_BATCH / 4][16];
+    __shared__ uint32_t mt[RS_BATCH / 4][4][4];  // [chunk][transposed row j][word]
+    __shared__ uint4 other[128];
+    const int b = threadIdx.x & 127, half = threadIdx.x >> 7;
+    const int j = b & 3, q = b >> 2, qw = q >> 3, qs = 4 * (q & 7);
+    const uint32_t *m_words = (const uint32_t *)m_rows;
+    const uint64_t stride = (uint64_t)gridDim.x * RS_BATCH;
+    uint4 acc = make_uint4(0, 0, 0, 0);
+    uint64_t r0 = (uint64_t)blockIdx.x * RS_BATCH;
+    uint4 m_next = make_uint4(0, 0, 0, 0), v_next = make_uint4(0, 0, 0, 0);
+    if (r0 + threadIdx.x < n) {
+        m_next = mat[r0 + threadIdx.x];
+        v_next = vec[r0 + threadIdx.x];
+    }
+    for (; r0 < n; r0 += stride) {
+        m_rows[threadIdx.x] = m_next;  // rows past n are zero and add nothing
+        v_rows[threadIdx.x] = v_next;
+        lds_barrier();
+        const uint64_t rn = r0 + stride + threadIdx.x;
+        m_next = rn < n ? mat[rn] : make_uint4(0, 0, 0, 0);
+        v_next = rn < n ? vec[rn] : make_uint4(0, 0, 0, 0);
+        {
+            const int g = threadIdx.x >> 2, w = threadIdx.x & 3;
+            uint32_t e0 = m_words[4 * (4 * g + 0) + w], e1 = m_words[4 * (4 * g + 1) + w];
+            uint32_t e2 = m_words[4 * (4 * g + 2) + w], e3 = m_words[4 * (4 * g + 3) + w];
+            transpose4_nibbles(e0, e1, e2, e3);
+            mt[g][0][w] = e0;
+            mt[g][1][w] = e1;
+            mt[g][2][w] = e2;
+            mt[g][3][w] = e3;
+        }
+        // expand_subset_sums_array (verifier/math/src/span.rs:43-57) of each chunk
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int e = threadIdx.x + k * BLOCK, g = e >> 4, m = e & 15;
+            uint4 x = make_uint4(0, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if ((m >> i) & 1) x = gxor(x, v_rows[4 * g + i]);
+            sums[g][m] = x;
+        }
+        lds_barrier();
+#pragma unroll 8
+        for (int g = half; g < RS_BATCH / 4; g += 2) acc = gxor(acc, sums[g][(mt[g][j][qw] >> qs) & 15]);
+        lds_barrier();
+    }
+    // combine the two halves of each output bit
+    if (half) other[b] = acc;
+    __syncthreads();
+    if (!half) partial[(uint64_t)blockIdx.x * 128 + b] = gxor(acc, other[b]);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rs_reduce(const uint4 *__restrict__ partial, int n_blocks,
+                                                     uint4 *__restrict__ out) {
+    // thread (b, quarter): quarter q XORs blocks q, q + 2, ... of output b (two per b)
+    const int b = threadIdx.x & 127, q = threadIdx.x >> 7;
+    __shared__ uint4 other[128];
+    uint4 acc = make_uint4(0, 0, 0, 0);
+    for (int k = q; k < n_blocks; k += 2) acc = gxor(acc, partial[(uint64_t)k * 128 + b]);
+    if (q) other[b] = acc;
+    __syncthreads();
+    if (!q) out[b] = gxor(acc, other[b]);
+}
+
+int rs_fold_1b_rows_blocks(uint64_t n) {
+    const uint64_t batches = (n + RS_BATCH - 1) / RS_BATCH;
+    return (int)(batches < 2048 ? batches : 2048);
+}
+
+hipError_t launch_rs_fold_1b_rows(const uint4 *mat, const uint4 *vec, uint64_t n, uint4 *partial, uint4 *out,
+                                  hipStream_t s) {
+    const int blocks = rs_fold_1b_rows_blocks(n);
+    k_rs_fold_1b_rows<<<blocks, BLOCK, 0, s>>>(mat, vec, n, partial);
+    k_rs_reduce<<<1, BLOCK, 0, s>>>(partial, blocks, out);
+    return hipGetLastError();
+}
+
 }  // namespace b64

@@ -279,6 +279,15 @@ void b64_server_destroy(b64_server *srv);
 /* B128 element i = word[2i+1] << 64 | word[2i]; zero-padded to 2^log_elems. */
 int b64_pack_witness(const void *d_words, uint64_t n_words, uint32_t log_elems, void *d_out, void *stream);
 
+/* ---- ring switch (prover/prover/src/ring_switch.rs) --------------------------- */
+/* fold_1b_rows_for_b128 (ring_switch.rs:197-265): d_out[b] (128 elements) = sum over
+ * rows r < 2^log_len of d_vec[r] * bit b of d_mat[r], bit b of the LE u128. */
+int b64_ring_switch_fold_1b_rows(const void *d_mat, const void *d_vec, uint32_t log_len, void *d_out,
+                                 void *stream);
+/* fold_b128_elems_inplace (ring_switch.rs:130-176), in place on 2^log_len elements:
+ * x <- sum over b of bit b of x times vec[b]; vec is 128 host elements. */
+int b64_ring_switch_fold_elems(void *d_elems, uint32_t log_len, const uint64_t *vec, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

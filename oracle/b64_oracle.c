@@ -850,6 +850,119 @@ void bo_pack_witness(const uint64_t *words, size_t n_elems, uint64_t *out) {
 }
 
 /* ===================================================================== */
+/* Ring switch (prover/prover/src/ring_switch.rs)                          */
+/* ===================================================================== */
+
+/* expand_subset_sums_array (verifier/math/src/span.rs:43-57): out[m] = sum of
+ * elems[i] over the set bits i of m. */
+static void subset_sums(const u128 *elems, int n, u128 *out) {
+    out[0] = 0;
+    for (int i = 0; i < n; i++)
+        for (size_t m = 0; m < ((size_t)1 << i); m++) out[m | ((size_t)1 << i)] = out[m] ^ elems[i];
+}
+
+/* square_transpose_const_size::<_, 2, 4> (ring_switch.rs:267-320) on four
+ * elements viewed as 128 B1 scalars: within each nibble q, bit q*4+r of element j
+ * becomes bit q*4+j of element r. */
+static void transpose4_nibbles(u128 m[4]) {
+    u128 t[4] = {0, 0, 0, 0};
+    for (int j = 0; j < 4; j++)
+        for (int q = 0; q < 32; q++)
+            for (int r = 0; r < 4; r++)
+                if ((m[j] >> (4 * q + r)) & 1) t[r] |= (u128)1 << (4 * q + j);
+    for (int j = 0; j < 4; j++) m[j] = t[j];
+}
+
+typedef struct {
+    const u128 *mat, *vec;
+    size_t n, chunk;
+    u128 *partial; /* per task: 128 accumulators */
+    int variant;
+} rs1b_ctx;
+
+static void rs1b_task(void *p, size_t task) {
+    rs1b_ctx *c = (rs1b_ctx *)p;
+    u128 *acc = c->partial + 128 * task;
+    for (int b = 0; b < 128; b++) acc[b] = 0;
+    const size_t r0 = task * c->chunk, r1 = r0 + c->chunk < c->n ? r0 + c->chunk : c->n;
+    if (c->variant == 1) {
+        for (size_t r = r0; r < r1; r++)
+            for (int b = 0; b < 128; b++)
+                if ((c->mat[r] >> b) & 1) acc[b] ^= c->vec[r];
+        return;
+    }
+    /* ring_switch.rs:216-257: chunks of 4 rows, subset sums of the vec chunk,
+     * transpose the mat chunk, then per byte i of transposed element j the low and
+     * high nibbles index the sums for output bits 8i+j and 8i+4+j. */
+    for (size_t r = r0; r < r1; r += 4) {
+        u128 v4[4] = {0, 0, 0, 0}, m4[4] = {0, 0, 0, 0}, look[16];
+        for (int t = 0; t < 4 && r + t < r1; t++) {
+            v4[t] = c->vec[r + t];
+            m4[t] = c->mat[r + t];
+        }
+        subset_sums(v4, 4, look);
+        transpose4_nibbles(m4);
+        for (int j = 0; j < 4; j++)
+            for (int i = 0; i < 16; i++) {
+                const unsigned byte = (unsigned)(m4[j] >> (8 * i)) & 0xFF;
+                acc[(i << 3) | j] ^= look[byte & 0x0F];
+                acc[(i << 3) | 4 | j] ^= look[byte >> 4];
+            }
+    }
+}
+
+void bo_rs_fold_1b_rows(const uint64_t *mat, const uint64_t *vec, int log_len, uint64_t *out, int variant,
+                        int nthreads) {
+    const size_t n = (size_t)1 << log_len;
+    const size_t chunk = n < 4096 ? n : 4096;
+    const size_t tasks = (n + chunk - 1) / chunk;
+    u128 *partial = (u128 *)malloc(sizeof(u128) * 128 * tasks);
+    rs1b_ctx c = {(const u128 *)mat, (const u128 *)vec, n, chunk, partial, variant};
+    par_for(nthreads, tasks, rs1b_task, &c);
+    u128 *o = (u128 *)out;
+    for (int b = 0; b < 128; b++) {
+        u128 a = 0;
+        for (size_t t = 0; t < tasks; t++) a ^= partial[128 * t + b];
+        o[b] = a;
+    }
+    free(partial);
+}
+
+typedef struct {
+    u128 *elems;
+    size_t n, chunk;
+    const u128 *vec;
+    const u128 *tables; /* 16 x 256 */
+    int variant;
+} rsel_ctx;
+
+static void rsel_task(void *p, size_t task) {
+    rsel_ctx *c = (rsel_ctx *)p;
+    const size_t x0 = task * c->chunk, x1 = x0 + c->chunk < c->n ? x0 + c->chunk : c->n;
+    for (size_t x = x0; x < x1; x++) {
+        const u128 e = c->elems[x];
+        u128 acc = 0;
+        if (c->variant == 1) {
+            for (int b = 0; b < 128; b++)
+                if ((e >> b) & 1) acc ^= c->vec[b];
+        } else {
+            for (int i = 0; i < 16; i++) acc ^= c->tables[256 * i + ((unsigned)(e >> (8 * i)) & 0xFF)];
+        }
+        c->elems[x] = acc;
+    }
+}
+
+void bo_rs_fold_elems(uint64_t *elems, size_t n, const uint64_t *vec, int variant, int nthreads) {
+    const u128 *v = (const u128 *)vec;
+    u128 *tables = (u128 *)malloc(sizeof(u128) * 16 * 256);
+    for (int i = 0; i < 16; i++) subset_sums(v + 8 * i, 8, tables + 256 * i); /* ring_switch.rs:144-157 */
+    const size_t chunk = 1 << 14;
+    rsel_ctx c = {(u128 *)elems, n, chunk, v, tables, variant};
+    par_for(nthreads, (n + chunk - 1) / chunk, rsel_task, &c);
+    free(tables);
+}
+
+/* ===================================================================== */
 /* Transcript: HasherChallenger<Sha256> (fiat_shamir/hasher_challenger.rs) */
 /* + ProverTranscript tape (transcript.rs:205-420)                         */
 /* ===================================================================== */

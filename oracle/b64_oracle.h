@@ -76,6 +76,19 @@ int bo_fold_codeword(const bo_domain *d, const uint64_t *cw, int log_len, const 
 void bo_evaluate_mle(const uint64_t *evals, int log_n, const uint64_t *point, uint64_t out[2]);
 void bo_pack_witness(const uint64_t *words, size_t n_elems, uint64_t *out);
 
+/* ---- ring switch (prover/prover/src/ring_switch.rs) ------------------------ */
+/* fold_1b_rows_for_b128 (:197-265): out[b] = sum over rows r of vec[r] * bit_b(mat[r]),
+ * b = 0..127; mat and vec hold 2^log_len elements, out 128.  variant 0 = the
+ * reference's method of four Russians over 4-row chunks with a 4x4 bit transpose,
+ * variant 1 = the definition, bit by bit. */
+void bo_rs_fold_1b_rows(const uint64_t *mat, const uint64_t *vec, int log_len, uint64_t *out, int variant,
+                        int nthreads);
+/* fold_b128_elems_inplace (:130-176) / fold_elems_inplace (:53-113) for B128:
+ * elems[x] <- sum over b of bit_b(elems[x]) * vec[b] (vec: 128 elements).
+ * variant 0 = 8-bit subset-sum tables (fold_b128_elems_inplace), variant 1 = the
+ * inner product over B1 components (fold_elems_inplace's fallback). */
+void bo_rs_fold_elems(uint64_t *elems, size_t n, const uint64_t *vec, int variant, int nthreads);
+
 /* Full FRI-only prover flow (prover/prover/src/fri/tests.rs:58-89 with
  * FRIFoldProver, fri/fold.rs:117-244, and FRIQueryProver, fri/query.rs:51-134).
  * challenges_in == NULL: challenges are sampled from the transcript; otherwise

@@ -68,6 +68,8 @@ def lib():
         L.bo_fold_codeword.argtypes = [C.c_void_p, u64p, C.c_int, u64p, C.c_int, u64p, C.c_int]
         L.bo_evaluate_mle.argtypes = [u64p, C.c_int, u64p, u64p]
         L.bo_pack_witness.argtypes = [u64p, C.c_size_t, u64p]
+        L.bo_rs_fold_1b_rows.argtypes = [u64p, u64p, C.c_int, u64p, C.c_int, C.c_int]
+        L.bo_rs_fold_elems.argtypes = [u64p, C.c_size_t, u64p, C.c_int, C.c_int]
         L.bo_fri_prove.argtypes = [C.c_void_p, u64p, C.c_int, C.c_int, C.c_int, intp, C.c_int, C.c_int,
                                    u64p, u8p, C.c_size_t, C.POINTER(C.c_size_t), u8p, u64p, u64p, C.c_int]
         L.bo_transcript_new.restype = C.c_void_p
@@ -332,6 +334,26 @@ def pack_witness(words: np.ndarray) -> np.ndarray:
     out = np.zeros((w.shape[0] // 2, 2), dtype=np.uint64)
     lib().bo_pack_witness(_p64(w), w.shape[0] // 2, _p64(out))
     return out
+
+
+def rs_fold_1b_rows(mat: np.ndarray, vec: np.ndarray, variant: int = 0, threads: int | None = None) -> np.ndarray:
+    """ring_switch::fold_1b_rows_for_b128: 128 elements."""
+    m = np.ascontiguousarray(mat, dtype=np.uint64).reshape(-1, 2)
+    v = np.ascontiguousarray(vec, dtype=np.uint64).reshape(-1, 2)
+    assert m.shape == v.shape and (m.shape[0] & (m.shape[0] - 1)) == 0
+    out = np.zeros((128, 2), dtype=np.uint64)
+    lib().bo_rs_fold_1b_rows(_p64(m), _p64(v), m.shape[0].bit_length() - 1, _p64(out), variant,
+                             threads or nthreads_default())
+    return out
+
+
+def rs_fold_elems(elems: np.ndarray, vec: np.ndarray, variant: int = 0, threads: int | None = None) -> np.ndarray:
+    """ring_switch::fold_b128_elems_inplace (variant 0) / fold_elems_inplace (variant 1)."""
+    e = np.array(elems, dtype=np.uint64).reshape(-1, 2)
+    v = np.ascontiguousarray(vec, dtype=np.uint64).reshape(-1, 2)
+    assert v.shape[0] == 128
+    lib().bo_rs_fold_elems(_p64(e), e.shape[0], _p64(v), variant, threads or nthreads_default())
+    return e
 
 
 def fri_prove(dom: Domain, msg: np.ndarray, log_dim: int, log_inv_rate: int, log_batch: int,
