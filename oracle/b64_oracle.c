@@ -807,6 +807,13 @@ static u128 fold_chunk(const bo_domain *d, int log_len, size_t chunk_index, u128
     return vals[0];
 }
 
+void bo_fold_chunk(const bo_domain *d, int log_len, uint64_t chunk_index, const uint64_t *vals,
+                   const uint64_t *chal, int n_chal, uint64_t out[2]) {
+    u128 scratch[1 << 12];
+    memcpy(scratch, vals, ((size_t)1 << n_chal) * sizeof(u128));
+    st(out, fold_chunk(d, log_len, chunk_index, scratch, (const u128 *)chal, n_chal));
+}
+
 static void fold_cw_task(void *p, size_t g) {
     fold_ctx *c = (fold_ctx *)p;
     size_t cs = (size_t)1 << c->n_chal;
@@ -1108,102 +1115,159 @@ static void prove_coset_opening(bo_transcript *t, const committed *c, size_t cos
     bo_transcript_decommit(t, br, 32 * (size_t)(c->tree_log_len - depth));
 }
 
+/* FRIFoldProver state (prover fri/fold.rs:34-82): the committed codeword c0
+ * (borrowed), the folded oracles committed so far, unprocessed challenges. */
+typedef struct {
+    const bo_domain *d;
+    int log_dim, lir, lbs, n_ar, n_queries, nthreads;
+    int arities[64];
+    int rs_log_len, log_len, n_final, n_rounds;
+    committed c0;
+    committed rc[65];
+    int n_rc;
+    u128 unproc[64];
+    int n_unproc, curr_round, next_commit, has_next;
+} fri_st;
+
+/* FRIFoldProver::new (fold.rs:57-82); c0 is borrowed. */
+static int fri_init(fri_st *f, const bo_domain *d, int log_dim, int lir, int lbs, const int *arities, int n_ar,
+                    int n_queries, const committed *c0, int nthreads) {
+    memset(f, 0, sizeof(*f));
+    if (n_ar > 63) return 1;
+    int sum = 0;
+    for (int i = 0; i < n_ar; i++) sum += arities[i];
+    if (sum > log_dim) return 2;
+    f->d = d;
+    f->log_dim = log_dim;
+    f->lir = lir;
+    f->lbs = lbs;
+    f->n_ar = n_ar;
+    memcpy(f->arities, arities, sizeof(int) * (size_t)n_ar);
+    f->n_queries = n_queries;
+    f->nthreads = nthreads;
+    f->rs_log_len = log_dim + lir;
+    f->log_len = f->rs_log_len + lbs;
+    f->n_final = log_dim - sum;
+    f->n_rounds = log_dim + lbs;
+    f->c0 = *c0;
+    f->next_commit = lbs;
+    f->has_next = 1;
+    return 0;
+}
+
+static void fri_free(fri_st *f) {
+    for (int i = 0; i < f->n_rc; i++) {
+        free(f->rc[i].cw);
+        free(f->rc[i].tree);
+    }
+    f->n_rc = 0;
+}
+
+/* receive_challenge (fold.rs:107-110) */
+static void fri_receive(fri_st *f, u128 ch) {
+    f->unproc[f->n_unproc++] = ch;
+    f->curr_round++;
+}
+
+/* execute_fold_round (fold.rs:117-175): returns 1 with root_out set when the
+ * round commits a folded oracle, 0 when it does not, < 0 on error. */
+static int fri_execute(fri_st *f, uint8_t root_out[32]) {
+    if (!(f->has_next && f->next_commit == f->curr_round)) return 0;
+    committed *nc = &f->rc[f->n_rc];
+    if (f->n_rc == 0) {
+        nc->log_len = f->rs_log_len;
+        nc->cw = (u128 *)malloc(((size_t)1 << f->rs_log_len) * sizeof(u128));
+        bo_fold_interleaved((const uint64_t *)f->c0.cw, f->rs_log_len, f->lbs, (const uint64_t *)f->unproc,
+                            (uint64_t *)nc->cw, f->nthreads);
+    } else {
+        committed *prev = &f->rc[f->n_rc - 1];
+        nc->log_len = prev->log_len - f->n_unproc;
+        nc->cw = (u128 *)malloc(((size_t)1 << nc->log_len) * sizeof(u128));
+        bo_fold_codeword(f->d, (const uint64_t *)prev->cw, prev->log_len, (const uint64_t *)f->unproc,
+                         f->n_unproc, (uint64_t *)nc->cw, f->nthreads);
+    }
+    f->n_unproc = 0;
+    int has_arity = f->n_rc < f->n_ar;
+    int log_coset = has_arity ? f->arities[f->n_rc] : f->n_final;
+    if (commit_vec(nc->cw, nc->log_len, log_coset, nc, f->nthreads)) return -1;
+    if (has_arity) f->next_commit += f->arities[f->n_rc];
+    else f->has_next = 0;
+    f->n_rc++;
+    memcpy(root_out, tree_root(nc), 32);
+    return 1;
+}
+
+/* finish_proof (fold.rs:221-244): terminal codeword, optimal layers, queries. */
+static int fri_finish(fri_st *f, bo_transcript *t, uint64_t *terminal_out) {
+    if (f->curr_round != f->n_rounds) return 6; /* EarlyProverFinish */
+    const committed *term = f->n_rc ? &f->rc[f->n_rc - 1] : &f->c0;
+    size_t term_len = (size_t)1 << term->log_len;
+    write_scalars(t, term->cw, term_len);
+    if (terminal_out) memcpy(terminal_out, term->cw, term_len * 16);
+
+    /* vcs_optimal_layers (fri/query.rs:93-109) with
+       vcs_optimal_layers_depths_iter (verifier fri/common.rs:154-168) */
+    int cap = log2_ceil((size_t)f->n_queries);
+    int depths[65];
+    int log_n_cosets = f->log_len;
+    for (int i = 0; i <= f->n_ar; i++) {
+        int a = i == 0 ? f->lbs : f->arities[i - 1];
+        log_n_cosets -= a;
+        depths[i] = cap < log_n_cosets ? cap : log_n_cosets;
+    }
+    for (int i = 0; i <= f->n_ar; i++) {
+        const committed *cc = i == 0 ? &f->c0 : &f->rc[i - 1];
+        bo_transcript_decommit(t, tree_layer(cc, depths[i]), 32 * ((size_t)1 << depths[i]));
+    }
+    for (int q = 0; q < f->n_queries; q++) {
+        size_t index = bo_transcript_sample_bits(t, f->rs_log_len);
+        prove_coset_opening(t, &f->c0, index, f->lbs, depths[0]);
+        for (int i = 0; i < f->n_ar; i++) {
+            index >>= f->arities[i];
+            prove_coset_opening(t, &f->rc[i], index, f->arities[i], depths[i + 1]);
+        }
+    }
+    return 0;
+}
+
 int bo_fri_prove(const bo_domain *d, const uint64_t *msg, int log_dim, int lir, int lbs,
                  const int *arities, int n_ar, int n_queries, const uint64_t *chal_in, uint8_t *tape,
                  size_t tape_cap, size_t *tape_len, uint8_t *roots_out, uint64_t *chal_out,
                  uint64_t *terminal_out, int nthreads) {
-    int rs_log_len = log_dim + lir;
-    int log_len = rs_log_len + lbs;
-    int sum = 0;
-    for (int i = 0; i < n_ar; i++) sum += arities[i];
-    if (sum > log_dim) return 2;
-    int n_final = log_dim - sum;
-    int n_rounds = log_dim + lbs;
-    int n_commits = n_ar + 1;
-
+    int log_len = log_dim + lir + lbs;
     committed c0;
     c0.cw = (u128 *)malloc(((size_t)1 << log_len) * sizeof(u128));
     int rc = bo_encode_batch(d, msg, log_dim, lir, lbs, (uint64_t *)c0.cw, nthreads);
     if (rc) return rc;
     rc = commit_vec(c0.cw, log_len, lbs, &c0, nthreads);
     if (rc) return rc;
+    fri_st f;
+    rc = fri_init(&f, d, log_dim, lir, lbs, arities, n_ar, n_queries, &c0, nthreads);
+    if (rc) return rc;
 
     bo_transcript *t = bo_transcript_new();
     bo_transcript_message(t, tree_root(&c0), 32);
     memcpy(roots_out, tree_root(&c0), 32);
-
-    committed *rc_list = (committed *)calloc((size_t)n_commits, sizeof(committed));
-    int n_rc = 0;
-    u128 unproc[64];
-    int n_unproc = 0;
-    int curr_round = 0;
-    int next_commit = lbs, has_next = 1;
-
-    for (int step = 0; step <= n_rounds; step++) {
+    int n_roots = 1;
+    for (int step = 0; step <= f.n_rounds; step++) {
         if (step > 0) {
             u128 ch;
             if (chal_in) ch = ld(chal_in + 2 * (step - 1));
             else bo_transcript_sample(t, (uint8_t *)&ch, 16);
             st(chal_out + 2 * (step - 1), ch);
-            unproc[n_unproc++] = ch;
-            curr_round++;
+            fri_receive(&f, ch);
         }
-        /* execute_fold_round (prover fri/fold.rs:117-175) */
-        if (!(has_next && next_commit == curr_round)) continue;
-        committed *nc = &rc_list[n_rc];
-        if (n_rc == 0) {
-            nc->log_len = rs_log_len;
-            nc->cw = (u128 *)malloc(((size_t)1 << rs_log_len) * sizeof(u128));
-            bo_fold_interleaved((const uint64_t *)c0.cw, rs_log_len, lbs, (const uint64_t *)unproc,
-                                (uint64_t *)nc->cw, nthreads);
-        } else {
-            committed *prev = &rc_list[n_rc - 1];
-            nc->log_len = prev->log_len - n_unproc;
-            nc->cw = (u128 *)malloc(((size_t)1 << nc->log_len) * sizeof(u128));
-            bo_fold_codeword(d, (const uint64_t *)prev->cw, prev->log_len, (const uint64_t *)unproc,
-                             n_unproc, (uint64_t *)nc->cw, nthreads);
-        }
-        n_unproc = 0;
-        int has_arity = n_rc < n_ar;
-        int log_coset = has_arity ? arities[n_rc] : n_final;
-        rc = commit_vec(nc->cw, nc->log_len, log_coset, nc, nthreads);
-        if (rc) return rc;
-        if (has_arity) next_commit += arities[n_rc];
-        else has_next = 0;
-        bo_transcript_message(t, tree_root(nc), 32);
-        memcpy(roots_out + 32 * (size_t)(n_rc + 1), tree_root(nc), 32);
-        n_rc++;
-    }
-    if (curr_round != n_rounds || n_rc != n_commits) return 6; /* EarlyProverFinish */
-
-    /* finish_proof (prover fri/fold.rs:221-244) */
-    committed *term = &rc_list[n_rc - 1];
-    size_t term_len = (size_t)1 << term->log_len;
-    write_scalars(t, term->cw, term_len);
-    memcpy(terminal_out, term->cw, term_len * 16);
-
-    /* vcs_optimal_layers (fri/query.rs:93-109) with
-       vcs_optimal_layers_depths_iter (verifier fri/common.rs:154-168) */
-    int cap = log2_ceil((size_t)n_queries);
-    int depths[65];
-    int log_n_cosets = log_len;
-    for (int i = 0; i <= n_ar; i++) {
-        int a = i == 0 ? lbs : arities[i - 1];
-        log_n_cosets -= a;
-        depths[i] = cap < log_n_cosets ? cap : log_n_cosets;
-    }
-    for (int i = 0; i <= n_ar; i++) {
-        const committed *cc = i == 0 ? &c0 : &rc_list[i - 1];
-        bo_transcript_decommit(t, tree_layer(cc, depths[i]), 32 * ((size_t)1 << depths[i]));
-    }
-    for (int q = 0; q < n_queries; q++) {
-        size_t index = bo_transcript_sample_bits(t, rs_log_len);
-        prove_coset_opening(t, &c0, index, lbs, depths[0]);
-        for (int i = 0; i < n_ar; i++) {
-            index >>= arities[i];
-            prove_coset_opening(t, &rc_list[i], index, arities[i], depths[i + 1]);
+        uint8_t root[32];
+        int c = fri_execute(&f, root);
+        if (c < 0) return 5;
+        if (c) {
+            bo_transcript_message(t, root, 32);
+            memcpy(roots_out + 32 * (size_t)n_roots++, root, 32);
         }
     }
+    if (f.n_rc != n_ar + 1) return 6; /* EarlyProverFinish */
+    rc = fri_finish(&f, t, terminal_out);
+    if (rc) return rc;
 
     *tape_len = t->tape_len;
     int ret = 0;
@@ -1214,12 +1278,202 @@ int bo_fri_prove(const bo_domain *d, const uint64_t *msg, int log_dim, int lir, 
     bo_transcript_free(t);
     free(c0.cw);
     free(c0.tree);
-    for (int i = 0; i < n_rc; i++) {
-        free(rc_list[i].cw);
-        free(rc_list[i].tree);
-    }
-    free(rc_list);
+    fri_free(&f);
     return ret;
+}
+
+/* ===================================================================== */
+/* Bivariate product sumcheck, BaseFold and the one-bit PCS opening        */
+/* ===================================================================== */
+
+/* BivariateProductSumcheckProver (prover protocols/sumcheck/bivariate_product.rs):
+ * two multilinears folded in place, binding the highest variable first. */
+typedef struct {
+    u128 *a, *b;
+    int n_vars, nthreads;
+    int has_coeffs; /* RoundCoeffsOrSum: 1 = Coeffs, 0 = Sum */
+    u128 sum, coeffs[3];
+} bp_st;
+
+typedef struct {
+    const u128 *a, *b;
+    size_t half;
+    u128 *y1, *yinf; /* per task */
+    u128 r;
+    u128 *wa, *wb;
+} bp_ctx;
+
+#define BP_GRAIN 8192
+
+static void bp_eval_task(void *p, size_t g) {
+    bp_ctx *c = (bp_ctx *)p;
+    size_t lo = g * BP_GRAIN, hi = lo + BP_GRAIN;
+    if (hi > c->half) hi = c->half;
+    u128 y1 = 0, yinf = 0;
+    for (size_t i = lo; i < hi; i++) {
+        u128 a0 = c->a[i], a1 = c->a[c->half + i], b0 = c->b[i], b1 = c->b[c->half + i];
+        y1 ^= gf_mul(a1, b1);
+        yinf ^= gf_mul(a0 ^ a1, b0 ^ b1);
+    }
+    c->y1[g] = y1;
+    c->yinf[g] = yinf;
+}
+
+/* execute (bivariate_product.rs:43-82): F(1) and F(inf) summed over the low
+ * half, then RoundEvals2::interpolate (round_evals.rs:73-78) with
+ * calculate_round_coeffs_from_evals_2 (:126-140). */
+static int bp_execute(bp_st *s, u128 coeffs[3]) {
+    if (s->has_coeffs) return 1; /* ExpectedFold */
+    if (s->n_vars == 0) return 2;
+    size_t half = (size_t)1 << (s->n_vars - 1);
+    size_t n_tasks = (half + BP_GRAIN - 1) / BP_GRAIN;
+    u128 *y1 = (u128 *)calloc(n_tasks, sizeof(u128)), *yinf = (u128 *)calloc(n_tasks, sizeof(u128));
+    bp_ctx c = {s->a, s->b, half, y1, yinf, 0, NULL, NULL};
+    if (n_tasks == 1) bp_eval_task(&c, 0);
+    else par_for(s->nthreads, n_tasks, bp_eval_task, &c);
+    u128 s1 = 0, sinf = 0;
+    for (size_t g = 0; g < n_tasks; g++) {
+        s1 ^= y1[g];
+        sinf ^= yinf[g];
+    }
+    free(y1);
+    free(yinf);
+    u128 y0 = s->sum ^ s1;
+    s->coeffs[0] = y0;
+    s->coeffs[2] = sinf;
+    s->coeffs[1] = s1 ^ y0 ^ sinf;
+    s->has_coeffs = 1;
+    memcpy(coeffs, s->coeffs, sizeof(s->coeffs));
+    return 0;
+}
+
+static void bp_fold_task(void *p, size_t g) {
+    bp_ctx *c = (bp_ctx *)p;
+    size_t lo = g * BP_GRAIN, hi = lo + BP_GRAIN;
+    if (hi > c->half) hi = c->half;
+    for (size_t i = lo; i < hi; i++) {
+        c->wa[i] ^= gf_mul(c->r, c->wa[i] ^ c->wa[c->half + i]);
+        c->wb[i] ^= gf_mul(c->r, c->wb[i] ^ c->wb[c->half + i]);
+    }
+}
+
+/* evaluate_univariate (Horner) for RoundCoeffs::evaluate (verifier sumcheck/common.rs:22-24) */
+static u128 poly_eval(const u128 *c, int n, u128 x) {
+    u128 acc = 0;
+    for (int i = n - 1; i >= 0; i--) acc = gf_mul(acc, x) ^ c[i];
+    return acc;
+}
+
+/* fold (bivariate_product.rs:84-98) with fold_highest_var_inplace
+ * (multilinear/fold.rs:14-29): zero += r * (one - zero). */
+static int bp_fold(bp_st *s, u128 r) {
+    if (!s->has_coeffs) return 1; /* ExpectedExecute */
+    size_t half = (size_t)1 << (s->n_vars - 1);
+    bp_ctx c = {NULL, NULL, half, NULL, NULL, r, s->a, s->b};
+    size_t n_tasks = (half + BP_GRAIN - 1) / BP_GRAIN;
+    if (n_tasks == 1) bp_fold_task(&c, 0);
+    else par_for(s->nthreads, n_tasks, bp_fold_task, &c);
+    s->n_vars--;
+    s->sum = poly_eval(s->coeffs, 3, r);
+    s->has_coeffs = 0;
+    return 0;
+}
+
+/* prove_single (prover protocols/sumcheck/prove.rs:17-60) over a bivariate
+ * product prover; a and b are folded in place (the prover owns them). */
+int bo_sumcheck_prove(uint64_t *a, uint64_t *b, int n_vars, const uint64_t sum[2], bo_transcript *t,
+                      uint64_t *challenges_out, uint64_t evals_out[4], int nthreads) {
+    bp_st s = {(u128 *)a, (u128 *)b, n_vars, nthreads, 0, ld(sum), {0, 0, 0}};
+    for (int i = 0; i < n_vars; i++) {
+        u128 co[3];
+        if (bp_execute(&s, co)) return 1;
+        bo_transcript_message(t, (const uint8_t *)co, 32); /* truncate(): c0, c1 */
+        u128 r;
+        bo_transcript_sample(t, (uint8_t *)&r, 16);
+        if (challenges_out) st(challenges_out + 2 * i, r);
+        if (bp_fold(&s, r)) return 1;
+    }
+    st(evals_out, s.a[0]);
+    st(evals_out + 2, s.b[0]);
+    return 0;
+}
+
+/* BaseFoldProver::prove (prover protocols/basefold.rs:131-151) and finish
+ * (:158-166).  multilinear / transparent (2^(log_dim+lbs) each) are folded in
+ * place; codeword / tree are the committed interleaved codeword and its tree. */
+int bo_basefold_prove(const bo_domain *d, uint64_t *multilinear, uint64_t *transparent, const uint64_t claim[2],
+                      const uint64_t *codeword, const uint8_t *tree, int log_dim, int lir, int lbs,
+                      const int *arities, int n_ar, int n_queries, bo_transcript *t, uint64_t *challenges_out,
+                      int nthreads) {
+    committed c0 = {(u128 *)codeword, log_dim + lir + lbs, (uint8_t *)tree, log_dim + lir};
+    fri_st f;
+    int rc = fri_init(&f, d, log_dim, lir, lbs, arities, n_ar, n_queries, &c0, nthreads);
+    if (rc) return rc;
+    int n_vars = log_dim + lbs;
+    bp_st s = {(u128 *)multilinear, (u128 *)transparent, n_vars, nthreads, 0, ld(claim), {0, 0, 0}};
+    for (int i = 0; i < n_vars; i++) {
+        /* execute (:99-113): sumcheck round, then FRI fold round */
+        u128 co[3];
+        if (bp_execute(&s, co)) return 1;
+        uint8_t root[32];
+        int c = fri_execute(&f, root);
+        if (c < 0) return 5;
+        bo_transcript_message(t, (const uint8_t *)co, 32);
+        if (c) bo_transcript_message(t, root, 32);
+        u128 r;
+        bo_transcript_sample(t, (uint8_t *)&r, 16);
+        if (challenges_out) st(challenges_out + 2 * i, r);
+        /* fold (:119-123) */
+        if (bp_fold(&s, r)) return 1;
+        fri_receive(&f, r);
+    }
+    uint8_t root[32];
+    int c = fri_execute(&f, root);
+    if (c < 0) return 5;
+    if (c) bo_transcript_message(t, root, 32);
+    rc = fri_finish(&f, t, NULL);
+    fri_free(&f);
+    return rc;
+}
+
+/* TensorAlgebra::transpose for B1 x B128 (verifier/math/src/tensor_algebra.rs:137-141 ->
+ * square_transforms_extension_field, verifier/field/src/transpose.rs:30-75):
+ * the 128 x 128 bit matrix of the 128 elements transposed, out[i] bit j = in[j] bit i. */
+static void transpose128(const u128 *in, u128 *out) {
+    for (int i = 0; i < 128; i++) {
+        u128 v = 0;
+        for (int j = 0; j < 128; j++) v |= ((in[j] >> i) & 1) << j;
+        out[i] = v;
+    }
+}
+void bo_tensor_transpose(const uint64_t *in, uint64_t *out) { transpose128((const u128 *)in, (u128 *)out); }
+
+/* OneBitPCSProver::prove (prover/prover/src/pcs.rs:107-170): ring-switch
+ * reduction to a B128 sumcheck claim, then BaseFold.  packed (2^log_len
+ * elements, log_len = log_dim + lbs) is consumed; eval_point has log_len + 7
+ * coordinates. */
+int bo_pcs_prove(const bo_domain *d, uint64_t *packed, const uint64_t *eval_point, const uint64_t *codeword,
+                 const uint8_t *tree, int log_dim, int lir, int lbs, const int *arities, int n_ar, int n_queries,
+                 bo_transcript *t, int nthreads) {
+    int log_len = log_dim + lbs;
+    size_t n = (size_t)1 << log_len;
+    u128 *suffix = (u128 *)malloc(n * sizeof(u128));
+    eq_ind((const u128 *)eval_point + 7, log_len, suffix);
+    u128 s_hat_v[128], s_hat_u[128], rpp[7], eq_rpp[128];
+    bo_rs_fold_1b_rows(packed, (const uint64_t *)suffix, log_len, (uint64_t *)s_hat_v, 0, nthreads);
+    bo_transcript_message(t, (const uint8_t *)s_hat_v, sizeof(s_hat_v));
+    transpose128(s_hat_v, s_hat_u);
+    bo_transcript_sample(t, (uint8_t *)rpp, sizeof(rpp));
+    eq_ind(rpp, 7, eq_rpp);
+    u128 claim = 0;
+    for (int i = 0; i < 128; i++) claim ^= gf_mul(s_hat_u[i], eq_rpp[i]);
+    bo_rs_fold_elems((uint64_t *)suffix, n, (const uint64_t *)eq_rpp, 0, nthreads);
+    uint64_t cl[2];
+    st(cl, claim);
+    int rc = bo_basefold_prove(d, packed, (uint64_t *)suffix, cl, codeword, tree, log_dim, lir, lbs, arities, n_ar,
+                               n_queries, t, NULL, nthreads);
+    free(suffix);
+    return rc;
 }
 
 /* ===================================================================== */

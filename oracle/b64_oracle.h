@@ -73,6 +73,10 @@ int bo_fold_interleaved(const uint64_t *cw, int log_len, int log_batch, const ui
                         uint64_t *out, int nthreads);
 int bo_fold_codeword(const bo_domain *d, const uint64_t *cw, int log_len, const uint64_t *chal,
                      int n_chal, uint64_t *out, int nthreads);
+/* fold_chunk (verifier fri/fold.rs:61-99): one coset of 2^n_chal values of a
+ * codeword of length 2^log_len folded to a single value. */
+void bo_fold_chunk(const bo_domain *d, int log_len, uint64_t chunk_index, const uint64_t *vals,
+                   const uint64_t *chal, int n_chal, uint64_t out[2]);
 void bo_evaluate_mle(const uint64_t *evals, int log_n, const uint64_t *point, uint64_t out[2]);
 void bo_pack_witness(const uint64_t *words, size_t n_elems, uint64_t *out);
 
@@ -103,6 +107,30 @@ int bo_fri_prove(const bo_domain *d, const uint64_t *msg, int log_dim, int log_i
 
 /* ---- transcript primitives (verifier/transcript/src) --------------------- */
 typedef struct bo_transcript bo_transcript;
+
+/* ---- sumcheck, BaseFold, one-bit PCS opening ------------------------------ */
+/* prove_single (prover protocols/sumcheck/prove.rs:17-60) over
+ * BivariateProductSumcheckProver (sumcheck/bivariate_product.rs): a and b
+ * (2^n_vars elements each) are folded in place; writes each round's c0, c1 as a
+ * message, samples the challenge; evals_out = (a(r), b(r)). */
+int bo_sumcheck_prove(uint64_t *a, uint64_t *b, int n_vars, const uint64_t sum[2], bo_transcript *t,
+                      uint64_t *challenges_out, uint64_t evals_out[4], int nthreads);
+/* BaseFoldProver::prove (prover protocols/basefold.rs:131-166).  The caller has
+ * written the codeword root.  multilinear and transparent (2^(log_dim+lbs)) are
+ * folded in place; codeword / tree are the committed interleaved codeword and
+ * its Merkle tree (bo_encode_batch / bo_merkle_build). */
+int bo_basefold_prove(const bo_domain *d, uint64_t *multilinear, uint64_t *transparent, const uint64_t claim[2],
+                      const uint64_t *codeword, const uint8_t *tree, int log_dim, int lir, int lbs,
+                      const int *arities, int n_ar, int n_queries, bo_transcript *t, uint64_t *challenges_out,
+                      int nthreads);
+/* TensorAlgebra<B1, B128>::transpose (verifier/math/src/tensor_algebra.rs:137-141):
+ * out[i] bit j = in[j] bit i over 128 elements. */
+void bo_tensor_transpose(const uint64_t *in, uint64_t *out);
+/* OneBitPCSProver::prove (prover/prover/src/pcs.rs:107-170); packed
+ * (2^(log_dim+lbs)) is consumed; eval_point has log_dim + lbs + 7 coordinates. */
+int bo_pcs_prove(const bo_domain *d, uint64_t *packed, const uint64_t *eval_point, const uint64_t *codeword,
+                 const uint8_t *tree, int log_dim, int lir, int lbs, const int *arities, int n_ar, int n_queries,
+                 bo_transcript *t, int nthreads);
 bo_transcript *bo_transcript_new(void);
 void bo_transcript_free(bo_transcript *t);
 void bo_transcript_message(bo_transcript *t, const uint8_t *bytes, size_t n);

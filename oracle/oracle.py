@@ -66,12 +66,19 @@ def lib():
         L.bo_eq_ind.argtypes = [u64p, C.c_int, u64p]
         L.bo_fold_interleaved.argtypes = [u64p, C.c_int, C.c_int, u64p, u64p, C.c_int]
         L.bo_fold_codeword.argtypes = [C.c_void_p, u64p, C.c_int, u64p, C.c_int, u64p, C.c_int]
+        L.bo_fold_chunk.argtypes = [C.c_void_p, C.c_int, C.c_uint64, u64p, u64p, C.c_int, u64p]
         L.bo_evaluate_mle.argtypes = [u64p, C.c_int, u64p, u64p]
         L.bo_pack_witness.argtypes = [u64p, C.c_size_t, u64p]
         L.bo_rs_fold_1b_rows.argtypes = [u64p, u64p, C.c_int, u64p, C.c_int, C.c_int]
         L.bo_rs_fold_elems.argtypes = [u64p, C.c_size_t, u64p, C.c_int, C.c_int]
         L.bo_fri_prove.argtypes = [C.c_void_p, u64p, C.c_int, C.c_int, C.c_int, intp, C.c_int, C.c_int,
                                    u64p, u8p, C.c_size_t, C.POINTER(C.c_size_t), u8p, u64p, u64p, C.c_int]
+        L.bo_sumcheck_prove.argtypes = [u64p, u64p, C.c_int, u64p, C.c_void_p, u64p, u64p, C.c_int]
+        L.bo_basefold_prove.argtypes = [C.c_void_p, u64p, u64p, u64p, u64p, u8p, C.c_int, C.c_int, C.c_int,
+                                        intp, C.c_int, C.c_int, C.c_void_p, u64p, C.c_int]
+        L.bo_tensor_transpose.argtypes = [u64p, u64p]
+        L.bo_pcs_prove.argtypes = [C.c_void_p, u64p, u64p, u64p, u8p, C.c_int, C.c_int, C.c_int, intp, C.c_int,
+                                   C.c_int, C.c_void_p, C.c_int]
         L.bo_transcript_new.restype = C.c_void_p
         L.bo_transcript_free.argtypes = [C.c_void_p]
         for name in ("bo_transcript_message", "bo_transcript_observe", "bo_transcript_decommit",
@@ -321,6 +328,15 @@ def fold_codeword(dom: Domain, cw: np.ndarray, log_len: int, chal: np.ndarray, t
     return out
 
 
+def fold_chunk(dom: Domain, log_len: int, chunk_index: int, vals: np.ndarray, chal: np.ndarray) -> int:
+    v = np.ascontiguousarray(vals, dtype=np.uint64).reshape(-1, 2)
+    ch = np.ascontiguousarray(chal, dtype=np.uint64).reshape(-1, 2)
+    assert v.shape[0] == 1 << ch.shape[0]
+    out = np.zeros(2, dtype=np.uint64)
+    lib().bo_fold_chunk(dom.ptr, log_len, chunk_index, _p64(v), _p64(ch), ch.shape[0], _p64(out))
+    return int(out[0]) | (int(out[1]) << 64)
+
+
 def evaluate_mle(evals: np.ndarray, point: np.ndarray) -> int:
     e = np.ascontiguousarray(evals, dtype=np.uint64).reshape(-1, 2)
     p = np.ascontiguousarray(point, dtype=np.uint64).reshape(-1, 2)
@@ -421,3 +437,65 @@ class Transcript:
         out = np.zeros(max(n, 1), dtype=np.uint8)
         lib().bo_transcript_tape(self.ptr, _p8(out), n)
         return out[:n].tobytes()
+
+
+# ---- sumcheck / BaseFold / one-bit PCS ----------------------------------------
+def sumcheck_prove(a: np.ndarray, b: np.ndarray, claim: int, transcript: "Transcript",
+                   threads: int = 1) -> dict:
+    """prove_single over BivariateProductSumcheckProver: challenges and (a(r), b(r))."""
+    wa = np.array(a, dtype=np.uint64).reshape(-1, 2)
+    wb = np.array(b, dtype=np.uint64).reshape(-1, 2)
+    assert wa.shape == wb.shape
+    n_vars = wa.shape[0].bit_length() - 1
+    ch = np.zeros((max(n_vars, 1), 2), dtype=np.uint64)
+    ev = np.zeros(4, dtype=np.uint64)
+    s = to_elems([claim])
+    if lib().bo_sumcheck_prove(_p64(wa), _p64(wb), n_vars, _p64(s), transcript.ptr, _p64(ch), _p64(ev), threads):
+        raise ValueError("sumcheck error")
+    return {"challenges": ch[:n_vars], "evals": to_ints(ev)}
+
+
+def _arities(arities):
+    return (C.c_int * max(len(arities), 1))(*arities)
+
+
+def basefold_prove(dom: Domain, multilinear: np.ndarray, transparent: np.ndarray, claim: int,
+                   codeword: np.ndarray, tree: np.ndarray, log_dim: int, log_inv_rate: int, log_batch: int,
+                   arities, n_queries: int, transcript: "Transcript", threads: int = 1) -> np.ndarray:
+    """BaseFoldProver::prove after the caller wrote the codeword root; returns the challenges."""
+    wa = np.array(multilinear, dtype=np.uint64).reshape(-1, 2)
+    wb = np.array(transparent, dtype=np.uint64).reshape(-1, 2)
+    cw = np.ascontiguousarray(codeword, dtype=np.uint64).reshape(-1, 2)
+    tr = np.ascontiguousarray(tree, dtype=np.uint8).reshape(-1)
+    n_vars = log_dim + log_batch
+    assert wa.shape[0] == wb.shape[0] == 1 << n_vars
+    ch = np.zeros((max(n_vars, 1), 2), dtype=np.uint64)
+    rc = lib().bo_basefold_prove(dom.ptr, _p64(wa), _p64(wb), _p64(to_elems([claim])), _p64(cw), _p8(tr), log_dim,
+                                 log_inv_rate, log_batch, _arities(arities), len(arities), n_queries,
+                                 transcript.ptr, _p64(ch), threads)
+    if rc:
+        raise ValueError(f"basefold_prove error {rc}")
+    return ch[:n_vars]
+
+
+def tensor_transpose(elems: np.ndarray) -> np.ndarray:
+    e = np.ascontiguousarray(elems, dtype=np.uint64).reshape(-1, 2)
+    assert e.shape[0] == 128
+    out = np.zeros_like(e)
+    lib().bo_tensor_transpose(_p64(e), _p64(out))
+    return out
+
+
+def pcs_prove(dom: Domain, packed: np.ndarray, eval_point: np.ndarray, codeword: np.ndarray, tree: np.ndarray,
+              log_dim: int, log_inv_rate: int, log_batch: int, arities, n_queries: int,
+              transcript: "Transcript", threads: int = 1) -> None:
+    """OneBitPCSProver::prove after the caller wrote the codeword root."""
+    w = np.array(packed, dtype=np.uint64).reshape(-1, 2)
+    pt = np.ascontiguousarray(eval_point, dtype=np.uint64).reshape(-1, 2)
+    assert w.shape[0] == 1 << (log_dim + log_batch) and pt.shape[0] == log_dim + log_batch + 7
+    cw = np.ascontiguousarray(codeword, dtype=np.uint64).reshape(-1, 2)
+    tr = np.ascontiguousarray(tree, dtype=np.uint8).reshape(-1)
+    rc = lib().bo_pcs_prove(dom.ptr, _p64(w), _p64(pt), _p64(cw), _p8(tr), log_dim, log_inv_rate, log_batch,
+                            _arities(arities), len(arities), n_queries, transcript.ptr, threads)
+    if rc:
+        raise ValueError(f"pcs_prove error {rc}")
