@@ -446,6 +446,95 @@ def pack_witness(words: DeviceBuffer, n_words: int, log_elems: int, out: Optiona
     return out
 
 
+def eq_ind_partial_eval(point, out: Optional[DeviceBuffer] = None, stream=None) -> DeviceBuffer:
+    """eq_ind_partial_eval (verifier/math/src/multilinear/eq.rs:123-131) into HBM."""
+    pt = np.ascontiguousarray(_elems(point))
+    n = pt.shape[0]
+    out = out or DeviceBuffer(16 << n)
+    check(lib().b64_eq_ind_partial_eval(_p64(pt), n, out.ptr, _stream(stream)), "eq_ind_partial_eval")
+    return out
+
+
+def _ints(arr: np.ndarray) -> List[int]:
+    return [int(lo) | (int(hi) << 64) for lo, hi in np.asarray(arr, dtype=np.uint64).reshape(-1, 2)]
+
+
+class BivariateProductSumcheckProver:
+    """BivariateProductSumcheckProver (prover protocols/sumcheck/bivariate_product.rs).
+
+    The prover folds both multilinears in place: like the reference, which takes
+    the buffers by value, it consumes a and b."""
+
+    def __init__(self, a: DeviceBuffer, b: DeviceBuffer, n_vars: int, claim: int, stream=None):
+        self._keep = (a, b)
+        out = C.c_void_p()
+        check(lib().b64_sumcheck_create(a.ptr, b.ptr, n_vars, _p64(_u128(claim)), _stream(stream), C.byref(out)),
+              "sumcheck_create")
+        self.ptr = out.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().b64_sumcheck_destroy(self.ptr)
+            self.ptr = None
+
+    def n_vars(self) -> int:
+        return lib().b64_sumcheck_n_vars(self.ptr)
+
+    def execute(self) -> List[int]:
+        co = np.zeros(6, dtype=np.uint64)
+        check(lib().b64_sumcheck_execute(self.ptr, _p64(co)), "sumcheck_execute")
+        return _ints(co)
+
+    def fold(self, challenge: int):
+        check(lib().b64_sumcheck_fold(self.ptr, _p64(_u128(challenge))), "sumcheck_fold")
+
+    def finish(self) -> List[int]:
+        ev = np.zeros(4, dtype=np.uint64)
+        check(lib().b64_sumcheck_finish(self.ptr, _p64(ev)), "sumcheck_finish")
+        return _ints(ev)
+
+
+def prove_single(prover: BivariateProductSumcheckProver, transcript: ProverTranscript) -> dict:
+    """prove_single (prover protocols/sumcheck/prove.rs:17-60)."""
+    challenges = []
+    for _ in range(prover.n_vars()):
+        c0, c1, _c2 = prover.execute()
+        transcript.message(c0.to_bytes(16, "little") + c1.to_bytes(16, "little"))
+        r = transcript.sample()
+        challenges.append(r)
+        prover.fold(r)
+    return {"challenges": challenges, "multilinear_evals": prover.finish()}
+
+
+def basefold_prove(params: FRIParams, domain: DomainContext, multilinear: DeviceBuffer, transparent: DeviceBuffer,
+                   claim: int, codeword: DeviceBuffer, tree: DeviceBuffer, transcript: ProverTranscript,
+                   stream=None) -> None:
+    """BaseFoldProver::new(...).prove(transcript) (prover protocols/basefold.rs:64-151).
+    multilinear and transparent are consumed."""
+    pc = params.to_c()
+    check(lib().b64_basefold_prove(C.byref(pc), domain.ptr, multilinear.ptr, transparent.ptr, _p64(_u128(claim)),
+                                   codeword.ptr, tree.ptr, transcript.ptr, _stream(stream)), "basefold_prove")
+
+
+class OneBitPCSProver:
+    """OneBitPCSProver (prover/prover/src/pcs.rs:35-171): commit and prove an
+    evaluation of a multilinear over B1 packed 128 bits per B128 element."""
+
+    def __init__(self, domain: DomainContext, params: FRIParams):
+        self.domain, self.params = domain, params
+
+    def commit(self, packed: DeviceBuffer, stream=None) -> CommitOutput:
+        return commit_interleaved(self.params, self.domain, packed, stream)
+
+    def prove(self, codeword: DeviceBuffer, tree: DeviceBuffer, packed: DeviceBuffer, evaluation_point,
+              transcript: ProverTranscript, stream=None) -> None:
+        """packed is consumed (folded in place by the sumcheck)."""
+        pt = np.ascontiguousarray(_elems(evaluation_point))
+        pc = self.params.to_c()
+        check(lib().b64_pcs_prove(C.byref(pc), self.domain.ptr, codeword.ptr, tree.ptr, packed.ptr, _p64(pt),
+                                  pt.shape[0], transcript.ptr, _stream(stream)), "pcs_prove")
+
+
 class CommitJob:
     """One FRI commit phase with pre-seeded challenges (b64_job_desc): encode,
     Merkle commit, every fold round and its commitment."""

@@ -125,6 +125,15 @@ SIGNATURES = {
     "b64_pack_witness": (C.c_int, [vp, C.c_uint64, C.c_uint32, vp, vp]),
     "b64_ring_switch_fold_1b_rows": (C.c_int, [vp, vp, C.c_uint32, vp, vp]),
     "b64_ring_switch_fold_elems": (C.c_int, [vp, C.c_uint32, u64p, vp]),
+    "b64_eq_ind_partial_eval": (C.c_int, [u64p, C.c_uint32, vp, vp]),
+    "b64_sumcheck_create": (C.c_int, [vp, vp, C.c_uint32, u64p, vp, C.POINTER(vp)]),
+    "b64_sumcheck_destroy": (None, [vp]),
+    "b64_sumcheck_n_vars": (C.c_uint32, [vp]),
+    "b64_sumcheck_execute": (C.c_int, [vp, u64p]),
+    "b64_sumcheck_fold": (C.c_int, [vp, u64p]),
+    "b64_sumcheck_finish": (C.c_int, [vp, u64p]),
+    "b64_basefold_prove": (C.c_int, [C.POINTER(FriParamsC), vp, vp, vp, u64p, vp, vp, vp, vp]),
+    "b64_pcs_prove": (C.c_int, [C.POINTER(FriParamsC), vp, vp, vp, vp, u64p, C.c_uint32, vp, vp]),
 }
 
 _lib = None

@@ -1,0 +1,50 @@
+// Host-side helpers shared by the C ABI sources (api.hip, batch.hip, pcs.hip):
+// stream casts, element packing and the status-code macros.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/b64hip.h"
+#include "host_util.hpp"
+
+namespace b64 {
+namespace abi {
+
+using host::u128;
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+inline u128 ld128(const uint64_t *p) { return ((u128)p[1] << 64) | p[0]; }
+inline void st128(uint64_t *p, u128 v) {
+    p[0] = (uint64_t)v;
+    p[1] = (uint64_t)(v >> 64);
+}
+inline uint4 to_u4(u128 v) {
+    return make_uint4((uint32_t)v, (uint32_t)(v >> 32), (uint32_t)(v >> 64), (uint32_t)(v >> 96));
+}
+inline uint4 to_u4(const uint64_t *p) { return to_u4(ld128(p)); }
+inline u128 from_u4(uint4 v) {
+    return ((u128)v.w << 96) | ((u128)v.z << 64) | ((u128)v.y << 32) | (u128)v.x;
+}
+inline int log2_ceil_u64(uint64_t x) {
+    int r = 0;
+    while ((1ull << r) < x) r++;
+    return r;
+}
+inline int hip_status(hipError_t e) {
+    return e == hipSuccess ? B64_OK : e == hipErrorOutOfMemory ? B64_ERR_OUT_OF_MEMORY : B64_ERR_HIP;
+}
+
+}  // namespace abi
+}  // namespace b64
+
+#define HIP_TRY(expr)                                     \
+    do {                                                  \
+        const int _rc = b64::abi::hip_status((expr));     \
+        if (_rc != B64_OK) return _rc;                    \
+    } while (0)
+
+#define B64_TRY(expr)                  \
+    do {                               \
+        const int _rc = (expr);        \
+        if (_rc != B64_OK) return _rc; \
+    } while (0)

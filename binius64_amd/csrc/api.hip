@@ -11,6 +11,7 @@
 #include <new>
 #include <vector>
 
+#include "abi_util.hpp"
 #include "host_util.hpp"
 #include "internal.hpp"
 
@@ -18,40 +19,18 @@ using b64::host::u128;
 
 #define B64_VERSION "0.1.0"
 
-namespace {
+using b64::abi::as_stream;
+using b64::abi::ld128;
+using b64::abi::log2_ceil_u64;
+using b64::abi::st128;
+using b64::abi::to_u4;
 
-inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
-inline u128 ld128(const uint64_t *p) { return ((u128)p[1] << 64) | p[0]; }
-inline void st128(uint64_t *p, u128 v) {
-    p[0] = (uint64_t)v;
-    p[1] = (uint64_t)(v >> 64);
-}
-inline uint4 to_u4(u128 v) {
-    return make_uint4((uint32_t)v, (uint32_t)(v >> 32), (uint32_t)(v >> 64), (uint32_t)(v >> 96));
-}
+namespace {
 
 int device_ok() {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
     return 1;
-}
-
-#define HIP_TRY(expr)                                                   \
-    do {                                                                \
-        hipError_t _e = (expr);                                         \
-        if (_e != hipSuccess) return _e == hipErrorOutOfMemory ? B64_ERR_OUT_OF_MEMORY : B64_ERR_HIP; \
-    } while (0)
-
-#define B64_TRY(expr)              \
-    do {                           \
-        int _rc = (expr);          \
-        if (_rc != B64_OK) return _rc; \
-    } while (0)
-
-int log2_ceil_u64(uint64_t x) {
-    int r = 0;
-    while ((1ull << r) < x) r++;
-    return r;
 }
 
 b64::CompressIV compress_iv() {
@@ -125,6 +104,8 @@ const char *b64_status_string(int s) {
         case B64_ERR_INVALID_FOLD_ARITY_SEQUENCE: return "invalid fold arity sequence";
         case B64_ERR_BUFFER_TOO_SMALL: return "buffer too small";
         case B64_ERR_NOT_COMMIT_ROUND: return "not a commit round";
+        case B64_ERR_EXPECTED_FOLD: return "expected a call to fold";
+        case B64_ERR_EXPECTED_EXECUTE: return "expected a call to execute";
         case B64_ERR_NO_DEVICE: return "no HIP device";
         case B64_ERR_HIP: return "HIP runtime error";
         case B64_ERR_OUT_OF_MEMORY: return "out of device memory";

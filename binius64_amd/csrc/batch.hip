@@ -18,30 +18,19 @@
 #include <new>
 #include <vector>
 
+#include "abi_util.hpp"
 #include "host_util.hpp"
 #include "internal.hpp"
 
+using b64::abi::as_stream;
+using b64::abi::to_u4;
 using b64::host::u128;
 
 namespace {
 
-inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
-inline uint4 to_u4(const uint64_t *p) {
-    return make_uint4((uint32_t)p[0], (uint32_t)(p[0] >> 32), (uint32_t)p[1], (uint32_t)(p[1] >> 32));
-}
 constexpr size_t ALIGN = 256;
 inline size_t align_up(size_t x) { return (x + ALIGN - 1) & ~(ALIGN - 1); }
 
-#define HIP_TRY(expr)                                                                               \
-    do {                                                                                            \
-        hipError_t _e = (expr);                                                                     \
-        if (_e != hipSuccess) return _e == hipErrorOutOfMemory ? B64_ERR_OUT_OF_MEMORY : B64_ERR_HIP; \
-    } while (0)
-#define B64_TRY(expr)                  \
-    do {                               \
-        int _rc = (expr);              \
-        if (_rc != B64_OK) return _rc; \
-    } while (0)
 
 b64::CompressIV job_iv() {
     static b64::CompressIV iv;

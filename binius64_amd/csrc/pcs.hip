@@ -1,0 +1,452 @@
+// Evaluation-proof side of the polynomial commitment on the device:
+//  * eq_ind_partial_eval (verifier/math/src/multilinear/eq.rs:123-131)
+//  * BivariateProductSumcheckProver (prover/prover/src/protocols/sumcheck/bivariate_product.rs)
+//  * BaseFoldProver (prover/prover/src/protocols/basefold.rs), driving the
+//    sumcheck and the FRI fold prover round by round on one transcript
+//  * OneBitPCSProver::prove (prover/prover/src/pcs.rs:107-170): the ring-switch
+//    reduction to a B128 sumcheck claim, then BaseFold.
+// The multilinears stay in HBM from the first round to the last; per round the
+// host reads back two field elements (the round evaluations), as the
+// Fiat-Shamir transcript needs them before the next challenge exists.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <new>
+#include <vector>
+
+#include "abi_util.hpp"
+#include "gf128_dev.hpp"
+#include "internal.hpp"
+
+using b64::abi::as_stream;
+using b64::abi::from_u4;
+using b64::abi::ld128;
+using b64::abi::st128;
+using b64::abi::to_u4;
+using b64::host::u128;
+
+namespace b64 {
+namespace {
+
+constexpr int SC_BLOCK = 256;
+constexpr int SC_WAVES = SC_BLOCK / 64;
+constexpr int SC_MAX_BLOCKS = 2048;
+
+__device__ __forceinline__ uint4 wave_xor(uint4 v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        v.x ^= __shfl_xor(v.x, m);
+        v.y ^= __shfl_xor(v.y, m);
+        v.z ^= __shfl_xor(v.z, m);
+        v.w ^= __shfl_xor(v.w, m);
+    }
+    return v;
+}
+
+// XOR of (p, q) over the block; thread 0 gets the result.
+__device__ __forceinline__ void block_xor2(uint4 &p, uint4 &q) {
+    __shared__ uint4 red[2][SC_WAVES];
+    p = wave_xor(p);
+    q = wave_xor(q);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][w] = p;
+        red[1][w] = q;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < SC_WAVES; k++) {
+            p = gxor(p, red[0][k]);
+            q = gxor(q, red[1][k]);
+        }
+    }
+}
+
+// One sumcheck round of the bivariate product A*B over `half` pairs
+// (bivariate_product.rs:43-82): y1 = sum A(v||1) B(v||1), yinf = sum
+// (A(v||0)+A(v||1)) (B(v||0)+B(v||1)), v over the low variables.  With fold set,
+// the previous round's fold_highest_var_inplace (multilinear/fold.rs:14-29,
+// zero += r (one - zero)) is applied first, in place, to arrays of 4*half
+// elements: every lane reads the four elements whose folds it needs and writes
+// the two it owns, so the in-place update is race-free.  The multiplies by r go
+// through a window table of r; the products A*B are general products.
+__global__ __launch_bounds__(SC_BLOCK) void k_sc_round(uint4 *__restrict__ a, uint4 *__restrict__ b, uint64_t half,
+                                                       int fold, uint4 r, uint4 *__restrict__ partial) {
+    __shared__ __attribute__((aligned(256))) uint4 tab[WIN_ENTRIES];
+    if (fold) {
+        if (threadIdx.x < WIN_ROWS) win_build_row(tab, r, threadIdx.x);
+        __syncthreads();
+    }
+    const uint32_t tb = lds_addr(tab);
+    const uint64_t len = 2 * half;
+    uint4 y1 = make_uint4(0, 0, 0, 0), yi = make_uint4(0, 0, 0, 0);
+    for (uint64_t i = (uint64_t)blockIdx.x * SC_BLOCK + threadIdx.x; i < half; i += (uint64_t)gridDim.x * SC_BLOCK) {
+        uint4 a0, a1, b0, b1;
+        if (fold) {
+            const uint4 x0 = a[i], x1 = a[i + half], x2 = a[i + len], x3 = a[i + len + half];
+            const uint4 z0 = b[i], z1 = b[i + half], z2 = b[i + len], z3 = b[i + len + half];
+            a0 = gxor(x0, win_mul_perm(tb, gxor(x0, x2)));
+            a1 = gxor(x1, win_mul_perm(tb, gxor(x1, x3)));
+            b0 = gxor(z0, win_mul_perm(tb, gxor(z0, z2)));
+            b1 = gxor(z1, win_mul_perm(tb, gxor(z1, z3)));
+            a[i] = a0;
+            a[i + half] = a1;
+            b[i] = b0;
+            b[i + half] = b1;
+        } else {
+            a0 = a[i];
+            a1 = a[i + half];
+            b0 = b[i];
+            b1 = b[i + half];
+        }
+        y1 = gxor(y1, gf_mul(a1, b1));
+        yi = gxor(yi, gf_mul(gxor(a0, a1), gxor(b0, b1)));
+    }
+    block_xor2(y1, yi);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = y1;
+        partial[2 * blockIdx.x + 1] = yi;
+    }
+}
+
+__global__ __launch_bounds__(SC_BLOCK) void k_sc_reduce(const uint4 *__restrict__ partial, int n_blocks,
+                                                        uint4 *__restrict__ out) {
+    uint4 y1 = make_uint4(0, 0, 0, 0), yi = make_uint4(0, 0, 0, 0);
+    for (int k = threadIdx.x; k < n_blocks; k += SC_BLOCK) {
+        y1 = gxor(y1, partial[2 * k]);
+        yi = gxor(yi, partial[2 * k + 1]);
+    }
+    block_xor2(y1, yi);
+    if (threadIdx.x == 0) {
+        out[0] = y1;
+        out[1] = yi;
+    }
+}
+
+// fold_highest_var_inplace on both multilinears: x[i] += r (x[i] + x[i + len]), i < len.
+__global__ __launch_bounds__(SC_BLOCK) void k_sc_fold(uint4 *__restrict__ a, uint4 *__restrict__ b, uint64_t len,
+                                                      uint4 r) {
+    __shared__ __attribute__((aligned(256))) uint4 tab[WIN_ENTRIES];
+    if (threadIdx.x < WIN_ROWS) win_build_row(tab, r, threadIdx.x);
+    __syncthreads();
+    const uint32_t tb = lds_addr(tab);
+    for (uint64_t i = (uint64_t)blockIdx.x * SC_BLOCK + threadIdx.x; i < len; i += (uint64_t)gridDim.x * SC_BLOCK) {
+        const uint4 x0 = a[i], x1 = a[i + len], z0 = b[i], z1 = b[i + len];
+        a[i] = gxor(x0, win_mul_perm(tb, gxor(x0, x1)));
+        b[i] = gxor(z0, win_mul_perm(tb, gxor(z0, z1)));
+    }
+}
+
+// ---- eq_ind_partial_eval -----------------------------------------------------
+// eq(r)[x] = prod_k (bit k of x ? r_k : 1 + r_k).  The coordinates are cut into
+// groups of 8; group g's table holds the eq values of its coordinates (256
+// entries, built by doubling as tensor_prod_eq_ind does, eq.rs:39-71).  Then
+// eq[x] = G0[x & 255] * c(x >> 8) with c(y) = prod_{g>=1} G_g[byte g-1 of y]:
+// each run of 256 outputs shares c(y), so the run is one window table of c(y)
+// applied to G0.  A block computes 256 values c(y) (one per lane), then its
+// waves take one run each per step.
+constexpr int EQ_MAX_VARS = 40;
+constexpr int EQ_GROUPS = (EQ_MAX_VARS + 7) / 8;
+struct EqPoint {
+    uint4 r[EQ_MAX_VARS];
+};
+
+__global__ __launch_bounds__(SC_BLOCK) void k_eq_expand(EqPoint p, int n, uint4 *__restrict__ out) {
+    __shared__ uint4 g[EQ_GROUPS][256];
+    __shared__ __attribute__((aligned(256))) uint4 wtab[SC_WAVES][WIN_ENTRIES];
+    __shared__ uint4 cbuf[SC_BLOCK];
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const int n_groups = n ? (n + 7) / 8 : 1;  // n = 0: the single entry 1
+    for (int gi = 0; gi < n_groups; gi++) {
+        const int len = (n - 8 * gi) < 8 ? (n - 8 * gi) : 8;
+        if (t == 0) g[gi][0] = make_uint4(1, 0, 0, 0);
+        __syncthreads();
+        for (int k = 0; k < len; k++) {
+            if (t < (1 << k)) {
+                const uint4 prod = gf_mul(g[gi][t], p.r[8 * gi + k]);
+                g[gi][t + (1 << k)] = prod;
+                g[gi][t] = gxor(g[gi][t], prod);
+            }
+            __syncthreads();
+        }
+    }
+    if (n <= 8) {
+        if (blockIdx.x == 0 && t < (1 << n)) out[t] = g[0][t];
+        return;
+    }
+    const uint64_t n_runs = 1ull << (n - 8);
+    for (uint64_t y0 = (uint64_t)blockIdx.x * SC_BLOCK; y0 < n_runs; y0 += (uint64_t)gridDim.x * SC_BLOCK) {
+        {
+            const uint64_t y = y0 + t;
+            uint4 c = make_uint4(1, 0, 0, 0);
+            if (y < n_runs)
+                for (int gi = 1; gi < n_groups; gi++) c = gf_mul(c, g[gi][(y >> (8 * (gi - 1))) & 255]);
+            cbuf[t] = c;
+        }
+        __syncthreads();
+        for (int j = 0; j < 64; j++) {
+            const int slot = 64 * w + j;
+            const uint64_t y = y0 + slot;
+            if (lane < WIN_ROWS) win_build_row(wtab[w], cbuf[slot], lane);
+            __syncthreads();
+            if (y < n_runs) {
+                const uint32_t tb = lds_addr(wtab[w]);
+                uint4 *o = out + (y << 8);
+#pragma unroll
+                for (int k = 0; k < 4; k++) o[lane + 64 * k] = win_mul_perm(tb, g[0][lane + 64 * k]);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+hipError_t launch_eq_expand(const EqPoint &p, int n, uint4 *out, hipStream_t s) {
+    uint64_t blocks = n <= 8 ? 1 : ((1ull << (n - 8)) + SC_BLOCK - 1) / SC_BLOCK;
+    if (blocks > 1024) blocks = 1024;
+    k_eq_expand<<<(unsigned)blocks, SC_BLOCK, 0, s>>>(p, n, out);
+    return hipGetLastError();
+}
+
+int sc_blocks(uint64_t half) {
+    const uint64_t b = (half + SC_BLOCK - 1) / SC_BLOCK;
+    return (int)(b < SC_MAX_BLOCKS ? (b ? b : 1) : SC_MAX_BLOCKS);
+}
+
+// 128 x 128 bit transpose: out[i] bit j = in[j] bit i
+// (TensorAlgebra<B1, B128>::transpose, verifier/math/src/tensor_algebra.rs:137-141).
+void transpose128(const u128 *in, u128 *out) {
+    for (int i = 0; i < 128; i++) {
+        u128 v = 0;
+        for (int j = 0; j < 128; j++) v |= ((in[j] >> i) & 1) << j;
+        out[i] = v;
+    }
+}
+
+// eq_ind_partial_eval on the host for a handful of coordinates.
+void eq_host(const u128 *r, int n, u128 *out) {
+    out[0] = 1;
+    for (int k = 0; k < n; k++)
+        for (size_t j = 0; j < ((size_t)1 << k); j++) {
+            const u128 prod = host::gf_mul(out[j], r[k]);
+            out[j + ((size_t)1 << k)] = prod;
+            out[j] ^= prod;
+        }
+}
+
+}  // namespace
+}  // namespace b64
+
+// ===========================================================================
+// BivariateProductSumcheckProver
+// ===========================================================================
+struct b64_sumcheck {
+    uint4 *a = nullptr, *b = nullptr;  // caller's buffers, folded in place
+    uint32_t n_vars = 0;
+    bool has_coeffs = false;  // RoundCoeffsOrSum::Coeffs
+    bool pending = false;     // a fold by r_pending not yet applied on the device
+    u128 sum = 0, r_pending = 0;
+    u128 coeffs[3] = {0, 0, 0};
+    uint4 *d_partial = nullptr;  // 2 * SC_MAX_BLOCKS partial sums, then the 2 totals
+    uint4 *h_res = nullptr;      // pinned, 2 elements
+    hipStream_t s = nullptr;
+
+    ~b64_sumcheck() {
+        if (d_partial) (void)hipFree(d_partial);
+        if (h_res) (void)hipHostFree(h_res);
+    }
+};
+
+extern "C" {
+
+int b64_sumcheck_create(void *d_a, void *d_b, uint32_t n_vars, const uint64_t sum[2], void *stream,
+                        b64_sumcheck **out) {
+    if (!d_a || !d_b || !sum || !out || n_vars > 40) return B64_ERR_INVALID_ARGS;
+    int n_dev = 0;
+    if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev <= 0) return B64_ERR_NO_DEVICE;
+    b64_sumcheck *sc = new (std::nothrow) b64_sumcheck();
+    if (!sc) return B64_ERR_OUT_OF_MEMORY;
+    sc->a = (uint4 *)d_a;
+    sc->b = (uint4 *)d_b;
+    sc->n_vars = n_vars;
+    sc->sum = ld128(sum);
+    sc->s = as_stream(stream);
+    if (hipMalloc(&sc->d_partial, (2 * b64::SC_MAX_BLOCKS + 2) * sizeof(uint4)) != hipSuccess ||
+        hipHostMalloc(&sc->h_res, 2 * sizeof(uint4)) != hipSuccess) {
+        delete sc;
+        return B64_ERR_OUT_OF_MEMORY;
+    }
+    *out = sc;
+    return B64_OK;
+}
+
+void b64_sumcheck_destroy(b64_sumcheck *sc) { delete sc; }
+
+uint32_t b64_sumcheck_n_vars(const b64_sumcheck *sc) { return sc ? sc->n_vars : 0; }
+
+int b64_sumcheck_execute(b64_sumcheck *sc, uint64_t coeffs_out[6]) {
+    if (!sc || !coeffs_out) return B64_ERR_INVALID_ARGS;
+    if (sc->has_coeffs) return B64_ERR_EXPECTED_FOLD;
+    if (sc->n_vars == 0) return B64_ERR_INVALID_ARGS;  // assert!(n_vars_remaining > 0)
+    const uint64_t half = 1ull << (sc->n_vars - 1);
+    const int blocks = b64::sc_blocks(half);
+    uint4 *totals = sc->d_partial + 2 * b64::SC_MAX_BLOCKS;
+    b64::k_sc_round<<<blocks, b64::SC_BLOCK, 0, sc->s>>>(sc->a, sc->b, half, sc->pending ? 1 : 0,
+                                                          to_u4(sc->r_pending), sc->d_partial);
+    b64::k_sc_reduce<<<1, b64::SC_BLOCK, 0, sc->s>>>(sc->d_partial, blocks, totals);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(sc->h_res, totals, 2 * sizeof(uint4), hipMemcpyDeviceToHost, sc->s));
+    HIP_TRY(hipStreamSynchronize(sc->s));
+    sc->pending = false;
+    // RoundEvals2::interpolate (round_evals.rs:73-78, :126-140)
+    const u128 y1 = from_u4(sc->h_res[0]), yinf = from_u4(sc->h_res[1]);
+    const u128 y0 = sc->sum ^ y1;
+    sc->coeffs[0] = y0;
+    sc->coeffs[2] = yinf;
+    sc->coeffs[1] = y1 ^ y0 ^ yinf;
+    sc->has_coeffs = true;
+    for (int k = 0; k < 3; k++) st128(coeffs_out + 2 * k, sc->coeffs[k]);
+    return B64_OK;
+}
+
+int b64_sumcheck_fold(b64_sumcheck *sc, const uint64_t challenge[2]) {
+    if (!sc || !challenge) return B64_ERR_INVALID_ARGS;
+    if (!sc->has_coeffs) return B64_ERR_EXPECTED_EXECUTE;
+    const u128 r = ld128(challenge);
+    // RoundCoeffs::evaluate (Horner)
+    u128 acc = 0;
+    for (int k = 2; k >= 0; k--) acc = b64::host::gf_mul(acc, r) ^ sc->coeffs[k];
+    sc->sum = acc;
+    sc->n_vars--;
+    sc->pending = true;  // applied by the next execute, fused with its reduction, or by finish
+    sc->r_pending = r;
+    sc->has_coeffs = false;
+    return B64_OK;
+}
+
+int b64_sumcheck_finish(b64_sumcheck *sc, uint64_t evals_out[4]) {
+    if (!sc || !evals_out) return B64_ERR_INVALID_ARGS;
+    if (sc->n_vars > 0) return sc->has_coeffs ? B64_ERR_EXPECTED_FOLD : B64_ERR_EXPECTED_EXECUTE;
+    if (sc->has_coeffs) return B64_ERR_EXPECTED_FOLD;
+    if (sc->pending) {
+        b64::k_sc_fold<<<1, b64::SC_BLOCK, 0, sc->s>>>(sc->a, sc->b, 1, to_u4(sc->r_pending));
+        HIP_TRY(hipGetLastError());
+        sc->pending = false;
+    }
+    HIP_TRY(hipMemcpyAsync(sc->h_res, sc->a, sizeof(uint4), hipMemcpyDeviceToHost, sc->s));
+    HIP_TRY(hipMemcpyAsync(sc->h_res + 1, sc->b, sizeof(uint4), hipMemcpyDeviceToHost, sc->s));
+    HIP_TRY(hipStreamSynchronize(sc->s));
+    st128(evals_out, from_u4(sc->h_res[0]));
+    st128(evals_out + 2, from_u4(sc->h_res[1]));
+    return B64_OK;
+}
+
+// ===========================================================================
+// eq_ind_partial_eval
+// ===========================================================================
+int b64_eq_ind_partial_eval(const uint64_t *point, uint32_t n, void *d_out, void *stream) {
+    if ((!point && n) || !d_out || n > (uint32_t)b64::EQ_MAX_VARS) return B64_ERR_INVALID_ARGS;
+    int n_dev = 0;
+    if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev <= 0) return B64_ERR_NO_DEVICE;
+    b64::EqPoint p;
+    memset(&p, 0, sizeof(p));
+    for (uint32_t k = 0; k < n; k++) p.r[k] = to_u4(ld128(point + 2 * k));
+    HIP_TRY(b64::launch_eq_expand(p, (int)n, (uint4 *)d_out, as_stream(stream)));
+    return B64_OK;
+}
+
+// ===========================================================================
+// BaseFoldProver::prove (protocols/basefold.rs:131-166)
+// ===========================================================================
+int b64_basefold_prove(const b64_fri_params *p, const b64_domain *d, void *d_multilinear, void *d_transparent,
+                       const uint64_t claim[2], const void *d_codeword, const void *d_tree, b64_transcript *t,
+                       void *stream) {
+    if (!p || !d || !d_multilinear || !d_transparent || !claim || !d_codeword || !d_tree || !t)
+        return B64_ERR_INVALID_ARGS;
+    const uint32_t n_vars = p->log_dim + p->log_batch_size;
+    b64_sumcheck *sc = nullptr;
+    B64_TRY(b64_sumcheck_create(d_multilinear, d_transparent, n_vars, claim, stream, &sc));
+    b64_fri_prover *fp = nullptr;
+    int rc = b64_fri_prover_create(p, d, d_codeword, d_tree, stream, &fp);
+    if (rc != B64_OK) {
+        b64_sumcheck_destroy(sc);
+        return rc;
+    }
+    if (b64_fri_prover_n_rounds(fp) != n_vars) rc = B64_ERR_INVALID_ARGS;
+    for (uint32_t i = 0; rc == B64_OK && i < n_vars; i++) {
+        // execute (:99-113): the sumcheck round, then the FRI fold round
+        uint64_t co[6];
+        rc = b64_sumcheck_execute(sc, co);
+        if (rc != B64_OK) break;
+        uint8_t root[32];
+        int committed = 0;
+        rc = b64_fri_prover_execute_fold_round(fp, root, &committed);
+        if (rc != B64_OK) break;
+        b64_transcript_message(t, (const uint8_t *)co, 32);  // RoundCoeffs::truncate: c0, c1
+        if (committed) b64_transcript_message(t, root, 32);
+        uint64_t r[2];
+        b64_transcript_sample(t, (uint8_t *)r, 16);
+        // fold (:119-123)
+        rc = b64_sumcheck_fold(sc, r);
+        if (rc == B64_OK) rc = b64_fri_prover_receive_challenge(fp, r);
+    }
+    if (rc == B64_OK) {
+        // finish (:158-166)
+        uint8_t root[32];
+        int committed = 0;
+        rc = b64_fri_prover_execute_fold_round(fp, root, &committed);
+        if (rc == B64_OK && committed) b64_transcript_message(t, root, 32);
+        if (rc == B64_OK) rc = b64_fri_prover_finish_proof(fp, t);
+    }
+    b64_fri_prover_destroy(fp);
+    b64_sumcheck_destroy(sc);
+    return rc;
+}
+
+// ===========================================================================
+// OneBitPCSProver::prove (prover/prover/src/pcs.rs:107-170)
+// ===========================================================================
+int b64_pcs_prove(const b64_fri_params *p, const b64_domain *d, const void *d_codeword, const void *d_tree,
+                  void *d_packed, const uint64_t *eval_point, uint32_t n_point, b64_transcript *t, void *stream) {
+    if (!p || !d || !d_codeword || !d_tree || !d_packed || !eval_point || !t) return B64_ERR_INVALID_ARGS;
+    const uint32_t log_len = p->log_dim + p->log_batch_size;
+    constexpr uint32_t KAPPA = 7;  // log2 of B128's degree over B1
+    if (n_point != log_len + KAPPA || log_len > (uint32_t)b64::EQ_MAX_VARS) return B64_ERR_INVALID_ARGS;
+    const hipStream_t s = as_stream(stream);
+    void *d_suffix = nullptr, *d_shv = nullptr;
+    HIP_TRY(hipMalloc(&d_suffix, (16ull << log_len) + 128 * 16));
+    d_shv = (uint8_t *)d_suffix + (16ull << log_len);
+    int rc = B64_OK;
+    do {
+        // suffix tensor eq(eval_point[7..]) and the partial evaluations s_hat_v
+        rc = b64_eq_ind_partial_eval(eval_point + 2 * KAPPA, log_len, d_suffix, stream);
+        if (rc != B64_OK) break;
+        rc = b64_ring_switch_fold_1b_rows(d_packed, d_suffix, log_len, d_shv, stream);
+        if (rc != B64_OK) break;
+        u128 s_hat_v[128], s_hat_u[128], rpp[KAPPA], eq_rpp[128];
+        if (hipMemcpyAsync(s_hat_v, d_shv, sizeof(s_hat_v), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            rc = B64_ERR_HIP;
+            break;
+        }
+        b64_transcript_message(t, (const uint8_t *)s_hat_v, sizeof(s_hat_v));
+        b64::transpose128(s_hat_v, s_hat_u);
+        // row-batching challenges
+        b64_transcript_sample(t, (uint8_t *)rpp, sizeof(rpp));
+        b64::eq_host(rpp, KAPPA, eq_rpp);
+        u128 claim = 0;
+        for (int i = 0; i < 128; i++) claim ^= b64::host::gf_mul(s_hat_u[i], eq_rpp[i]);
+        // rs_eq_ind = fold_b128_elems_inplace(suffix tensor, eq(r''))
+        uint64_t vec[256];
+        for (int i = 0; i < 128; i++) st128(vec + 2 * i, eq_rpp[i]);
+        rc = b64_ring_switch_fold_elems(d_suffix, log_len, vec, stream);
+        if (rc != B64_OK) break;
+        uint64_t cl[2];
+        st128(cl, claim);
+        rc = b64_basefold_prove(p, d, d_packed, d_suffix, cl, d_codeword, d_tree, t, stream);
+    } while (false);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(d_suffix);
+    return rc;
+}
+
+}  // extern "C"

@@ -41,6 +41,8 @@ extern "C" {
 #define B64_ERR_INVALID_FOLD_ARITY_SEQUENCE 9 /* fri::Error::InvalidFoldAritySequence */
 #define B64_ERR_BUFFER_TOO_SMALL 10
 #define B64_ERR_NOT_COMMIT_ROUND 11
+#define B64_ERR_EXPECTED_FOLD 12              /* sumcheck::Error::ExpectedFold */
+#define B64_ERR_EXPECTED_EXECUTE 13           /* sumcheck::Error::ExpectedExecute */
 #define B64_ERR_NO_DEVICE 20
 #define B64_ERR_HIP 21
 #define B64_ERR_OUT_OF_MEMORY 22
@@ -49,6 +51,7 @@ extern "C" {
 typedef struct b64_domain b64_domain;
 typedef struct b64_transcript b64_transcript;
 typedef struct b64_fri_prover b64_fri_prover;
+typedef struct b64_sumcheck b64_sumcheck;
 
 /* FRIParams (verifier/verifier/src/fri/common.rs:15-31). */
 #define B64_MAX_FOLD_ARITIES 32
@@ -287,6 +290,47 @@ int b64_ring_switch_fold_1b_rows(const void *d_mat, const void *d_vec, uint32_t 
 /* fold_b128_elems_inplace (ring_switch.rs:130-176), in place on 2^log_len elements:
  * x <- sum over b of bit b of x times vec[b]; vec is 128 host elements. */
 int b64_ring_switch_fold_elems(void *d_elems, uint32_t log_len, const uint64_t *vec, void *stream);
+
+/* ---- eq_ind_partial_eval (verifier/math/src/multilinear/eq.rs:123-131) ------ */
+/* d_out[x] = prod over k < n of (bit k of x ? point[k] : 1 + point[k]),
+ * 2^n elements; n <= 40. */
+int b64_eq_ind_partial_eval(const uint64_t *point, uint32_t n, void *d_out, void *stream);
+
+/* ---- BivariateProductSumcheckProver (prover/prover/src/protocols/sumcheck/
+ *      bivariate_product.rs) through the SumcheckProver round interface
+ *      (sumcheck/common.rs:28-53).  Binds the highest variable first. ------- */
+/* new (:25-35): the prover takes d_a and d_b (2^n_vars elements each) as its
+ * multilinears and folds them in place, as the reference takes the buffers by
+ * value; their contents are consumed. */
+int b64_sumcheck_create(void *d_a, void *d_b, uint32_t n_vars, const uint64_t sum[2], void *stream,
+                        b64_sumcheck **out);
+void b64_sumcheck_destroy(b64_sumcheck *sc);
+uint32_t b64_sumcheck_n_vars(const b64_sumcheck *sc);                  /* :39-41 */
+/* execute (:43-82): the round polynomial's coefficients c0, c1, c2.
+ * EXPECTED_FOLD if called twice without a fold. */
+int b64_sumcheck_execute(b64_sumcheck *sc, uint64_t coeffs_out[6]);
+/* fold (:84-98).  EXPECTED_EXECUTE unless execute came first. */
+int b64_sumcheck_fold(b64_sumcheck *sc, const uint64_t challenge[2]);
+/* finish (:100-121): the two multilinears' evaluations at the challenge point. */
+int b64_sumcheck_finish(b64_sumcheck *sc, uint64_t evals_out[4]);
+
+/* ---- BaseFoldProver::prove (prover/prover/src/protocols/basefold.rs:131-166) -- */
+/* The caller has committed the multilinear (b64_fri_commit_interleaved) and
+ * written the root to the transcript.  d_multilinear and d_transparent
+ * (2^(log_dim+log_batch_size) elements each) are consumed; claim is the sum of
+ * their product.  Writes every round polynomial, round commitment and the FRI
+ * query proof to t. */
+int b64_basefold_prove(const b64_fri_params *p, const b64_domain *d, void *d_multilinear, void *d_transparent,
+                       const uint64_t claim[2], const void *d_codeword, const void *d_tree, b64_transcript *t,
+                       void *stream);
+
+/* ---- OneBitPCSProver::prove (prover/prover/src/pcs.rs:107-170) ---------------- */
+/* Proves the evaluation of the committed B1 multilinear (d_packed: 2^(log_dim +
+ * log_batch_size) B128 elements, 128 bits each; consumed) at eval_point
+ * (n_point = log_dim + log_batch_size + 7 coordinates): ring-switch partial
+ * evaluations, row-batching challenges, then BaseFold. */
+int b64_pcs_prove(const b64_fri_params *p, const b64_domain *d, const void *d_codeword, const void *d_tree,
+                  void *d_packed, const uint64_t *eval_point, uint32_t n_point, b64_transcript *t, void *stream);
 
 #ifdef __cplusplus
 }
