@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-ring-switch", action="store_true", help="skip the ring-switch folds leg")
+    ap.add_argument("--no-pcs", action="store_true", help="skip the PCS evaluation-proof leg")
     ap.add_argument("--cpu-log-n", type=int, default=24, help="CPU baseline sample size")
     ap.add_argument("--ntt", default=None, metavar="V,K,T,M,G[,W]",
                     help="NTT schedule for every size in this run (replaces the autotuned table): "
@@ -176,6 +177,7 @@ def main():
     batch = None if args.no_batch else batch_bench(b64, torch, stream, sp)
     serve = None if args.no_serve else serve_bench(b64, torch, sp)
     ring = None if args.no_ring_switch else ring_switch_bench(b64, torch, stream, sp, with_cpu=not args.no_cpu)
+    pcs = None if args.no_pcs else pcs_bench(b64, torch, sp, log_n=n, with_cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
@@ -206,6 +208,7 @@ def main():
         "batch": batch,
         "serve": serve,
         "ring_switch": ring,
+        "pcs_prove": pcs,
     }
     print(json.dumps(out), flush=True)
 
@@ -431,6 +434,78 @@ def ring_switch_bench(b64, torch, stream, sp, log_len: int = 24, reps: int = 10,
         res["cpu_baseline"] = {"cores": cores, "kind": "port", "sample": f"same 2^{log_len} inputs, median of 3",
                                "fold_1b_rows_GBs": round((16 << log_len) / sorted(ts)[1] / 1e9, 3),
                                "fold_elems_GBs": round((16 << log_len) / sorted(ts2)[1] / 1e9, 3)}
+    return res
+
+
+def pcs_bench(b64, torch, sp, log_n: int = 24, reps: int = 3, with_cpu: bool = True):
+    """The evaluation proof of the polynomial commitment (OneBitPCSProver::prove,
+    prover/prover/src/pcs.rs:107-170) for a committed multilinear of 2^log_n B128
+    elements (2^(log_n+7) bits) with the Verifier::setup parameters: ring-switch
+    partial evaluations, log_n BaseFold rounds (sumcheck + FRI fold and
+    commitment), 232 FRI queries.  Wall time per proof, transcript included; the
+    commitment is made once beforehand."""
+    import numpy as np
+
+    L = b64.lib()
+    params, dom = b64.setup_commit_params(log_n, 1, 96)
+    pc = params.to_c()
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(11)
+    msg = torch.randint(-(2 ** 63), 2 ** 63 - 1, (2 << log_n,), dtype=torch.int64, device=dev, generator=g)
+    packed = torch.empty_like(msg)
+    cw = torch.empty((2 << params.log_len(),), dtype=torch.int64, device=dev)
+    n_nodes = (2 << params.rs_log_len()) - 1
+    tree = torch.empty((4 * n_nodes,), dtype=torch.int64, device=dev)
+    root = np.zeros(32, dtype=np.uint8)
+    point = np.random.default_rng(12).integers(0, 2 ** 63, size=(log_n + 7, 2), dtype=np.uint64)
+    pp = point.ctypes.data_as(C.POINTER(C.c_uint64))
+
+    def commit():
+        b64.check(L.b64_fri_commit_interleaved(C.byref(pc), dom.ptr, msg.data_ptr(), cw.data_ptr(), tree.data_ptr(),
+                                               b64._p8(root), sp), "commit")
+
+    commit()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        commit()
+    torch.cuda.synchronize()
+    commit_ms = 1e3 * (time.perf_counter() - t) / reps
+
+    def prove():
+        packed.copy_(msg)  # the proof consumes its multilinear
+        torch.cuda.synchronize()
+        tr = b64.ProverTranscript()
+        tr.message(root.tobytes())
+        t0 = time.perf_counter()
+        b64.check(L.b64_pcs_prove(C.byref(pc), dom.ptr, cw.data_ptr(), tree.data_ptr(), packed.data_ptr(), pp,
+                                  log_n + 7, tr.ptr, sp), "pcs_prove")
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, tr.finalize()
+
+    _, tape = prove()
+    runs = [prove() for _ in range(reps)]
+    same = all(r[1] == tape for r in runs)
+    med = sorted(r[0] for r in runs)[reps // 2]
+    res = {"log_n": log_n, "proof_ms": round(1e3 * med, 3), "proofs_per_s": round(1 / med, 2),
+           "commit_ms": round(commit_ms, 3), "tape_bytes": len(tape), "deterministic": bool(same)}
+    if with_cpu:
+        from oracle import oracle as orc
+        cores = min(16, os.cpu_count() or 1)
+        m_h = msg.cpu().numpy().view(np.uint64).reshape(-1, 2)
+        cw_h = cw.cpu().numpy().view(np.uint64).reshape(-1, 2)
+        tree_h = tree.cpu().numpy().view(np.uint8).reshape(n_nodes, 32)
+        ref = orc.Domain(dom.subspace(params.rs_log_len()))
+        ot = orc.Transcript()
+        ot.message(root.tobytes())
+        t = time.perf_counter()
+        orc.pcs_prove(ref, m_h, point, cw_h, tree_h, params.log_dim, 1, params.log_batch_size, params.fold_arities,
+                      params.n_test_queries, ot, threads=cores)
+        cpu_s = time.perf_counter() - t
+        res["cpu_baseline"] = {"proof_s": round(cpu_s, 3), "cores": cores, "kind": "port",
+                               "sample": f"same 2^{log_n}-element multilinear and point, one proof"}
+        res["tape_identical_to_cpu"] = ot.tape() == tape
     return res
 
 

@@ -74,6 +74,19 @@ def _u128(v: int) -> np.ndarray:
     return np.array([v & 0xFFFFFFFFFFFFFFFF, v >> 64], dtype=np.uint64)
 
 
+def _release(obj, destroy: str) -> None:
+    """Frees obj.ptr with the named destroy call.  During interpreter teardown
+    the module's globals may already be gone; the process exit then frees the
+    library's objects itself."""
+    ptr = getattr(obj, "ptr", None)
+    obj.ptr = None
+    if ptr:
+        try:
+            getattr(lib(), destroy)(ptr)
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------------------
 class DeviceBuffer:
     """An HBM allocation owned by the library's allocator."""
@@ -165,9 +178,7 @@ class DomainContext:
         return cls(dim=dim)
 
     def __del__(self):
-        if getattr(self, "ptr", None):
-            lib().b64_domain_destroy(self.ptr)
-            self.ptr = None
+        _release(self, "b64_domain_destroy")
 
     def log_domain_size(self) -> int:
         return lib().b64_domain_log_size(self.ptr)
@@ -360,9 +371,7 @@ class ProverTranscript:
         self.ptr = out.value
 
     def __del__(self):
-        if getattr(self, "ptr", None):
-            lib().b64_transcript_destroy(self.ptr)
-            self.ptr = None
+        _release(self, "b64_transcript_destroy")
 
     def _buf(self, data: bytes):
         return np.frombuffer(data, dtype=np.uint8).copy() if data else np.zeros(1, dtype=np.uint8)
@@ -408,9 +417,7 @@ class FRIFoldProver:
         self.ptr = out.value
 
     def __del__(self):
-        if getattr(self, "ptr", None):
-            lib().b64_fri_prover_destroy(self.ptr)
-            self.ptr = None
+        _release(self, "b64_fri_prover_destroy")
 
     def n_rounds(self) -> int:
         return lib().b64_fri_prover_n_rounds(self.ptr)
@@ -473,9 +480,7 @@ class BivariateProductSumcheckProver:
         self.ptr = out.value
 
     def __del__(self):
-        if getattr(self, "ptr", None):
-            lib().b64_sumcheck_destroy(self.ptr)
-            self.ptr = None
+        _release(self, "b64_sumcheck_destroy")
 
     def n_vars(self) -> int:
         return lib().b64_sumcheck_n_vars(self.ptr)
@@ -600,9 +605,7 @@ class BatchPlan:
         check(lib().b64_batch_plan_run(self.ptr, _stream(stream)), "batch_plan_run")
 
     def __del__(self):
-        if getattr(self, "ptr", None):
-            lib().b64_batch_plan_destroy(self.ptr)
-            self.ptr = None
+        _release(self, "b64_batch_plan_destroy")
 
 
 class Server:

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../include/b64hip.h"
 #include "host_util.hpp"
 
@@ -32,6 +34,28 @@ inline int log2_ceil_u64(uint64_t x) {
 }
 inline int hip_status(hipError_t e) {
     return e == hipSuccess ? B64_OK : e == hipErrorOutOfMemory ? B64_ERR_OUT_OF_MEMORY : B64_ERR_HIP;
+}
+
+// Scratch buffers of the provers (folded oracles, their trees, reduction
+// partials, gather lists) come from the device's default stream-ordered pool.
+// Left at its default release threshold the pool hands freed memory back to the
+// driver at every synchronisation, and each allocation then costs a full
+// driver allocation again; keeping up to this many freed bytes cached makes the
+// per-round allocations cheap.
+constexpr uint64_t POOL_KEEP_BYTES = 8ull << 30;
+inline void keep_pool_memory() {
+    static std::mutex mu;
+    static bool done[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+    std::lock_guard<std::mutex> lock(mu);
+    if (done[dev]) return;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t keep = POOL_KEEP_BYTES;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+    done[dev] = true;
 }
 
 }  // namespace abi

@@ -439,6 +439,7 @@ int b64_ring_switch_fold_1b_rows(const void *d_mat, const void *d_vec, uint32_t 
     // per-call partial sums from the stream-ordered pool, so concurrent calls on
     // different streams never share them
     void *partial = nullptr;
+    b64::abi::keep_pool_memory();
     HIP_TRY(hipMallocAsync(&partial, (size_t)b64::rs_fold_1b_rows_blocks(n) * 128 * sizeof(uint4), s));
     const hipError_t e = b64::launch_rs_fold_1b_rows((const uint4 *)d_mat, (const uint4 *)d_vec, n,
                                                      (uint4 *)partial, (uint4 *)d_out, s);
@@ -684,8 +685,8 @@ struct b64_fri_prover {
 
     ~b64_fri_prover() {
         for (auto &o : committed) {
-            if (o.cw) (void)hipFree(o.cw);
-            if (o.tree) (void)hipFree(o.tree);
+            if (o.cw) (void)hipFreeAsync(o.cw, s);
+            if (o.tree) (void)hipFreeAsync(o.tree, s);
         }
     }
     uint32_t n_rounds() const { return p.log_dim + p.log_batch_size; }
@@ -696,27 +697,6 @@ namespace {
 
 int fetch(void *dst, const void *d_src, size_t bytes, hipStream_t s) {
     HIP_TRY(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, s));
-    return B64_OK;
-}
-
-// Gather openings for all queries of one oracle with a single device->host
-// transfer per oracle: the coset values and the branch digests.
-int gather_rows(const b64_fri_prover::Oracle &o, const std::vector<uint64_t> &coset_idx, int log_coset, int depth,
-                std::vector<uint8_t> &vals, std::vector<uint8_t> &branches, hipStream_t s) {
-    const size_t q = coset_idx.size();
-    const size_t coset_bytes = 16ull << log_coset;
-    const int br_len = o.tree_log_len - depth;
-    vals.resize(q * coset_bytes);
-    branches.resize(q * 32 * (size_t)(br_len > 0 ? br_len : 0));
-    for (size_t i = 0; i < q; i++) {
-        B64_TRY(fetch(vals.data() + i * coset_bytes, (const uint8_t *)o.cw + (coset_idx[i] << log_coset) * 16,
-                      coset_bytes, s));
-        for (int j = 0; j < br_len; j++) {
-            const uint64_t node = (((1ull << j) - 1) << (o.tree_log_len + 1 - j)) | ((coset_idx[i] >> j) ^ 1);
-            B64_TRY(fetch(branches.data() + (i * br_len + j) * 32, (const uint8_t *)o.tree + 32 * node, 32, s));
-        }
-    }
-    HIP_TRY(hipStreamSynchronize(s));
     return B64_OK;
 }
 
@@ -738,6 +718,7 @@ int b64_fri_prover_create(const b64_fri_params *p, const b64_domain *d, const vo
     fp->first.tree_log_len = (int)(p->log_dim + p->log_inv_rate);
     fp->s = as_stream(stream);
     fp->next_commit = p->log_batch_size;  // FRIFoldProver::new (fold.rs:70)
+    b64::abi::keep_pool_memory();
     *out = fp;
     return B64_OK;
 }
@@ -767,10 +748,10 @@ int b64_fri_prover_execute_fold_round(b64_fri_prover *fp, uint8_t root_out[32], 
         // fold_interleaved (fold.rs:260-282)
         o.log_len = (int)fp->rs_log_len();
         if (n_ch != fp->p.log_batch_size || n_ch > 6) return B64_ERR_INVALID_ARGS;
-        HIP_TRY(hipMalloc(&o.cw, 16ull << o.log_len));
+        HIP_TRY(hipMallocAsync((void **)&o.cw, 16ull << o.log_len, fp->s));
         hipError_t e = b64::launch_fold_interleaved(fp->first.cw, o.cw, 1ull << o.log_len, (int)n_ch, ch.data(), fp->s);
         if (e != hipSuccess) {
-            (void)hipFree(o.cw);
+            (void)hipFreeAsync(o.cw, fp->s);
             return B64_ERR_HIP;
         }
     } else {
@@ -778,10 +759,10 @@ int b64_fri_prover_execute_fold_round(b64_fri_prover *fp, uint8_t root_out[32], 
         const b64_fri_prover::Oracle &prev = fp->committed.back();
         if ((int)n_ch > prev.log_len || n_ch > 8) return B64_ERR_INVALID_ARGS;
         o.log_len = prev.log_len - (int)n_ch;
-        HIP_TRY(hipMalloc(&o.cw, 16ull << o.log_len));
+        HIP_TRY(hipMallocAsync((void **)&o.cw, 16ull << o.log_len, fp->s));
         hipError_t e = b64::launch_fold_codeword(dd, prev.cw, o.cw, prev.log_len, ch.data(), (int)n_ch, fp->s);
         if (e != hipSuccess) {
-            (void)hipFree(o.cw);
+            (void)hipFreeAsync(o.cw, fp->s);
             return B64_ERR_HIP;
         }
     }
@@ -790,12 +771,12 @@ int b64_fri_prover_execute_fold_round(b64_fri_prover *fp, uint8_t root_out[32], 
     const bool has_arity = k < fp->p.n_fold_arities;
     const int log_coset = has_arity ? (int)fp->p.fold_arities[k] : (int)fp->p.log_terminal_dim;
     if (log_coset > o.log_len) {
-        (void)hipFree(o.cw);
+        (void)hipFreeAsync(o.cw, fp->s);
         return B64_ERR_INCORRECT_BATCH_SIZE;
     }
     o.tree_log_len = o.log_len - log_coset;
-    if (hipMalloc(&o.tree, 32ull * ((2ull << o.tree_log_len) - 1)) != hipSuccess) {
-        (void)hipFree(o.cw);
+    if (hipMallocAsync((void **)&o.tree, 32ull * ((2ull << o.tree_log_len) - 1), fp->s) != hipSuccess) {
+        (void)hipFreeAsync(o.cw, fp->s);
         return B64_ERR_OUT_OF_MEMORY;
     }
     fp->committed.push_back(o);  // owned from here on
@@ -858,27 +839,47 @@ int b64_fri_prover_finish_proof(b64_fri_prover *fp, b64_transcript *t) {
         HIP_TRY(hipStreamSynchronize(s));
         b64_transcript_decommit(t, layer.data(), layer.size());
     }
-    // queries: decommitments are not observed, so every index can be sampled first
+    // queries: decommitments are not observed, so every index can be sampled
+    // first.  Each opening (prove_coset_opening, query.rs:112-134: the coset's
+    // values, then the branch digests from the leaf up) is listed as 16-byte
+    // granules in tape order and gathered on the device in one launch.
     const uint32_t nq = p.n_test_queries;
-    std::vector<uint64_t> idx0(nq);
-    for (uint32_t q = 0; q < nq; q++) idx0[q] = b64_transcript_sample_bits(t, fp->rs_log_len());
-    std::vector<std::vector<uint8_t>> vals(n_open), brs(n_open);
-    std::vector<int> log_cosets(n_open);
-    std::vector<uint64_t> cur = idx0;
-    for (size_t i = 0; i < n_open; i++) {
-        if (i > 0)
-            for (auto &x : cur) x >>= p.fold_arities[i - 1];
-        log_cosets[i] = (int)(i == 0 ? p.log_batch_size : p.fold_arities[i - 1]);
-        B64_TRY(gather_rows(*opened[i], cur, log_cosets[i], depths[i], vals[i], brs[i], s));
-    }
+    std::vector<uint64_t> idx(nq);
+    for (uint32_t q = 0; q < nq; q++) idx[q] = b64_transcript_sample_bits(t, fp->rs_log_len());
+    std::vector<uint64_t> src;
     for (uint32_t q = 0; q < nq; q++) {
+        uint64_t cur = idx[q];
         for (size_t i = 0; i < n_open; i++) {
-            const size_t cb = 16ull << log_cosets[i];
-            b64_transcript_decommit(t, vals[i].data() + q * cb, cb);
-            const int br_len = opened[i]->tree_log_len - depths[i];
-            if (br_len > 0) b64_transcript_decommit(t, brs[i].data() + (size_t)q * br_len * 32, 32ull * br_len);
+            const b64_fri_prover::Oracle &o = *opened[i];
+            const int log_coset = (int)(i == 0 ? p.log_batch_size : p.fold_arities[i - 1]);
+            if (i > 0) cur >>= log_coset;
+            if (cur >> o.tree_log_len) return B64_ERR_INDEX_OUT_OF_RANGE;
+            const uint64_t vals = (uint64_t)(uintptr_t)o.cw + ((cur << log_coset) << 4);
+            for (uint64_t k = 0; k < (1ull << log_coset); k++) src.push_back(vals + 16 * k);
+            const int br_len = o.tree_log_len - depths[i];
+            for (int j = 0; j < br_len; j++) {
+                // sibling at level j of BinaryMerkleTree::branch (binary_merkle_tree.rs:154-169)
+                const uint64_t node = (((1ull << j) - 1) << (o.tree_log_len + 1 - j)) | ((cur >> j) ^ 1);
+                const uint64_t at = (uint64_t)(uintptr_t)o.tree + 32 * node;
+                src.push_back(at);
+                src.push_back(at + 16);
+            }
         }
     }
+    if (src.empty()) return B64_OK;
+    const size_t n = src.size();
+    uint8_t *d_buf = nullptr;
+    HIP_TRY(hipMallocAsync((void **)&d_buf, n * (sizeof(uint64_t) + 16), s));
+    std::vector<uint8_t> out(n * 16);
+    int rc = B64_OK;
+    if (hipMemcpyAsync(d_buf, src.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, s) != hipSuccess ||
+        b64::launch_gather16((const uint64_t *)d_buf, (uint4 *)(d_buf + n * sizeof(uint64_t)), n, s) != hipSuccess ||
+        hipMemcpyAsync(out.data(), d_buf + n * sizeof(uint64_t), n * 16, hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = B64_ERR_HIP;
+    (void)hipFreeAsync(d_buf, s);
+    HIP_TRY(hipStreamSynchronize(s));
+    if (rc != B64_OK) return rc;
+    b64_transcript_decommit(t, out.data(), out.size());
     return B64_OK;
 }
 

@@ -126,6 +126,8 @@ hipError_t launch_rs_fold_elems(uint4 *elems, uint64_t n, const RsVec &v, hipStr
 int rs_fold_1b_rows_blocks(uint64_t n);
 hipError_t launch_rs_fold_1b_rows(const uint4 *mat, const uint4 *vec, uint64_t n, uint4 *partial, uint4 *out,
                                   hipStream_t s);
+// dst[i] = the 16 bytes at device address src[i] (src itself in device memory).
+hipError_t launch_gather16(const uint64_t *src, uint4 *dst, uint64_t n, hipStream_t s);
 hipError_t launch_pack_witness(const uint64_t *words, uint64_t n_words, uint4 *out, uint64_t n_elems,
                                hipStream_t s);
 

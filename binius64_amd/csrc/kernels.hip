@@ -935,6 +935,23 @@ hipError_t launch_pack_witness(const uint64_t *words, uint64_t n_words, uint4 *o
 }
 
 // ---------------------------------------------------------------------------
+// Gather of 16-byte granules from device addresses: dst[i] = *src[i].  The FRI
+// query phase lists every opened coset value and branch digest in tape order,
+// so the whole opening proof comes back in one copy.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_gather16(const uint64_t *__restrict__ src, uint4 *__restrict__ dst,
+                                                    uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) dst[i] = *reinterpret_cast<const uint4 *>(src[i]);
+}
+
+hipError_t launch_gather16(const uint64_t *src, uint4 *dst, uint64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    k_gather16<<<grid_for(n), BLOCK, 0, s>>>(src, dst, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Ring switch (prover/prover/src/ring_switch.rs)
 // ---------------------------------------------------------------------------
 
@@ -1053,16 +1070,27 @@ __global__ __launch_bounds__(BLOCK) void k_rs_fold_1b_rows(const uint4 *__restri
     if (!half) partial[(uint64_t)blockIdx.x * 128 + b] = gxor(acc, other[b]);
 }
 
+// out[b] = XOR over blocks k of partial[k][b]: one block per output b, its
+// lanes striding over the partial sums.
 __global__ __launch_bounds__(BLOCK) void k_rs_reduce(const uint4 *__restrict__ partial, int n_blocks,
                                                      uint4 *__restrict__ out) {
-    // thread (b, quarter): quarter q XORs blocks q, q + 2, ... of output b (two per b)
-    const int b = threadIdx.x & 127, q = threadIdx.x >> 7;
-    __shared__ uint4 other[128];
+    __shared__ uint4 red[BLOCK / 64];
+    const int b = blockIdx.x;
     uint4 acc = make_uint4(0, 0, 0, 0);
-    for (int k = q; k < n_blocks; k += 2) acc = gxor(acc, partial[(uint64_t)k * 128 + b]);
-    if (q) other[b] = acc;
+    for (int k = threadIdx.x; k < n_blocks; k += BLOCK) acc = gxor(acc, partial[(uint64_t)k * 128 + b]);
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        acc.x ^= __shfl_xor(acc.x, m);
+        acc.y ^= __shfl_xor(acc.y, m);
+        acc.z ^= __shfl_xor(acc.z, m);
+        acc.w ^= __shfl_xor(acc.w, m);
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (!q) out[b] = gxor(acc, other[b]);
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < BLOCK / 64; w++) acc = gxor(acc, red[w]);
+        out[b] = acc;
+    }
 }
 
 int rs_fold_1b_rows_blocks(uint64_t n) {
@@ -1074,7 +1102,7 @@ hipError_t launch_rs_fold_1b_rows(const uint4 *mat, const uint4 *vec, uint64_t n
                                   hipStream_t s) {
     const int blocks = rs_fold_1b_rows_blocks(n);
     k_rs_fold_1b_rows<<<blocks, BLOCK, 0, s>>>(mat, vec, n, partial);
-    k_rs_reduce<<<1, BLOCK, 0, s>>>(partial, blocks, out);
+    k_rs_reduce<<<128, BLOCK, 0, s>>>(partial, blocks, out);
     return hipGetLastError();
 }
 

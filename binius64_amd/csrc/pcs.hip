@@ -70,9 +70,14 @@ __device__ __forceinline__ void block_xor2(uint4 &p, uint4 &q) {
 // elements: every lane reads the four elements whose folds it needs and writes
 // the two it owns, so the in-place update is race-free.  The multiplies by r go
 // through a window table of r; the products A*B are general products.
+// Each block writes its two partial sums; the block that finishes last (a
+// self-resetting atomicInc counter) XORs all of them and writes the round's
+// totals to `result`, host-visible pinned memory read after the stream syncs.
 __global__ __launch_bounds__(SC_BLOCK) void k_sc_round(uint4 *__restrict__ a, uint4 *__restrict__ b, uint64_t half,
-                                                       int fold, uint4 r, uint4 *__restrict__ partial) {
+                                                       int fold, uint4 r, uint4 *__restrict__ partial,
+                                                       unsigned int *__restrict__ counter, uint4 *__restrict__ result) {
     __shared__ __attribute__((aligned(256))) uint4 tab[WIN_ENTRIES];
+    __shared__ int is_last;
     if (fold) {
         if (threadIdx.x < WIN_ROWS) win_build_row(tab, r, threadIdx.x);
         __syncthreads();
@@ -106,35 +111,38 @@ __global__ __launch_bounds__(SC_BLOCK) void k_sc_round(uint4 *__restrict__ a, ui
     if (threadIdx.x == 0) {
         partial[2 * blockIdx.x] = y1;
         partial[2 * blockIdx.x + 1] = yi;
+        __threadfence();  // publish the partials before counting this block done
+        is_last = atomicInc(counter, gridDim.x - 1) == gridDim.x - 1;
     }
-}
-
-__global__ __launch_bounds__(SC_BLOCK) void k_sc_reduce(const uint4 *__restrict__ partial, int n_blocks,
-                                                        uint4 *__restrict__ out) {
-    uint4 y1 = make_uint4(0, 0, 0, 0), yi = make_uint4(0, 0, 0, 0);
-    for (int k = threadIdx.x; k < n_blocks; k += SC_BLOCK) {
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();  // every other block's partials are visible from here
+    y1 = make_uint4(0, 0, 0, 0);
+    yi = make_uint4(0, 0, 0, 0);
+    for (unsigned k = threadIdx.x; k < gridDim.x; k += SC_BLOCK) {
         y1 = gxor(y1, partial[2 * k]);
         yi = gxor(yi, partial[2 * k + 1]);
     }
+    __syncthreads();  // block_xor2's scratch is reused
     block_xor2(y1, yi);
     if (threadIdx.x == 0) {
-        out[0] = y1;
-        out[1] = yi;
+        result[0] = y1;
+        result[1] = yi;
     }
 }
 
-// fold_highest_var_inplace on both multilinears: x[i] += r (x[i] + x[i + len]), i < len.
-__global__ __launch_bounds__(SC_BLOCK) void k_sc_fold(uint4 *__restrict__ a, uint4 *__restrict__ b, uint64_t len,
-                                                      uint4 r) {
-    __shared__ __attribute__((aligned(256))) uint4 tab[WIN_ENTRIES];
-    if (threadIdx.x < WIN_ROWS) win_build_row(tab, r, threadIdx.x);
-    __syncthreads();
-    const uint32_t tb = lds_addr(tab);
-    for (uint64_t i = (uint64_t)blockIdx.x * SC_BLOCK + threadIdx.x; i < len; i += (uint64_t)gridDim.x * SC_BLOCK) {
-        const uint4 x0 = a[i], x1 = a[i + len], z0 = b[i], z1 = b[i + len];
-        a[i] = gxor(x0, win_mul_perm(tb, gxor(x0, x1)));
-        b[i] = gxor(z0, win_mul_perm(tb, gxor(z0, z1)));
+// finish (bivariate_product.rs:100-121): the pending fold of both length-2
+// multilinears, if any, and their single values written to host-visible memory.
+__global__ void k_sc_final(const uint4 *__restrict__ a, const uint4 *__restrict__ b, int fold, uint4 r,
+                           uint4 *__restrict__ result) {
+    if (threadIdx.x != 0) return;
+    uint4 x = a[0], z = b[0];
+    if (fold) {
+        x = gxor(x, gf_mul(r, gxor(x, a[1])));
+        z = gxor(z, gf_mul(r, gxor(z, b[1])));
     }
+    result[0] = x;
+    result[1] = z;
 }
 
 // ---- eq_ind_partial_eval -----------------------------------------------------
@@ -246,12 +254,13 @@ struct b64_sumcheck {
     bool pending = false;     // a fold by r_pending not yet applied on the device
     u128 sum = 0, r_pending = 0;
     u128 coeffs[3] = {0, 0, 0};
-    uint4 *d_partial = nullptr;  // 2 * SC_MAX_BLOCKS partial sums, then the 2 totals
-    uint4 *h_res = nullptr;      // pinned, 2 elements
+    uint4 *d_partial = nullptr;  // 2 * SC_MAX_BLOCKS partial sums, then the block counter
+    uint4 *h_res = nullptr;      // pinned, 2 elements, written by the kernels
+    uint4 *h_res_dev = nullptr;  // h_res as the device addresses it
     hipStream_t s = nullptr;
 
     ~b64_sumcheck() {
-        if (d_partial) (void)hipFree(d_partial);
+        if (d_partial) (void)hipFreeAsync(d_partial, s);
         if (h_res) (void)hipHostFree(h_res);
     }
 };
@@ -270,10 +279,17 @@ int b64_sumcheck_create(void *d_a, void *d_b, uint32_t n_vars, const uint64_t su
     sc->n_vars = n_vars;
     sc->sum = ld128(sum);
     sc->s = as_stream(stream);
-    if (hipMalloc(&sc->d_partial, (2 * b64::SC_MAX_BLOCKS + 2) * sizeof(uint4)) != hipSuccess ||
-        hipHostMalloc(&sc->h_res, 2 * sizeof(uint4)) != hipSuccess) {
+    b64::abi::keep_pool_memory();
+    if (hipMallocAsync((void **)&sc->d_partial, (2 * b64::SC_MAX_BLOCKS + 1) * sizeof(uint4), sc->s) != hipSuccess ||
+        hipHostMalloc((void **)&sc->h_res, 2 * sizeof(uint4), hipHostMallocDefault) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&sc->h_res_dev, sc->h_res, 0) != hipSuccess) {
         delete sc;
         return B64_ERR_OUT_OF_MEMORY;
+    }
+    // the block counter starts at zero and resets itself after every round
+    if (hipMemsetAsync(sc->d_partial + 2 * b64::SC_MAX_BLOCKS, 0, sizeof(uint4), sc->s) != hipSuccess) {
+        delete sc;
+        return B64_ERR_HIP;
     }
     *out = sc;
     return B64_OK;
@@ -289,12 +305,10 @@ int b64_sumcheck_execute(b64_sumcheck *sc, uint64_t coeffs_out[6]) {
     if (sc->n_vars == 0) return B64_ERR_INVALID_ARGS;  // assert!(n_vars_remaining > 0)
     const uint64_t half = 1ull << (sc->n_vars - 1);
     const int blocks = b64::sc_blocks(half);
-    uint4 *totals = sc->d_partial + 2 * b64::SC_MAX_BLOCKS;
-    b64::k_sc_round<<<blocks, b64::SC_BLOCK, 0, sc->s>>>(sc->a, sc->b, half, sc->pending ? 1 : 0,
-                                                          to_u4(sc->r_pending), sc->d_partial);
-    b64::k_sc_reduce<<<1, b64::SC_BLOCK, 0, sc->s>>>(sc->d_partial, blocks, totals);
+    b64::k_sc_round<<<blocks, b64::SC_BLOCK, 0, sc->s>>>(
+        sc->a, sc->b, half, sc->pending ? 1 : 0, to_u4(sc->r_pending), sc->d_partial,
+        (unsigned int *)(sc->d_partial + 2 * b64::SC_MAX_BLOCKS), sc->h_res_dev);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(sc->h_res, totals, 2 * sizeof(uint4), hipMemcpyDeviceToHost, sc->s));
     HIP_TRY(hipStreamSynchronize(sc->s));
     sc->pending = false;
     // RoundEvals2::interpolate (round_evals.rs:73-78, :126-140)
@@ -327,14 +341,10 @@ int b64_sumcheck_finish(b64_sumcheck *sc, uint64_t evals_out[4]) {
     if (!sc || !evals_out) return B64_ERR_INVALID_ARGS;
     if (sc->n_vars > 0) return sc->has_coeffs ? B64_ERR_EXPECTED_FOLD : B64_ERR_EXPECTED_EXECUTE;
     if (sc->has_coeffs) return B64_ERR_EXPECTED_FOLD;
-    if (sc->pending) {
-        b64::k_sc_fold<<<1, b64::SC_BLOCK, 0, sc->s>>>(sc->a, sc->b, 1, to_u4(sc->r_pending));
-        HIP_TRY(hipGetLastError());
-        sc->pending = false;
-    }
-    HIP_TRY(hipMemcpyAsync(sc->h_res, sc->a, sizeof(uint4), hipMemcpyDeviceToHost, sc->s));
-    HIP_TRY(hipMemcpyAsync(sc->h_res + 1, sc->b, sizeof(uint4), hipMemcpyDeviceToHost, sc->s));
+    b64::k_sc_final<<<1, 64, 0, sc->s>>>(sc->a, sc->b, sc->pending ? 1 : 0, to_u4(sc->r_pending), sc->h_res_dev);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(sc->s));
+    sc->pending = false;
     st128(evals_out, from_u4(sc->h_res[0]));
     st128(evals_out + 2, from_u4(sc->h_res[1]));
     return B64_OK;
@@ -413,7 +423,8 @@ int b64_pcs_prove(const b64_fri_params *p, const b64_domain *d, const void *d_co
     if (n_point != log_len + KAPPA || log_len > (uint32_t)b64::EQ_MAX_VARS) return B64_ERR_INVALID_ARGS;
     const hipStream_t s = as_stream(stream);
     void *d_suffix = nullptr, *d_shv = nullptr;
-    HIP_TRY(hipMalloc(&d_suffix, (16ull << log_len) + 128 * 16));
+    b64::abi::keep_pool_memory();
+    HIP_TRY(hipMallocAsync(&d_suffix, (16ull << log_len) + 128 * 16, s));
     d_shv = (uint8_t *)d_suffix + (16ull << log_len);
     int rc = B64_OK;
     do {
@@ -444,8 +455,7 @@ int b64_pcs_prove(const b64_fri_params *p, const b64_domain *d, const void *d_co
         st128(cl, claim);
         rc = b64_basefold_prove(p, d, d_packed, d_suffix, cl, d_codeword, d_tree, t, stream);
     } while (false);
-    (void)hipStreamSynchronize(s);
-    (void)hipFree(d_suffix);
+    (void)hipFreeAsync(d_suffix, s);
     return rc;
 }
 
