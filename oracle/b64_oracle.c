@@ -1477,6 +1477,314 @@ int bo_pcs_prove(const bo_domain *d, uint64_t *packed, const uint64_t *eval_poin
 }
 
 /* ===================================================================== */
+/* Vision-M hash (verifier/verifier/src/hash/vision_{4,6})                 */
+/* ===================================================================== */
+#define VIS_ROUNDS 8
+#define VIS_MAXM 6
+
+struct bo_vision {
+    int m;
+    u128 b_fwd[4];                /* B(x) = c0 + c1 x + c2 x^2 + c3 x^4 (constants.rs B_FWD_COEFFS) */
+    u128 b_inv_cols[128], b_inv0; /* B^-1(y) = sum_j bit_j(y) cols[j] + b_inv0 */
+    u128 rc[1 + 2 * VIS_ROUNDS][VIS_MAXM];
+    u128 mds[VIS_MAXM][VIS_MAXM];
+};
+
+static inline u128 gf_sqr(u128 a) { return gf_mul(a, a); }
+
+static u128 gf_sqr_n(u128 a, int n) {
+    while (n--) a = gf_sqr(a);
+    return a;
+}
+
+/* a^(2^128 - 2) by the Itoh-Tsujii chain: b_k = a^(2^k - 1) with
+ * b_(i+j) = b_i^(2^j) * b_j over k = 1, 2, 3, 6, 7, 14, 15, 30, 31, 62, 63, 126, 127,
+ * then one more squaring. */
+static u128 gf_inv_chain(u128 a) {
+    u128 b1 = a;
+    u128 b2 = gf_mul(gf_sqr_n(b1, 1), b1);
+    u128 b3 = gf_mul(gf_sqr_n(b2, 1), b1);
+    u128 b6 = gf_mul(gf_sqr_n(b3, 3), b3);
+    u128 b7 = gf_mul(gf_sqr_n(b6, 1), b1);
+    u128 b14 = gf_mul(gf_sqr_n(b7, 7), b7);
+    u128 b15 = gf_mul(gf_sqr_n(b14, 1), b1);
+    u128 b30 = gf_mul(gf_sqr_n(b15, 15), b15);
+    u128 b31 = gf_mul(gf_sqr_n(b30, 1), b1);
+    u128 b62 = gf_mul(gf_sqr_n(b31, 31), b31);
+    u128 b63 = gf_mul(gf_sqr_n(b62, 1), b1);
+    u128 b126 = gf_mul(gf_sqr_n(b63, 63), b63);
+    u128 b127 = gf_mul(gf_sqr_n(b126, 1), b1);
+    return gf_sqr(b127);
+}
+
+/* batch_invert (verifier/math/src/batch_invert.rs): invert_or_zero of every
+ * element with one field inversion (zeros stand in as ones and stay zero). */
+static void batch_inv(u128 *x, int n) {
+    u128 pre[VIS_MAXM], acc = 1;
+    for (int i = 0; i < n; i++) {
+        pre[i] = acc;
+        acc = gf_mul(acc, x[i] ? x[i] : 1);
+    }
+    u128 inv = gf_inv_chain(acc);
+    for (int i = n - 1; i >= 0; i--) {
+        const u128 xi = x[i] ? x[i] : 1;
+        const u128 out = gf_mul(inv, pre[i]);
+        inv = gf_mul(inv, xi);
+        x[i] = x[i] ? out : 0;
+    }
+}
+
+void bo_vision_batch_invert(uint64_t *x, int n) {
+    u128 t[VIS_MAXM];
+    if (n > VIS_MAXM) return;
+    for (int i = 0; i < n; i++) t[i] = ld(x + 2 * i);
+    batch_inv(t, n);
+    for (int i = 0; i < n; i++) st(x + 2 * i, t[i]);
+}
+
+static u128 b_fwd_linear(const bo_vision *v, u128 x) {
+    u128 x2 = gf_sqr(x), x4 = gf_sqr(x2);
+    return gf_mul(v->b_fwd[1], x) ^ gf_mul(v->b_fwd[2], x2) ^ gf_mul(v->b_fwd[3], x4);
+}
+
+static u128 b_fwd(const bo_vision *v, u128 x) { return v->b_fwd[0] ^ b_fwd_linear(v, x); }
+
+static u128 b_inv(const bo_vision *v, u128 y) {
+    u128 acc = v->b_inv0;
+    for (int j = 0; j < 128; j++)
+        if ((y >> j) & 1) acc ^= v->b_inv_cols[j];
+    return acc;
+}
+
+/* The inverse of B's linear part, column by column: Gauss-Jordan elimination
+ * over GF(2) on [L | I] with L's columns L(x^j).  B^-1(y) = L^-1(y) + L^-1(c0)
+ * (the affine form that constants.rs B_INV_COEFFS encodes as a linearized
+ * polynomial).  Returns nonzero if L is singular. */
+static int invert_b_linear(bo_vision *v) {
+    /* rows r of the augmented matrix: bits of row r of L (as u128 over columns j) and of I */
+    u128 a[128], e[128];
+    for (int r = 0; r < 128; r++) {
+        a[r] = 0;
+        e[r] = (u128)1 << r;
+    }
+    for (int j = 0; j < 128; j++) {
+        u128 col = b_fwd_linear(v, (u128)1 << j);
+        for (int r = 0; r < 128; r++)
+            if ((col >> r) & 1) a[r] |= (u128)1 << j;
+    }
+    for (int c = 0; c < 128; c++) {
+        int p = -1;
+        for (int r = c; r < 128; r++)
+            if ((a[r] >> c) & 1) {
+                p = r;
+                break;
+            }
+        if (p < 0) return 1;
+        u128 t = a[p]; a[p] = a[c]; a[c] = t;
+        t = e[p]; e[p] = e[c]; e[c] = t;
+        for (int r = 0; r < 128; r++)
+            if (r != c && ((a[r] >> c) & 1)) {
+                a[r] ^= a[c];
+                e[r] ^= e[c];
+            }
+    }
+    /* now e = L^-1 as rows: bit j of row r = entry (r, j); column j = L^-1(x^j) */
+    for (int j = 0; j < 128; j++) {
+        u128 col = 0;
+        for (int r = 0; r < 128; r++)
+            if ((e[r] >> j) & 1) col |= (u128)1 << r;
+        v->b_inv_cols[j] = col;
+    }
+    v->b_inv0 = 0;
+    v->b_inv0 = b_inv(v, v->b_fwd[0]);
+    return 0;
+}
+
+/* sbox (permutation.rs): batch inversion of the state, then B or B^-1. */
+static void vis_sbox(const bo_vision *v, u128 *s, int inverse) {
+    batch_inv(s, v->m);
+    for (int i = 0; i < v->m; i++) s[i] = inverse ? b_inv(v, s[i]) : b_fwd(v, s[i]);
+}
+
+/* mds_mul as the matrix it implements (permutation.rs test_mds for M = 4; the
+ * circulant rows of vision_6/permutation.rs mds_mul for M = 6). */
+static void vis_mds(const bo_vision *v, u128 *s) {
+    u128 o[VIS_MAXM];
+    for (int i = 0; i < v->m; i++) {
+        o[i] = 0;
+        for (int j = 0; j < v->m; j++) o[i] ^= gf_mul(v->mds[i][j], s[j]);
+    }
+    memcpy(s, o, sizeof(u128) * (size_t)v->m);
+}
+
+static void vis_add(const bo_vision *v, u128 *s, const u128 *c) {
+    for (int i = 0; i < v->m; i++) s[i] ^= c[i];
+}
+
+/* permutation (permutation.rs): RC[0], then 8 rounds of
+ * (S-box with B^-1, MDS, RC) and (S-box with B, MDS, RC). */
+static void vis_permute(const bo_vision *v, u128 *s) {
+    vis_add(v, s, v->rc[0]);
+    for (int r = 0; r < VIS_ROUNDS; r++) {
+        vis_sbox(v, s, 1);
+        vis_mds(v, s);
+        vis_add(v, s, v->rc[1 + 2 * r]);
+        vis_sbox(v, s, 0);
+        vis_mds(v, s);
+        vis_add(v, s, v->rc[2 + 2 * r]);
+    }
+}
+
+bo_vision *bo_vision_new(int m, const uint64_t *b_fwd_coeffs, const uint64_t *initial, const uint64_t *matrix,
+                         const uint64_t *constant) {
+    if (m != 4 && m != 6) return NULL;
+    bo_vision *v = (bo_vision *)calloc(1, sizeof(bo_vision));
+    v->m = m;
+    for (int i = 0; i < 4; i++) v->b_fwd[i] = ld(b_fwd_coeffs + 2 * i);
+    if (invert_b_linear(v)) {
+        free(v);
+        return NULL;
+    }
+    const u128 two = 2, three = 3, inv_two = gf_inv(2);
+    if (m == 4) {
+        const u128 rows[4][4] = {{two, three, 1, 1}, {1, two, three, 1}, {1, 1, two, three}, {three, 1, 1, two}};
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) v->mds[i][j] = rows[i][j];
+    } else {
+        const u128 row0[6] = {inv_two, inv_two, two, 1, two, three};
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) v->mds[i][j] = row0[(j - i + 6) % 6];
+    }
+    /* compute_round_constants (constants.rs test module): a key schedule run
+     * through the permutation's own S-boxes and MDS. */
+    u128 key_state[VIS_MAXM], key_inj[VIS_MAXM], cmat[VIS_MAXM * VIS_MAXM], cconst[VIS_MAXM];
+    for (int i = 0; i < m; i++) {
+        key_state[i] = key_inj[i] = v->rc[0][i] = ld(initial + 2 * i);
+        cconst[i] = ld(constant + 2 * i);
+    }
+    for (int i = 0; i < m * m; i++) cmat[i] = ld(matrix + 2 * i);
+    int idx = 1;
+    for (int r = 0; r < VIS_ROUNDS; r++)
+        for (int half = 0; half < 2; half++) {
+            u128 t[VIS_MAXM];
+            for (int i = 0; i < m; i++) {
+                t[i] = cconst[i];
+                for (int j = 0; j < m; j++) t[i] ^= gf_mul(cmat[i * m + j], key_inj[j]);
+            }
+            memcpy(key_inj, t, sizeof(u128) * (size_t)m);
+            vis_sbox(v, key_state, half == 0);
+            vis_mds(v, key_state);
+            vis_add(v, key_state, key_inj);
+            memcpy(v->rc[idx++], key_state, sizeof(u128) * (size_t)m);
+        }
+    return v;
+}
+
+void bo_vision_free(bo_vision *v) { free(v); }
+
+void bo_vision_round_constants(const bo_vision *v, uint64_t *out) {
+    for (int r = 0; r < 1 + 2 * VIS_ROUNDS; r++)
+        for (int i = 0; i < v->m; i++) st(out + 2 * (r * v->m + i), v->rc[r][i]);
+}
+
+void bo_vision_b_maps(const bo_vision *v, const uint64_t *x, uint64_t *fwd, uint64_t *inv, size_t n) {
+    for (size_t i = 0; i < n; i++) {
+        st(fwd + 2 * i, b_fwd(v, ld(x + 2 * i)));
+        st(inv + 2 * i, b_inv(v, ld(x + 2 * i)));
+    }
+}
+
+void bo_vision_permutation(const bo_vision *v, uint64_t *states, size_t n_states) {
+    for (size_t k = 0; k < n_states; k++) {
+        u128 s[VIS_MAXM];
+        for (int i = 0; i < v->m; i++) s[i] = ld(states + 2 * (k * v->m + i));
+        vis_permute(v, s);
+        for (int i = 0; i < v->m; i++) st(states + 2 * (k * v->m + i), s[i]);
+    }
+}
+
+/* VisionHasherDigest (digest.rs): sponge with rate M - 2 elements; each full
+ * block overwrites state[0 .. rate) and permutes; the last partial block is
+ * padded 0x80 .. 0x01 (or a whole padding block is absorbed); the digest is
+ * state[0] || state[1]. */
+void bo_vision_digest(const bo_vision *v, const uint8_t *data, size_t len, uint8_t out[32]) {
+    const int rate = v->m - 2;
+    const size_t rb = 16 * (size_t)rate;
+    u128 s[VIS_MAXM] = {0};
+    uint8_t blk[64];
+    size_t off = 0;
+    for (; off + rb <= len; off += rb) {
+        for (int i = 0; i < rate; i++) memcpy(&s[i], data + off + 16 * i, 16);
+        vis_permute(v, s);
+    }
+    size_t rem = len - off;
+    memset(blk, 0, rb);
+    memcpy(blk, data + off, rem);
+    blk[rem] |= 0x80;
+    blk[rb - 1] |= 0x01;
+    for (int i = 0; i < rate; i++) memcpy(&s[i], blk + 16 * i, 16);
+    vis_permute(v, s);
+    memcpy(out, &s[0], 16);
+    memcpy(out + 16, &s[1], 16);
+}
+
+/* VisionCompression (vision_4/compression.rs): permute [l0, l1, r0, r1],
+ * add the inputs' first two elements back, keep the first two. */
+void bo_vision_compress(const bo_vision *v4, const uint8_t left[32], const uint8_t right[32], uint8_t out[32]) {
+    u128 s[4];
+    memcpy(&s[0], left, 16);
+    memcpy(&s[1], left + 16, 16);
+    memcpy(&s[2], right, 16);
+    memcpy(&s[3], right + 16, 16);
+    u128 o0 = s[0], o1 = s[1];
+    vis_permute(v4, s);
+    s[0] ^= o0;
+    s[1] ^= o1;
+    memcpy(out, &s[0], 16);
+    memcpy(out + 16, &s[1], 16);
+}
+
+typedef struct {
+    const bo_vision *leaf, *node;
+    const u128 *elems;
+    size_t batch;
+    uint8_t *tree;
+    size_t in_off, out_off;
+} vmk_ctx;
+
+static void vmk_leaf_task(void *p, size_t i) {
+    vmk_ctx *c = (vmk_ctx *)p;
+    bo_vision_digest(c->leaf, (const uint8_t *)(c->elems + i * c->batch), 16 * c->batch, c->tree + 32 * i);
+}
+static void vmk_node_task(void *p, size_t i) {
+    vmk_ctx *c = (vmk_ctx *)p;
+    bo_vision_compress(c->node, c->tree + 32 * (c->in_off + 2 * i), c->tree + 32 * (c->in_off + 2 * i + 1),
+                       c->tree + 32 * (c->out_off + i));
+}
+
+/* BinaryMerkleTreeProver with a Vision leaf digest (Vision-4 or Vision-6
+ * sponge, prover/prover/benches/binary_merkle_tree.rs) and VisionCompression
+ * for the nodes; same flattened layout as bo_merkle_build. */
+int bo_merkle_build_vision(const bo_vision *leaf, const bo_vision *node4, const uint64_t *elems, size_t n_elems,
+                           size_t batch, uint8_t *tree, int nthreads) {
+    if (!batch || n_elems % batch) return 4;
+    size_t n_leaves = n_elems / batch;
+    if (!n_leaves || (n_leaves & (n_leaves - 1))) return 5;
+    if (node4->m != 4) return 1;
+    vmk_ctx c = {leaf, node4, (const u128 *)elems, batch, tree, 0, 0};
+    par_for(nthreads, n_leaves, vmk_leaf_task, &c);
+    size_t in_off = 0, n = n_leaves;
+    while (n > 1) {
+        c.in_off = in_off;
+        c.out_off = in_off + n;
+        par_for(nthreads, n / 2, vmk_node_task, &c);
+        in_off += n;
+        n /= 2;
+    }
+    return 0;
+}
+
+/* ===================================================================== */
 /* SplitMix64 input generator                                              */
 /* ===================================================================== */
 void bo_splitmix_fill(uint64_t seed, uint64_t *out, size_t n) {

@@ -140,6 +140,24 @@ void bo_transcript_sample(bo_transcript *t, uint8_t *out, size_t n);
 uint32_t bo_transcript_sample_bits(bo_transcript *t, int bits);
 size_t bo_transcript_tape(const bo_transcript *t, uint8_t *out, size_t cap);
 
+/* ---- Vision-M hash (verifier/verifier/src/hash/vision_{4,6}) --------------- */
+/* m = 4 or 6; b_fwd_coeffs: 4 elements (B_FWD_COEFFS); initial / matrix /
+ * constant: the round-key schedule's INITIAL_CONSTANT (m), CONSTANTS_MATRIX
+ * (m*m, row-major), CONSTANTS_CONSTANT (m).  Round constants and B^-1 are
+ * derived from them.  NULL for a bad m or a singular B. */
+typedef struct bo_vision bo_vision;
+bo_vision *bo_vision_new(int m, const uint64_t *b_fwd_coeffs, const uint64_t *initial, const uint64_t *matrix,
+                         const uint64_t *constant);
+void bo_vision_free(bo_vision *v);
+void bo_vision_round_constants(const bo_vision *v, uint64_t *out); /* (1 + 2*8) * m elements */
+void bo_vision_b_maps(const bo_vision *v, const uint64_t *x, uint64_t *fwd, uint64_t *inv, size_t n);
+void bo_vision_batch_invert(uint64_t *x, int n); /* batch_invert: invert_or_zero of n <= 6 elements */
+void bo_vision_permutation(const bo_vision *v, uint64_t *states, size_t n_states);
+void bo_vision_digest(const bo_vision *v, const uint8_t *data, size_t len, uint8_t out[32]);
+void bo_vision_compress(const bo_vision *v4, const uint8_t left[32], const uint8_t right[32], uint8_t out[32]);
+int bo_merkle_build_vision(const bo_vision *leaf, const bo_vision *node4, const uint64_t *elems, size_t n_elems,
+                           size_t batch, uint8_t *tree, int nthreads);
+
 /* ---- seeded input generator (SplitMix64; fixed for the whole repository) -- */
 void bo_splitmix_fill(uint64_t seed, uint64_t *out, size_t n_words);
 

@@ -79,6 +79,16 @@ def lib():
         L.bo_tensor_transpose.argtypes = [u64p, u64p]
         L.bo_pcs_prove.argtypes = [C.c_void_p, u64p, u64p, u64p, u8p, C.c_int, C.c_int, C.c_int, intp, C.c_int,
                                    C.c_int, C.c_void_p, C.c_int]
+        L.bo_vision_new.restype = C.c_void_p
+        L.bo_vision_new.argtypes = [C.c_int, u64p, u64p, u64p, u64p]
+        L.bo_vision_free.argtypes = [C.c_void_p]
+        L.bo_vision_round_constants.argtypes = [C.c_void_p, u64p]
+        L.bo_vision_b_maps.argtypes = [C.c_void_p, u64p, u64p, u64p, C.c_size_t]
+        L.bo_vision_batch_invert.argtypes = [u64p, C.c_int]
+        L.bo_vision_permutation.argtypes = [C.c_void_p, u64p, C.c_size_t]
+        L.bo_vision_digest.argtypes = [C.c_void_p, u8p, C.c_size_t, u8p]
+        L.bo_vision_compress.argtypes = [C.c_void_p, u8p, u8p, u8p]
+        L.bo_merkle_build_vision.argtypes = [C.c_void_p, C.c_void_p, u64p, C.c_size_t, C.c_size_t, u8p, C.c_int]
         L.bo_transcript_new.restype = C.c_void_p
         L.bo_transcript_free.argtypes = [C.c_void_p]
         for name in ("bo_transcript_message", "bo_transcript_observe", "bo_transcript_decommit",
@@ -499,3 +509,70 @@ def pcs_prove(dom: Domain, packed: np.ndarray, eval_point: np.ndarray, codeword:
                             _arities(arities), len(arities), n_queries, transcript.ptr, threads)
     if rc:
         raise ValueError(f"pcs_prove error {rc}")
+
+
+# ---- Vision-M hash -------------------------------------------------------------
+class Vision:
+    """Vision-M permutation, sponge digest and (M = 4) compression, built from the
+    hash's defining data (B_FWD_COEFFS and the round-key schedule inputs)."""
+
+    def __init__(self, m: int, data: dict):
+        def arr(key):
+            return to_elems(int(x, 16) for x in data[key])
+        self.m = m
+        self._b, self._i, self._mat, self._c = (arr("b_fwd_coeffs"), arr("initial_constant"),
+                                                arr("constants_matrix"), arr("constants_constant"))
+        self.ptr = lib().bo_vision_new(m, _p64(self._b), _p64(self._i), _p64(self._mat), _p64(self._c))
+        if not self.ptr:
+            raise ValueError("vision setup failed")
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().bo_vision_free(self.ptr)
+            self.ptr = None
+
+    def round_constants(self):
+        out = np.zeros((17 * self.m, 2), dtype=np.uint64)
+        lib().bo_vision_round_constants(self.ptr, _p64(out))
+        return to_ints(out)
+
+    def b_maps(self, x: np.ndarray):
+        x = np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, 2)
+        f, i = np.zeros_like(x), np.zeros_like(x)
+        lib().bo_vision_b_maps(self.ptr, _p64(x), _p64(f), _p64(i), x.shape[0])
+        return f, i
+
+    def permutation(self, states: np.ndarray) -> np.ndarray:
+        st = np.array(states, dtype=np.uint64).reshape(-1, 2)
+        assert st.shape[0] % self.m == 0
+        lib().bo_vision_permutation(self.ptr, _p64(st), st.shape[0] // self.m)
+        return st
+
+    def digest(self, data: bytes) -> bytes:
+        buf = np.frombuffer(data, dtype=np.uint8).copy() if data else np.zeros(1, dtype=np.uint8)
+        out = np.zeros(32, dtype=np.uint8)
+        lib().bo_vision_digest(self.ptr, _p8(buf), len(data), _p8(out))
+        return out.tobytes()
+
+    def compress(self, left: bytes, right: bytes) -> bytes:
+        assert self.m == 4
+        out = np.zeros(32, dtype=np.uint8)
+        lib().bo_vision_compress(self.ptr, _p8(np.frombuffer(left, dtype=np.uint8).copy()),
+                                 _p8(np.frombuffer(right, dtype=np.uint8).copy()), _p8(out))
+        return out.tobytes()
+
+
+def vision_batch_invert(x: np.ndarray) -> np.ndarray:
+    v = np.array(x, dtype=np.uint64).reshape(-1, 2)
+    lib().bo_vision_batch_invert(_p64(v), v.shape[0])
+    return v
+
+
+def merkle_build_vision(leaf: Vision, node: Vision, elems: np.ndarray, batch: int, threads: int = 1) -> np.ndarray:
+    e = np.ascontiguousarray(elems, dtype=np.uint64).reshape(-1, 2)
+    n_leaves = e.shape[0] // batch if batch else 0
+    tree = np.zeros((max(2 * n_leaves - 1, 1), 32), dtype=np.uint8)
+    rc = lib().bo_merkle_build_vision(leaf.ptr, node.ptr, _p64(e), e.shape[0], batch, _p8(tree), threads)
+    if rc:
+        raise ValueError(f"merkle error {rc}")
+    return tree

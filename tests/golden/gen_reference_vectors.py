@@ -13,12 +13,70 @@ Writes tests/golden/reference_vectors.json with
   * multiplicative_generator: BinaryField128bGhash::MULTIPLICATIVE_GENERATOR (ghash.rs:502).
   * fri_kats: calculate_n_test_queries / estimate_optimal_arity expectations
     (verifier/verifier/src/fri/common.rs:225-247).
+  * vision4 / vision6: the Vision-M hash data of verifier/verifier/src/hash/vision_{4,6}/:
+    B_FWD_COEFFS, B_INV_COEFFS and ROUND_CONSTANTS (constants.rs), the round-key
+    schedule inputs INITIAL_CONSTANT, CONSTANTS_MATRIX, CONSTANTS_CONSTANT of its
+    test module, and the permutation test vectors of test_permutation
+    (permutation.rs; the case whose input comes from a seeded RNG is skipped).
 Only numbers are copied; no source text is kept.
 """
 import json
 import re
 import sys
 from pathlib import Path
+
+
+GHASH_RE = r"Ghash::new\((0x[0-9a-fA-F]+)\)"
+
+
+def _static_values(src: str, name: str):
+    start = src.index(name)
+    start = src.index("= [", start)
+    depth, i = 0, start + 2
+    while True:  # the matching closing bracket of the initializer
+        if src[i] == "[":
+            depth += 1
+        elif src[i] == "]":
+            depth -= 1
+            if depth == 0:
+                break
+        i += 1
+    return [hex(int(v, 16)) for v in re.findall(GHASH_RE, src[start:i])]
+
+
+def vision_data(root: Path, m: int) -> dict:
+    base = root / f"verifier/verifier/src/hash/vision_{m}"
+    const = (base / "constants.rs").read_text()
+    perm = (base / "permutation.rs").read_text()
+    data = {
+        "m": m,
+        "b_fwd_coeffs": _static_values(const, "static B_FWD_COEFFS"),
+        "b_inv_coeffs": _static_values(const, "static B_INV_COEFFS"),
+        "round_constants": _static_values(const, "ROUND_CONSTANTS: [[Ghash"),
+        "initial_constant": _static_values(const, "static INITIAL_CONSTANT"),
+        "constants_matrix": _static_values(const, "static CONSTANTS_MATRIX"),
+        "constants_constant": _static_values(const, "static CONSTANTS_CONSTANT"),
+    }
+    assert len(data["b_fwd_coeffs"]) == 4 and len(data["b_inv_coeffs"]) == 129
+    assert len(data["round_constants"]) == 17 * m and len(data["constants_matrix"]) == m * m
+    # test_permutation cases: (input, expected) pairs in source order
+    body = perm[perm.index("fn test_permutation()"):]
+    body = body[: body.index("for (input, expected) in cases")]
+    kats = []
+    for case in re.split(r"\n\t\t\t\(\n", body)[1:]:
+        vals = re.findall(GHASH_RE, case)
+        if "Ghash::random" in case:
+            continue
+        if "array::from_fn(|_| Ghash::new(" in case:
+            inp = [vals[0]] * m
+            exp = vals[1:]
+        else:
+            inp, exp = vals[:m], vals[m:]
+        assert len(inp) == m and len(exp) == m, (m, case)
+        kats.append([[hex(int(v, 16)) for v in inp], [hex(int(v, 16)) for v in exp]])
+    assert len(kats) == 2, kats
+    data["permutation_kats"] = kats
+    return data
 
 
 def main(ref_root: str) -> None:
@@ -59,7 +117,10 @@ def main(ref_root: str) -> None:
         vals = re.findall(r"^\s*(0x[0-9a-fA-F]+)(?:u128)?,", seg, re.M)
         return [hex(int(v, 16)) for v in vals]
 
+    out_vision = {f"vision{m}": vision_data(Path(ref_root), m) for m in (4, 6)}
+
     out = {
+        **out_vision,
         "aes_to_ghash": [hex(v) for v in table],
         "ghash_mul_kats": kats,
         "mul_x_cases": cases("test_mul_x"),
