@@ -540,6 +540,36 @@ class OneBitPCSProver:
                                   pt.shape[0], transcript.ptr, _stream(stream)), "pcs_prove")
 
 
+class VisionHasher:
+    """Vision-M (M = 4 or 6) permutation and sponge digest on device buffers
+    (verifier/verifier/src/hash/vision_{4,6})."""
+
+    def __init__(self, m: int):
+        if m not in (4, 6):
+            raise ValueError("Vision width must be 4 or 6")
+        self.m = m
+
+    def permute(self, states: DeviceBuffer, n: int, stream=None) -> None:
+        check(lib().b64_vision_permute(self.m, states.ptr, n, _stream(stream)), "vision_permute")
+
+    def digest_many(self, msgs: DeviceBuffer, n: int, msg_elems: int, out: Optional[DeviceBuffer] = None,
+                    stream=None) -> DeviceBuffer:
+        out = out or DeviceBuffer(32 * max(n, 1))
+        check(lib().b64_vision_digest(self.m, msgs.ptr, n, msg_elems, out.ptr, _stream(stream)), "vision_digest")
+        return out
+
+
+def merkle_commit_vision(data: DeviceBuffer, n_elems: int, batch_size: int, leaf_width: int = 4,
+                         tree: Optional[DeviceBuffer] = None, stream=None):
+    """BinaryMerkleTreeProver::commit with Vision leaves and VisionCompression nodes."""
+    n_leaves = n_elems // batch_size if batch_size else 0
+    tree = tree or DeviceBuffer(32 * max(2 * n_leaves - 1, 1))
+    root = np.zeros(32, dtype=np.uint8)
+    check(lib().b64_merkle_commit_vision(leaf_width, data.ptr, n_elems, batch_size, tree.ptr, _p8(root),
+                                         _stream(stream)), "merkle_commit_vision")
+    return root.tobytes(), tree
+
+
 class CommitJob:
     """One FRI commit phase with pre-seeded challenges (b64_job_desc): encode,
     Merkle commit, every fold round and its commitment."""

@@ -133,6 +133,9 @@ SIGNATURES = {
     "b64_sumcheck_fold": (C.c_int, [vp, u64p]),
     "b64_sumcheck_finish": (C.c_int, [vp, u64p]),
     "b64_basefold_prove": (C.c_int, [C.POINTER(FriParamsC), vp, vp, vp, u64p, vp, vp, vp, vp]),
+    "b64_vision_permute": (C.c_int, [C.c_uint32, vp, C.c_uint64, vp]),
+    "b64_vision_digest": (C.c_int, [C.c_uint32, vp, C.c_uint64, C.c_uint64, vp, vp]),
+    "b64_merkle_commit_vision": (C.c_int, [C.c_uint32, vp, C.c_uint64, C.c_uint64, vp, u8p, vp]),
     "b64_pcs_prove": (C.c_int, [C.POINTER(FriParamsC), vp, vp, vp, vp, u64p, C.c_uint32, vp, vp]),
 }
 

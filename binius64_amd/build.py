@@ -14,7 +14,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libb64hip.so"
-SOURCES = ["api.hip", "kernels.hip", "batch.hip", "server.hip", "pcs.hip"]
+SOURCES = ["api.hip", "kernels.hip", "batch.hip", "server.hip", "pcs.hip", "vision.hip"]
 ARCH = os.environ.get("B64_OFFLOAD_ARCH", "gfx950")
 
 

@@ -1,0 +1,554 @@
+// Vision-4 / Vision-6 hash on the device: the permutation
+// (verifier/verifier/src/hash/vision_{4,6}/permutation.rs), the sponge digest
+// (digest.rs), the Vision-4 compression (vision_4/compression.rs) and Merkle
+// trees built from them (BinaryMerkleTreeProver with VisionHasherDigest leaves,
+// prover/examples/src/lib.rs setup_vision4, prover/prover/benches/binary_merkle_tree.rs).
+//
+// Only the hash's defining data is held here: the coefficients of the affine
+// map B(x) = c0 + c1 x + c2 x^2 + c3 x^4 and the inputs of the round-key
+// schedule.  The round constants, B^-1 and every lookup table are derived on
+// the host when a device first uses a width:
+//  * round constants: the key schedule of the reference's constants tests, run
+//    through this file's own S-box and MDS;
+//  * B^-1: B's linear part inverted over GF(2) (Gauss-Jordan on its 128 x 128
+//    bit matrix), plus the constant B^-1(0) = L^-1(c0);
+//  * window tables (the 4-bit layout of gf128_dev.hpp's win_mul_perm) for the
+//    F2-linear maps the S-box needs: x^(2^k) for k = 1, 3, 7, 15, 31 (the
+//    Itoh-Tsujii inversion chain) and the linear parts of B and B^-1.
+//
+// On the device one lane owns one state.  The S-box's batch inversion
+// (verifier/math/src/batch_invert.rs: invert_or_zero of every element) uses one
+// field inversion per S-box for the whole state (prefix products, then back
+// substitution); the inversion is a^(2^128-2) by the Itoh-Tsujii chain, 12
+// general products and 15 table maps.  The seven tables (56 KiB) and the round
+// constants sit in LDS for the whole launch.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <mutex>
+#include <new>
+
+#include "abi_util.hpp"
+#include "gf128_dev.hpp"
+#include "internal.hpp"
+
+using b64::abi::as_stream;
+using b64::host::u128;
+
+namespace b64 {
+namespace vision {
+namespace {
+
+struct u64x2 {
+    uint64_t lo, hi;
+};
+
+// B(x) = c0 + c1 x + c2 x^2 + c3 x^4 (constants.rs B_FWD_COEFFS, both widths)
+constexpr u64x2 B_FWD[4] = {
+    {0x4c29e4d892ca33ddull, 0xb0b849b207a0f1c7ull}, {0x855eeb8ce24c9523ull, 0xf2240891aac3c3a5ull},
+    {0x9d79908388dd768dull, 0xb017e96797ef1fbeull}, {0x00dbb86a2cb472bbull, 0x2bf9f0b94c2b2cebull},
+};
+// Vision-4 round-key schedule inputs (vision_4/constants.rs)
+constexpr u64x2 INITIAL4[4] = {
+    {0xf4a1d28234ffdf87ull, 0xd0e015190f5e0795ull}, {0xb93ec3a23ec7681bull, 0x6111731acd9a89f6ull},
+    {0x18a34a96728b8d29ull, 0x15da8de707ee3f39ull}, {0xed2a216b1d232bfbull, 0x2ea920e89fbb13a1ull},
+};
+constexpr u64x2 MATRIX4[16] = {
+    {0x8ef96344bc8cbd0full, 0x1cac9af051191d2cull}, {0x4750ab0a854c3a4dull, 0x3ceb350e8c2d2f4dull},
+    {0x8e092e899156946bull, 0x09cc78f7eafef94dull}, {0xa3d58f7eedb30dbbull, 0x1d1ad17c8c7ee715ull},
+    {0xdd0cdae9dff5dfc5ull, 0x06bee572113950b3ull}, {0x8b4bfa1a5cd3d295ull, 0x4799c3743a356042ull},
+    {0x4cb89ac6bf636d0bull, 0x516883bf97b07fcfull}, {0xdf1a9113b04d755bull, 0x0b681d1621d6aa1full},
+    {0x3ddfe74c125e99e5ull, 0xe9ba2f01051f6b4bull}, {0x33e31cfb26b916d1ull, 0x1f58efc5ecf2e1b9ull},
+    {0xf67efc573241fe19ull, 0x36da671249b5444cull}, {0x615a130556e5eb99ull, 0x056cae18b867d486ull},
+    {0x8dd9df84956bcf11ull, 0x1d8645dd4b1e46b7ull}, {0x95efe6255339375bull, 0x3efa34e48e3b2183ull},
+    {0x97d0812db56c5effull, 0x79bf44bae0d23793ull}, {0x9e538151daddae85ull, 0x7fb3b7fce84776e3ull},
+};
+constexpr u64x2 CONSTANT4[4] = {
+    {0x93ca17e6bb10203full, 0x0d42981a71a7c2e4ull}, {0xca17ed48e18717b9ull, 0x9ceed672ccb7030full},
+    {0x95c6ecdb4a349d15ull, 0x0cec5c463024f2baull}, {0x990418f3a23e3c99ull, 0xa46023576301b995ull},
+};
+// Vision-6 round-key schedule inputs (vision_6/constants.rs)
+constexpr u64x2 INITIAL6[6] = {
+    {0xf4a1d28234ffdf87ull, 0xd0e015190f5e0795ull}, {0xb93ec3a23ec7681bull, 0x6111731acd9a89f6ull},
+    {0x18a34a96728b8d29ull, 0x15da8de707ee3f39ull}, {0xed2a216b1d232bfbull, 0x2ea920e89fbb13a1ull},
+    {0x8ef96344bc8cbd0full, 0x1cac9af051191d2cull}, {0x4750ab0a854c3a4dull, 0x3ceb350e8c2d2f4dull},
+};
+constexpr u64x2 MATRIX6[36] = {
+    {0x8e092e899156946bull, 0x09cc78f7eafef94dull}, {0xa3d58f7eedb30dbbull, 0x1d1ad17c8c7ee715ull},
+    {0xdd0cdae9dff5dfc5ull, 0x06bee572113950b3ull}, {0x8b4bfa1a5cd3d295ull, 0x4799c3743a356042ull},
+    {0x4cb89ac6bf636d0bull, 0x516883bf97b07fcfull}, {0xdf1a9113b04d755bull, 0x0b681d1621d6aa1full},
+    {0x3ddfe74c125e99e5ull, 0xe9ba2f01051f6b4bull}, {0x33e31cfb26b916d1ull, 0x1f58efc5ecf2e1b9ull},
+    {0xf67efc573241fe19ull, 0x36da671249b5444cull}, {0x615a130556e5eb99ull, 0x056cae18b867d486ull},
+    {0x8dd9df84956bcf11ull, 0x1d8645dd4b1e46b7ull}, {0x95efe6255339375bull, 0x3efa34e48e3b2183ull},
+    {0x97d0812db56c5effull, 0x79bf44bae0d23793ull}, {0x9e538151daddae85ull, 0x7fb3b7fce84776e3ull},
+    {0x93ca17e6bb10203full, 0x0d42981a71a7c2e4ull}, {0xca17ed48e18717b9ull, 0x9ceed672ccb7030full},
+    {0x95c6ecdb4a349d15ull, 0x0cec5c463024f2baull}, {0x990418f3a23e3c99ull, 0xa46023576301b995ull},
+    {0x5d33ae5d97eb83afull, 0x6227628fea1dc546ull}, {0x6d94bf331473f6f1ull, 0x2ef550b7cda1e285ull},
+    {0x698bc6ae2e3a8499ull, 0x375f4232fa7ca41aull}, {0x012b9ab22c303529ull, 0x2d2f859caba39ed7ull},
+    {0x0d0bf7b6399257b7ull, 0x5672a0afa29cac30ull}, {0xf3ae1aee7aa41593ull, 0xebd30b6903f1fe11ull},
+    {0x6f66f0ef2f18bd8dull, 0x39c54aa0738c373aull}, {0xb89afcd0d6a1a9f9ull, 0x083a52acda7952eaull},
+    {0xbafc31f177df4359ull, 0x07c782c6beef83baull}, {0xe5ca6278e76adc93ull, 0x1ed204d4affa05b8ull},
+    {0xa543ee1ec99323cbull, 0x5ce4a5c3be5e0bc5ull}, {0xe5f49f51af3c2df7ull, 0x0d54d1bf8ea15dafull},
+    {0xdc4e99bc400b9fa9ull, 0x22510fa8fffa288aull}, {0xf8135f4f4b2b4491ull, 0x78ca292f525f6a68ull},
+    {0x79afa9e365bfa159ull, 0x0ef5f99b1b4a36e9ull}, {0x3a2f3f77630cffa1ull, 0x7a999d6e5f0b133dull},
+    {0xd21d9ad73aef8469ull, 0xb1ae2573d4a15bb6ull}, {0x2535d2146c4869e1ull, 0x9c348d3213f5a078ull},
+};
+constexpr u64x2 CONSTANT6[6] = {
+    {0xb4ee8dd307ef621full, 0xabd5e3e6c590952dull}, {0x2cf0fc6f51bedcebull, 0x22681ba7c4bf168cull},
+    {0x1a74f42c784257a1ull, 0x41baa8130ac4ddf6ull}, {0x11ac77cfb11d7ccdull, 0x26f2ab43dac65cb0ull},
+    {0x1552b09907e977e3ull, 0x29218b482b8935faull}, {0xfe818a702d07fb9dull, 0x001cb764573d0acdull},
+};
+
+constexpr int ROUNDS = 8;
+constexpr int RC_ROWS = 1 + 2 * ROUNDS;
+constexpr int MAXM = 6;
+constexpr int N_TABLES = 7;
+enum Table { T_SQR = 0, T_POW3, T_POW7, T_POW15, T_POW31, T_BFWD, T_BINV };
+constexpr int VBLOCK = 256;
+// device data of one width: tables, then round constants (RC_ROWS * MAXM), then c0 and B^-1(0)
+constexpr int DEV_TAB = N_TABLES * WIN_ENTRIES;
+constexpr int DEV_RC = DEV_TAB;
+constexpr int DEV_C = DEV_RC + RC_ROWS * MAXM;
+constexpr int DEV_ELEMS = DEV_C + 2;
+
+inline u128 to128(u64x2 v) { return ((u128)v.hi << 64) | v.lo; }
+inline uint4 u4(u128 v) {
+    return make_uint4((uint32_t)v, (uint32_t)(v >> 32), (uint32_t)(v >> 64), (uint32_t)(v >> 96));
+}
+
+// ---- host side: the derived data -------------------------------------------
+struct HostVision {
+    int m = 0;
+    u128 c[4];              // B's coefficients
+    u128 binv_cols[128];    // L^-1(x^j)
+    u128 binv0 = 0;         // B^-1(0)
+    u128 rc[RC_ROWS][MAXM];
+
+    u128 sqr(u128 x) const { return host::gf_mul(x, x); }
+    u128 b_fwd_linear(u128 x) const {
+        const u128 x2 = sqr(x), x4 = sqr(x2);
+        return host::gf_mul(c[1], x) ^ host::gf_mul(c[2], x2) ^ host::gf_mul(c[3], x4);
+    }
+    u128 b_inv_linear(u128 y) const {
+        u128 acc = 0;
+        for (int j = 0; j < 128; j++)
+            if ((y >> j) & 1) acc ^= binv_cols[j];
+        return acc;
+    }
+    void sbox(u128 *s, bool inverse) const {
+        for (int i = 0; i < m; i++) {
+            const u128 x = s[i] ? host::gf_inv(s[i]) : 0;
+            s[i] = inverse ? (b_inv_linear(x) ^ binv0) : (b_fwd_linear(x) ^ c[0]);
+        }
+    }
+    void mds(u128 *a) const {
+        u128 o[MAXM];
+        if (m == 4) {
+            // rows [2 3 1 1], [1 2 3 1], [1 1 2 3], [3 1 1 2] with 2 = x, 3 = x + 1
+            for (int i = 0; i < 4; i++)
+                o[i] = host::gf_mul(a[i] ^ a[(i + 1) & 3], 2) ^ a[(i + 1) & 3] ^ a[(i + 2) & 3] ^ a[(i + 3) & 3];
+        } else {
+            // circulant, first row [x^-1, x^-1, x, 1, x, x + 1]
+            const u128 inv_x = host::gf_inv(2);
+            for (int i = 0; i < 6; i++) {
+                const u128 *r = a;
+                auto at = [&](int k) { return r[(i + k) % 6]; };
+                o[i] = host::gf_mul(at(0) ^ at(1), inv_x) ^ host::gf_mul(at(2) ^ at(4) ^ at(5), 2) ^ at(3) ^ at(5);
+            }
+        }
+        memcpy(a, o, sizeof(u128) * (size_t)m);
+    }
+
+    // Gauss-Jordan over GF(2) on the columns L(x^j) of B's linear part.
+    bool invert_b() {
+        u128 rows[128], inv[128];
+        for (int r = 0; r < 128; r++) {
+            rows[r] = 0;
+            inv[r] = (u128)1 << r;
+        }
+        for (int j = 0; j < 128; j++) {
+            const u128 col = b_fwd_linear((u128)1 << j);
+            for (int r = 0; r < 128; r++)
+                if ((col >> r) & 1) rows[r] |= (u128)1 << j;
+        }
+        for (int c0 = 0; c0 < 128; c0++) {
+            int p = c0;
+            while (p < 128 && !((rows[p] >> c0) & 1)) p++;
+            if (p == 128) return false;
+            std::swap(rows[p], rows[c0]);
+            std::swap(inv[p], inv[c0]);
+            for (int r = 0; r < 128; r++)
+                if (r != c0 && ((rows[r] >> c0) & 1)) {
+                    rows[r] ^= rows[c0];
+                    inv[r] ^= inv[c0];
+                }
+        }
+        for (int j = 0; j < 128; j++) {
+            u128 col = 0;
+            for (int r = 0; r < 128; r++)
+                if ((inv[r] >> j) & 1) col |= (u128)1 << r;
+            binv_cols[j] = col;
+        }
+        binv0 = b_inv_linear(c[0]);
+        return true;
+    }
+
+    // The round-key schedule: key_injection <- CONSTANTS_MATRIX * key_injection
+    // + CONSTANTS_CONSTANT, key_state <- MDS(S-box(key_state)) + key_injection,
+    // alternating the S-box's B^-1 and B halves.
+    void round_constants(const u64x2 *initial, const u64x2 *matrix, const u64x2 *constant) {
+        u128 state[MAXM], inj[MAXM];
+        for (int i = 0; i < m; i++) state[i] = inj[i] = rc[0][i] = to128(initial[i]);
+        int idx = 1;
+        for (int r = 0; r < ROUNDS; r++)
+            for (int half = 0; half < 2; half++) {
+                u128 t[MAXM];
+                for (int i = 0; i < m; i++) {
+                    t[i] = to128(constant[i]);
+                    for (int j = 0; j < m; j++) t[i] ^= host::gf_mul(to128(matrix[i * m + j]), inj[j]);
+                }
+                memcpy(inj, t, sizeof(u128) * (size_t)m);
+                sbox(state, half == 0);
+                mds(state);
+                for (int i = 0; i < m; i++) state[i] ^= inj[i];
+                memcpy(rc[idx++], state, sizeof(u128) * (size_t)m);
+            }
+    }
+};
+
+// Window table of a linear map given by its column images col[j] = L(x^j):
+// row k, entry n = L(n x^(4k)), stored at slot n ^ (k & 15) of the row.
+void window_table(const u128 *col, uint4 *tab) {
+    for (int k = 0; k < 32; k++)
+        for (int n = 0; n < 16; n++) {
+            u128 v = 0;
+            for (int b = 0; b < 4; b++)
+                if ((n >> b) & 1) v ^= col[4 * k + b];
+            tab[k * 16 + (n ^ (k & 15))] = u4(v);
+        }
+}
+
+bool build_device_data(int m, uint4 *out) {
+    HostVision hv;
+    hv.m = m;
+    for (int i = 0; i < 4; i++) hv.c[i] = to128(B_FWD[i]);
+    if (!hv.invert_b()) return false;
+    if (m == 4)
+        hv.round_constants(INITIAL4, MATRIX4, CONSTANT4);
+    else
+        hv.round_constants(INITIAL6, MATRIX6, CONSTANT6);
+    u128 col[128];
+    const int pows[5] = {1, 3, 7, 15, 31};
+    for (int t = 0; t < 5; t++) {
+        for (int j = 0; j < 128; j++) {
+            u128 x = (u128)1 << j;
+            for (int k = 0; k < pows[t]; k++) x = hv.sqr(x);
+            col[j] = x;
+        }
+        window_table(col, out + t * WIN_ENTRIES);
+    }
+    for (int j = 0; j < 128; j++) col[j] = hv.b_fwd_linear((u128)1 << j);
+    window_table(col, out + T_BFWD * WIN_ENTRIES);
+    window_table(hv.binv_cols, out + T_BINV * WIN_ENTRIES);
+    for (int r = 0; r < RC_ROWS; r++)
+        for (int i = 0; i < MAXM; i++) out[DEV_RC + r * MAXM + i] = u4(i < m ? hv.rc[r][i] : 0);
+    out[DEV_C] = u4(hv.c[0]);
+    out[DEV_C + 1] = u4(hv.binv0);
+    return true;
+}
+
+// Per device and width: the derived data, uploaded once.
+const uint4 *device_data(int m) {
+    static std::mutex mu;
+    static const uint4 *cache[64][2] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    const int w = m == 4 ? 0 : 1;
+    std::lock_guard<std::mutex> lock(mu);
+    if (cache[dev][w]) return cache[dev][w];
+    uint4 *host = new (std::nothrow) uint4[DEV_ELEMS];
+    if (!host) return nullptr;
+    uint4 *d = nullptr;
+    const bool ok = build_device_data(m, host) && hipMalloc(&d, DEV_ELEMS * sizeof(uint4)) == hipSuccess &&
+                    hipMemcpy(d, host, DEV_ELEMS * sizeof(uint4), hipMemcpyHostToDevice) == hipSuccess;
+    delete[] host;
+    if (!ok) {
+        if (d) (void)hipFree(d);
+        return nullptr;
+    }
+    cache[dev][w] = d;  // held until the process ends
+    return d;
+}
+
+// ---- device side -------------------------------------------------------------
+__device__ __forceinline__ uint4 vmap(uint32_t tabs, int t, uint4 v) {
+    return win_mul_perm(tabs + (uint32_t)(t * WIN_ENTRIES * 16), v);
+}
+
+// v * x^-1 (ghash.rs mul_inv_x)
+__device__ __forceinline__ uint4 gf_mul_inv_x(uint4 v) {
+    const uint32_t m = 0u - (v.x & 1u);
+    return make_uint4(((v.x >> 1) | (v.y << 31)) ^ (m & 0x43u), (v.y >> 1) | (v.z << 31), (v.z >> 1) | (v.w << 31),
+                      (v.w >> 1) ^ (m & 0x80000000u));
+}
+
+// a^(2^128 - 2): b_k = a^(2^k - 1), b_(i+j) = b_i^(2^j) b_j along
+// k = 1, 2, 3, 6, 7, 14, 15, 30, 31, 62, 63, 126, 127, then one squaring
+// (x^(2^63) as two x^(2^31) maps and a squaring).
+__device__ uint4 vis_inv(uint32_t tabs, uint4 a) {
+    const uint4 b1 = a;
+    const uint4 b2 = gf_mul(vmap(tabs, T_SQR, b1), b1);
+    const uint4 b3 = gf_mul(vmap(tabs, T_SQR, b2), b1);
+    const uint4 b6 = gf_mul(vmap(tabs, T_POW3, b3), b3);
+    const uint4 b7 = gf_mul(vmap(tabs, T_SQR, b6), b1);
+    const uint4 b14 = gf_mul(vmap(tabs, T_POW7, b7), b7);
+    const uint4 b15 = gf_mul(vmap(tabs, T_SQR, b14), b1);
+    const uint4 b30 = gf_mul(vmap(tabs, T_POW15, b15), b15);
+    const uint4 b31 = gf_mul(vmap(tabs, T_SQR, b30), b1);
+    const uint4 b62 = gf_mul(vmap(tabs, T_POW31, b31), b31);
+    const uint4 b63 = gf_mul(vmap(tabs, T_SQR, b62), b1);
+    const uint4 b126 = gf_mul(vmap(tabs, T_SQR, vmap(tabs, T_POW31, vmap(tabs, T_POW31, b63))), b63);
+    const uint4 b127 = gf_mul(vmap(tabs, T_SQR, b126), b1);
+    return vmap(tabs, T_SQR, b127);
+}
+
+// invert_or_zero of every element with one inversion (zeros enter as ones and
+// come back as zeros).
+template <int M>
+__device__ __forceinline__ void vis_batch_inv(uint32_t tabs, uint4 (&s)[M]) {
+    const uint4 one = make_uint4(1, 0, 0, 0), zero = make_uint4(0, 0, 0, 0);
+    uint4 pre[M], x[M];
+    bool z[M];
+    uint4 acc = one;
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+        z[i] = gzero(s[i]);
+        x[i] = z[i] ? one : s[i];
+        pre[i] = acc;
+        acc = i == 0 ? x[0] : gf_mul(acc, x[i]);
+    }
+    uint4 inv = vis_inv(tabs, acc);
+#pragma unroll
+    for (int i = M - 1; i >= 0; i--) {
+        const uint4 out = i == 0 ? inv : gf_mul(inv, pre[i]);
+        if (i > 0) inv = gf_mul(inv, x[i]);
+        s[i] = z[i] ? zero : out;
+    }
+}
+
+template <int M>
+__device__ __forceinline__ void vis_mds(uint4 (&a)[M]) {
+    if constexpr (M == 4) {
+        const uint4 sum = gxor(gxor(a[0], a[1]), gxor(a[2], a[3]));
+        const uint4 a0 = a[0];
+        a[0] = gxor(gxor(a[0], sum), gf_mul_x(gxor(a[0], a[1])));
+        a[1] = gxor(gxor(a[1], sum), gf_mul_x(gxor(a[1], a[2])));
+        a[2] = gxor(gxor(a[2], sum), gf_mul_x(gxor(a[2], a[3])));
+        a[3] = gxor(gxor(a[3], sum), gf_mul_x(gxor(a[3], a0)));
+    } else {
+        uint4 o[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const uint4 p = gf_mul_inv_x(gxor(a[i], a[(i + 1) % 6]));
+            const uint4 q = gf_mul_x(gxor(gxor(a[(i + 2) % 6], a[(i + 4) % 6]), a[(i + 5) % 6]));
+            o[i] = gxor(gxor(p, q), gxor(a[(i + 3) % 6], a[(i + 5) % 6]));
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++) a[i] = o[i];
+    }
+}
+
+// LDS view of one width's data.
+struct VisionLds {
+    uint32_t tabs;      // LDS address of the 8 tables
+    const uint4 *rc;    // RC_ROWS * MAXM
+    uint4 c0, binv0;
+};
+
+template <int M>
+__device__ void vis_permute(const VisionLds &v, uint4 (&s)[M]) {
+#pragma unroll
+    for (int i = 0; i < M; i++) s[i] = gxor(s[i], v.rc[i]);
+#pragma unroll 1
+    for (int r = 0; r < ROUNDS; r++) {
+        vis_batch_inv<M>(v.tabs, s);
+#pragma unroll
+        for (int i = 0; i < M; i++) s[i] = gxor(vmap(v.tabs, T_BINV, s[i]), v.binv0);
+        vis_mds<M>(s);
+#pragma unroll
+        for (int i = 0; i < M; i++) s[i] = gxor(s[i], v.rc[(1 + 2 * r) * MAXM + i]);
+        vis_batch_inv<M>(v.tabs, s);
+#pragma unroll
+        for (int i = 0; i < M; i++) s[i] = gxor(vmap(v.tabs, T_BFWD, s[i]), v.c0);
+        vis_mds<M>(s);
+#pragma unroll
+        for (int i = 0; i < M; i++) s[i] = gxor(s[i], v.rc[(2 + 2 * r) * MAXM + i]);
+    }
+}
+
+// Stages one width's tables and constants into LDS (58 KiB).
+__device__ VisionLds vis_stage(const uint4 *__restrict__ dev, uint4 *lds) {
+    for (int e = threadIdx.x; e < DEV_ELEMS; e += VBLOCK) lds[e] = dev[e];
+    __syncthreads();
+    return VisionLds{lds_addr(lds), lds + DEV_RC, lds[DEV_C], lds[DEV_C + 1]};
+}
+
+template <int M>
+__global__ __launch_bounds__(VBLOCK) void k_vision_permute(const uint4 *__restrict__ dev, uint4 *__restrict__ states,
+                                                         uint64_t n) {
+    __shared__ __attribute__((aligned(256))) uint4 vis_lds[DEV_ELEMS];
+    const VisionLds v = vis_stage(dev, vis_lds);
+    for (uint64_t k = (uint64_t)blockIdx.x * VBLOCK + threadIdx.x; k < n; k += (uint64_t)gridDim.x * VBLOCK) {
+        uint4 s[M];
+#pragma unroll
+        for (int i = 0; i < M; i++) s[i] = states[k * M + i];
+        vis_permute<M>(v, s);
+#pragma unroll
+        for (int i = 0; i < M; i++) states[k * M + i] = s[i];
+    }
+}
+
+// VisionHasherDigest of n messages of len elements each (msg k at
+// msgs + k * len): rate R = M - 2 elements; each full block overwrites
+// state[0 .. R) and permutes; the final block holds the remaining elements, a
+// 0x80 byte after them and 0x01 in its last byte (a whole padding block when
+// nothing remains); the digest is state[0] || state[1].
+template <int M>
+__global__ __launch_bounds__(VBLOCK) void k_vision_digest(const uint4 *__restrict__ dev,
+                                                        const uint4 *__restrict__ msgs, uint64_t n, uint32_t len,
+                                                        uint4 *__restrict__ out) {
+    constexpr int R = M - 2;
+    __shared__ __attribute__((aligned(256))) uint4 vis_lds[DEV_ELEMS];
+    const VisionLds v = vis_stage(dev, vis_lds);
+    for (uint64_t k = (uint64_t)blockIdx.x * VBLOCK + threadIdx.x; k < n; k += (uint64_t)gridDim.x * VBLOCK) {
+        const uint4 *p = msgs + k * len;
+        uint4 s[M];
+#pragma unroll
+        for (int i = 0; i < M; i++) s[i] = make_uint4(0, 0, 0, 0);
+        uint32_t e = 0;
+        for (; e + R <= len; e += R) {
+#pragma unroll
+            for (int i = 0; i < R; i++) s[i] = p[e + i];
+            vis_permute<M>(v, s);
+        }
+        const uint32_t rem = len - e;
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            s[i] = (uint32_t)i < rem ? p[e + i] : make_uint4(0, 0, 0, 0);
+            if ((uint32_t)i == rem) s[i].x ^= 0x80u;
+        }
+        s[R - 1].w ^= 0x01000000u;
+        vis_permute<M>(v, s);
+        out[2 * k] = s[0];
+        out[2 * k + 1] = s[1];
+    }
+}
+
+// VisionCompression of one tree level: node i of the level above is
+// compress(in[2i], in[2i+1]) = first half of permute(l0, l1, r0, r1) + (l0, l1).
+__global__ __launch_bounds__(VBLOCK) void k_vision_compress(const uint4 *__restrict__ dev, uint4 *__restrict__ tree,
+                                                          uint64_t in_off, uint64_t n_out) {
+    __shared__ __attribute__((aligned(256))) uint4 vis_lds[DEV_ELEMS];
+    const VisionLds v = vis_stage(dev, vis_lds);
+    for (uint64_t i = (uint64_t)blockIdx.x * VBLOCK + threadIdx.x; i < n_out; i += (uint64_t)gridDim.x * VBLOCK) {
+        const uint4 *in = tree + 2 * (in_off + 2 * i);
+        uint4 s[4] = {in[0], in[1], in[2], in[3]};
+        const uint4 l0 = s[0], l1 = s[1];
+        vis_permute<4>(v, s);
+        uint4 *o = tree + 2 * (in_off + 2 * n_out + i);
+        o[0] = gxor(s[0], l0);
+        o[1] = gxor(s[1], l1);
+    }
+}
+
+unsigned vis_grid(uint64_t n) {
+    const uint64_t b = (n + VBLOCK - 1) / VBLOCK;
+    return (unsigned)(b < 4096 ? (b ? b : 1) : 4096);
+}
+
+hipError_t launch_permute(int m, const uint4 *dev, uint4 *states, uint64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (m == 4) {
+        k_vision_permute<4><<<vis_grid(n), VBLOCK, 0, s>>>(dev, states, n);
+    } else {
+        k_vision_permute<6><<<vis_grid(n), VBLOCK, 0, s>>>(dev, states, n);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_digest(int m, const uint4 *dev, const uint4 *msgs, uint64_t n, uint32_t len, uint4 *out,
+                         hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (m == 4) {
+        k_vision_digest<4><<<vis_grid(n), VBLOCK, 0, s>>>(dev, msgs, n, len, out);
+    } else {
+        k_vision_digest<6><<<vis_grid(n), VBLOCK, 0, s>>>(dev, msgs, n, len, out);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_tree_levels(const uint4 *dev4, uint4 *tree, uint64_t n_leaves, hipStream_t s) {
+    uint64_t in_off = 0;
+    for (uint64_t n = n_leaves; n > 1; n /= 2) {
+        k_vision_compress<<<vis_grid(n / 2), VBLOCK, 0, s>>>(dev4, tree, in_off, n / 2);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        in_off += n;
+    }
+    return hipSuccess;
+}
+
+int check_device() {
+    int n = 0;
+    return (hipGetDeviceCount(&n) == hipSuccess && n > 0) ? B64_OK : B64_ERR_NO_DEVICE;
+}
+
+}  // namespace
+}  // namespace vision
+}  // namespace b64
+
+extern "C" {
+
+int b64_vision_permute(uint32_t m, void *d_states, uint64_t n, void *stream) {
+    if ((m != 4 && m != 6) || (!d_states && n)) return B64_ERR_INVALID_ARGS;
+    B64_TRY(b64::vision::check_device());
+    const uint4 *dev = b64::vision::device_data((int)m);
+    if (!dev) return B64_ERR_HIP;
+    HIP_TRY(b64::vision::launch_permute((int)m, dev, (uint4 *)d_states, n, as_stream(stream)));
+    return B64_OK;
+}
+
+int b64_vision_digest(uint32_t m, const void *d_msgs, uint64_t n, uint64_t msg_elems, void *d_out, void *stream) {
+    if ((m != 4 && m != 6) || ((!d_msgs && n * msg_elems) || (!d_out && n)) || msg_elems > 0xFFFFFFFFull)
+        return B64_ERR_INVALID_ARGS;
+    B64_TRY(b64::vision::check_device());
+    const uint4 *dev = b64::vision::device_data((int)m);
+    if (!dev) return B64_ERR_HIP;
+    HIP_TRY(b64::vision::launch_digest((int)m, dev, (const uint4 *)d_msgs, n, (uint32_t)msg_elems, (uint4 *)d_out,
+                                       as_stream(stream)));
+    return B64_OK;
+}
+
+int b64_merkle_commit_vision(uint32_t leaf_m, const void *d_elems, uint64_t n_elems, uint64_t batch, void *d_tree,
+                             uint8_t root_out[32], void *stream) {
+    if ((leaf_m != 4 && leaf_m != 6) || !d_elems || !d_tree) return B64_ERR_INVALID_ARGS;
+    if (batch == 0 || n_elems % batch != 0 || batch > 0xFFFFFFFFull) return B64_ERR_INCORRECT_BATCH_SIZE;
+    const uint64_t n_leaves = n_elems / batch;
+    if (n_leaves == 0 || (n_leaves & (n_leaves - 1))) return B64_ERR_POWER_OF_TWO_LENGTH_REQUIRED;
+    B64_TRY(b64::vision::check_device());
+    const uint4 *leaf = b64::vision::device_data((int)leaf_m);
+    const uint4 *node = b64::vision::device_data(4);
+    if (!leaf || !node) return B64_ERR_HIP;
+    const hipStream_t s = as_stream(stream);
+    HIP_TRY(b64::vision::launch_digest((int)leaf_m, leaf, (const uint4 *)d_elems, n_leaves, (uint32_t)batch,
+                                       (uint4 *)d_tree, s));
+    HIP_TRY(b64::vision::launch_tree_levels(node, (uint4 *)d_tree, n_leaves, s));
+    if (root_out) {
+        HIP_TRY(hipMemcpyAsync(root_out, (const uint8_t *)d_tree + 32 * (2 * n_leaves - 2), 32,
+                               hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    return B64_OK;
+}
+
+}  // extern "C"

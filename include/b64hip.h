@@ -332,6 +332,23 @@ int b64_basefold_prove(const b64_fri_params *p, const b64_domain *d, void *d_mul
 int b64_pcs_prove(const b64_fri_params *p, const b64_domain *d, const void *d_codeword, const void *d_tree,
                   void *d_packed, const uint64_t *eval_point, uint32_t n_point, b64_transcript *t, void *stream);
 
+/* ---- Vision-4 / Vision-6 hash (verifier/verifier/src/hash/vision_{4,6},
+ *      prover/prover/src/hash/vision_{4,6}) ------------------------------------ */
+/* The permutation (vision_{m}/permutation.rs) of n states of m = 4 or 6
+ * elements each, in place (d_states: n * m elements). */
+int b64_vision_permute(uint32_t m, void *d_states, uint64_t n, void *stream);
+/* VisionHasherDigest (vision_{m}/digest.rs; the prover's multi-digest gives the
+ * same values) of n messages of msg_elems B128 elements each, message k at
+ * d_msgs + 16 * k * msg_elems bytes; d_out receives n 32-byte digests. */
+int b64_vision_digest(uint32_t m, const void *d_msgs, uint64_t n, uint64_t msg_elems, void *d_out, void *stream);
+/* BinaryMerkleTreeProver::commit (merkle_tree/prover.rs) with VisionHasherDigest
+ * leaves of width leaf_m (4: prover/examples/src/lib.rs setup_vision4; 6: the
+ * leaf hasher of prover/prover/benches/binary_merkle_tree.rs) and
+ * VisionCompression (vision_4/compression.rs) nodes.  Layout and error codes as
+ * b64_merkle_commit. */
+int b64_merkle_commit_vision(uint32_t leaf_m, const void *d_elems, uint64_t n_elems, uint64_t batch, void *d_tree,
+                             uint8_t root_out[32], void *stream);
+
 #ifdef __cplusplus
 }
 #endif
