@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-ring-switch", action="store_true", help="skip the ring-switch folds leg")
     ap.add_argument("--no-pcs", action="store_true", help="skip the PCS evaluation-proof leg")
+    ap.add_argument("--no-vision", action="store_true", help="skip the Vision Merkle-commit leg")
     ap.add_argument("--cpu-log-n", type=int, default=24, help="CPU baseline sample size")
     ap.add_argument("--ntt", default=None, metavar="V,K,T,M,G[,W]",
                     help="NTT schedule for every size in this run (replaces the autotuned table): "
@@ -178,6 +179,7 @@ def main():
     serve = None if args.no_serve else serve_bench(b64, torch, sp)
     ring = None if args.no_ring_switch else ring_switch_bench(b64, torch, stream, sp, with_cpu=not args.no_cpu)
     pcs = None if args.no_pcs else pcs_bench(b64, torch, sp, log_n=n, with_cpu=not args.no_cpu)
+    vision = None if args.no_vision else vision_bench(b64, torch, stream, sp, with_cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
@@ -209,6 +211,7 @@ def main():
         "serve": serve,
         "ring_switch": ring,
         "pcs_prove": pcs,
+        "vision_merkle": vision,
     }
     print(json.dumps(out), flush=True)
 
@@ -506,6 +509,55 @@ def pcs_bench(b64, torch, sp, log_n: int = 24, reps: int = 3, with_cpu: bool = T
         res["cpu_baseline"] = {"proof_s": round(cpu_s, 3), "cores": cores, "kind": "port",
                                "sample": f"same 2^{log_n}-element multilinear and point, one proof"}
         res["tape_identical_to_cpu"] = ot.tape() == tape
+    return res
+
+
+def vision_bench(b64, torch, stream, sp, log_leaves: int = 17, reps: int = 3, with_cpu: bool = True,
+                 cpu_log_leaves: int = 13):
+    """Vision Merkle commits at the shape of the reference's Merkle bench
+    (prover/prover/benches/binary_merkle_tree.rs: 2^17 leaves of 16 B128 elements):
+    Vision-6 leaves with Vision-4 compression (that bench's hasher), and Vision-4
+    throughout (setup_vision4, prover/examples/src/lib.rs).  Throughput is the
+    bench's convention: committed bytes per second."""
+    import numpy as np
+
+    L = b64.lib()
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(13)
+    n_elems = 16 << log_leaves
+    data = torch.randint(-(2 ** 63), 2 ** 63 - 1, (2 * n_elems,), dtype=torch.int64, device=dev, generator=g)
+    tree = torch.empty((4 * ((2 << log_leaves) - 1),), dtype=torch.int64, device=dev)
+    res = {"log_leaves": log_leaves, "leaf_elems": 16}
+    for leaf_m in (6, 4):
+        def run():
+            b64.check(L.b64_merkle_commit_vision(leaf_m, data.data_ptr(), n_elems, 16, tree.data_ptr(), None, sp),
+                      "merkle_commit_vision")
+        run()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            run()
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / reps
+        res[f"vision{leaf_m}_leaves_ms"] = round(ms, 3)
+        res[f"vision{leaf_m}_leaves_MBs"] = round((16 * n_elems) / (ms * 1e-3) / 1e6, 2)
+    if with_cpu:
+        import json as _json
+
+        from oracle import oracle as orc
+        gold = _json.loads((ROOT / "tests" / "golden" / "reference_vectors.json").read_text())
+        v4, v6 = orc.Vision(4, gold["vision4"]), orc.Vision(6, gold["vision6"])
+        cores = min(16, os.cpu_count() or 1)
+        sample = data[: 2 * (16 << cpu_log_leaves)].cpu().numpy().view(np.uint64).reshape(-1, 2)
+        t = time.perf_counter()
+        orc.merkle_build_vision(v6, v4, sample, 16, threads=cores)
+        cpu_s = time.perf_counter() - t
+        res["cpu_baseline"] = {"vision6_leaves_MBs": round(16 * sample.shape[0] / cpu_s / 1e6, 3), "cores": cores,
+                               "kind": "port", "sample": f"2^{cpu_log_leaves} leaves of the same data, one commit",
+                               "seconds": round(cpu_s, 3)}
     return res
 
 

@@ -70,14 +70,13 @@ __device__ __forceinline__ void block_xor2(uint4 &p, uint4 &q) {
 // elements: every lane reads the four elements whose folds it needs and writes
 // the two it owns, so the in-place update is race-free.  The multiplies by r go
 // through a window table of r; the products A*B are general products.
-// Each block writes its two partial sums; the block that finishes last (a
-// self-resetting atomicInc counter) XORs all of them and writes the round's
-// totals to `result`, host-visible pinned memory read after the stream syncs.
+// Each block writes its two partial sums; k_sc_reduce adds them up.  (Letting
+// the last block to finish do the reduction costs more than the launch it
+// saves: each block's device-scope release then writes back the L2 the round's
+// folded arrays are dirty in.)
 __global__ __launch_bounds__(SC_BLOCK) void k_sc_round(uint4 *__restrict__ a, uint4 *__restrict__ b, uint64_t half,
-                                                       int fold, uint4 r, uint4 *__restrict__ partial,
-                                                       unsigned int *__restrict__ counter, uint4 *__restrict__ result) {
+                                                       int fold, uint4 r, uint4 *__restrict__ partial) {
     __shared__ __attribute__((aligned(256))) uint4 tab[WIN_ENTRIES];
-    __shared__ int is_last;
     if (fold) {
         if (threadIdx.x < WIN_ROWS) win_build_row(tab, r, threadIdx.x);
         __syncthreads();
@@ -111,19 +110,18 @@ __global__ __launch_bounds__(SC_BLOCK) void k_sc_round(uint4 *__restrict__ a, ui
     if (threadIdx.x == 0) {
         partial[2 * blockIdx.x] = y1;
         partial[2 * blockIdx.x + 1] = yi;
-        __threadfence();  // publish the partials before counting this block done
-        is_last = atomicInc(counter, gridDim.x - 1) == gridDim.x - 1;
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();  // every other block's partials are visible from here
-    y1 = make_uint4(0, 0, 0, 0);
-    yi = make_uint4(0, 0, 0, 0);
-    for (unsigned k = threadIdx.x; k < gridDim.x; k += SC_BLOCK) {
+}
+
+// The round's totals: XOR of the n_blocks partial pairs, written to `result`,
+// host-visible pinned memory read after the stream synchronises.
+__global__ __launch_bounds__(SC_BLOCK) void k_sc_reduce(const uint4 *__restrict__ partial, int n_blocks,
+                                                        uint4 *__restrict__ result) {
+    uint4 y1 = make_uint4(0, 0, 0, 0), yi = make_uint4(0, 0, 0, 0);
+    for (int k = threadIdx.x; k < n_blocks; k += SC_BLOCK) {
         y1 = gxor(y1, partial[2 * k]);
         yi = gxor(yi, partial[2 * k + 1]);
     }
-    __syncthreads();  // block_xor2's scratch is reused
     block_xor2(y1, yi);
     if (threadIdx.x == 0) {
         result[0] = y1;
@@ -254,7 +252,7 @@ struct b64_sumcheck {
     bool pending = false;     // a fold by r_pending not yet applied on the device
     u128 sum = 0, r_pending = 0;
     u128 coeffs[3] = {0, 0, 0};
-    uint4 *d_partial = nullptr;  // 2 * SC_MAX_BLOCKS partial sums, then the block counter
+    uint4 *d_partial = nullptr;  // 2 * SC_MAX_BLOCKS partial sums
     uint4 *h_res = nullptr;      // pinned, 2 elements, written by the kernels
     uint4 *h_res_dev = nullptr;  // h_res as the device addresses it
     hipStream_t s = nullptr;
@@ -280,16 +278,11 @@ int b64_sumcheck_create(void *d_a, void *d_b, uint32_t n_vars, const uint64_t su
     sc->sum = ld128(sum);
     sc->s = as_stream(stream);
     b64::abi::keep_pool_memory();
-    if (hipMallocAsync((void **)&sc->d_partial, (2 * b64::SC_MAX_BLOCKS + 1) * sizeof(uint4), sc->s) != hipSuccess ||
+    if (hipMallocAsync((void **)&sc->d_partial, 2 * b64::SC_MAX_BLOCKS * sizeof(uint4), sc->s) != hipSuccess ||
         hipHostMalloc((void **)&sc->h_res, 2 * sizeof(uint4), hipHostMallocDefault) != hipSuccess ||
         hipHostGetDevicePointer((void **)&sc->h_res_dev, sc->h_res, 0) != hipSuccess) {
         delete sc;
         return B64_ERR_OUT_OF_MEMORY;
-    }
-    // the block counter starts at zero and resets itself after every round
-    if (hipMemsetAsync(sc->d_partial + 2 * b64::SC_MAX_BLOCKS, 0, sizeof(uint4), sc->s) != hipSuccess) {
-        delete sc;
-        return B64_ERR_HIP;
     }
     *out = sc;
     return B64_OK;
@@ -305,9 +298,9 @@ int b64_sumcheck_execute(b64_sumcheck *sc, uint64_t coeffs_out[6]) {
     if (sc->n_vars == 0) return B64_ERR_INVALID_ARGS;  // assert!(n_vars_remaining > 0)
     const uint64_t half = 1ull << (sc->n_vars - 1);
     const int blocks = b64::sc_blocks(half);
-    b64::k_sc_round<<<blocks, b64::SC_BLOCK, 0, sc->s>>>(
-        sc->a, sc->b, half, sc->pending ? 1 : 0, to_u4(sc->r_pending), sc->d_partial,
-        (unsigned int *)(sc->d_partial + 2 * b64::SC_MAX_BLOCKS), sc->h_res_dev);
+    b64::k_sc_round<<<blocks, b64::SC_BLOCK, 0, sc->s>>>(sc->a, sc->b, half, sc->pending ? 1 : 0,
+                                                          to_u4(sc->r_pending), sc->d_partial);
+    b64::k_sc_reduce<<<1, b64::SC_BLOCK, 0, sc->s>>>(sc->d_partial, blocks, sc->h_res_dev);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(sc->s));
     sc->pending = false;
