@@ -352,13 +352,18 @@ class CommitOutput:
     tree: DeviceBuffer
 
 
-def commit_interleaved(params: FRIParams, domain: DomainContext, message: DeviceBuffer, stream=None) -> CommitOutput:
+# Merkle schemes (the MerkleTreeProver parameter of the reference's provers)
+MERKLE_SHA256, MERKLE_VISION4, MERKLE_VISION6 = 0, 4, 6
+
+
+def commit_interleaved(params: FRIParams, domain: DomainContext, message: DeviceBuffer, stream=None,
+                       merkle_hash: int = MERKLE_SHA256) -> CommitOutput:
     cw = DeviceBuffer(16 << params.log_len())
     tree = DeviceBuffer(32 * ((2 << params.rs_log_len()) - 1))
     root = np.zeros(32, dtype=np.uint8)
     pc = params.to_c()
-    check(lib().b64_fri_commit_interleaved(C.byref(pc), domain.ptr, message.ptr, cw.ptr, tree.ptr, _p8(root),
-                                           _stream(stream)), "commit_interleaved")
+    check(lib().b64_fri_commit_interleaved_ex(C.byref(pc), domain.ptr, merkle_hash, message.ptr, cw.ptr, tree.ptr,
+                                              _p8(root), _stream(stream)), "commit_interleaved")
     return CommitOutput(root.tobytes(), cw, tree)
 
 
@@ -408,12 +413,12 @@ class FRIFoldProver:
     """FRIFoldProver (prover/prover/src/fri/fold.rs:31-245) over device-resident oracles."""
 
     def __init__(self, params: FRIParams, domain: DomainContext, codeword: DeviceBuffer, tree: DeviceBuffer,
-                 stream=None):
+                 stream=None, merkle_hash: int = MERKLE_SHA256):
         self._keep = (domain, codeword, tree)
         self._pc = params.to_c()
         out = C.c_void_p()
-        check(lib().b64_fri_prover_create(C.byref(self._pc), domain.ptr, codeword.ptr, tree.ptr, _stream(stream),
-                                          C.byref(out)), "fri_prover_create")
+        check(lib().b64_fri_prover_create_ex(C.byref(self._pc), domain.ptr, merkle_hash, codeword.ptr, tree.ptr,
+                                             _stream(stream), C.byref(out)), "fri_prover_create")
         self.ptr = out.value
 
     def __del__(self):
@@ -513,31 +518,32 @@ def prove_single(prover: BivariateProductSumcheckProver, transcript: ProverTrans
 
 def basefold_prove(params: FRIParams, domain: DomainContext, multilinear: DeviceBuffer, transparent: DeviceBuffer,
                    claim: int, codeword: DeviceBuffer, tree: DeviceBuffer, transcript: ProverTranscript,
-                   stream=None) -> None:
+                   stream=None, merkle_hash: int = MERKLE_SHA256) -> None:
     """BaseFoldProver::new(...).prove(transcript) (prover protocols/basefold.rs:64-151).
     multilinear and transparent are consumed."""
     pc = params.to_c()
-    check(lib().b64_basefold_prove(C.byref(pc), domain.ptr, multilinear.ptr, transparent.ptr, _p64(_u128(claim)),
-                                   codeword.ptr, tree.ptr, transcript.ptr, _stream(stream)), "basefold_prove")
+    check(lib().b64_basefold_prove_ex(C.byref(pc), domain.ptr, merkle_hash, multilinear.ptr, transparent.ptr,
+                                      _p64(_u128(claim)), codeword.ptr, tree.ptr, transcript.ptr, _stream(stream)),
+          "basefold_prove")
 
 
 class OneBitPCSProver:
     """OneBitPCSProver (prover/prover/src/pcs.rs:35-171): commit and prove an
     evaluation of a multilinear over B1 packed 128 bits per B128 element."""
 
-    def __init__(self, domain: DomainContext, params: FRIParams):
-        self.domain, self.params = domain, params
+    def __init__(self, domain: DomainContext, params: FRIParams, merkle_hash: int = MERKLE_SHA256):
+        self.domain, self.params, self.merkle_hash = domain, params, merkle_hash
 
     def commit(self, packed: DeviceBuffer, stream=None) -> CommitOutput:
-        return commit_interleaved(self.params, self.domain, packed, stream)
+        return commit_interleaved(self.params, self.domain, packed, stream, merkle_hash=self.merkle_hash)
 
     def prove(self, codeword: DeviceBuffer, tree: DeviceBuffer, packed: DeviceBuffer, evaluation_point,
               transcript: ProverTranscript, stream=None) -> None:
         """packed is consumed (folded in place by the sumcheck)."""
         pt = np.ascontiguousarray(_elems(evaluation_point))
         pc = self.params.to_c()
-        check(lib().b64_pcs_prove(C.byref(pc), self.domain.ptr, codeword.ptr, tree.ptr, packed.ptr, _p64(pt),
-                                  pt.shape[0], transcript.ptr, _stream(stream)), "pcs_prove")
+        check(lib().b64_pcs_prove_ex(C.byref(pc), self.domain.ptr, self.merkle_hash, codeword.ptr, tree.ptr,
+                                     packed.ptr, _p64(pt), pt.shape[0], transcript.ptr, _stream(stream)), "pcs_prove")
 
 
 class VisionHasher:
@@ -714,13 +720,14 @@ def ring_switch_fold_elems(elems: DeviceBuffer, log_len: int, vec: np.ndarray, s
     return elems
 
 
-def fri_prove(params: FRIParams, domain: DomainContext, message: np.ndarray, challenges=None):
+def fri_prove(params: FRIParams, domain: DomainContext, message: np.ndarray, challenges=None,
+              merkle_hash: int = MERKLE_SHA256):
     """FRI-only prover flow (prover/prover/src/fri/tests.rs:58-89): commit, fold
     rounds with transcript challenges (or the given ones), finish_proof.
     Returns (proof tape, roots, challenges)."""
     msg = DeviceBuffer.from_host(_elems(message))
-    co = commit_interleaved(params, domain, msg)
-    prover = FRIFoldProver(params, domain, co.codeword, co.tree)
+    co = commit_interleaved(params, domain, msg, merkle_hash=merkle_hash)
+    prover = FRIFoldProver(params, domain, co.codeword, co.tree, merkle_hash=merkle_hash)
     t = ProverTranscript()
     t.message(co.commitment)
     roots = [co.commitment]

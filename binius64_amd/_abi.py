@@ -136,6 +136,10 @@ SIGNATURES = {
     "b64_vision_permute": (C.c_int, [C.c_uint32, vp, C.c_uint64, vp]),
     "b64_vision_digest": (C.c_int, [C.c_uint32, vp, C.c_uint64, C.c_uint64, vp, vp]),
     "b64_merkle_commit_vision": (C.c_int, [C.c_uint32, vp, C.c_uint64, C.c_uint64, vp, u8p, vp]),
+    "b64_fri_commit_interleaved_ex": (C.c_int, [C.POINTER(FriParamsC), vp, C.c_uint32, vp, vp, vp, u8p, vp]),
+    "b64_fri_prover_create_ex": (C.c_int, [C.POINTER(FriParamsC), vp, C.c_uint32, vp, vp, vp, C.POINTER(vp)]),
+    "b64_basefold_prove_ex": (C.c_int, [C.POINTER(FriParamsC), vp, C.c_uint32, vp, vp, u64p, vp, vp, vp, vp]),
+    "b64_pcs_prove_ex": (C.c_int, [C.POINTER(FriParamsC), vp, C.c_uint32, vp, vp, vp, u64p, C.c_uint32, vp, vp]),
     "b64_pcs_prove": (C.c_int, [C.POINTER(FriParamsC), vp, vp, vp, vp, u64p, C.c_uint32, vp, vp]),
 }
 

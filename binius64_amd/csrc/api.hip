@@ -545,14 +545,24 @@ int b64_setup_commit_params(uint32_t log_witness_elems, uint32_t lir, uint32_t s
     return rc;
 }
 
-int b64_fri_commit_interleaved(const b64_fri_params *p, const b64_domain *d, const void *d_msg, void *d_codeword,
-                               void *d_tree, uint8_t root_out[32], void *stream) {
+int b64_fri_commit_interleaved_ex(const b64_fri_params *p, const b64_domain *d, uint32_t merkle_hash,
+                                  const void *d_msg, void *d_codeword, void *d_tree, uint8_t root_out[32],
+                                  void *stream) {
     if (!p || !d || !d_msg || !d_codeword || !d_tree) return B64_ERR_INVALID_ARGS;
+    if (merkle_hash != B64_MERKLE_SHA256 && merkle_hash != B64_MERKLE_VISION4 && merkle_hash != B64_MERKLE_VISION6)
+        return B64_ERR_INVALID_ARGS;
     // encode_batch requires ntt.subspace(rs_log_len) == rs_code.subspace(); the
     // code built over `d` satisfies it by construction (reed_solomon.rs:54-65).
     B64_TRY(b64_rs_encode_batch(d, p->log_dim, p->log_inv_rate, d_msg, p->log_batch_size, d_codeword, stream));
     const uint64_t n = 1ull << (p->log_dim + p->log_inv_rate + p->log_batch_size);
-    return b64_merkle_commit(d_codeword, n, 1ull << p->log_batch_size, d_tree, root_out, stream);
+    if (merkle_hash == B64_MERKLE_SHA256)
+        return b64_merkle_commit(d_codeword, n, 1ull << p->log_batch_size, d_tree, root_out, stream);
+    return b64_merkle_commit_vision(merkle_hash, d_codeword, n, 1ull << p->log_batch_size, d_tree, root_out, stream);
+}
+
+int b64_fri_commit_interleaved(const b64_fri_params *p, const b64_domain *d, const void *d_msg, void *d_codeword,
+                               void *d_tree, uint8_t root_out[32], void *stream) {
+    return b64_fri_commit_interleaved_ex(p, d, B64_MERKLE_SHA256, d_msg, d_codeword, d_tree, root_out, stream);
 }
 
 int b64_fri_commit_fold(const b64_fri_params *p, const b64_domain *d, const void *d_msg, void *d_codeword,
@@ -678,6 +688,7 @@ struct b64_fri_prover {
     Oracle first;  // the committed interleaved codeword (not owned)
     std::vector<Oracle> committed;
     hipStream_t s = nullptr;
+    uint32_t merkle_hash = B64_MERKLE_SHA256;  // scheme of the folded oracles' trees
     uint32_t curr_round = 0;
     bool has_next = true;
     uint32_t next_commit = 0;
@@ -706,7 +717,14 @@ extern "C" {
 
 int b64_fri_prover_create(const b64_fri_params *p, const b64_domain *d, const void *d_codeword, const void *d_tree,
                           void *stream, b64_fri_prover **out) {
+    return b64_fri_prover_create_ex(p, d, B64_MERKLE_SHA256, d_codeword, d_tree, stream, out);
+}
+
+int b64_fri_prover_create_ex(const b64_fri_params *p, const b64_domain *d, uint32_t merkle_hash, const void *d_codeword,
+                             const void *d_tree, void *stream, b64_fri_prover **out) {
     if (!p || !d || !d_codeword || !d_tree || !out) return B64_ERR_INVALID_ARGS;
+    if (merkle_hash != B64_MERKLE_SHA256 && merkle_hash != B64_MERKLE_VISION4 && merkle_hash != B64_MERKLE_VISION6)
+        return B64_ERR_INVALID_ARGS;
     if (p->log_dim + p->log_inv_rate > (uint32_t)d->dim) return B64_ERR_INVALID_ARGS;
     b64_fri_prover *fp = new (std::nothrow) b64_fri_prover();
     if (!fp) return B64_ERR_OUT_OF_MEMORY;
@@ -718,6 +736,7 @@ int b64_fri_prover_create(const b64_fri_params *p, const b64_domain *d, const vo
     fp->first.tree_log_len = (int)(p->log_dim + p->log_inv_rate);
     fp->s = as_stream(stream);
     fp->next_commit = p->log_batch_size;  // FRIFoldProver::new (fold.rs:70)
+    fp->merkle_hash = merkle_hash;
     b64::abi::keep_pool_memory();
     *out = fp;
     return B64_OK;
@@ -780,7 +799,11 @@ int b64_fri_prover_execute_fold_round(b64_fri_prover *fp, uint8_t root_out[32], 
         return B64_ERR_OUT_OF_MEMORY;
     }
     fp->committed.push_back(o);  // owned from here on
-    B64_TRY(merkle_commit_impl(o.cw, 1ull << o.log_len, 1ull << log_coset, o.tree, fp->s));
+    if (fp->merkle_hash == B64_MERKLE_SHA256)
+        B64_TRY(merkle_commit_impl(o.cw, 1ull << o.log_len, 1ull << log_coset, o.tree, fp->s));
+    else
+        B64_TRY(b64::vision_merkle_commit((int)fp->merkle_hash, o.cw, 1ull << o.log_len, 1ull << log_coset, o.tree,
+                                          fp->s));
     if (has_arity)
         fp->next_commit += fp->p.fold_arities[k];
     else

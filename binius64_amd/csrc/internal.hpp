@@ -149,5 +149,12 @@ struct BatchStorage {
 int b64_batch_plan_build(const b64_job_desc *jobs, uint32_t n_jobs, int use_graph, b64::BatchStorage *st,
                          hipStream_t upload_stream, b64_batch_plan **out);
 
+// Vision Merkle tree (leaf_m = 4 or 6 leaves, Vision-4 compression nodes) into
+// tree; a b64 status code.
+namespace b64 {
+int vision_merkle_commit(int leaf_m, const uint4 *elems, uint64_t n_elems, uint64_t batch, uint4 *tree,
+                         hipStream_t s);
+}
+
 // Device view of a domain (uploads its twiddle basis on first use).
 int b64_domain_device_view_internal(b64_domain *d, b64::DeviceDomain *dd);

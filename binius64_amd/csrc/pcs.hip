@@ -363,13 +363,20 @@ int b64_eq_ind_partial_eval(const uint64_t *point, uint32_t n, void *d_out, void
 int b64_basefold_prove(const b64_fri_params *p, const b64_domain *d, void *d_multilinear, void *d_transparent,
                        const uint64_t claim[2], const void *d_codeword, const void *d_tree, b64_transcript *t,
                        void *stream) {
+    return b64_basefold_prove_ex(p, d, B64_MERKLE_SHA256, d_multilinear, d_transparent, claim, d_codeword, d_tree, t,
+                                 stream);
+}
+
+int b64_basefold_prove_ex(const b64_fri_params *p, const b64_domain *d, uint32_t merkle_hash, void *d_multilinear,
+                          void *d_transparent, const uint64_t claim[2], const void *d_codeword, const void *d_tree,
+                          b64_transcript *t, void *stream) {
     if (!p || !d || !d_multilinear || !d_transparent || !claim || !d_codeword || !d_tree || !t)
         return B64_ERR_INVALID_ARGS;
     const uint32_t n_vars = p->log_dim + p->log_batch_size;
     b64_sumcheck *sc = nullptr;
     B64_TRY(b64_sumcheck_create(d_multilinear, d_transparent, n_vars, claim, stream, &sc));
     b64_fri_prover *fp = nullptr;
-    int rc = b64_fri_prover_create(p, d, d_codeword, d_tree, stream, &fp);
+    int rc = b64_fri_prover_create_ex(p, d, merkle_hash, d_codeword, d_tree, stream, &fp);
     if (rc != B64_OK) {
         b64_sumcheck_destroy(sc);
         return rc;
@@ -410,6 +417,12 @@ int b64_basefold_prove(const b64_fri_params *p, const b64_domain *d, void *d_mul
 // ===========================================================================
 int b64_pcs_prove(const b64_fri_params *p, const b64_domain *d, const void *d_codeword, const void *d_tree,
                   void *d_packed, const uint64_t *eval_point, uint32_t n_point, b64_transcript *t, void *stream) {
+    return b64_pcs_prove_ex(p, d, B64_MERKLE_SHA256, d_codeword, d_tree, d_packed, eval_point, n_point, t, stream);
+}
+
+int b64_pcs_prove_ex(const b64_fri_params *p, const b64_domain *d, uint32_t merkle_hash, const void *d_codeword,
+                     const void *d_tree, void *d_packed, const uint64_t *eval_point, uint32_t n_point,
+                     b64_transcript *t, void *stream) {
     if (!p || !d || !d_codeword || !d_tree || !d_packed || !eval_point || !t) return B64_ERR_INVALID_ARGS;
     const uint32_t log_len = p->log_dim + p->log_batch_size;
     constexpr uint32_t KAPPA = 7;  // log2 of B128's degree over B1
@@ -446,7 +459,7 @@ int b64_pcs_prove(const b64_fri_params *p, const b64_domain *d, const void *d_co
         if (rc != B64_OK) break;
         uint64_t cl[2];
         st128(cl, claim);
-        rc = b64_basefold_prove(p, d, d_packed, d_suffix, cl, d_codeword, d_tree, t, stream);
+        rc = b64_basefold_prove_ex(p, d, merkle_hash, d_packed, d_suffix, cl, d_codeword, d_tree, t, stream);
     } while (false);
     (void)hipFreeAsync(d_suffix, s);
     return rc;

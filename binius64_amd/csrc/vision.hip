@@ -507,6 +507,25 @@ int check_device() {
 }  // namespace vision
 }  // namespace b64
 
+namespace b64 {
+
+int vision_merkle_commit(int leaf_m, const uint4 *elems, uint64_t n_elems, uint64_t batch, uint4 *tree,
+                         hipStream_t s) {
+    if (leaf_m != 4 && leaf_m != 6) return B64_ERR_INVALID_ARGS;
+    if (batch == 0 || n_elems % batch != 0 || batch > 0xFFFFFFFFull) return B64_ERR_INCORRECT_BATCH_SIZE;
+    const uint64_t n_leaves = n_elems / batch;
+    if (n_leaves == 0 || (n_leaves & (n_leaves - 1))) return B64_ERR_POWER_OF_TWO_LENGTH_REQUIRED;
+    B64_TRY(vision::check_device());
+    const uint4 *leaf = vision::device_data(leaf_m);
+    const uint4 *node = vision::device_data(4);
+    if (!leaf || !node) return B64_ERR_HIP;
+    HIP_TRY(vision::launch_digest(leaf_m, leaf, elems, n_leaves, (uint32_t)batch, tree, s));
+    HIP_TRY(vision::launch_tree_levels(node, tree, n_leaves, s));
+    return B64_OK;
+}
+
+}  // namespace b64
+
 extern "C" {
 
 int b64_vision_permute(uint32_t m, void *d_states, uint64_t n, void *stream) {
@@ -531,18 +550,10 @@ int b64_vision_digest(uint32_t m, const void *d_msgs, uint64_t n, uint64_t msg_e
 
 int b64_merkle_commit_vision(uint32_t leaf_m, const void *d_elems, uint64_t n_elems, uint64_t batch, void *d_tree,
                              uint8_t root_out[32], void *stream) {
-    if ((leaf_m != 4 && leaf_m != 6) || !d_elems || !d_tree) return B64_ERR_INVALID_ARGS;
-    if (batch == 0 || n_elems % batch != 0 || batch > 0xFFFFFFFFull) return B64_ERR_INCORRECT_BATCH_SIZE;
-    const uint64_t n_leaves = n_elems / batch;
-    if (n_leaves == 0 || (n_leaves & (n_leaves - 1))) return B64_ERR_POWER_OF_TWO_LENGTH_REQUIRED;
-    B64_TRY(b64::vision::check_device());
-    const uint4 *leaf = b64::vision::device_data((int)leaf_m);
-    const uint4 *node = b64::vision::device_data(4);
-    if (!leaf || !node) return B64_ERR_HIP;
+    if (!d_elems || !d_tree) return B64_ERR_INVALID_ARGS;
     const hipStream_t s = as_stream(stream);
-    HIP_TRY(b64::vision::launch_digest((int)leaf_m, leaf, (const uint4 *)d_elems, n_leaves, (uint32_t)batch,
-                                       (uint4 *)d_tree, s));
-    HIP_TRY(b64::vision::launch_tree_levels(node, (uint4 *)d_tree, n_leaves, s));
+    B64_TRY(b64::vision_merkle_commit((int)leaf_m, (const uint4 *)d_elems, n_elems, batch, (uint4 *)d_tree, s));
+    const uint64_t n_leaves = n_elems / batch;
     if (root_out) {
         HIP_TRY(hipMemcpyAsync(root_out, (const uint8_t *)d_tree + 32 * (2 * n_leaves - 2), 32,
                                hipMemcpyDeviceToHost, s));

@@ -48,6 +48,15 @@ extern "C" {
 #define B64_ERR_OUT_OF_MEMORY 22
 #define B64_ERR_ADMISSION_REJECTED 23
 
+/* Merkle schemes of the commitments (the MerkleTreeProver type parameter of the
+ * reference's provers): SHA-256 leaves with Sha256Compression nodes (the
+ * default, verifier/verifier/src/config.rs), or VisionHasherDigest leaves of
+ * width 4 or 6 with VisionCompression nodes (prover/examples/src/lib.rs
+ * setup_vision4; Vision-6 leaves as in prover/prover/benches/binary_merkle_tree.rs). */
+#define B64_MERKLE_SHA256 0u
+#define B64_MERKLE_VISION4 4u
+#define B64_MERKLE_VISION6 6u
+
 typedef struct b64_domain b64_domain;
 typedef struct b64_transcript b64_transcript;
 typedef struct b64_fri_prover b64_fri_prover;
@@ -177,6 +186,11 @@ int b64_setup_commit_params(uint32_t log_witness_elems, uint32_t log_inv_rate, u
 int b64_fri_commit_interleaved(const b64_fri_params *p, const b64_domain *d, const void *d_msg,
                                void *d_codeword, void *d_tree, uint8_t root_out[32], void *stream);
 
+/* The same with the Merkle scheme merkle_hash (B64_MERKLE_*). */
+int b64_fri_commit_interleaved_ex(const b64_fri_params *p, const b64_domain *d, uint32_t merkle_hash,
+                                  const void *d_msg, void *d_codeword, void *d_tree, uint8_t root_out[32],
+                                  void *stream);
+
 /* commit_interleaved followed by the first fold_interleaved (prover fri/fold.rs:260-282)
  * with log_batch_size challenges supplied up front, fused so the codeword is read
  * once for leaf hashing and folding.  In the transcript-driven protocol the
@@ -202,6 +216,9 @@ size_t b64_transcript_tape(const b64_transcript *t, uint8_t *out, size_t cap);  
  * codeword and tree (device) and owns the folded oracles it commits. */
 int b64_fri_prover_create(const b64_fri_params *p, const b64_domain *d, const void *d_codeword,
                           const void *d_tree, void *stream, b64_fri_prover **out);
+/* The same with the Merkle scheme merkle_hash (B64_MERKLE_*) for the folded oracles. */
+int b64_fri_prover_create_ex(const b64_fri_params *p, const b64_domain *d, uint32_t merkle_hash, const void *d_codeword,
+                             const void *d_tree, void *stream, b64_fri_prover **out);
 void b64_fri_prover_destroy(b64_fri_prover *fp);
 uint32_t b64_fri_prover_n_rounds(const b64_fri_prover *fp);              /* fold.rs:85-87 */
 uint32_t b64_fri_prover_curr_round(const b64_fri_prover *fp);            /* fold.rs:90-92 */
@@ -324,6 +341,10 @@ int b64_basefold_prove(const b64_fri_params *p, const b64_domain *d, void *d_mul
                        const uint64_t claim[2], const void *d_codeword, const void *d_tree, b64_transcript *t,
                        void *stream);
 
+int b64_basefold_prove_ex(const b64_fri_params *p, const b64_domain *d, uint32_t merkle_hash, void *d_multilinear,
+                          void *d_transparent, const uint64_t claim[2], const void *d_codeword, const void *d_tree,
+                          b64_transcript *t, void *stream);
+
 /* ---- OneBitPCSProver::prove (prover/prover/src/pcs.rs:107-170) ---------------- */
 /* Proves the evaluation of the committed B1 multilinear (d_packed: 2^(log_dim +
  * log_batch_size) B128 elements, 128 bits each; consumed) at eval_point
@@ -331,6 +352,10 @@ int b64_basefold_prove(const b64_fri_params *p, const b64_domain *d, void *d_mul
  * evaluations, row-batching challenges, then BaseFold. */
 int b64_pcs_prove(const b64_fri_params *p, const b64_domain *d, const void *d_codeword, const void *d_tree,
                   void *d_packed, const uint64_t *eval_point, uint32_t n_point, b64_transcript *t, void *stream);
+/* The same over a codeword committed with the Merkle scheme merkle_hash (B64_MERKLE_*). */
+int b64_pcs_prove_ex(const b64_fri_params *p, const b64_domain *d, uint32_t merkle_hash, const void *d_codeword,
+                     const void *d_tree, void *d_packed, const uint64_t *eval_point, uint32_t n_point,
+                     b64_transcript *t, void *stream);
 
 /* ---- Vision-4 / Vision-6 hash (verifier/verifier/src/hash/vision_{4,6},
  *      prover/prover/src/hash/vision_{4,6}) ------------------------------------ */

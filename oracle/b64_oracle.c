@@ -1080,11 +1080,24 @@ typedef struct {
     int tree_log_len;
 } committed;
 
-static int commit_vec(const u128 *v, int log_len, int log_batch, committed *o, int nthreads) {
+/* The Merkle scheme of the commitments: SHA-256 leaves with Sha256Compression
+ * nodes (both NULL), or Vision leaves with VisionCompression nodes
+ * (prover/examples/src/lib.rs setup_vision4). */
+typedef struct bo_vision bo_vision;
+typedef struct {
+    const bo_vision *leaf, *node;
+} mk_scheme;
+int bo_merkle_build_vision(const bo_vision *leaf, const bo_vision *node4, const uint64_t *elems, size_t n_elems,
+                           size_t batch, uint8_t *tree, int nthreads);
+
+static int commit_vec(const u128 *v, int log_len, int log_batch, committed *o, int nthreads, mk_scheme sc) {
     o->log_len = log_len;
     o->tree_log_len = log_len - log_batch;
     size_t n_leaves = (size_t)1 << o->tree_log_len;
     o->tree = (uint8_t *)malloc(32 * (2 * n_leaves - 1));
+    if (sc.leaf)
+        return bo_merkle_build_vision(sc.leaf, sc.node, (const uint64_t *)v, (size_t)1 << log_len,
+                                      (size_t)1 << log_batch, o->tree, nthreads);
     return bo_merkle_build((const uint64_t *)v, (size_t)1 << log_len, (size_t)1 << log_batch, o->tree,
                            nthreads);
 }
@@ -1127,11 +1140,12 @@ typedef struct {
     int n_rc;
     u128 unproc[64];
     int n_unproc, curr_round, next_commit, has_next;
+    mk_scheme sc;
 } fri_st;
 
 /* FRIFoldProver::new (fold.rs:57-82); c0 is borrowed. */
 static int fri_init(fri_st *f, const bo_domain *d, int log_dim, int lir, int lbs, const int *arities, int n_ar,
-                    int n_queries, const committed *c0, int nthreads) {
+                    int n_queries, const committed *c0, int nthreads, mk_scheme sc) {
     memset(f, 0, sizeof(*f));
     if (n_ar > 63) return 1;
     int sum = 0;
@@ -1152,6 +1166,7 @@ static int fri_init(fri_st *f, const bo_domain *d, int log_dim, int lir, int lbs
     f->c0 = *c0;
     f->next_commit = lbs;
     f->has_next = 1;
+    f->sc = sc;
     return 0;
 }
 
@@ -1189,7 +1204,7 @@ static int fri_execute(fri_st *f, uint8_t root_out[32]) {
     f->n_unproc = 0;
     int has_arity = f->n_rc < f->n_ar;
     int log_coset = has_arity ? f->arities[f->n_rc] : f->n_final;
-    if (commit_vec(nc->cw, nc->log_len, log_coset, nc, f->nthreads)) return -1;
+    if (commit_vec(nc->cw, nc->log_len, log_coset, nc, f->nthreads, f->sc)) return -1;
     if (has_arity) f->next_commit += f->arities[f->n_rc];
     else f->has_next = 0;
     f->n_rc++;
@@ -1230,19 +1245,20 @@ static int fri_finish(fri_st *f, bo_transcript *t, uint64_t *terminal_out) {
     return 0;
 }
 
-int bo_fri_prove(const bo_domain *d, const uint64_t *msg, int log_dim, int lir, int lbs,
-                 const int *arities, int n_ar, int n_queries, const uint64_t *chal_in, uint8_t *tape,
-                 size_t tape_cap, size_t *tape_len, uint8_t *roots_out, uint64_t *chal_out,
-                 uint64_t *terminal_out, int nthreads) {
+int bo_fri_prove_ex(const bo_domain *d, const uint64_t *msg, int log_dim, int lir, int lbs,
+                    const int *arities, int n_ar, int n_queries, const uint64_t *chal_in, uint8_t *tape,
+                    size_t tape_cap, size_t *tape_len, uint8_t *roots_out, uint64_t *chal_out,
+                    uint64_t *terminal_out, int nthreads, const bo_vision *leaf, const bo_vision *node) {
+    const mk_scheme sc = {leaf, node};
     int log_len = log_dim + lir + lbs;
     committed c0;
     c0.cw = (u128 *)malloc(((size_t)1 << log_len) * sizeof(u128));
     int rc = bo_encode_batch(d, msg, log_dim, lir, lbs, (uint64_t *)c0.cw, nthreads);
     if (rc) return rc;
-    rc = commit_vec(c0.cw, log_len, lbs, &c0, nthreads);
+    rc = commit_vec(c0.cw, log_len, lbs, &c0, nthreads, sc);
     if (rc) return rc;
     fri_st f;
-    rc = fri_init(&f, d, log_dim, lir, lbs, arities, n_ar, n_queries, &c0, nthreads);
+    rc = fri_init(&f, d, log_dim, lir, lbs, arities, n_ar, n_queries, &c0, nthreads, sc);
     if (rc) return rc;
 
     bo_transcript *t = bo_transcript_new();
@@ -1280,6 +1296,13 @@ int bo_fri_prove(const bo_domain *d, const uint64_t *msg, int log_dim, int lir, 
     free(c0.tree);
     fri_free(&f);
     return ret;
+}
+
+int bo_fri_prove(const bo_domain *d, const uint64_t *msg, int log_dim, int lir, int lbs, const int *arities, int n_ar,
+                 int n_queries, const uint64_t *chal_in, uint8_t *tape, size_t tape_cap, size_t *tape_len,
+                 uint8_t *roots_out, uint64_t *chal_out, uint64_t *terminal_out, int nthreads) {
+    return bo_fri_prove_ex(d, msg, log_dim, lir, lbs, arities, n_ar, n_queries, chal_in, tape, tape_cap, tape_len,
+                           roots_out, chal_out, terminal_out, nthreads, NULL, NULL);
 }
 
 /* ===================================================================== */
@@ -1401,13 +1424,14 @@ int bo_sumcheck_prove(uint64_t *a, uint64_t *b, int n_vars, const uint64_t sum[2
 /* BaseFoldProver::prove (prover protocols/basefold.rs:131-151) and finish
  * (:158-166).  multilinear / transparent (2^(log_dim+lbs) each) are folded in
  * place; codeword / tree are the committed interleaved codeword and its tree. */
-int bo_basefold_prove(const bo_domain *d, uint64_t *multilinear, uint64_t *transparent, const uint64_t claim[2],
-                      const uint64_t *codeword, const uint8_t *tree, int log_dim, int lir, int lbs,
-                      const int *arities, int n_ar, int n_queries, bo_transcript *t, uint64_t *challenges_out,
-                      int nthreads) {
+int bo_basefold_prove_ex(const bo_domain *d, uint64_t *multilinear, uint64_t *transparent, const uint64_t claim[2],
+                         const uint64_t *codeword, const uint8_t *tree, int log_dim, int lir, int lbs,
+                         const int *arities, int n_ar, int n_queries, bo_transcript *t, uint64_t *challenges_out,
+                         int nthreads, const bo_vision *leaf, const bo_vision *node) {
+    const mk_scheme sc = {leaf, node};
     committed c0 = {(u128 *)codeword, log_dim + lir + lbs, (uint8_t *)tree, log_dim + lir};
     fri_st f;
-    int rc = fri_init(&f, d, log_dim, lir, lbs, arities, n_ar, n_queries, &c0, nthreads);
+    int rc = fri_init(&f, d, log_dim, lir, lbs, arities, n_ar, n_queries, &c0, nthreads, sc);
     if (rc) return rc;
     int n_vars = log_dim + lbs;
     bp_st s = {(u128 *)multilinear, (u128 *)transparent, n_vars, nthreads, 0, ld(claim), {0, 0, 0}};
@@ -1436,6 +1460,14 @@ int bo_basefold_prove(const bo_domain *d, uint64_t *multilinear, uint64_t *trans
     return rc;
 }
 
+int bo_basefold_prove(const bo_domain *d, uint64_t *multilinear, uint64_t *transparent, const uint64_t claim[2],
+                      const uint64_t *codeword, const uint8_t *tree, int log_dim, int lir, int lbs,
+                      const int *arities, int n_ar, int n_queries, bo_transcript *t, uint64_t *challenges_out,
+                      int nthreads) {
+    return bo_basefold_prove_ex(d, multilinear, transparent, claim, codeword, tree, log_dim, lir, lbs, arities, n_ar,
+                                n_queries, t, challenges_out, nthreads, NULL, NULL);
+}
+
 /* TensorAlgebra::transpose for B1 x B128 (verifier/math/src/tensor_algebra.rs:137-141 ->
  * square_transforms_extension_field, verifier/field/src/transpose.rs:30-75):
  * the 128 x 128 bit matrix of the 128 elements transposed, out[i] bit j = in[j] bit i. */
@@ -1452,9 +1484,9 @@ void bo_tensor_transpose(const uint64_t *in, uint64_t *out) { transpose128((cons
  * reduction to a B128 sumcheck claim, then BaseFold.  packed (2^log_len
  * elements, log_len = log_dim + lbs) is consumed; eval_point has log_len + 7
  * coordinates. */
-int bo_pcs_prove(const bo_domain *d, uint64_t *packed, const uint64_t *eval_point, const uint64_t *codeword,
-                 const uint8_t *tree, int log_dim, int lir, int lbs, const int *arities, int n_ar, int n_queries,
-                 bo_transcript *t, int nthreads) {
+int bo_pcs_prove_ex(const bo_domain *d, uint64_t *packed, const uint64_t *eval_point, const uint64_t *codeword,
+                    const uint8_t *tree, int log_dim, int lir, int lbs, const int *arities, int n_ar, int n_queries,
+                    bo_transcript *t, int nthreads, const bo_vision *leaf, const bo_vision *node) {
     int log_len = log_dim + lbs;
     size_t n = (size_t)1 << log_len;
     u128 *suffix = (u128 *)malloc(n * sizeof(u128));
@@ -1470,10 +1502,17 @@ int bo_pcs_prove(const bo_domain *d, uint64_t *packed, const uint64_t *eval_poin
     bo_rs_fold_elems((uint64_t *)suffix, n, (const uint64_t *)eq_rpp, 0, nthreads);
     uint64_t cl[2];
     st(cl, claim);
-    int rc = bo_basefold_prove(d, packed, (uint64_t *)suffix, cl, codeword, tree, log_dim, lir, lbs, arities, n_ar,
-                               n_queries, t, NULL, nthreads);
+    int rc = bo_basefold_prove_ex(d, packed, (uint64_t *)suffix, cl, codeword, tree, log_dim, lir, lbs, arities,
+                                  n_ar, n_queries, t, NULL, nthreads, leaf, node);
     free(suffix);
     return rc;
+}
+
+int bo_pcs_prove(const bo_domain *d, uint64_t *packed, const uint64_t *eval_point, const uint64_t *codeword,
+                 const uint8_t *tree, int log_dim, int lir, int lbs, const int *arities, int n_ar, int n_queries,
+                 bo_transcript *t, int nthreads) {
+    return bo_pcs_prove_ex(d, packed, eval_point, codeword, tree, log_dim, lir, lbs, arities, n_ar, n_queries, t,
+                           nthreads, NULL, NULL);
 }
 
 /* ===================================================================== */

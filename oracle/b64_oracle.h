@@ -107,6 +107,15 @@ int bo_fri_prove(const bo_domain *d, const uint64_t *msg, int log_dim, int log_i
 
 /* ---- transcript primitives (verifier/transcript/src) --------------------- */
 typedef struct bo_transcript bo_transcript;
+typedef struct bo_vision bo_vision;
+
+/* The same provers with the Vision Merkle scheme (leaf: Vision-4 or -6 digest,
+ * node: Vision-4 compression, prover/examples/src/lib.rs setup_vision4); NULL
+ * for both selects SHA-256 leaves and Sha256Compression nodes. */
+int bo_fri_prove_ex(const bo_domain *d, const uint64_t *msg, int log_dim, int log_inv_rate, int log_batch,
+                    const int *arities, int n_arities, int n_queries, const uint64_t *challenges_in, uint8_t *tape,
+                    size_t tape_cap, size_t *tape_len, uint8_t *roots_out, uint64_t *challenges_out,
+                    uint64_t *terminal_out, int nthreads, const bo_vision *leaf, const bo_vision *node);
 
 /* ---- sumcheck, BaseFold, one-bit PCS opening ------------------------------ */
 /* prove_single (prover protocols/sumcheck/prove.rs:17-60) over
@@ -123,6 +132,10 @@ int bo_basefold_prove(const bo_domain *d, uint64_t *multilinear, uint64_t *trans
                       const uint64_t *codeword, const uint8_t *tree, int log_dim, int lir, int lbs,
                       const int *arities, int n_ar, int n_queries, bo_transcript *t, uint64_t *challenges_out,
                       int nthreads);
+int bo_basefold_prove_ex(const bo_domain *d, uint64_t *multilinear, uint64_t *transparent, const uint64_t claim[2],
+                         const uint64_t *codeword, const uint8_t *tree, int log_dim, int lir, int lbs,
+                         const int *arities, int n_ar, int n_queries, bo_transcript *t, uint64_t *challenges_out,
+                         int nthreads, const bo_vision *leaf, const bo_vision *node);
 /* TensorAlgebra<B1, B128>::transpose (verifier/math/src/tensor_algebra.rs:137-141):
  * out[i] bit j = in[j] bit i over 128 elements. */
 void bo_tensor_transpose(const uint64_t *in, uint64_t *out);
@@ -131,6 +144,9 @@ void bo_tensor_transpose(const uint64_t *in, uint64_t *out);
 int bo_pcs_prove(const bo_domain *d, uint64_t *packed, const uint64_t *eval_point, const uint64_t *codeword,
                  const uint8_t *tree, int log_dim, int lir, int lbs, const int *arities, int n_ar, int n_queries,
                  bo_transcript *t, int nthreads);
+int bo_pcs_prove_ex(const bo_domain *d, uint64_t *packed, const uint64_t *eval_point, const uint64_t *codeword,
+                    const uint8_t *tree, int log_dim, int lir, int lbs, const int *arities, int n_ar, int n_queries,
+                    bo_transcript *t, int nthreads, const bo_vision *leaf, const bo_vision *node);
 bo_transcript *bo_transcript_new(void);
 void bo_transcript_free(bo_transcript *t);
 void bo_transcript_message(bo_transcript *t, const uint8_t *bytes, size_t n);
@@ -145,7 +161,6 @@ size_t bo_transcript_tape(const bo_transcript *t, uint8_t *out, size_t cap);
  * constant: the round-key schedule's INITIAL_CONSTANT (m), CONSTANTS_MATRIX
  * (m*m, row-major), CONSTANTS_CONSTANT (m).  Round constants and B^-1 are
  * derived from them.  NULL for a bad m or a singular B. */
-typedef struct bo_vision bo_vision;
 bo_vision *bo_vision_new(int m, const uint64_t *b_fwd_coeffs, const uint64_t *initial, const uint64_t *matrix,
                          const uint64_t *constant);
 void bo_vision_free(bo_vision *v);

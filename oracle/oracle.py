@@ -73,12 +73,15 @@ def lib():
         L.bo_rs_fold_elems.argtypes = [u64p, C.c_size_t, u64p, C.c_int, C.c_int]
         L.bo_fri_prove.argtypes = [C.c_void_p, u64p, C.c_int, C.c_int, C.c_int, intp, C.c_int, C.c_int,
                                    u64p, u8p, C.c_size_t, C.POINTER(C.c_size_t), u8p, u64p, u64p, C.c_int]
+        L.bo_fri_prove_ex.argtypes = L.bo_fri_prove.argtypes + [C.c_void_p, C.c_void_p]
         L.bo_sumcheck_prove.argtypes = [u64p, u64p, C.c_int, u64p, C.c_void_p, u64p, u64p, C.c_int]
         L.bo_basefold_prove.argtypes = [C.c_void_p, u64p, u64p, u64p, u64p, u8p, C.c_int, C.c_int, C.c_int,
                                         intp, C.c_int, C.c_int, C.c_void_p, u64p, C.c_int]
         L.bo_tensor_transpose.argtypes = [u64p, u64p]
         L.bo_pcs_prove.argtypes = [C.c_void_p, u64p, u64p, u64p, u8p, C.c_int, C.c_int, C.c_int, intp, C.c_int,
                                    C.c_int, C.c_void_p, C.c_int]
+        L.bo_basefold_prove_ex.argtypes = L.bo_basefold_prove.argtypes + [C.c_void_p, C.c_void_p]
+        L.bo_pcs_prove_ex.argtypes = L.bo_pcs_prove.argtypes + [C.c_void_p, C.c_void_p]
         L.bo_vision_new.restype = C.c_void_p
         L.bo_vision_new.argtypes = [C.c_int, u64p, u64p, u64p, u64p]
         L.bo_vision_free.argtypes = [C.c_void_p]
@@ -382,8 +385,14 @@ def rs_fold_elems(elems: np.ndarray, vec: np.ndarray, variant: int = 0, threads:
     return e
 
 
+def _scheme(scheme):
+    """(leaf Vision, node Vision) pointers for the Vision Merkle scheme, (None, None) for SHA-256."""
+    return (None, None) if scheme is None else (scheme[0].ptr, scheme[1].ptr)
+
+
 def fri_prove(dom: Domain, msg: np.ndarray, log_dim: int, log_inv_rate: int, log_batch: int,
-              arities, n_queries: int, challenges: np.ndarray | None = None, threads: int = 1) -> dict:
+              arities, n_queries: int, challenges: np.ndarray | None = None, threads: int = 1,
+              scheme=None) -> dict:
     m = np.ascontiguousarray(msg, dtype=np.uint64).reshape(-1, 2)
     ar = (C.c_int * max(len(arities), 1))(*arities)
     n_rounds = log_dim + log_batch
@@ -398,12 +407,13 @@ def fri_prove(dom: Domain, msg: np.ndarray, log_dim: int, log_inv_rate: int, log
     # first call sizes the tape
     args = [dom.ptr, _p64(m), log_dim, log_inv_rate, log_batch, ar, len(arities), n_queries,
             _p64(ch_in) if ch_in is not None else None]
-    rc = lib().bo_fri_prove(*args, None, 0, C.byref(tape_len), _p8(roots), _p64(ch_out), _p64(term), threads)
+    rc = lib().bo_fri_prove_ex(*args, None, 0, C.byref(tape_len), _p8(roots), _p64(ch_out), _p64(term), threads,
+                               *_scheme(scheme))
     if rc:
         raise ValueError(f"fri_prove error {rc}")
     tape = np.zeros(tape_len.value, dtype=np.uint8)
-    rc = lib().bo_fri_prove(*args, _p8(tape), tape.size, C.byref(tape_len), _p8(roots), _p64(ch_out),
-                            _p64(term), threads)
+    rc = lib().bo_fri_prove_ex(*args, _p8(tape), tape.size, C.byref(tape_len), _p8(roots), _p64(ch_out),
+                               _p64(term), threads, *_scheme(scheme))
     if rc:
         raise ValueError(f"fri_prove error {rc}")
     return {"tape": tape.tobytes(), "roots": [roots[i].tobytes() for i in range(len(arities) + 2)],
@@ -471,7 +481,7 @@ def _arities(arities):
 
 def basefold_prove(dom: Domain, multilinear: np.ndarray, transparent: np.ndarray, claim: int,
                    codeword: np.ndarray, tree: np.ndarray, log_dim: int, log_inv_rate: int, log_batch: int,
-                   arities, n_queries: int, transcript: "Transcript", threads: int = 1) -> np.ndarray:
+                   arities, n_queries: int, transcript: "Transcript", threads: int = 1, scheme=None) -> np.ndarray:
     """BaseFoldProver::prove after the caller wrote the codeword root; returns the challenges."""
     wa = np.array(multilinear, dtype=np.uint64).reshape(-1, 2)
     wb = np.array(transparent, dtype=np.uint64).reshape(-1, 2)
@@ -480,9 +490,9 @@ def basefold_prove(dom: Domain, multilinear: np.ndarray, transparent: np.ndarray
     n_vars = log_dim + log_batch
     assert wa.shape[0] == wb.shape[0] == 1 << n_vars
     ch = np.zeros((max(n_vars, 1), 2), dtype=np.uint64)
-    rc = lib().bo_basefold_prove(dom.ptr, _p64(wa), _p64(wb), _p64(to_elems([claim])), _p64(cw), _p8(tr), log_dim,
-                                 log_inv_rate, log_batch, _arities(arities), len(arities), n_queries,
-                                 transcript.ptr, _p64(ch), threads)
+    rc = lib().bo_basefold_prove_ex(dom.ptr, _p64(wa), _p64(wb), _p64(to_elems([claim])), _p64(cw), _p8(tr),
+                                    log_dim, log_inv_rate, log_batch, _arities(arities), len(arities), n_queries,
+                                    transcript.ptr, _p64(ch), threads, *_scheme(scheme))
     if rc:
         raise ValueError(f"basefold_prove error {rc}")
     return ch[:n_vars]
@@ -498,15 +508,15 @@ def tensor_transpose(elems: np.ndarray) -> np.ndarray:
 
 def pcs_prove(dom: Domain, packed: np.ndarray, eval_point: np.ndarray, codeword: np.ndarray, tree: np.ndarray,
               log_dim: int, log_inv_rate: int, log_batch: int, arities, n_queries: int,
-              transcript: "Transcript", threads: int = 1) -> None:
+              transcript: "Transcript", threads: int = 1, scheme=None) -> None:
     """OneBitPCSProver::prove after the caller wrote the codeword root."""
     w = np.array(packed, dtype=np.uint64).reshape(-1, 2)
     pt = np.ascontiguousarray(eval_point, dtype=np.uint64).reshape(-1, 2)
     assert w.shape[0] == 1 << (log_dim + log_batch) and pt.shape[0] == log_dim + log_batch + 7
     cw = np.ascontiguousarray(codeword, dtype=np.uint64).reshape(-1, 2)
     tr = np.ascontiguousarray(tree, dtype=np.uint8).reshape(-1)
-    rc = lib().bo_pcs_prove(dom.ptr, _p64(w), _p64(pt), _p64(cw), _p8(tr), log_dim, log_inv_rate, log_batch,
-                            _arities(arities), len(arities), n_queries, transcript.ptr, threads)
+    rc = lib().bo_pcs_prove_ex(dom.ptr, _p64(w), _p64(pt), _p64(cw), _p8(tr), log_dim, log_inv_rate, log_batch,
+                               _arities(arities), len(arities), n_queries, transcript.ptr, threads, *_scheme(scheme))
     if rc:
         raise ValueError(f"pcs_prove error {rc}")
 

@@ -99,16 +99,44 @@ def compress(orc, left: bytes, right: bytes) -> bytes:
     return struct.pack("<8I", *orc.sha256_compress_block(_IV, left + right))
 
 
-def fold_digests(orc, digests):
+class Sha256Scheme:
+    """SHA-256 leaves, Sha256Compression nodes (verifier/verifier/src/hash)."""
+
+    def __init__(self, orc):
+        self.orc = orc
+
+    def leaf(self, data: bytes) -> bytes:
+        return hashlib.sha256(data).digest()
+
+    def compress(self, left: bytes, right: bytes) -> bytes:
+        return compress(self.orc, left, right)
+
+
+class VisionScheme:
+    """VisionHasherDigest leaves (Vision-4 or -6), VisionCompression nodes."""
+
+    def __init__(self, leaf_vision, node_vision):
+        self.leaf_v, self.node_v = leaf_vision, node_vision
+
+    def leaf(self, data: bytes) -> bytes:
+        return self.leaf_v.digest(data)
+
+    def compress(self, left: bytes, right: bytes) -> bytes:
+        return self.node_v.compress(left, right)
+
+
+def fold_digests(orc, digests, scheme=None):
+    scheme = scheme or Sha256Scheme(orc)
     nodes = list(digests)
     while len(nodes) > 1:
-        nodes = [compress(orc, nodes[2 * i], nodes[2 * i + 1]) for i in range(len(nodes) // 2)]
+        nodes = [scheme.compress(nodes[2 * i], nodes[2 * i + 1]) for i in range(len(nodes) // 2)]
     return nodes[0]
 
 
 def verify_fri_queries(orc, dom, vt: VerifierTape, log_dim: int, lir: int, lbs: int, arities, n_queries: int,
-                       codeword_root: bytes, round_roots, challenges) -> int:
+                       codeword_root: bytes, round_roots, challenges, scheme=None) -> int:
     """Returns the final FRI value; raises AssertionError on a rejected proof."""
+    scheme = scheme or Sha256Scheme(orc)
     n_final = log_dim - sum(arities)
     rs_log_len = log_dim + lir
     assert len(round_roots) == len(arities) + 1
@@ -120,9 +148,9 @@ def verify_fri_queries(orc, dom, vt: VerifierTape, log_dim: int, lir: int, lbs: 
     # verify_last_oracle: the terminal codeword hashes to the last root, and
     # folding each coset with the final challenges gives one repeated value.
     term = bytes_elems(vt.decommit(16 << (n_final + lir)))
-    leaves = [hashlib.sha256(b"".join(elem_bytes(v) for v in term[i:i + (1 << n_final)])).digest()
+    leaves = [scheme.leaf(b"".join(elem_bytes(v) for v in term[i:i + (1 << n_final)]))
               for i in range(0, len(term), 1 << n_final)]
-    assert fold_digests(orc, leaves) == round_roots[-1], "terminal codeword root"
+    assert fold_digests(orc, leaves, scheme) == round_roots[-1], "terminal codeword root"
     final_ch = orc.to_elems(fold_ch[len(fold_ch) - n_final:]) if n_final else None
     rep = []
     for i in range(len(term) >> n_final):
@@ -139,15 +167,15 @@ def verify_fri_queries(orc, dom, vt: VerifierTape, log_dim: int, lir: int, lbs: 
         depths.append(min(cap, log_n_cosets))
     layers = [[vt.decommit(32) for _ in range(1 << d)] for d in depths]
     for root, layer in zip([codeword_root] + list(round_roots), layers):
-        assert fold_digests(orc, layer) == root, "layer does not hash to its root"
+        assert fold_digests(orc, layer, scheme) == root, "layer does not hash to its root"
 
     def opening(index, log_coset, depth, tree_depth, layer):
         vals = bytes_elems(vt.decommit(16 << log_coset))
-        h = hashlib.sha256(b"".join(elem_bytes(v) for v in vals)).digest()
+        h = scheme.leaf(b"".join(elem_bytes(v) for v in vals))
         j = index
         for _ in range(tree_depth - depth):
             node = vt.decommit(32)
-            h = compress(orc, h, node) if j & 1 == 0 else compress(orc, node, h)
+            h = scheme.compress(h, node) if j & 1 == 0 else scheme.compress(node, h)
             j >>= 1
         assert h == layer[j], "opening does not verify"
         return vals
@@ -173,7 +201,7 @@ def verify_fri_queries(orc, dom, vt: VerifierTape, log_dim: int, lir: int, lbs: 
 
 
 def basefold_verify(orc, dom, vt: VerifierTape, log_dim: int, lir: int, lbs: int, arities, n_queries: int,
-                    codeword_root: bytes, claim: int):
+                    codeword_root: bytes, claim: int, scheme=None):
     """basefold::verify (verifier protocols/basefold.rs): returns
     (final_fri_value, final_sumcheck_value, challenges)."""
     n_vars = log_dim + lbs
@@ -190,7 +218,7 @@ def basefold_verify(orc, dom, vt: VerifierTape, log_dim: int, lir: int, lbs: int
     if commits[n_vars]:
         roots.append(vt.message(32))
     final_fri = verify_fri_queries(orc, dom, vt, log_dim, lir, lbs, arities, n_queries, codeword_root, roots,
-                                   challenges)
+                                   challenges, scheme)
     return final_fri, s, challenges
 
 

@@ -87,3 +87,42 @@ def test_merkle_vision_errors(b64):
     with pytest.raises(b64.B64Error) as e:
         b64.merkle_commit_vision(buf, 32, 16, 5)
     assert e.value.code == 1
+
+
+@pytest.mark.parametrize("leaf_m,log_dim,lir,lbs,arities,nq", [
+    (4, 6, 1, 2, [2, 2], 3), (6, 8, 1, 4, [4, 1], 232), (4, 10, 1, 4, [4, 4, 1], 232)])
+def test_fri_proof_tape_with_vision_merkle(b64, orc, oracles, leaf_m, log_dim, lir, lbs, arities, nq):
+    """FRI-only flow with the Vision Merkle scheme: roots, challenges and the proof
+    tape equal the oracle's."""
+    say("fri vision", leaf_m, log_dim, lbs, arities)
+    dom, ref = b64.DomainContext.standard(log_dim + lir), orc.Domain(dim=log_dim + lir)
+    msg = orc.random_elems(950 + log_dim, 1 << (log_dim + lbs))
+    params = b64.FRIParams.new(log_dim, lir, lbs, arities, nq)
+    tape, roots, chal = b64.fri_prove(params, dom, msg, merkle_hash=leaf_m)
+    want = orc.fri_prove(ref, msg, log_dim, lir, lbs, arities, nq, threads=8, scheme=(oracles[leaf_m], oracles[4]))
+    assert roots == want["roots"]
+    assert orc.to_ints(want["challenges"]) == chal
+    assert tape == want["tape"]
+
+
+def test_pcs_with_vision_merkle(b64, orc, oracles):
+    """The evaluation proof over a Vision-4-committed codeword (setup_vision4)
+    with the Verifier::setup shape at 2^12 elements equals the oracle's."""
+    n = 12
+    params, dom = b64.setup_commit_params(n, 1, 96)
+    packed = orc.random_elems(1212, 1 << n)
+    point = orc.random_elems(1213, n + 7)
+    pcs = b64.OneBitPCSProver(dom, params, merkle_hash=b64.MERKLE_VISION4)
+    co = pcs.commit(b64.DeviceBuffer.from_host(packed))
+    ref = orc.Domain(dom.subspace(params.rs_log_len()))
+    cw = orc.encode_batch(ref, packed, params.log_dim, 1, params.log_batch_size)
+    tree = orc.merkle_build_vision(oracles[4], oracles[4], cw, 1 << params.log_batch_size, threads=8)
+    assert co.commitment == tree[-1].tobytes()
+    t = b64.ProverTranscript()
+    t.message(co.commitment)
+    pcs.prove(co.codeword, co.tree, b64.DeviceBuffer.from_host(packed), point, t)
+    ot = orc.Transcript()
+    ot.message(co.commitment)
+    orc.pcs_prove(ref, packed, point, cw, tree, params.log_dim, 1, params.log_batch_size, params.fold_arities,
+                  params.n_test_queries, ot, threads=8, scheme=(oracles[4], oracles[4]))
+    assert t.finalize() == ot.tape()

@@ -197,3 +197,59 @@ def test_pcs_prove_verifies(orc, log_dim, lir, lbs, arities, nq):
     bad = point.copy()
     bad[0, 0] ^= 1
     assert not pcs_verify(orc, dom, tape, root, claim, bad, log_dim, lir, lbs, arities, nq)
+
+
+@pytest.fixture(scope="module")
+def visions(orc):
+    import json
+    from pathlib import Path
+    gold = json.loads((Path(__file__).parent / "golden" / "reference_vectors.json").read_text())
+    return {m: orc.Vision(m, gold[f"vision{m}"]) for m in (4, 6)}
+
+
+@pytest.mark.parametrize("leaf_m", [4, 6])
+def test_fri_and_basefold_with_vision_merkle(orc, visions, leaf_m):
+    """The Vision Merkle scheme (setup_vision4; Vision-6 leaves as in the Merkle
+    bench) through the FRI-only flow and BaseFold: the restated verifiers accept
+    with the same scheme and reject with SHA-256."""
+    scheme = (visions[leaf_m], visions[4])
+    vs = fv.VisionScheme(*scheme)
+    log_dim, lir, lbs, arities, nq = 6, 1, 2, [2, 2], 3
+    n_vars = log_dim + lbs
+    dom = orc.Domain(dim=log_dim + lir)
+    msg = orc.random_elems(900 + leaf_m, 1 << n_vars)
+    res = orc.fri_prove(dom, msg, log_dim, lir, lbs, arities, nq, scheme=scheme)
+    # replay the FRI-only flow (fri/tests.rs:58-89): root, then per round a
+    # challenge and, on commit rounds, that round's root
+    vt = fv.VerifierTape(orc, res["tape"])
+    roots, chal = [vt.message(32)], []
+    commits = fv.commit_rounds(lbs, arities, n_vars + 1)
+    for rnd in range(n_vars + 1):
+        if rnd:
+            chal.append(vt.sample_elem())
+        if commits[rnd]:
+            roots.append(vt.message(32))
+    assert chal == orc.to_ints(res["challenges"]) and roots == res["roots"]
+    final = fv.verify_fri_queries(orc, dom, vt, log_dim, lir, lbs, arities, nq, roots[0], roots[1:], chal, vs)
+    assert vt.done()
+    assert final == orc.evaluate_mle(msg, orc.to_elems(chal[::-1]))
+    # BaseFold over a Vision-committed codeword
+    point = orc.random_elems(901 + leaf_m, n_vars)
+    eq = orc.eq_ind(point)
+    claim = orc.evaluate_mle(msg, point)
+    cw = orc.encode_batch(dom, msg, log_dim, lir, lbs)
+    tree = orc.merkle_build_vision(scheme[0], scheme[1], cw, 1 << lbs)
+    root = tree[-1].tobytes()
+    t = orc.Transcript()
+    t.message(root)
+    orc.basefold_prove(dom, msg, eq, claim, cw, tree, log_dim, lir, lbs, arities, nq, t, scheme=scheme)
+    tape = t.tape()
+    vt = fv.VerifierTape(orc, tape)
+    vt.message(32)
+    final_fri, final_sum, chals = fv.basefold_verify(orc, dom, vt, log_dim, lir, lbs, arities, nq, root, claim, vs)
+    assert vt.done()
+    assert final_sum == fv.fmul(orc, final_fri, orc.evaluate_mle(eq, orc.to_elems(chals[::-1])))
+    vt = fv.VerifierTape(orc, tape)
+    vt.message(32)
+    with pytest.raises(AssertionError):
+        fv.basefold_verify(orc, dom, vt, log_dim, lir, lbs, arities, nq, root, claim)  # SHA-256 scheme
