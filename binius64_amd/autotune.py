@@ -31,13 +31,14 @@ Candidate = Tuple[int, int, int, int, int, int]
 
 def default_candidates() -> List[Candidate]:
     c: List[Candidate] = [(0, k, 11, 1, 2, 0) for k in (5, 6, 7)]
-    for k in (5, 6, 7):
+    for k in (4, 5, 6, 7):
         for tile in (10, 11):
             for tab_min in (32, 64, 128):
                 c.append((1, k, tile, tab_min, 2, 0))
+            c.append((1, k, tile, 64, 1, 0))
             c.append((1, k, tile, 64, 2, 4))
         c.append((1, k, 11, 64, 4, 0))
-    c += [(1, 4, 11, 32, 2, 0), (1, 4, 11, 64, 2, 0), (1, 4, 10, 32, 2, 4)]
+    c += [(1, 3, 11, 32, 2, 0), (1, 3, 11, 64, 1, 0), (1, 3, 10, 32, 2, 4)]
     return c
 
 

@@ -200,7 +200,7 @@ int b64_config_set(const char *key, int64_t value) {
         if (value < 1 || value > (1 << 30)) return B64_ERR_INVALID_ARGS;
         t.tab_min = (int)value;
     } else if (!strcmp(field, "ntt.tab_group")) {
-        if (value != 2 && value != 4 && value != 8) return B64_ERR_INVALID_ARGS;
+        if (value != 1 && value != 2 && value != 4 && value != 8) return B64_ERR_INVALID_ARGS;
         t.tab_group = (int)value;
     } else if (!strcmp(field, "ntt.waves")) {
         if (value != 0 && value != 4) return B64_ERR_INVALID_ARGS;

@@ -49,6 +49,9 @@ struct NttPassArgs {
     // out[r*2^m + i] = msg[rev_m(i)] (reed_solomon.rs:164-187), instead of data.
     const uint4 *msg;
     int log_d, layer0, k, c, m;
+    // index bits lb+k .. lb+k+e-1, just above the flipped bits, also held in the
+    // tile (independent butterfly groups), when the c low bits cannot fill it
+    int e;
     int tab_min;  // layers whose twiddle covers >= tab_min butterflies use window tables
 };
 struct LeafArgs {
@@ -84,7 +87,7 @@ struct NttTuning {
     int max_k;     // layers per pass
     int tile_log;  // log2 elements per LDS tile
     int tab_min;   // variant 1: butterflies per twiddle (in a tile) needed to build a table
-    int tab_group; // variant 1: window tables resident in LDS at once (2, 4 or 8; 8 KiB each)
+    int tab_group; // variant 1: window tables resident in LDS at once (1, 2, 4 or 8; 8 KiB each)
     int waves;     // tables-only passes: 4 = kernel limited to 128 VGPRs (4 waves per SIMD), 0 = default
 };
 extern NttTuning g_ntt_tuning;  // defaults
@@ -111,8 +114,10 @@ hipError_t launch_fold_codeword(const DeviceDomain &dd, const uint4 *cw, uint4 *
 // Subtree-reduction schedule of a tree with n_leaves leaves: per launch the
 // input level offset (in digests), its node count and the levels reduced.
 int merkle_reduce_steps(uint64_t n_leaves, uint64_t *offs, uint64_t *ns, int *levels);
-// Pass schedule of a forward NTT (layer0, k, c per pass); returns the count.
-int ntt_pass_plan(int log_d, int skip_early, int skip_late, int *layer0, int *k, int *c);
+// Pass schedule of a forward NTT (layer0, k, c, e per pass); returns the count.
+// A pass that gathers from the message (the first forward pass of an encode,
+// msg_first) keeps e = 0.
+int ntt_pass_plan(int log_d, int skip_early, int skip_late, bool msg_first, int *layer0, int *k, int *c, int *e);
 // One launch of `kind` over n_jobs jobs: d_jobs is a device array of the
 // kind's args, d_start[j] the first block of job j, `blocks` the total.
 hipError_t launch_multi(MultiKind kind, const void *d_jobs, const uint32_t *d_start, int n_jobs, uint32_t blocks,

@@ -72,6 +72,9 @@ hipError_t launch_bitrev_replicate(const uint4 *msg, uint4 *out, int m, int lir,
 // of the c lowest index bits (contiguous in HBM, so loads coalesce); the bits in
 // between and above come from the tile id.  Twiddles only depend on bits above
 // the flipped bit, so a tile needs 2^j distinct twiddles in its j-th layer.
+// Where the low bits are too few to fill a tile (the last passes, lb small),
+// the tile also takes the e bits just above the flipped ones: 2^e independent
+// butterfly groups, and 2^(j+e) twiddles in layer j.
 // ---------------------------------------------------------------------------
 constexpr int NTT_TILE_LOG = 11;
 constexpr int NTT_MAX_K = 7;
@@ -107,8 +110,10 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Position of a tile: it covers index bits lb .. lb+k-1 (flipped) and 0 .. c-1;
-// the bits in between (mid) and above (hi) come from the tile id.
+// Position of a tile: it covers index bits lb .. lb+k+e-1 (k flipped, e above
+// them) and 0 .. c-1; the bits in between (mid) and above (hi) come from the
+// tile id.  Tile element L = x << (k+c) | bf << c | low sits at index
+// base | (L >> c) << lb | low.
 struct NttTile {
     uint64_t base, mid, hi;
     int lb, n_el;
@@ -119,8 +124,8 @@ __device__ __forceinline__ NttTile ntt_tile_at(const NttPassArgs &a, uint64_t ti
     const int mid_bits = t.lb - a.c;
     t.mid = tile & ((1ull << mid_bits) - 1);
     t.hi = tile >> mid_bits;
-    t.base = (t.hi << (t.lb + a.k)) | (t.mid << a.c);
-    t.n_el = 1 << (a.k + a.c);
+    t.base = (t.hi << (t.lb + a.k + a.e)) | (t.mid << a.c);
+    t.n_el = 1 << (a.k + a.c + a.e);
     return t;
 }
 
@@ -210,25 +215,28 @@ __device__ __forceinline__ void ntt_store_tile(const uint4 *__restrict__ s, cons
 }
 
 // Twiddles of every layer of the pass for this tile:
-// tw[2^j - 1 + loc] = twiddle(layer0 + j, hi << j | loc) = XOR of W[layer][i]
-// over the set bits i of the block index.  The part from hi (bits above the
-// tile) is shared by a layer's 2^j twiddles and is summed once per layer, with
-// the loads of a layer issued together.  Ends with a barrier.
+// tw[(2^j - 1) << e | B] = twiddle(layer0 + j, hi << (j+e) | B) = XOR of
+// W[layer][i] over the set bits i of the block index, for the 2^(j+e) blocks B
+// of layer j in the tile.  The part from hi (bits above the tile) is shared by
+// a layer's twiddles and is summed once per layer, with the loads of a layer
+// issued together.  Needs k + e <= NTT_MAX_K.  Ends with a barrier.
 constexpr int NTT_TW = (1 << NTT_MAX_K) - 1;
+__device__ __forceinline__ int ntt_tw_base(int j, int e) { return ((1 << j) - 1) << e; }
 
 __device__ __forceinline__ void ntt_tile_twiddles(const NttPassArgs &a, uint64_t hi, uint4 *__restrict__ tw,
                                                   uint4 *__restrict__ thi) {
-    const int k = a.k;
+    const int k = a.k, e = a.e;
     if ((int)threadIdx.x < k) {
         const int j = threadIdx.x;
-        const uint4 *Wl = a.W + (a.layer0 + j) * W_STRIDE + j;
+        const uint4 *Wl = a.W + (a.layer0 + j) * W_STRIDE + j + e;
+        const int n_hi = a.layer0 - e;
         uint4 acc = make_uint4(0, 0, 0, 0);
-        for (int i0 = 0; i0 < a.layer0; i0 += 8) {
+        for (int i0 = 0; i0 < n_hi; i0 += 8) {
             uint4 w[8];
 #pragma unroll
             for (int u = 0; u < 8; u++) {
                 const int i = i0 + u;
-                w[u] = (i < a.layer0 && ((hi >> i) & 1)) ? Wl[i] : make_uint4(0, 0, 0, 0);
+                w[u] = (i < n_hi && ((hi >> i) & 1)) ? Wl[i] : make_uint4(0, 0, 0, 0);
             }
 #pragma unroll
             for (int u = 0; u < 8; u++) acc = gxor(acc, w[u]);
@@ -236,14 +244,14 @@ __device__ __forceinline__ void ntt_tile_twiddles(const NttPassArgs &a, uint64_t
         thi[j] = acc;
     }
     lds_barrier();
-    for (int t = threadIdx.x; t < (1 << k) - 1; t += BLOCK) {
-        const int j = 31 - __clz(t + 1);  // t + 1 in [2^j, 2^(j+1))
-        const int loc = t + 1 - (1 << j);
+    for (int t = threadIdx.x; t < ntt_tw_base(k, e); t += BLOCK) {
+        const int j = 31 - __clz((t >> e) + 1);  // (t >> e) + 1 in [2^j, 2^(j+1))
+        const int loc = t - ntt_tw_base(j, e);    // block index, j + e bits
         const uint4 *Wl = a.W + (a.layer0 + j) * W_STRIDE;
         uint4 w[NTT_MAX_K - 1];
 #pragma unroll
         for (int i = 0; i < NTT_MAX_K - 1; i++)
-            w[i] = (i < j && ((loc >> i) & 1)) ? Wl[i] : make_uint4(0, 0, 0, 0);
+            w[i] = (i < j + e && ((loc >> i) & 1)) ? Wl[i] : make_uint4(0, 0, 0, 0);
         uint4 acc = thi[j];
 #pragma unroll
         for (int i = 0; i < NTT_MAX_K - 1; i++) acc = gxor(acc, w[i]);
@@ -266,7 +274,7 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass(NttPassArgs a) {
 
     for (int jj = 0; jj < k; jj++) {
         const int j = INVERSE ? (k - 1 - jj) : jj;
-        const uint4 *twl = tw + (1 << j) - 1;
+        const uint4 *twl = tw + ntt_tw_base(j, a.e);
         const int pos = c + (k - 1 - j);
         const int lowmask = (1 << pos) - 1;
         for (int P = threadIdx.x; P < (n_el >> 1); P += BLOCK) {
@@ -315,11 +323,11 @@ __device__ __forceinline__ void ntt_tab_layers(const NttPassArgs &a, int n_el, S
     const int k = a.k, c = a.c;
     for (int jj = 0; jj < k; jj++) {
         const int j = INVERSE ? (k - 1 - jj) : jj;
-        const uint4 *__restrict__ tw = sm.tw + (1 << j) - 1;
+        const uint4 *__restrict__ tw = sm.tw + ntt_tw_base(j, a.e);
         const int pos = c + (k - 1 - j);
         const int per_blk = 1 << pos;
         const int lowmask = per_blk - 1;
-        const int nblk = 1 << j;
+        const int nblk = 1 << (j + a.e);
         if (HYBRID && per_blk < a.tab_min) {
             // few butterflies per twiddle: a table would not pay for itself
             for (int P = threadIdx.x; P < (n_el >> 1); P += BLOCK) {
@@ -398,6 +406,7 @@ static void launch_ntt_tab_h(const NttPassArgs &a, uint32_t n_tiles, int tab_gro
     switch (tab_group) {
         case 8: k_ntt_pass_tab<INVERSE, 8, HYBRID><<<n_tiles, BLOCK, 0, s>>>(a); break;
         case 4: k_ntt_pass_tab<INVERSE, 4, HYBRID><<<n_tiles, BLOCK, 0, s>>>(a); break;
+        case 1: k_ntt_pass_tab<INVERSE, 1, HYBRID><<<n_tiles, BLOCK, 0, s>>>(a); break;
         default: k_ntt_pass_tab<INVERSE, 2, HYBRID><<<n_tiles, BLOCK, 0, s>>>(a); break;
     }
 }
@@ -407,6 +416,7 @@ static void launch_ntt_tab_w4(const NttPassArgs &a, uint32_t n_tiles, int tab_gr
     switch (tab_group) {
         case 8: k_ntt_pass_tab_w4<INVERSE, 8><<<n_tiles, BLOCK, 0, s>>>(a); break;
         case 4: k_ntt_pass_tab_w4<INVERSE, 4><<<n_tiles, BLOCK, 0, s>>>(a); break;
+        case 1: k_ntt_pass_tab_w4<INVERSE, 1><<<n_tiles, BLOCK, 0, s>>>(a); break;
         default: k_ntt_pass_tab_w4<INVERSE, 2><<<n_tiles, BLOCK, 0, s>>>(a); break;
     }
 }
@@ -415,7 +425,7 @@ static void launch_ntt_tab_w4(const NttPassArgs &a, uint32_t n_tiles, int tab_gr
 template <bool INVERSE>
 static void launch_ntt_tab(const NttPassArgs &a, uint32_t n_tiles, const NttTuning &tu, hipStream_t s) {
     if ((1 << a.c) >= a.tab_min) {
-        if (tu.waves == 4 && a.k + a.c <= NTT_W4_TILE_LOG)
+        if (tu.waves == 4 && a.k + a.c + a.e <= NTT_W4_TILE_LOG)
             launch_ntt_tab_w4<INVERSE>(a, n_tiles, tu.tab_group, s);
         else
             launch_ntt_tab_h<INVERSE, false>(a, n_tiles, tu.tab_group, s);
@@ -425,7 +435,10 @@ static void launch_ntt_tab(const NttPassArgs &a, uint32_t n_tiles, const NttTuni
 }
 
 
-int ntt_pass_plan(int log_d, int skip_early, int skip_late, int *layer0, int *kk, int *cc) {
+constexpr int NTT_MIN_TILES_LOG = 10;
+
+int ntt_pass_plan(int log_d, int skip_early, int skip_late, bool msg_first, int *layer0, int *kk, int *cc,
+                  int *ee) {
     const int first = skip_early, end = log_d - skip_late;
     const int n_layers = end - first;
     if (n_layers <= 0) return 0;
@@ -444,6 +457,14 @@ int ntt_pass_plan(int log_d, int skip_early, int skip_late, int *layer0, int *kk
         layer0[p] = at;
         kk[p] = k;
         cc[p] = lb < (tile_log - k) ? lb : (tile_log - k);
+        // fill the tile from the bits above the flipped ones (there are `at`),
+        // as long as 2^NTT_MIN_TILES_LOG tiles remain to spread over the device
+        int e = tile_log - k - cc[p];
+        if (e > at) e = at;
+        if (e > NTT_MAX_K - k) e = NTT_MAX_K - k;
+        const int spare = log_d - k - cc[p] - NTT_MIN_TILES_LOG;
+        if (e > spare) e = spare > 0 ? spare : 0;
+        ee[p] = (p == 0 && msg_first) ? 0 : e;
         at += k;
     }
     return n_passes;
@@ -453,9 +474,9 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
                       bool inverse, hipStream_t s, const uint4 *msg, int msg_log) {
     // message mode needs the first pass to start at layer lir = log_d - msg_log
     if (msg && (inverse || skip_early != log_d - msg_log)) return hipErrorInvalidValue;
-    int starts[64], ks[64], cs[64];
+    int starts[64], ks[64], cs[64], es[64];
     const NttTuning &tu = ntt_tuning_for(log_d);
-    const int n_passes = ntt_pass_plan(log_d, skip_early, skip_late, starts, ks, cs);
+    const int n_passes = ntt_pass_plan(log_d, skip_early, skip_late, msg != nullptr, starts, ks, cs, es);
     for (int q = 0; q < n_passes; q++) {
         const int p = inverse ? (n_passes - 1 - q) : q;
         NttPassArgs a;
@@ -467,8 +488,9 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
         a.layer0 = starts[p];
         a.k = ks[p];
         a.c = cs[p];
+        a.e = es[p];
         a.tab_min = tu.tab_min;
-        const uint64_t n_tiles = 1ull << (log_d - a.k - a.c);
+        const uint64_t n_tiles = 1ull << (log_d - a.k - a.c - a.e);
         if (tu.variant == 1) {
             if (inverse)
                 launch_ntt_tab<true>(a, (unsigned)n_tiles, tu, s);

@@ -81,7 +81,8 @@ def test_ntt_forward_windows(b64, orc, log_d):
     (0, 7, 11, 1, 2, 0), (1, 7, 11, 1, 2, 0), (1, 7, 11, 128, 2, 0), (1, 7, 11, 1 << 20, 2, 0), (1, 3, 6, 4, 2, 0),
     (1, 1, 1, 1, 2, 0), (1, 5, 9, 64, 2, 0), (0, 2, 4, 1, 2, 0), (1, 4, 11, 32, 2, 0), (1, 7, 11, 1, 4, 0),
     (1, 6, 11, 16, 8, 0), (1, 5, 10, 8, 8, 0), (1, 3, 11, 1, 4, 0), (1, 5, 10, 16, 2, 4), (1, 4, 9, 1, 4, 4),
-    (1, 7, 11, 1, 2, 4), (1, 3, 10, 1, 8, 4)])
+    (1, 7, 11, 1, 2, 4), (1, 3, 10, 1, 8, 4), (1, 4, 11, 64, 1, 0), (1, 7, 11, 1, 1, 0),
+    (1, 3, 6, 4, 1, 4), (1, 5, 10, 16, 1, 0)])
 def test_ntt_tuning_variants(b64, orc, variant, max_k, tile_log, tab_min, tab_group, waves):
     """Every multiply variant and pass/tile shape gives the same transform."""
     old = {k: b64.config_get(k) for k in ("ntt.variant", "ntt.max_k", "ntt.tile_log", "ntt.tab_min", "ntt.tab_group",
@@ -104,6 +105,30 @@ def test_ntt_tuning_variants(b64, orc, variant, max_k, tile_log, tab_min, tab_gr
         for k, v in old.items():
             b64.config_set(k, v)
         from binius64_amd._abi import _apply_tuning
+        _apply_tuning(b64.lib())
+
+
+@pytest.mark.parametrize("variant,max_k,tile_log,tab_min,tab_group", [(1, 4, 6, 8, 2), (1, 7, 11, 128, 2),
+                                                                    (1, 4, 11, 64, 1), (1, 5, 10, 64, 2)])
+def test_ntt_schedules_log_d20(b64, orc, variant, max_k, tile_log, tab_min, tab_group):
+    """Schedules at a size where the last passes take extra index bits into
+    their tiles: forward, inverse and the message-gathering encode."""
+    log_d = 20
+    cand = (variant, max_k, tile_log, tab_min, tab_group)
+    dom, ref = b64.DomainContext.standard(log_d), orc.Domain(dim=log_d)
+    data = orc.random_elems(2024, 1 << log_d)
+    try:
+        for key, v in zip(("ntt.variant", "ntt.max_k", "ntt.tile_log", "ntt.tab_min", "ntt.tab_group"), cand):
+            b64.config_set(f"{key}:{log_d}", v)
+        say("ntt-many-tiles", cand)
+        assert np.array_equal(ntt_dev(b64, dom, data, 1, 4), orc.ntt_forward(ref, data, 1, 4))
+        assert np.array_equal(ntt_dev(b64, dom, data, 0, 0, inverse=True), orc.ntt_inverse(ref, data, 0, 0))
+        msg = data[: 1 << (log_d - 1)]
+        out = b64.ReedSolomonCode(dom, log_d - 5, 1).encode_batch(b64.DeviceBuffer.from_host(msg), 4)
+        assert np.array_equal(out.to_host(1 << log_d), orc.encode_batch(ref, msg, log_d - 5, 1, 4, threads=8))
+    finally:
+        from binius64_amd._abi import _apply_tuning
+        b64.config_set("ntt.clear_sizes", 0)
         _apply_tuning(b64.lib())
 
 
