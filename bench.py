@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # copy) and VALU issue rate (SQ_INSTS_VALU per CU per clock).  Keyed by kernel,
 # log2 codeword length, layers per pass and tile size; the counters cannot be
 # read from inside the timed run, so other schedules report null.
-TRAFFIC = {("k_ntt_pass_tab", 25, 7, 11): {"bytes": 1073929374, "valu_issue_per_cu_clk": 0.91}}
+TRAFFIC = {("k_ntt_pass_tab", 25, 4, 11): {"bytes": 1073876727, "valu_issue_per_cu_clk": 0.64}}
 
 
 def parse():
