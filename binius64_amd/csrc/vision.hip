@@ -27,6 +27,7 @@
 
 #include <mutex>
 #include <new>
+#include <type_traits>
 
 #include "abi_util.hpp"
 #include "gf128_dev.hpp"
@@ -281,9 +282,64 @@ const uint4 *device_data(int m) {
 }
 
 // ---- device side -------------------------------------------------------------
-__device__ __forceinline__ uint4 vmap(uint32_t tabs, int t, uint4 v) {
-    return win_mul_perm(tabs + (uint32_t)(t * WIN_ENTRIES * 16), v);
+// The permutation is written once over an "ops" policy that supplies the
+// general product and the table maps:
+//  * LaneOps: one lane owns a state and does all of its work (throughput form);
+//  * QuadOps: the four lanes of a quad (lanes 4i .. 4i+3) hold the same state and
+//    split every product and every map four ways, exchanging partial results
+//    with DPP quad permutes.  A product costs each lane one 64x64 carry-less
+//    product (a_(q>>1) * b_(q&1)) instead of three, a map 8 table reads instead
+//    of 32, so a state's dependent chain is about 2.5x shorter (latency form,
+//    for launches with too few states to fill the device).
+struct LaneOps {
+    __device__ __forceinline__ uint4 mul(uint4 a, uint4 b) const { return gf_mul(a, b); }
+    __device__ __forceinline__ uint4 map(uint32_t tabs, int t, uint4 v) const {
+        return win_mul_perm(tabs + (uint32_t)(t * WIN_ENTRIES * 16), v);
+    }
+    __device__ __forceinline__ bool writer() const { return true; }
+};
+
+// Lane K of the quad, to every lane of the quad.
+template <int K>
+__device__ __forceinline__ uint32_t quad_bcast(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, K * 0x55, 0xF, 0xF, true);
 }
+template <int K>
+__device__ __forceinline__ uint64_t quad_bcast64(uint64_t v) {
+    return ((uint64_t)quad_bcast<K>((uint32_t)(v >> 32)) << 32) | quad_bcast<K>((uint32_t)v);
+}
+// XOR of the four lanes' values, in every lane (quad_perm [1,0,3,2], then [2,3,0,1]).
+__device__ __forceinline__ uint32_t quad_xor(uint32_t v) {
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true);
+    return v ^ (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, true);
+}
+
+struct QuadOps {
+    uint32_t q;  // lane within the quad
+    // a b = a0 b0 + (a0 b1 + a1 b0) X + a1 b1 X^2 with X = x^64; lane q computes
+    // a_(q>>1) b_(q&1).
+    __device__ __forceinline__ uint4 mul(uint4 a, uint4 b) const {
+        const bool ah = q & 2, bh = q & 1;
+        const uint64_t av = ah ? (((uint64_t)a.w << 32) | a.z) : (((uint64_t)a.y << 32) | a.x);
+        const uint64_t bv = bh ? (((uint64_t)b.w << 32) | b.z) : (((uint64_t)b.y << 32) | b.x);
+        uint64_t lo, hi;
+        clmul64(av, bv, lo, hi);
+        const uint64_t p0l = quad_bcast64<0>(lo), p0h = quad_bcast64<0>(hi);
+        const uint64_t p1l = quad_bcast64<1>(lo), p1h = quad_bcast64<1>(hi);
+        const uint64_t p2l = quad_bcast64<2>(lo), p2h = quad_bcast64<2>(hi);
+        const uint64_t p3l = quad_bcast64<3>(lo), p3h = quad_bcast64<3>(hi);
+        return gf_reduce(p0l, xor3_64(p0h, p1l, p2l), xor3_64(p1h, p2h, p3l), p3h);
+    }
+    // Lane q looks up word q of v in rows 8q .. 8q+7 of the table (those rows'
+    // slot swizzle is the nibble index XOR 8 (q & 1)), then the quad sums.
+    __device__ __forceinline__ uint4 map(uint32_t tabs, int t, uint4 v) const {
+        const uint32_t w = q == 0 ? v.x : (q == 1 ? v.y : (q == 2 ? v.z : v.w));
+        const uint32_t base = tabs + (uint32_t)(t * WIN_ENTRIES * 16) + q * (8 * 16 * 16);
+        const uint4 acc = win_word_perm(base, w ^ ((q & 1) ? 0x88888888u : 0u), 0, make_uint4(0, 0, 0, 0));
+        return make_uint4(quad_xor(acc.x), quad_xor(acc.y), quad_xor(acc.z), quad_xor(acc.w));
+    }
+    __device__ __forceinline__ bool writer() const { return q == 0; }
+};
 
 // v * x^-1 (ghash.rs mul_inv_x)
 __device__ __forceinline__ uint4 gf_mul_inv_x(uint4 v) {
@@ -295,27 +351,28 @@ __device__ __forceinline__ uint4 gf_mul_inv_x(uint4 v) {
 // a^(2^128 - 2): b_k = a^(2^k - 1), b_(i+j) = b_i^(2^j) b_j along
 // k = 1, 2, 3, 6, 7, 14, 15, 30, 31, 62, 63, 126, 127, then one squaring
 // (x^(2^63) as two x^(2^31) maps and a squaring).
-__device__ uint4 vis_inv(uint32_t tabs, uint4 a) {
+template <class Ops>
+__device__ uint4 vis_inv(const Ops &o, uint32_t tabs, uint4 a) {
     const uint4 b1 = a;
-    const uint4 b2 = gf_mul(vmap(tabs, T_SQR, b1), b1);
-    const uint4 b3 = gf_mul(vmap(tabs, T_SQR, b2), b1);
-    const uint4 b6 = gf_mul(vmap(tabs, T_POW3, b3), b3);
-    const uint4 b7 = gf_mul(vmap(tabs, T_SQR, b6), b1);
-    const uint4 b14 = gf_mul(vmap(tabs, T_POW7, b7), b7);
-    const uint4 b15 = gf_mul(vmap(tabs, T_SQR, b14), b1);
-    const uint4 b30 = gf_mul(vmap(tabs, T_POW15, b15), b15);
-    const uint4 b31 = gf_mul(vmap(tabs, T_SQR, b30), b1);
-    const uint4 b62 = gf_mul(vmap(tabs, T_POW31, b31), b31);
-    const uint4 b63 = gf_mul(vmap(tabs, T_SQR, b62), b1);
-    const uint4 b126 = gf_mul(vmap(tabs, T_SQR, vmap(tabs, T_POW31, vmap(tabs, T_POW31, b63))), b63);
-    const uint4 b127 = gf_mul(vmap(tabs, T_SQR, b126), b1);
-    return vmap(tabs, T_SQR, b127);
+    const uint4 b2 = o.mul(o.map(tabs, T_SQR, b1), b1);
+    const uint4 b3 = o.mul(o.map(tabs, T_SQR, b2), b1);
+    const uint4 b6 = o.mul(o.map(tabs, T_POW3, b3), b3);
+    const uint4 b7 = o.mul(o.map(tabs, T_SQR, b6), b1);
+    const uint4 b14 = o.mul(o.map(tabs, T_POW7, b7), b7);
+    const uint4 b15 = o.mul(o.map(tabs, T_SQR, b14), b1);
+    const uint4 b30 = o.mul(o.map(tabs, T_POW15, b15), b15);
+    const uint4 b31 = o.mul(o.map(tabs, T_SQR, b30), b1);
+    const uint4 b62 = o.mul(o.map(tabs, T_POW31, b31), b31);
+    const uint4 b63 = o.mul(o.map(tabs, T_SQR, b62), b1);
+    const uint4 b126 = o.mul(o.map(tabs, T_SQR, o.map(tabs, T_POW31, o.map(tabs, T_POW31, b63))), b63);
+    const uint4 b127 = o.mul(o.map(tabs, T_SQR, b126), b1);
+    return o.map(tabs, T_SQR, b127);
 }
 
 // invert_or_zero of every element with one inversion (zeros enter as ones and
 // come back as zeros).
-template <int M>
-__device__ __forceinline__ void vis_batch_inv(uint32_t tabs, uint4 (&s)[M]) {
+template <int M, class Ops>
+__device__ __forceinline__ void vis_batch_inv(const Ops &o, uint32_t tabs, uint4 (&s)[M]) {
     const uint4 one = make_uint4(1, 0, 0, 0), zero = make_uint4(0, 0, 0, 0);
     uint4 pre[M], x[M];
     bool z[M];
@@ -325,13 +382,13 @@ __device__ __forceinline__ void vis_batch_inv(uint32_t tabs, uint4 (&s)[M]) {
         z[i] = gzero(s[i]);
         x[i] = z[i] ? one : s[i];
         pre[i] = acc;
-        acc = i == 0 ? x[0] : gf_mul(acc, x[i]);
+        acc = i == 0 ? x[0] : o.mul(acc, x[i]);
     }
-    uint4 inv = vis_inv(tabs, acc);
+    uint4 inv = vis_inv(o, tabs, acc);
 #pragma unroll
     for (int i = M - 1; i >= 0; i--) {
-        const uint4 out = i == 0 ? inv : gf_mul(inv, pre[i]);
-        if (i > 0) inv = gf_mul(inv, x[i]);
+        const uint4 out = i == 0 ? inv : o.mul(inv, pre[i]);
+        if (i > 0) inv = o.mul(inv, x[i]);
         s[i] = z[i] ? zero : out;
     }
 }
@@ -365,21 +422,21 @@ struct VisionLds {
     uint4 c0, binv0;
 };
 
-template <int M>
-__device__ void vis_permute(const VisionLds &v, uint4 (&s)[M]) {
+template <int M, class Ops = LaneOps>
+__device__ void vis_permute(const VisionLds &v, uint4 (&s)[M], const Ops &o = Ops()) {
 #pragma unroll
     for (int i = 0; i < M; i++) s[i] = gxor(s[i], v.rc[i]);
 #pragma unroll 1
     for (int r = 0; r < ROUNDS; r++) {
-        vis_batch_inv<M>(v.tabs, s);
+        vis_batch_inv<M>(o, v.tabs, s);
 #pragma unroll
-        for (int i = 0; i < M; i++) s[i] = gxor(vmap(v.tabs, T_BINV, s[i]), v.binv0);
+        for (int i = 0; i < M; i++) s[i] = gxor(o.map(v.tabs, T_BINV, s[i]), v.binv0);
         vis_mds<M>(s);
 #pragma unroll
         for (int i = 0; i < M; i++) s[i] = gxor(s[i], v.rc[(1 + 2 * r) * MAXM + i]);
-        vis_batch_inv<M>(v.tabs, s);
+        vis_batch_inv<M>(o, v.tabs, s);
 #pragma unroll
-        for (int i = 0; i < M; i++) s[i] = gxor(vmap(v.tabs, T_BFWD, s[i]), v.c0);
+        for (int i = 0; i < M; i++) s[i] = gxor(o.map(v.tabs, T_BFWD, s[i]), v.c0);
         vis_mds<M>(s);
 #pragma unroll
         for (int i = 0; i < M; i++) s[i] = gxor(s[i], v.rc[(2 + 2 * r) * MAXM + i]);
@@ -413,14 +470,19 @@ __global__ __launch_bounds__(VBLOCK) void k_vision_permute(const uint4 *__restri
 // state[0 .. R) and permutes; the final block holds the remaining elements, a
 // 0x80 byte after them and 0x01 in its last byte (a whole padding block when
 // nothing remains); the digest is state[0] || state[1].
-template <int M>
+template <int M, bool QUAD>
 __global__ __launch_bounds__(VBLOCK) void k_vision_digest(const uint4 *__restrict__ dev,
                                                         const uint4 *__restrict__ msgs, uint64_t n, uint32_t len,
                                                         uint4 *__restrict__ out) {
     constexpr int R = M - 2;
+    constexpr int SHIFT = QUAD ? 2 : 0;
     __shared__ __attribute__((aligned(256))) uint4 vis_lds[DEV_ELEMS];
     const VisionLds v = vis_stage(dev, vis_lds);
-    for (uint64_t k = (uint64_t)blockIdx.x * VBLOCK + threadIdx.x; k < n; k += (uint64_t)gridDim.x * VBLOCK) {
+    typename std::conditional<QUAD, QuadOps, LaneOps>::type ops;
+    if constexpr (QUAD) ops.q = threadIdx.x & 3;
+    for (uint64_t t = (uint64_t)blockIdx.x * VBLOCK + threadIdx.x; (t >> SHIFT) < n;
+         t += (uint64_t)gridDim.x * VBLOCK) {
+        const uint64_t k = t >> SHIFT;
         const uint4 *p = msgs + k * len;
         uint4 s[M];
 #pragma unroll
@@ -429,7 +491,7 @@ __global__ __launch_bounds__(VBLOCK) void k_vision_digest(const uint4 *__restric
         for (; e + R <= len; e += R) {
 #pragma unroll
             for (int i = 0; i < R; i++) s[i] = p[e + i];
-            vis_permute<M>(v, s);
+            vis_permute<M>(v, s, ops);
         }
         const uint32_t rem = len - e;
 #pragma unroll
@@ -438,26 +500,37 @@ __global__ __launch_bounds__(VBLOCK) void k_vision_digest(const uint4 *__restric
             if ((uint32_t)i == rem) s[i].x ^= 0x80u;
         }
         s[R - 1].w ^= 0x01000000u;
-        vis_permute<M>(v, s);
-        out[2 * k] = s[0];
-        out[2 * k + 1] = s[1];
+        vis_permute<M>(v, s, ops);
+        if (ops.writer()) {
+            out[2 * k] = s[0];
+            out[2 * k + 1] = s[1];
+        }
     }
 }
 
 // VisionCompression of one tree level: node i of the level above is
 // compress(in[2i], in[2i+1]) = first half of permute(l0, l1, r0, r1) + (l0, l1).
+// QUAD: four lanes per node (QuadOps); the loop bound is uniform over a quad.
+template <bool QUAD>
 __global__ __launch_bounds__(VBLOCK) void k_vision_compress(const uint4 *__restrict__ dev, uint4 *__restrict__ tree,
                                                           uint64_t in_off, uint64_t n_out) {
+    constexpr int SHIFT = QUAD ? 2 : 0;
     __shared__ __attribute__((aligned(256))) uint4 vis_lds[DEV_ELEMS];
     const VisionLds v = vis_stage(dev, vis_lds);
-    for (uint64_t i = (uint64_t)blockIdx.x * VBLOCK + threadIdx.x; i < n_out; i += (uint64_t)gridDim.x * VBLOCK) {
+    typename std::conditional<QUAD, QuadOps, LaneOps>::type ops;
+    if constexpr (QUAD) ops.q = threadIdx.x & 3;
+    for (uint64_t t = (uint64_t)blockIdx.x * VBLOCK + threadIdx.x; (t >> SHIFT) < n_out;
+         t += (uint64_t)gridDim.x * VBLOCK) {
+        const uint64_t i = t >> SHIFT;
         const uint4 *in = tree + 2 * (in_off + 2 * i);
         uint4 s[4] = {in[0], in[1], in[2], in[3]};
         const uint4 l0 = s[0], l1 = s[1];
-        vis_permute<4>(v, s);
-        uint4 *o = tree + 2 * (in_off + 2 * n_out + i);
-        o[0] = gxor(s[0], l0);
-        o[1] = gxor(s[1], l1);
+        vis_permute<4>(v, s, ops);
+        if (ops.writer()) {
+            uint4 *o = tree + 2 * (in_off + 2 * n_out + i);
+            o[0] = gxor(s[0], l0);
+            o[1] = gxor(s[1], l1);
+        }
     }
 }
 
@@ -476,13 +549,28 @@ hipError_t launch_permute(int m, const uint4 *dev, uint4 *states, uint64_t n, hi
     return hipGetLastError();
 }
 
+// Launches with at most this many states run four lanes per state (QuadOps).
+// A state's permutations form one dependent chain; with fewer states than
+// lanes on the device the quad form's 2.6x shorter chain wins, with more the
+// lane form's 1.8x smaller instruction count does (measured at 2^16 and 2^15
+// Vision-4 compressions per launch: 0.60 vs 0.52 ms, 0.31 vs 0.49 ms).
+constexpr uint64_t QUAD_MAX_STATES = 1ull << 15;
+
 hipError_t launch_digest(int m, const uint4 *dev, const uint4 *msgs, uint64_t n, uint32_t len, uint4 *out,
                          hipStream_t s) {
     if (n == 0) return hipSuccess;
+    const bool quad = n <= QUAD_MAX_STATES;
+    const unsigned grid = vis_grid(quad ? 4 * n : n);
     if (m == 4) {
-        k_vision_digest<4><<<vis_grid(n), VBLOCK, 0, s>>>(dev, msgs, n, len, out);
+        if (quad)
+            k_vision_digest<4, true><<<grid, VBLOCK, 0, s>>>(dev, msgs, n, len, out);
+        else
+            k_vision_digest<4, false><<<grid, VBLOCK, 0, s>>>(dev, msgs, n, len, out);
     } else {
-        k_vision_digest<6><<<vis_grid(n), VBLOCK, 0, s>>>(dev, msgs, n, len, out);
+        if (quad)
+            k_vision_digest<6, true><<<grid, VBLOCK, 0, s>>>(dev, msgs, n, len, out);
+        else
+            k_vision_digest<6, false><<<grid, VBLOCK, 0, s>>>(dev, msgs, n, len, out);
     }
     return hipGetLastError();
 }
@@ -490,7 +578,10 @@ hipError_t launch_digest(int m, const uint4 *dev, const uint4 *msgs, uint64_t n,
 hipError_t launch_tree_levels(const uint4 *dev4, uint4 *tree, uint64_t n_leaves, hipStream_t s) {
     uint64_t in_off = 0;
     for (uint64_t n = n_leaves; n > 1; n /= 2) {
-        k_vision_compress<<<vis_grid(n / 2), VBLOCK, 0, s>>>(dev4, tree, in_off, n / 2);
+        if (n / 2 <= QUAD_MAX_STATES)
+            k_vision_compress<true><<<vis_grid(2 * n), VBLOCK, 0, s>>>(dev4, tree, in_off, n / 2);
+        else
+            k_vision_compress<false><<<vis_grid(n / 2), VBLOCK, 0, s>>>(dev4, tree, in_off, n / 2);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         in_off += n;

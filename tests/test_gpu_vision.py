@@ -63,7 +63,10 @@ def test_digest_lengths(b64, orc, oracles, m):
         assert got == want, msg_elems
 
 
-@pytest.mark.parametrize("leaf_m,log_leaves,batch", [(4, 0, 16), (4, 6, 16), (6, 6, 16), (4, 5, 2), (6, 9, 16)])
+# (4, 17, 1): lane-form digests and widest level, quad-form upper levels;
+# (6, 15, 2): quad-form digests at the largest count that takes them
+@pytest.mark.parametrize("leaf_m,log_leaves,batch", [(4, 0, 16), (4, 6, 16), (6, 6, 16), (4, 5, 2), (6, 9, 16),
+                                                     (4, 17, 1), (6, 15, 2)])
 def test_merkle_vision(b64, orc, oracles, leaf_m, log_leaves, batch):
     say("vision merkle", leaf_m, log_leaves, batch)
     n = batch << log_leaves
