@@ -13,15 +13,15 @@
 //  * B^-1: B's linear part inverted over GF(2) (Gauss-Jordan on its 128 x 128
 //    bit matrix), plus the constant B^-1(0) = L^-1(c0);
 //  * window tables (the 4-bit layout of gf128_dev.hpp's win_mul_perm) for the
-//    F2-linear maps the S-box needs: x^(2^k) for k = 1, 3, 7, 15, 31 (the
+//    F2-linear maps the S-box needs: x^(2^k) for k = 1, 3, 6, 12, 24 (the
 //    Itoh-Tsujii inversion chain) and the linear parts of B and B^-1.
 //
 // On the device one lane owns one state.  The S-box's batch inversion
 // (verifier/math/src/batch_invert.rs: invert_or_zero of every element) uses one
 // field inversion per S-box for the whole state (prefix products, then back
-// substitution); the inversion is a^(2^128-2) by the Itoh-Tsujii chain, 12
-// general products and 15 table maps.  The seven tables (56 KiB) and the round
-// constants sit in LDS for the whole launch.
+// substitution); the inversion is a^(2^128-2) by the Itoh-Tsujii chain, 10
+// general products and 12 table maps.  The seven tables (56 KiB) and the round
+// constants sit in LDS for the whole launch (two blocks per CU).
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -104,8 +104,12 @@ constexpr int ROUNDS = 8;
 constexpr int RC_ROWS = 1 + 2 * ROUNDS;
 constexpr int MAXM = 6;
 constexpr int N_TABLES = 7;
-enum Table { T_SQR = 0, T_POW3, T_POW7, T_POW15, T_POW31, T_BFWD, T_BINV };
+// T_Pk: x -> x^(2^k)
+enum Table { T_P1 = 0, T_P3, T_P6, T_P12, T_P24, T_BFWD, T_BINV };
 constexpr int VBLOCK = 256;
+// Two blocks of VBLOCK threads share a CU (58 KiB of LDS each), so kernels are
+// held to two waves per SIMD: at most 256 VGPRs and AGPRs together.
+#define VIS_KERNEL __global__ __launch_bounds__(VBLOCK) __attribute__((amdgpu_waves_per_eu(2)))
 // device data of one width: tables, then round constants (RC_ROWS * MAXM), then c0 and B^-1(0)
 constexpr int DEV_TAB = N_TABLES * WIN_ENTRIES;
 constexpr int DEV_RC = DEV_TAB;
@@ -239,7 +243,7 @@ bool build_device_data(int m, uint4 *out) {
     else
         hv.round_constants(INITIAL6, MATRIX6, CONSTANT6);
     u128 col[128];
-    const int pows[5] = {1, 3, 7, 15, 31};
+    const int pows[5] = {1, 3, 6, 12, 24};
     for (int t = 0; t < 5; t++) {
         for (int j = 0; j < 128; j++) {
             u128 x = (u128)1 << j;
@@ -348,25 +352,23 @@ __device__ __forceinline__ uint4 gf_mul_inv_x(uint4 v) {
                       (v.w >> 1) ^ (m & 0x80000000u));
 }
 
-// a^(2^128 - 2): b_k = a^(2^k - 1), b_(i+j) = b_i^(2^j) b_j along
-// k = 1, 2, 3, 6, 7, 14, 15, 30, 31, 62, 63, 126, 127, then one squaring
-// (x^(2^63) as two x^(2^31) maps and a squaring).
+// a^(2^128 - 2): b_k = a^(2^k - 1), b_(i+j) = b_i^(2^j) b_j along the
+// addition chain 1, 2, 3, 6, 12, 24, 48, 96, 120, 126, 127 (10 products;
+// x^(2^48) as two x^(2^24) maps), then one squaring.
 template <class Ops>
 __device__ uint4 vis_inv(const Ops &o, uint32_t tabs, uint4 a) {
     const uint4 b1 = a;
-    const uint4 b2 = o.mul(o.map(tabs, T_SQR, b1), b1);
-    const uint4 b3 = o.mul(o.map(tabs, T_SQR, b2), b1);
-    const uint4 b6 = o.mul(o.map(tabs, T_POW3, b3), b3);
-    const uint4 b7 = o.mul(o.map(tabs, T_SQR, b6), b1);
-    const uint4 b14 = o.mul(o.map(tabs, T_POW7, b7), b7);
-    const uint4 b15 = o.mul(o.map(tabs, T_SQR, b14), b1);
-    const uint4 b30 = o.mul(o.map(tabs, T_POW15, b15), b15);
-    const uint4 b31 = o.mul(o.map(tabs, T_SQR, b30), b1);
-    const uint4 b62 = o.mul(o.map(tabs, T_POW31, b31), b31);
-    const uint4 b63 = o.mul(o.map(tabs, T_SQR, b62), b1);
-    const uint4 b126 = o.mul(o.map(tabs, T_SQR, o.map(tabs, T_POW31, o.map(tabs, T_POW31, b63))), b63);
-    const uint4 b127 = o.mul(o.map(tabs, T_SQR, b126), b1);
-    return o.map(tabs, T_SQR, b127);
+    const uint4 b2 = o.mul(o.map(tabs, T_P1, b1), b1);
+    const uint4 b3 = o.mul(o.map(tabs, T_P1, b2), b1);
+    const uint4 b6 = o.mul(o.map(tabs, T_P3, b3), b3);
+    const uint4 b12 = o.mul(o.map(tabs, T_P6, b6), b6);
+    const uint4 b24 = o.mul(o.map(tabs, T_P12, b12), b12);
+    const uint4 b48 = o.mul(o.map(tabs, T_P24, b24), b24);
+    const uint4 b96 = o.mul(o.map(tabs, T_P24, o.map(tabs, T_P24, b48)), b48);
+    const uint4 b120 = o.mul(o.map(tabs, T_P24, b96), b24);
+    const uint4 b126 = o.mul(o.map(tabs, T_P6, b120), b6);
+    const uint4 b127 = o.mul(o.map(tabs, T_P1, b126), b1);
+    return o.map(tabs, T_P1, b127);
 }
 
 // invert_or_zero of every element with one inversion (zeros enter as ones and
@@ -451,7 +453,7 @@ __device__ VisionLds vis_stage(const uint4 *__restrict__ dev, uint4 *lds) {
 }
 
 template <int M>
-__global__ __launch_bounds__(VBLOCK) void k_vision_permute(const uint4 *__restrict__ dev, uint4 *__restrict__ states,
+VIS_KERNEL void k_vision_permute(const uint4 *__restrict__ dev, uint4 *__restrict__ states,
                                                          uint64_t n) {
     __shared__ __attribute__((aligned(256))) uint4 vis_lds[DEV_ELEMS];
     const VisionLds v = vis_stage(dev, vis_lds);
@@ -471,7 +473,7 @@ __global__ __launch_bounds__(VBLOCK) void k_vision_permute(const uint4 *__restri
 // 0x80 byte after them and 0x01 in its last byte (a whole padding block when
 // nothing remains); the digest is state[0] || state[1].
 template <int M, bool QUAD>
-__global__ __launch_bounds__(VBLOCK) void k_vision_digest(const uint4 *__restrict__ dev,
+VIS_KERNEL void k_vision_digest(const uint4 *__restrict__ dev,
                                                         const uint4 *__restrict__ msgs, uint64_t n, uint32_t len,
                                                         uint4 *__restrict__ out) {
     constexpr int R = M - 2;
@@ -512,7 +514,7 @@ __global__ __launch_bounds__(VBLOCK) void k_vision_digest(const uint4 *__restric
 // compress(in[2i], in[2i+1]) = first half of permute(l0, l1, r0, r1) + (l0, l1).
 // QUAD: four lanes per node (QuadOps); the loop bound is uniform over a quad.
 template <bool QUAD>
-__global__ __launch_bounds__(VBLOCK) void k_vision_compress(const uint4 *__restrict__ dev, uint4 *__restrict__ tree,
+VIS_KERNEL void k_vision_compress(const uint4 *__restrict__ dev, uint4 *__restrict__ tree,
                                                           uint64_t in_off, uint64_t n_out) {
     constexpr int SHIFT = QUAD ? 2 : 0;
     __shared__ __attribute__((aligned(256))) uint4 vis_lds[DEV_ELEMS];
