@@ -39,6 +39,8 @@ def default_candidates() -> List[Candidate]:
             c.append((1, k, tile, 64, 2, 4))
         c.append((1, k, 11, 64, 4, 0))
     c += [(1, 3, 11, 32, 2, 0), (1, 3, 11, 64, 1, 0), (1, 3, 10, 32, 2, 4)]
+    # variant 2: 4-layer forward passes above index bit 10 in registers
+    c += [(2, 4, 11, 128, 2, 0), (2, 4, 11, 64, 2, 0), (2, 4, 10, 128, 2, 0), (2, 4, 11, 64, 2, 4)]
     return c
 
 

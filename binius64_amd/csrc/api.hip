@@ -188,7 +188,7 @@ int b64_config_set(const char *key, int64_t value) {
     if (!parse_tuning_key(key, field, sizeof field, &log_d)) return B64_ERR_INVALID_ARGS;
     b64::NttTuning t = b64::ntt_tuning_for(log_d);
     if (!strcmp(field, "ntt.variant")) {
-        if (value < 0 || value > 1) return B64_ERR_INVALID_ARGS;
+        if (value < 0 || value > 2) return B64_ERR_INVALID_ARGS;
         t.variant = (int)value;
     } else if (!strcmp(field, "ntt.max_k")) {
         if (value < 1 || value > 7) return B64_ERR_INVALID_ARGS;

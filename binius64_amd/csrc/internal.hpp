@@ -83,7 +83,8 @@ struct FoldCwArgs {
 enum class MultiKind { NttPass, Leaf16Fold, LeafHash, Subtree, FoldCodeword };
 
 struct NttTuning {
-    int variant;   // 0 = general product, 1 = per-layer choice (window tables where amortised)
+    int variant;   // 0 = general product, 1 = per-layer choice (window tables where amortised),
+                   // 2 = as 1, with forward 4-layer passes above index bit 10 in registers
     int max_k;     // layers per pass
     int tile_log;  // log2 elements per LDS tile
     int tab_min;   // variant 1: butterflies per twiddle (in a tile) needed to build a table

@@ -80,7 +80,8 @@ constexpr int NTT_TILE_LOG = 11;
 constexpr int NTT_MAX_K = 7;
 
 // Runtime knobs (set through b64_config_set): the multiply used by the NTT
-// passes (0 = general product, 1 = LDS window tables where they pay), the
+// passes (0 = general product, 1 = LDS window tables where they pay, 2 = as 1
+// with forward 4-layer passes run in registers, k_ntt_pass_r16), the
 // largest number of layers per pass, the tile size, the butterflies per
 // twiddle that justify a table, and the tables resident in LDS at once.
 NttTuning g_ntt_tuning = {1, NTT_MAX_K, NTT_TILE_LOG, 128, 2, 0};
@@ -421,6 +422,101 @@ static void launch_ntt_tab_w4(const NttPassArgs &a, uint32_t n_tiles, int tab_gr
     }
 }
 
+// Four-layer passes in registers (variant 2).  A thread owns the 16 elements
+// that differ only in the pass's flipped bits lb .. lb+3 and runs all four
+// layers on them in registers; the 15 twiddles of the pass depend only on the
+// bits above lb+3 (hi), so a block that stays inside one hi group builds the
+// 15 window tables (120 KiB of LDS) once and streams 1024-position slices
+// through them: no LDS traffic for the data, no block barrier between layers.
+// Used where a hi group has at least R16_BLOCK positions (lb >= 10), forward
+// only; the message-gathering first pass of an encode reads, per thread, one
+// 16-element run of the message (the flipped bits are the low 4 bits of the
+// bit-reversed message index).
+constexpr int R16_BLOCK = 1024;
+constexpr int R16_MIN_LB = 10;
+constexpr int R16_TABLES = 15;
+struct alignas(256) R16Smem {
+    uint4 tabs[R16_TABLES * WIN_ENTRIES];
+    uint4 tw[NTT_TW];
+    uint4 thi[NTT_MAX_K];
+};
+
+template <bool MSG>
+__global__ __launch_bounds__(R16_BLOCK) void k_ntt_pass_r16(NttPassArgs a, uint64_t per_block) {
+    __shared__ R16Smem sm;
+    const int lb = a.log_d - a.layer0 - 4;
+    const uint64_t p0 = (uint64_t)blockIdx.x * per_block;
+    const uint64_t hi = p0 >> lb;
+    ntt_tile_twiddles(a, hi, sm.tw, sm.thi);  // a.k = 4, a.e = 0; ends with a barrier
+    for (int r = threadIdx.x; r < R16_TABLES * WIN_ROWS; r += R16_BLOCK)
+        win_build_row(sm.tabs + (r >> 5) * WIN_ENTRIES, sm.tw[r >> 5], r & 31);
+    lds_barrier();
+    const uint32_t tabs = lds_addr(sm.tabs);
+    const uint64_t lowmask = (1ull << lb) - 1;
+    for (uint64_t p = p0 + threadIdx.x; p < p0 + per_block; p += R16_BLOCK) {
+        const uint64_t low = p & lowmask;
+        const uint64_t base = (hi << (lb + 4)) | low;
+        uint4 x[16];
+        if (MSG) {
+            // codeword index bits lb .. lb+3 are message bits 3 .. 0 (reversed)
+            const int mb = a.m - 4;
+            const uint64_t run = mb ? (__brevll(low) >> (64 - mb)) << 4 : 0;
+#pragma unroll
+            for (int e = 0; e < 16; e++) x[e] = a.msg[run + rev_bits(e, 4)];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 16; e++) x[e] = a.data[base | ((uint64_t)e << lb)];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int bit = 3 - j;  // layer j flips element bit 3 - j
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                if (e & (1 << bit)) continue;
+                const int blk = e >> (bit + 1);
+                const uint32_t tb = tabs + (uint32_t)(((1 << j) - 1 + blk) * WIN_ENTRIES * 16);
+                uint4 u = x[e], v = x[e | (1 << bit)];
+                u = gxor(u, win_mul_perm(tb, v));
+                v = gxor(v, u);
+                x[e] = u;
+                x[e | (1 << bit)] = v;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 16; e++) a.data[base | ((uint64_t)e << lb)] = x[e];
+    }
+}
+
+static uint32_t device_cus() {
+    static int cached[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (cached[dev] == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 1;
+        cached[dev] = cus;
+    }
+    return (uint32_t)cached[dev];
+}
+
+// One block per CU (the tables take 120 KiB); positions per block a power of
+// two that divides 2^lb, so a block never leaves its hi group.
+static void launch_ntt_r16(const NttPassArgs &a, hipStream_t s) {
+    const int lb = a.log_d - a.layer0 - 4;
+    const int total_log = a.log_d - 4;
+    int cus_log = 0;
+    while ((1u << cus_log) < device_cus()) cus_log++;
+    int per_log = total_log - cus_log;
+    if (per_log < R16_MIN_LB) per_log = R16_MIN_LB;
+    if (per_log > lb) per_log = lb;
+    const uint32_t blocks = 1u << (total_log - per_log);
+    if (a.msg)
+        k_ntt_pass_r16<true><<<blocks, R16_BLOCK, 0, s>>>(a, 1ull << per_log);
+    else
+        k_ntt_pass_r16<false><<<blocks, R16_BLOCK, 0, s>>>(a, 1ull << per_log);
+}
+
 // The deepest layer of a pass has the fewest butterflies per twiddle, 2^c.
 template <bool INVERSE>
 static void launch_ntt_tab(const NttPassArgs &a, uint32_t n_tiles, const NttTuning &tu, hipStream_t s) {
@@ -491,7 +587,9 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
         a.e = es[p];
         a.tab_min = tu.tab_min;
         const uint64_t n_tiles = 1ull << (log_d - a.k - a.c - a.e);
-        if (tu.variant == 1) {
+        if (tu.variant == 2 && !inverse && a.k == 4 && a.e == 0 && log_d - a.layer0 - 4 >= R16_MIN_LB) {
+            launch_ntt_r16(a, s);
+        } else if (tu.variant >= 1) {
             if (inverse)
                 launch_ntt_tab<true>(a, (unsigned)n_tiles, tu, s);
             else

@@ -82,7 +82,7 @@ def test_ntt_forward_windows(b64, orc, log_d):
     (1, 1, 1, 1, 2, 0), (1, 5, 9, 64, 2, 0), (0, 2, 4, 1, 2, 0), (1, 4, 11, 32, 2, 0), (1, 7, 11, 1, 4, 0),
     (1, 6, 11, 16, 8, 0), (1, 5, 10, 8, 8, 0), (1, 3, 11, 1, 4, 0), (1, 5, 10, 16, 2, 4), (1, 4, 9, 1, 4, 4),
     (1, 7, 11, 1, 2, 4), (1, 3, 10, 1, 8, 4), (1, 4, 11, 64, 1, 0), (1, 7, 11, 1, 1, 0),
-    (1, 3, 6, 4, 1, 4), (1, 5, 10, 16, 1, 0)])
+    (1, 3, 6, 4, 1, 4), (1, 5, 10, 16, 1, 0), (2, 4, 11, 128, 2, 0), (2, 4, 6, 8, 2, 0)])
 def test_ntt_tuning_variants(b64, orc, variant, max_k, tile_log, tab_min, tab_group, waves):
     """Every multiply variant and pass/tile shape gives the same transform."""
     old = {k: b64.config_get(k) for k in ("ntt.variant", "ntt.max_k", "ntt.tile_log", "ntt.tab_min", "ntt.tab_group",
@@ -109,10 +109,12 @@ def test_ntt_tuning_variants(b64, orc, variant, max_k, tile_log, tab_min, tab_gr
 
 
 @pytest.mark.parametrize("variant,max_k,tile_log,tab_min,tab_group", [(1, 4, 6, 8, 2), (1, 7, 11, 128, 2),
-                                                                    (1, 4, 11, 64, 1), (1, 5, 10, 64, 2)])
+                                                                    (1, 4, 11, 64, 1), (1, 5, 10, 64, 2),
+                                                                    (2, 4, 11, 128, 2), (2, 4, 10, 64, 2)])
 def test_ntt_schedules_log_d20(b64, orc, variant, max_k, tile_log, tab_min, tab_group):
     """Schedules at a size where the last passes take extra index bits into
-    their tiles: forward, inverse and the message-gathering encode."""
+    their tiles, and where variant 2 runs the upper 4-layer passes in registers
+    (including the message-gathering one): forward, inverse and the encode."""
     log_d = 20
     cand = (variant, max_k, tile_log, tab_min, tab_group)
     dom, ref = b64.DomainContext.standard(log_d), orc.Domain(dim=log_d)
