@@ -26,13 +26,40 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-# Per launch of the forward-NTT pass kernel, from rocprofv3 --pmc runs of
+# Per launch of the forward-NTT pass kernels, from rocprofv3 --pmc runs of
 # `bench.py --counters` (profiles/ntt_pass_log_n24_counters.md): HBM bytes
 # (FETCH_SIZE + WRITE_SIZE, each scaled by the same counter on a 512 MiB device
-# copy) and VALU issue rate (SQ_INSTS_VALU per CU per clock).  Keyed by kernel,
-# log2 codeword length, layers per pass and tile size; the counters cannot be
-# read from inside the timed run, so other schedules report null.
-TRAFFIC = {("k_ntt_pass_tab", 25, 4, 11): {"bytes": 1073876727, "valu_issue_per_cu_clk": 0.64}}
+# copy) and VALU issue rate (SQ_INSTS_VALU per CU per clock), averaged over the
+# passes of one forward NTT.  Keyed by log2 codeword length and the schedule
+# (variant, layers per pass, tile size); the counters cannot be read from
+# inside the timed run, so other schedules report null.
+TRAFFIC = {(25, 2, 4, 11): {"bytes": 1134438430, "valu_issue_per_cu_clk": 0.68}}
+
+
+def traffic_entry(log_cw, tuning):
+    return TRAFFIC.get((log_cw, tuning["variant"], tuning["max_k"], tuning["tile_log"])) or {}
+
+
+def ntt_pass_kernels(log_d, skip_early, skip_late, tuning):
+    """Kernels of a forward NTT's passes, as the library plans them: the
+    layer range split evenly into passes of at most max_k layers; with
+    variant 2, 4-layer passes whose flipped bits start at index bit 10 or
+    above run in k_ntt_pass_r16, the others in k_ntt_pass_tab (k_ntt_pass for
+    variant 0)."""
+    first, end = skip_early, log_d - skip_late
+    n = end - first
+    max_k = tuning["max_k"]
+    n_passes = (n + max_k - 1) // max_k
+    names, at = [], first
+    for p in range(n_passes):
+        k = (end - at + (n_passes - p) - 1) // (n_passes - p)
+        lb = log_d - at - k
+        if tuning["variant"] == 2 and k == 4 and lb >= 10:
+            names.append("k_ntt_pass_r16")
+        else:
+            names.append("k_ntt_pass_tab" if tuning["variant"] >= 1 else "k_ntt_pass")
+        at += k
+    return " + ".join(f"{nm} x{names.count(nm)}" for nm in dict.fromkeys(names))
 
 
 def parse():
@@ -149,7 +176,7 @@ def main():
               for k in ("variant", "max_k", "tile_log", "tab_min", "tab_group", "waves")}
     max_k = tuning["max_k"]
     n_passes = (n_layers + max_k - 1) // max_k  # the library's balanced pass split
-    kernel = "k_ntt_pass_tab" if tuning["variant"] == 1 else "k_ntt_pass"
+    kernel = ntt_pass_kernels(log_cw, lir, lbs, tuning)
     stages = {
         "encode_ms": timed(lambda: b64.check(L.b64_rs_encode_batch(dom.ptr, log_dim, lir, msg.data_ptr(), lbs,
                                                                    cw.data_ptr(), sp))),
@@ -165,9 +192,8 @@ def main():
     muls = n_layers * (1 << (log_cw - 1))  # one multiply per butterfly
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": (TRAFFIC.get((kernel, log_cw, tuning["max_k"], tuning["tile_log"])) or {}).get("bytes"),
-                "valu_issue_per_cu_clk": (TRAFFIC.get((kernel, log_cw, tuning["max_k"], tuning["tile_log"]))
-                                          or {}).get("valu_issue_per_cu_clk"),
+                "traffic": traffic_entry(log_cw, tuning).get("bytes"),
+                "valu_issue_per_cu_clk": traffic_entry(log_cw, tuning).get("valu_issue_per_cu_clk"),
                 "kernel": kernel, "ntt_tuning": tuning, "launches_per_ntt": n_passes,
                 "avg_launch_ms": round(ntt_pass_ms, 4),
                 "algorithmic_bytes_per_launch": pass_bytes,
