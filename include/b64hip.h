@@ -89,7 +89,8 @@ int b64_stream_synchronize(void *stream);
 /* Kernel tuning knobs (no counterpart in the reference; the autotuner's
  * interface).  Keys: "ntt.variant" (0 = general multiply, 1 = per layer, LDS
  * window tables where a twiddle covers at least "ntt.tab_min" butterflies of a
- * tile), "ntt.max_k" (layers per NTT pass, 1..7), "ntt.tile_log" (log2 elements
+ * tile, 2 = as 1 with the forward 4-layer passes whose flipped bits start at
+ * index bit 10 or above run in registers), "ntt.max_k" (layers per NTT pass, 1..7), "ntt.tile_log" (log2 elements
  * per LDS tile), "ntt.tab_group" (window tables resident in LDS at once: 1, 2,
  * 4 or 8), "ntt.waves" (4 = tables-only passes use a kernel limited to 128 VGPRs,
  * 0 = default).  "ntt.<knob>:<log_d>" sets the knob for transforms of size
