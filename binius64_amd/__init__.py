@@ -29,6 +29,7 @@ __all__ = [
     "ProverTranscript", "FRIFoldProver", "gf_mul", "gf_inv", "gf_mul_device", "gf_mul_const_device",
     "pack_witness", "device_count", "config_set", "config_get", "LIB_PATH", "lib",
     "CommitJob", "BatchPlan", "Server", "fri_prove", "ring_switch_fold_1b_rows", "ring_switch_fold_elems",
+    "Proof", "STD_CHALLENGER_NAME",
 ]
 
 
@@ -407,6 +408,74 @@ class ProverTranscript:
         out = np.zeros(max(n, 1), dtype=np.uint8)
         lib().b64_transcript_tape(self.ptr, _p8(out), n)
         return out[:n].tobytes()
+
+
+# StdChallenger::NAME (verifier/verifier/src/config.rs:19-21): the name the
+# reference's prover CLI puts in a Proof.
+STD_CHALLENGER_NAME = "HasherChallenger<Sha256>"
+
+
+class Proof:
+    """Proof (verifier/core/src/constraint_system.rs:1014-1100): transcript bytes
+    and the name of the challenger that produced them, with the reference's
+    wire format (b64_proof_serialize / b64_proof_deserialize).  Serialisation
+    errors raise B64Error with the SerializationError counterpart's code."""
+
+    SERIALIZATION_VERSION = 1
+
+    def __init__(self, data: bytes, challenger_type: str):
+        self._data = bytes(data)
+        self._challenger = str(challenger_type)
+
+    @classmethod
+    def owned(cls, data: bytes, challenger_type: str) -> "Proof":
+        return cls(data, challenger_type)
+
+    borrowed = owned
+
+    def as_slice(self) -> bytes:
+        return self._data
+
+    def challenger_type(self) -> str:
+        return self._challenger
+
+    def len(self) -> int:
+        return len(self._data)
+
+    __len__ = len
+
+    def is_empty(self) -> bool:
+        return not self._data
+
+    def into_owned(self):
+        return self._data, self._challenger
+
+    def to_owned(self) -> "Proof":
+        return Proof(self._data, self._challenger)
+
+    def __eq__(self, other) -> bool:
+        return isinstance(other, Proof) and self.into_owned() == other.into_owned()
+
+    def serialize(self) -> bytes:
+        ct = self._challenger.encode("utf-8")
+        if b"\0" in ct:
+            raise ValueError("challenger type with a NUL byte cannot cross the C ABI")
+        data = np.frombuffer(self._data, dtype=np.uint8).copy() if self._data else np.zeros(1, dtype=np.uint8)
+        n = C.c_uint64()
+        lib().b64_proof_serialize(_p8(data), len(self._data), ct, None, 0, C.byref(n))
+        out = np.zeros(max(n.value, 1), dtype=np.uint8)
+        check(lib().b64_proof_serialize(_p8(data), len(self._data), ct, _p8(out), n.value, C.byref(n)),
+              "proof_serialize")
+        return out[:n.value].tobytes()
+
+    @classmethod
+    def deserialize(cls, buf: bytes) -> "Proof":
+        arr = np.frombuffer(bytes(buf), dtype=np.uint8).copy() if buf else np.zeros(1, dtype=np.uint8)
+        co, cl, do, dl = (C.c_uint64() for _ in range(4))
+        check(lib().b64_proof_deserialize(_p8(arr), len(buf), C.byref(co), C.byref(cl), C.byref(do), C.byref(dl)),
+              "proof_deserialize")
+        raw = arr.tobytes()
+        return cls(raw[do.value:do.value + dl.value], raw[co.value:co.value + cl.value].decode("utf-8"))
 
 
 class FRIFoldProver:

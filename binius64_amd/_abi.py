@@ -114,6 +114,8 @@ SIGNATURES = {
     "b64_transcript_sample": (None, [vp, u8p, C.c_size_t]),
     "b64_transcript_sample_bits": (C.c_uint32, [vp, C.c_uint32]),
     "b64_transcript_tape": (C.c_size_t, [vp, u8p, C.c_size_t]),
+    "b64_proof_serialize": (C.c_int, [u8p, C.c_uint64, C.c_char_p, u8p, C.c_uint64, u64p]),
+    "b64_proof_deserialize": (C.c_int, [u8p, C.c_uint64, u64p, u64p, u64p, u64p]),
     "b64_fri_prover_create": (C.c_int, [C.POINTER(FriParamsC), vp, vp, vp, vp, C.POINTER(vp)]),
     "b64_fri_prover_destroy": (None, [vp]),
     "b64_fri_prover_n_rounds": (C.c_uint32, [vp]),

@@ -106,6 +106,10 @@ const char *b64_status_string(int s) {
         case B64_ERR_NOT_COMMIT_ROUND: return "not a commit round";
         case B64_ERR_EXPECTED_FOLD: return "expected a call to fold";
         case B64_ERR_EXPECTED_EXECUTE: return "expected a call to execute";
+        case B64_ERR_NOT_ENOUGH_BYTES: return "not enough data in read buffer to deserialize";
+        case B64_ERR_INVALID_CONSTRUCTION: return "invalid construction of Proof::version";
+        case B64_ERR_FROM_UTF8: return "string is not valid UTF-8";
+        case B64_ERR_USIZE_TOO_LARGE: return "length too large to serialize (max is u32::MAX)";
         case B64_ERR_NO_DEVICE: return "no HIP device";
         case B64_ERR_HIP: return "HIP runtime error";
         case B64_ERR_OUT_OF_MEMORY: return "out of device memory";
@@ -669,6 +673,103 @@ uint32_t b64_transcript_sample_bits(b64_transcript *t, uint32_t bits) {
 size_t b64_transcript_tape(const b64_transcript *t, uint8_t *out, size_t cap) {
     if (out && cap >= t->tape.size()) memcpy(out, t->tape.data(), t->tape.size());
     return t->tape.size();
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Proof wire format (verifier/core/src/constraint_system.rs:1014-1100): the
+// transcript bytes wrapped with a version and the challenger's name, using the
+// SerializeBytes rules of verifier/utils/src/serialization.rs (u32 LE; usize
+// as u32; str and Vec<u8> as a length then the bytes).
+// ===========================================================================
+namespace {
+
+void put_u32(uint8_t *p, uint32_t v) {
+    for (int i = 0; i < 4; i++) p[i] = (uint8_t)(v >> (8 * i));
+}
+uint32_t get_u32(const uint8_t *p) {
+    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+}
+
+// Well-formed UTF-8 as String::from_utf8 accepts it: no overlong forms, no
+// surrogates, nothing above U+10FFFF.
+bool valid_utf8(const uint8_t *s, uint64_t n) {
+    uint64_t i = 0;
+    while (i < n) {
+        const uint8_t c = s[i];
+        if (c < 0x80) {
+            i++;
+            continue;
+        }
+        int len;
+        uint8_t lo = 0x80, hi = 0xBF;  // range of the second byte
+        if (c >= 0xC2 && c <= 0xDF) {
+            len = 2;
+        } else if (c >= 0xE0 && c <= 0xEF) {
+            len = 3;
+            if (c == 0xE0) lo = 0xA0;
+            if (c == 0xED) hi = 0x9F;
+        } else if (c >= 0xF0 && c <= 0xF4) {
+            len = 4;
+            if (c == 0xF0) lo = 0x90;
+            if (c == 0xF4) hi = 0x8F;
+        } else {
+            return false;
+        }
+        if (n - i < (uint64_t)len) return false;
+        if (s[i + 1] < lo || s[i + 1] > hi) return false;
+        for (int k = 2; k < len; k++)
+            if ((s[i + k] & 0xC0) != 0x80) return false;
+        i += (uint64_t)len;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int b64_proof_serialize(const uint8_t *data, uint64_t data_len, const char *challenger_type, uint8_t *out,
+                        uint64_t out_cap, uint64_t *out_len) {
+    if (!challenger_type || (!data && data_len) || !out_len) return B64_ERR_INVALID_ARGS;
+    const uint64_t ct_len = strlen(challenger_type);
+    const uint64_t need = 4 + 4 + ct_len + 4 + data_len;
+    *out_len = need;
+    if (ct_len > 0xFFFFFFFFull || data_len > 0xFFFFFFFFull) return B64_ERR_USIZE_TOO_LARGE;
+    if (!out || out_cap < need) return B64_ERR_BUFFER_TOO_SMALL;
+    uint8_t *p = out;
+    put_u32(p, B64_PROOF_SERIALIZATION_VERSION);
+    put_u32(p + 4, (uint32_t)ct_len);
+    if (ct_len) memcpy(p + 8, challenger_type, ct_len);
+    put_u32(p + 8 + ct_len, (uint32_t)data_len);
+    if (data_len) memcpy(p + 12 + ct_len, data, data_len);
+    return B64_OK;
+}
+
+int b64_proof_deserialize(const uint8_t *buf, uint64_t buf_len, uint64_t *challenger_off, uint64_t *challenger_len,
+                          uint64_t *data_off, uint64_t *data_len) {
+    if ((!buf && buf_len) || !challenger_off || !challenger_len || !data_off || !data_len)
+        return B64_ERR_INVALID_ARGS;
+    uint64_t at = 0;
+    if (buf_len - at < 4) return B64_ERR_NOT_ENOUGH_BYTES;
+    if (get_u32(buf) != B64_PROOF_SERIALIZATION_VERSION) return B64_ERR_INVALID_CONSTRUCTION;
+    at += 4;
+    if (buf_len - at < 4) return B64_ERR_NOT_ENOUGH_BYTES;
+    const uint64_t ct = get_u32(buf + at);
+    at += 4;
+    if (buf_len - at < ct) return B64_ERR_NOT_ENOUGH_BYTES;
+    if (!valid_utf8(buf + at, ct)) return B64_ERR_FROM_UTF8;
+    *challenger_off = at;
+    *challenger_len = ct;
+    at += ct;
+    if (buf_len - at < 4) return B64_ERR_NOT_ENOUGH_BYTES;
+    const uint64_t dl = get_u32(buf + at);
+    at += 4;
+    if (buf_len - at < dl) return B64_ERR_NOT_ENOUGH_BYTES;
+    *data_off = at;
+    *data_len = dl;
+    return B64_OK;
 }
 
 }  // extern "C"

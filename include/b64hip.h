@@ -43,6 +43,10 @@ extern "C" {
 #define B64_ERR_NOT_COMMIT_ROUND 11
 #define B64_ERR_EXPECTED_FOLD 12              /* sumcheck::Error::ExpectedFold */
 #define B64_ERR_EXPECTED_EXECUTE 13           /* sumcheck::Error::ExpectedExecute */
+#define B64_ERR_NOT_ENOUGH_BYTES 14           /* SerializationError::NotEnoughBytes */
+#define B64_ERR_INVALID_CONSTRUCTION 15       /* SerializationError::InvalidConstruction ("Proof::version") */
+#define B64_ERR_FROM_UTF8 16                  /* SerializationError::FromUtf8Error */
+#define B64_ERR_USIZE_TOO_LARGE 17            /* SerializationError::UsizeTooLarge */
 #define B64_ERR_NO_DEVICE 20
 #define B64_ERR_HIP 21
 #define B64_ERR_OUT_OF_MEMORY 22
@@ -211,6 +215,27 @@ void b64_transcript_decommit(b64_transcript *t, const uint8_t *bytes, size_t n);
 void b64_transcript_sample(b64_transcript *t, uint8_t *out, size_t n);            /* CanSample, :403 */
 uint32_t b64_transcript_sample_bits(b64_transcript *t, uint32_t bits);            /* :413 */
 size_t b64_transcript_tape(const b64_transcript *t, uint8_t *out, size_t cap);    /* finalize, :245 */
+
+/* ---- Proof wire format (verifier/core/src/constraint_system.rs:1014-1100) ------ */
+/* Proof::SERIALIZATION_VERSION and the challenger name the reference's prover
+ * CLI writes (StdChallenger::NAME, verifier/verifier/src/config.rs:19-21). */
+#define B64_PROOF_SERIALIZATION_VERSION 1u
+#define B64_STD_CHALLENGER_NAME "HasherChallenger<Sha256>"
+/* Proof::owned(data, challenger_type).serialize: u32 LE version, then the
+ * challenger type and the transcript bytes, each as a u32 LE length and the
+ * bytes (usize serialises as u32, utils/serialization.rs:47-54).  *out_len
+ * always receives the encoded size; out may be NULL to ask for it.
+ * USIZE_TOO_LARGE if a length exceeds u32, BUFFER_TOO_SMALL if out_cap is less
+ * than the encoded size. */
+int b64_proof_serialize(const uint8_t *data, uint64_t data_len, const char *challenger_type, uint8_t *out,
+                        uint64_t out_cap, uint64_t *out_len);
+/* Proof::deserialize, zero-copy: the challenger type and the transcript data
+ * are returned as (offset, length) ranges of buf.  INVALID_CONSTRUCTION for a
+ * version other than 1, NOT_ENOUGH_BYTES for a truncated buffer, FROM_UTF8 if
+ * the challenger type is not UTF-8.  Bytes after the proof are ignored, as a
+ * read buffer's remainder is. */
+int b64_proof_deserialize(const uint8_t *buf, uint64_t buf_len, uint64_t *challenger_off, uint64_t *challenger_len,
+                          uint64_t *data_off, uint64_t *data_len);
 
 /* ---- FRIFoldProver / FRIQueryProver (prover/prover/src/fri/fold.rs, query.rs) - */
 /* FRIFoldProver::new (fold.rs:57-82).  The prover keeps pointers to the committed

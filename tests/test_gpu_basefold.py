@@ -148,4 +148,9 @@ def test_pcs_config2_setup_chain(b64, orc):
     ot.message(co.commitment)
     orc.pcs_prove(ref, packed, point, cw, tree, params.log_dim, 1, params.log_batch_size, params.fold_arities, 232,
                   ot, threads=8)
-    assert t.finalize() == ot.tape()
+    tape = t.finalize()
+    assert tape == ot.tape()
+    # the proof as the reference's prover CLI writes it (Proof::owned(transcript, StdChallenger::NAME))
+    wire = b64.Proof.owned(tape, b64.STD_CHALLENGER_NAME).serialize()
+    assert wire[:4] == (1).to_bytes(4, "little") and len(wire) == 12 + 24 + len(tape)
+    assert b64.Proof.deserialize(wire).as_slice() == ot.tape()
