@@ -794,12 +794,14 @@ struct b64_fri_prover {
     bool has_next = true;
     uint32_t next_commit = 0;
     std::vector<u128> unprocessed;
+    uint8_t *h_root = nullptr;  // pinned: the last committed root, copied by the fold round
 
     ~b64_fri_prover() {
         for (auto &o : committed) {
             if (o.cw) (void)hipFreeAsync(o.cw, s);
             if (o.tree) (void)hipFreeAsync(o.tree, s);
         }
+        if (h_root) (void)hipHostFree(h_root);
     }
     uint32_t n_rounds() const { return p.log_dim + p.log_batch_size; }
     uint32_t rs_log_len() const { return p.log_dim + p.log_inv_rate; }
@@ -838,6 +840,11 @@ int b64_fri_prover_create_ex(const b64_fri_params *p, const b64_domain *d, uint3
     fp->s = as_stream(stream);
     fp->next_commit = p->log_batch_size;  // FRIFoldProver::new (fold.rs:70)
     fp->merkle_hash = merkle_hash;
+    if (hipHostMalloc((void **)&fp->h_root, 32, hipHostMallocDefault) != hipSuccess) {
+        fp->h_root = nullptr;
+        delete fp;
+        return B64_ERR_OUT_OF_MEMORY;
+    }
     b64::abi::keep_pool_memory();
     *out = fp;
     return B64_OK;
@@ -854,7 +861,12 @@ int b64_fri_prover_receive_challenge(b64_fri_prover *fp, const uint64_t ch[2]) {
     return B64_OK;
 }
 
-int b64_fri_prover_execute_fold_round(b64_fri_prover *fp, uint8_t root_out[32], int *committed) {
+}  // extern "C"
+
+// The fold round's device work, enqueued on the prover's stream; a committed
+// round also enqueues the copy of its root into fp->h_root.  The caller
+// synchronises the stream before reading it.
+int b64::fri_fold_round_enqueue(b64_fri_prover *fp, int *committed, const uint8_t **root_host) {
     if (!fp || !committed) return B64_ERR_INVALID_ARGS;
     *committed = 0;
     if (!(fp->has_next && fp->next_commit == fp->curr_round)) return B64_OK;
@@ -909,11 +921,21 @@ int b64_fri_prover_execute_fold_round(b64_fri_prover *fp, uint8_t root_out[32], 
         fp->next_commit += fp->p.fold_arities[k];
     else
         fp->has_next = false;
-    if (root_out) {
-        B64_TRY(fetch(root_out, (const uint8_t *)o.tree + 32 * ((2ull << o.tree_log_len) - 2), 32, fp->s));
-        HIP_TRY(hipStreamSynchronize(fp->s));
-    }
+    B64_TRY(fetch(fp->h_root, (const uint8_t *)o.tree + 32 * ((2ull << o.tree_log_len) - 2), 32, fp->s));
+    if (root_host) *root_host = fp->h_root;
     *committed = 1;
+    return B64_OK;
+}
+
+extern "C" {
+
+int b64_fri_prover_execute_fold_round(b64_fri_prover *fp, uint8_t root_out[32], int *committed) {
+    const uint8_t *root = nullptr;
+    B64_TRY(b64::fri_fold_round_enqueue(fp, committed, &root));
+    if (*committed && root_out) {
+        HIP_TRY(hipStreamSynchronize(fp->s));
+        memcpy(root_out, root, 32);
+    }
     return B64_OK;
 }
 

@@ -115,6 +115,10 @@ hipError_t launch_fold_codeword(const DeviceDomain &dd, const uint4 *cw, uint4 *
 // Subtree-reduction schedule of a tree with n_leaves leaves: per launch the
 // input level offset (in digests), its node count and the levels reduced.
 int merkle_reduce_steps(uint64_t n_leaves, uint64_t *offs, uint64_t *ns, int *levels);
+// FRIFoldProver::execute_fold_round without the final wait: the fold, the
+// Merkle commit and, on a committed round, the copy of the root into pinned
+// host memory (*root_host, valid once the stream is synchronised).
+int fri_fold_round_enqueue(b64_fri_prover *fp, int *committed, const uint8_t **root_host);
 // Pass schedule of a forward NTT (layer0, k, c, e per pass); returns the count.
 // A pass that gathers from the message (the first forward pass of an encode,
 // msg_first) keeps e = 0.

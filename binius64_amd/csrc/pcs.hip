@@ -383,13 +383,16 @@ int b64_basefold_prove_ex(const b64_fri_params *p, const b64_domain *d, uint32_t
     }
     if (b64_fri_prover_n_rounds(fp) != n_vars) rc = B64_ERR_INVALID_ARGS;
     for (uint32_t i = 0; rc == B64_OK && i < n_vars; i++) {
-        // execute (:99-113): the sumcheck round, then the FRI fold round
+        // execute (:99-113): the sumcheck round and the FRI fold round.  The two
+        // are independent device work on one stream: the fold round is
+        // enqueued first and the sumcheck's wait for its coefficients also
+        // covers the fold round's root.
+        int committed = 0;
+        const uint8_t *root = nullptr;
+        rc = b64::fri_fold_round_enqueue(fp, &committed, &root);
+        if (rc != B64_OK) break;
         uint64_t co[6];
         rc = b64_sumcheck_execute(sc, co);
-        if (rc != B64_OK) break;
-        uint8_t root[32];
-        int committed = 0;
-        rc = b64_fri_prover_execute_fold_round(fp, root, &committed);
         if (rc != B64_OK) break;
         b64_transcript_message(t, (const uint8_t *)co, 32);  // RoundCoeffs::truncate: c0, c1
         if (committed) b64_transcript_message(t, root, 32);
