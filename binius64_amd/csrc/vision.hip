@@ -27,7 +27,6 @@
 
 #include <mutex>
 #include <new>
-#include <type_traits>
 
 #include "abi_util.hpp"
 #include "gf128_dev.hpp"
@@ -301,6 +300,7 @@ struct LaneOps {
         return win_mul_perm(tabs + (uint32_t)(t * WIN_ENTRIES * 16), v);
     }
     __device__ __forceinline__ bool writer() const { return true; }
+    __device__ __forceinline__ void set_lane(uint32_t) {}
 };
 
 // Lane K of the quad, to every lane of the quad.
@@ -343,7 +343,71 @@ struct QuadOps {
         return make_uint4(quad_xor(acc.x), quad_xor(acc.y), quad_xor(acc.z), quad_xor(acc.w));
     }
     __device__ __forceinline__ bool writer() const { return q == 0; }
+    __device__ __forceinline__ void set_lane(uint32_t x) { q = x & 3; }
 };
+
+// XOR of the 16 lanes of a DPP row, in every lane of the row (quad_perm
+// [1,0,3,2], quad_perm [2,3,0,1], row_ror:4, row_ror:8).
+__device__ __forceinline__ uint32_t row_xor(uint32_t v) {
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true);
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, true);
+    v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, true);
+    return v ^ (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, true);
+}
+
+__device__ __forceinline__ uint32_t word_of(uint4 v, uint32_t i) {
+    return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+
+// Sixteen lanes (a DPP row) per state: lane l computes the 32x32 carry-less
+// product of word l>>2 of a and word l&3 of b and places it in a 256-bit frame
+// at word (l>>2)+(l&3); the row XOR of the frames is the full product.  A map
+// takes one byte of v per lane (two table rows).  The DPP reduction chains
+// (four dependent steps per word) eat most of the gain: a narrow tree level
+// takes 0.146 ms against the quad form's 0.172 ms.
+struct RowOps {
+    uint32_t l;  // lane within the row
+    __device__ __forceinline__ uint4 mul(uint4 a, uint4 b) const {
+        const uint32_t i = l >> 2, j = l & 3, at = i + j;
+        const uint64_t p = clmul32(word_of(a, i), word_of(b, j));
+        const uint32_t lo = (uint32_t)p, hi = (uint32_t)(p >> 32);
+        uint32_t f[8];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; k++) f[k] = row_xor(k == at ? lo : (k == at + 1 ? hi : 0u));
+        return gf_reduce(((uint64_t)f[1] << 32) | f[0], ((uint64_t)f[3] << 32) | f[2], ((uint64_t)f[5] << 32) | f[4],
+                         ((uint64_t)f[7] << 32) | f[6]);
+    }
+    // rows 2l and 2l+1 of the table: nibbles 2l and 2l+1 of v; entry n of
+    // row r sits at slot n ^ (r & 15)
+    __device__ __forceinline__ uint4 map(uint32_t tabs, int t, uint4 v) const {
+        const uint32_t byte = (word_of(v, l >> 2) >> (8 * (l & 3))) & 0xFFu;
+        const uint32_t r0 = 2 * l, r1 = r0 + 1;
+        const uint32_t base = tabs + (uint32_t)(t * WIN_ENTRIES * 16);
+        const uint32_t a0 = base + r0 * 256 + (((byte & 15u) ^ (r0 & 15u)) << 4);
+        const uint32_t a1 = base + r1 * 256 + (((byte >> 4) ^ (r1 & 15u)) << 4);
+        const u32x4 e0 = *(const lds_cu32x4 *)(uintptr_t)a0;
+        const u32x4 e1 = *(const lds_cu32x4 *)(uintptr_t)a1;
+        return make_uint4(row_xor(e0.x ^ e1.x), row_xor(e0.y ^ e1.y), row_xor(e0.z ^ e1.z), row_xor(e0.w ^ e1.w));
+    }
+    __device__ __forceinline__ bool writer() const { return l == 0; }
+    __device__ __forceinline__ void set_lane(uint32_t x) { l = x & 15; }
+};
+
+template <int LANES>
+struct OpsFor;
+template <>
+struct OpsFor<1> {
+    using type = LaneOps;
+};
+template <>
+struct OpsFor<4> {
+    using type = QuadOps;
+};
+template <>
+struct OpsFor<16> {
+    using type = RowOps;
+};
+constexpr int lanes_log(int lanes) { return lanes == 16 ? 4 : (lanes == 4 ? 2 : 0); }
 
 // v * x^-1 (ghash.rs mul_inv_x)
 __device__ __forceinline__ uint4 gf_mul_inv_x(uint4 v) {
@@ -472,16 +536,17 @@ VIS_KERNEL void k_vision_permute(const uint4 *__restrict__ dev, uint4 *__restric
 // state[0 .. R) and permutes; the final block holds the remaining elements, a
 // 0x80 byte after them and 0x01 in its last byte (a whole padding block when
 // nothing remains); the digest is state[0] || state[1].
-template <int M, bool QUAD>
-VIS_KERNEL void k_vision_digest(const uint4 *__restrict__ dev,
-                                                        const uint4 *__restrict__ msgs, uint64_t n, uint32_t len,
-                                                        uint4 *__restrict__ out) {
+// LANES lanes per state (1, 4 or 16: LaneOps, QuadOps, RowOps); the loop bound
+// is uniform over a state's lanes.
+template <int M, int LANES>
+VIS_KERNEL void k_vision_digest(const uint4 *__restrict__ dev, const uint4 *__restrict__ msgs, uint64_t n,
+                                uint32_t len, uint4 *__restrict__ out) {
     constexpr int R = M - 2;
-    constexpr int SHIFT = QUAD ? 2 : 0;
+    constexpr int SHIFT = lanes_log(LANES);
     __shared__ __attribute__((aligned(256))) uint4 vis_lds[DEV_ELEMS];
     const VisionLds v = vis_stage(dev, vis_lds);
-    typename std::conditional<QUAD, QuadOps, LaneOps>::type ops;
-    if constexpr (QUAD) ops.q = threadIdx.x & 3;
+    typename OpsFor<LANES>::type ops;
+    ops.set_lane(threadIdx.x);
     for (uint64_t t = (uint64_t)blockIdx.x * VBLOCK + threadIdx.x; (t >> SHIFT) < n;
          t += (uint64_t)gridDim.x * VBLOCK) {
         const uint64_t k = t >> SHIFT;
@@ -512,15 +577,15 @@ VIS_KERNEL void k_vision_digest(const uint4 *__restrict__ dev,
 
 // VisionCompression of one tree level: node i of the level above is
 // compress(in[2i], in[2i+1]) = first half of permute(l0, l1, r0, r1) + (l0, l1).
-// QUAD: four lanes per node (QuadOps); the loop bound is uniform over a quad.
-template <bool QUAD>
-VIS_KERNEL void k_vision_compress(const uint4 *__restrict__ dev, uint4 *__restrict__ tree,
-                                                          uint64_t in_off, uint64_t n_out) {
-    constexpr int SHIFT = QUAD ? 2 : 0;
+// LANES lanes per node, as in k_vision_digest.
+template <int LANES>
+VIS_KERNEL void k_vision_compress(const uint4 *__restrict__ dev, uint4 *__restrict__ tree, uint64_t in_off,
+                                  uint64_t n_out) {
+    constexpr int SHIFT = lanes_log(LANES);
     __shared__ __attribute__((aligned(256))) uint4 vis_lds[DEV_ELEMS];
     const VisionLds v = vis_stage(dev, vis_lds);
-    typename std::conditional<QUAD, QuadOps, LaneOps>::type ops;
-    if constexpr (QUAD) ops.q = threadIdx.x & 3;
+    typename OpsFor<LANES>::type ops;
+    ops.set_lane(threadIdx.x);
     for (uint64_t t = (uint64_t)blockIdx.x * VBLOCK + threadIdx.x; (t >> SHIFT) < n_out;
          t += (uint64_t)gridDim.x * VBLOCK) {
         const uint64_t i = t >> SHIFT;
@@ -558,33 +623,45 @@ hipError_t launch_permute(int m, const uint4 *dev, uint4 *states, uint64_t n, hi
 // compressions per launch 0.60 ms quad vs 0.52 ms lane, 2^15: 0.31 vs 0.49 ms;
 // Vision-6 digests of 2^17 16-element messages 6.3 ms quad vs 5.2 ms lane).
 constexpr uint64_t QUAD_MAX_STATES = 1ull << 15;
+// ... and with at most this many, sixteen lanes per state (RowOps); measured
+// best of 2^10, 2^12 and 2^14 for a 2^17-leaf commit.
+constexpr uint64_t ROW_MAX_STATES = 1ull << 12;
+
+int lanes_for(uint64_t n) { return n <= ROW_MAX_STATES ? 16 : (n <= QUAD_MAX_STATES ? 4 : 1); }
+
+template <int M, int LANES>
+void launch_digest_l(const uint4 *dev, const uint4 *msgs, uint64_t n, uint32_t len, uint4 *out, hipStream_t s) {
+    k_vision_digest<M, LANES><<<vis_grid(LANES * n), VBLOCK, 0, s>>>(dev, msgs, n, len, out);
+}
+
+template <int M>
+void launch_digest_m(const uint4 *dev, const uint4 *msgs, uint64_t n, uint32_t len, uint4 *out, hipStream_t s) {
+    switch (lanes_for(n)) {
+        case 16: launch_digest_l<M, 16>(dev, msgs, n, len, out, s); break;
+        case 4: launch_digest_l<M, 4>(dev, msgs, n, len, out, s); break;
+        default: launch_digest_l<M, 1>(dev, msgs, n, len, out, s); break;
+    }
+}
 
 hipError_t launch_digest(int m, const uint4 *dev, const uint4 *msgs, uint64_t n, uint32_t len, uint4 *out,
                          hipStream_t s) {
     if (n == 0) return hipSuccess;
-    const bool quad = n <= QUAD_MAX_STATES;
-    const unsigned grid = vis_grid(quad ? 4 * n : n);
-    if (m == 4) {
-        if (quad)
-            k_vision_digest<4, true><<<grid, VBLOCK, 0, s>>>(dev, msgs, n, len, out);
-        else
-            k_vision_digest<4, false><<<grid, VBLOCK, 0, s>>>(dev, msgs, n, len, out);
-    } else {
-        if (quad)
-            k_vision_digest<6, true><<<grid, VBLOCK, 0, s>>>(dev, msgs, n, len, out);
-        else
-            k_vision_digest<6, false><<<grid, VBLOCK, 0, s>>>(dev, msgs, n, len, out);
-    }
+    if (m == 4)
+        launch_digest_m<4>(dev, msgs, n, len, out, s);
+    else
+        launch_digest_m<6>(dev, msgs, n, len, out, s);
     return hipGetLastError();
 }
 
 hipError_t launch_tree_levels(const uint4 *dev4, uint4 *tree, uint64_t n_leaves, hipStream_t s) {
     uint64_t in_off = 0;
     for (uint64_t n = n_leaves; n > 1; n /= 2) {
-        if (n / 2 <= QUAD_MAX_STATES)
-            k_vision_compress<true><<<vis_grid(2 * n), VBLOCK, 0, s>>>(dev4, tree, in_off, n / 2);
-        else
-            k_vision_compress<false><<<vis_grid(n / 2), VBLOCK, 0, s>>>(dev4, tree, in_off, n / 2);
+        const uint64_t nodes = n / 2;
+        switch (lanes_for(nodes)) {
+            case 16: k_vision_compress<16><<<vis_grid(16 * nodes), VBLOCK, 0, s>>>(dev4, tree, in_off, nodes); break;
+            case 4: k_vision_compress<4><<<vis_grid(4 * nodes), VBLOCK, 0, s>>>(dev4, tree, in_off, nodes); break;
+            default: k_vision_compress<1><<<vis_grid(nodes), VBLOCK, 0, s>>>(dev4, tree, in_off, nodes); break;
+        }
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         in_off += n;
