@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <mutex>
+#include <vector>
 
 #include "../../include/b64hip.h"
 #include "host_util.hpp"
@@ -56,6 +57,38 @@ inline void keep_pool_memory() {
         (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
     }
     done[dev] = true;
+}
+
+// Small pinned host blocks (PINNED_SMALL bytes) for per-prover results that
+// kernels or copies write while the host is not waiting: a process-wide free
+// list, so a prover created per proof does not pay a pinned allocation.
+constexpr size_t PINNED_SMALL = 256;
+struct PinnedSmallPool {
+    std::mutex mu;
+    std::vector<void *> free_blocks;
+};
+inline PinnedSmallPool &pinned_small_pool() {
+    static PinnedSmallPool *pool = new PinnedSmallPool();  // lives until the process ends
+    return *pool;
+}
+inline void *pinned_small_get() {
+    PinnedSmallPool &p = pinned_small_pool();
+    {
+        std::lock_guard<std::mutex> lock(p.mu);
+        if (!p.free_blocks.empty()) {
+            void *b = p.free_blocks.back();
+            p.free_blocks.pop_back();
+            return b;
+        }
+    }
+    void *b = nullptr;
+    return hipHostMalloc(&b, PINNED_SMALL, hipHostMallocDefault) == hipSuccess ? b : nullptr;
+}
+inline void pinned_small_put(void *b) {
+    if (!b) return;
+    PinnedSmallPool &p = pinned_small_pool();
+    std::lock_guard<std::mutex> lock(p.mu);
+    p.free_blocks.push_back(b);
 }
 
 }  // namespace abi

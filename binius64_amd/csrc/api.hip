@@ -801,7 +801,10 @@ struct b64_fri_prover {
             if (o.cw) (void)hipFreeAsync(o.cw, s);
             if (o.tree) (void)hipFreeAsync(o.tree, s);
         }
-        if (h_root) (void)hipHostFree(h_root);
+        if (h_root) {
+            (void)hipStreamSynchronize(s);  // a root copy may still be in flight
+            b64::abi::pinned_small_put(h_root);
+        }
     }
     uint32_t n_rounds() const { return p.log_dim + p.log_batch_size; }
     uint32_t rs_log_len() const { return p.log_dim + p.log_inv_rate; }
@@ -840,8 +843,8 @@ int b64_fri_prover_create_ex(const b64_fri_params *p, const b64_domain *d, uint3
     fp->s = as_stream(stream);
     fp->next_commit = p->log_batch_size;  // FRIFoldProver::new (fold.rs:70)
     fp->merkle_hash = merkle_hash;
-    if (hipHostMalloc((void **)&fp->h_root, 32, hipHostMallocDefault) != hipSuccess) {
-        fp->h_root = nullptr;
+    fp->h_root = (uint8_t *)b64::abi::pinned_small_get();
+    if (!fp->h_root) {
         delete fp;
         return B64_ERR_OUT_OF_MEMORY;
     }

@@ -1,5 +1,7 @@
 // gfx950 kernels for the B128 encode / commit / fold path and their launchers.
 #include <hip/hip_runtime.h>
+
+#include <atomic>
 #include <stdint.h>
 
 #include "gf128_dev.hpp"
@@ -488,16 +490,16 @@ __global__ __launch_bounds__(R16_BLOCK) void k_ntt_pass_r16(NttPassArgs a, uint6
 }
 
 static uint32_t device_cus() {
-    static int cached[64] = {};
+    static std::atomic<int> cached[64] = {};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-    if (cached[dev] == 0) {
-        int cus = 0;
+    int cus = cached[dev].load(std::memory_order_relaxed);
+    if (cus == 0) {
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
             cus = 1;
-        cached[dev] = cus;
+        cached[dev].store(cus, std::memory_order_relaxed);
     }
-    return (uint32_t)cached[dev];
+    return (uint32_t)cus;
 }
 
 // One block per CU (the tables take 120 KiB); positions per block a power of

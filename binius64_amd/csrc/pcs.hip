@@ -259,7 +259,10 @@ struct b64_sumcheck {
 
     ~b64_sumcheck() {
         if (d_partial) (void)hipFreeAsync(d_partial, s);
-        if (h_res) (void)hipHostFree(h_res);
+        if (h_res) {
+            (void)hipStreamSynchronize(s);  // the kernels write h_res
+            b64::abi::pinned_small_put(h_res);
+        }
     }
 };
 
@@ -279,7 +282,7 @@ int b64_sumcheck_create(void *d_a, void *d_b, uint32_t n_vars, const uint64_t su
     sc->s = as_stream(stream);
     b64::abi::keep_pool_memory();
     if (hipMallocAsync((void **)&sc->d_partial, 2 * b64::SC_MAX_BLOCKS * sizeof(uint4), sc->s) != hipSuccess ||
-        hipHostMalloc((void **)&sc->h_res, 2 * sizeof(uint4), hipHostMallocDefault) != hipSuccess ||
+        !(sc->h_res = (uint4 *)b64::abi::pinned_small_get()) ||
         hipHostGetDevicePointer((void **)&sc->h_res_dev, sc->h_res, 0) != hipSuccess) {
         delete sc;
         return B64_ERR_OUT_OF_MEMORY;
