@@ -301,9 +301,11 @@ int b64_sumcheck_execute(b64_sumcheck *sc, uint64_t coeffs_out[6]) {
     if (sc->n_vars == 0) return B64_ERR_INVALID_ARGS;  // assert!(n_vars_remaining > 0)
     const uint64_t half = 1ull << (sc->n_vars - 1);
     const int blocks = b64::sc_blocks(half);
+    // a one-block round writes its totals straight to the pinned result
     b64::k_sc_round<<<blocks, b64::SC_BLOCK, 0, sc->s>>>(sc->a, sc->b, half, sc->pending ? 1 : 0,
-                                                          to_u4(sc->r_pending), sc->d_partial);
-    b64::k_sc_reduce<<<1, b64::SC_BLOCK, 0, sc->s>>>(sc->d_partial, blocks, sc->h_res_dev);
+                                                          to_u4(sc->r_pending),
+                                                          blocks == 1 ? sc->h_res_dev : sc->d_partial);
+    if (blocks > 1) b64::k_sc_reduce<<<1, b64::SC_BLOCK, 0, sc->s>>>(sc->d_partial, blocks, sc->h_res_dev);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(sc->s));
     sc->pending = false;
