@@ -134,6 +134,31 @@ def test_ntt_schedules_log_d20(b64, orc, variant, max_k, tile_log, tab_min, tab_
         _apply_tuning(b64.lib())
 
 
+@pytest.mark.parametrize("log_dim,lir,lbs", [(16, 2, 0), (15, 1, 2), (14, 3, 1)])
+def test_encode_register_passes(b64, orc, log_dim, lir, lbs):
+    """Variant 2 on encodes whose message-gathering pass runs in registers, with
+    rates 1/2..1/8 (the replicas are the hi groups of that pass) and batches
+    of 1, 2 and 4 polynomials; a nonstandard basis for the first case."""
+    log_d = log_dim + lir + lbs
+    if log_dim == 16:
+        basis = [(0x9E3779B97F4A7C15 * (i + 1)) ^ (i << 70) for i in range(log_dim + lir)]
+        dom, ref = b64.DomainContext(orc.to_elems(basis)), orc.Domain(orc.to_elems(basis))
+    else:
+        dom, ref = b64.DomainContext.standard(log_dim + lir), orc.Domain(dim=log_dim + lir)
+    msg = orc.random_elems(4242 + log_d, 1 << (log_dim + lbs))
+    try:
+        for key, v in zip(("ntt.variant", "ntt.max_k", "ntt.tile_log", "ntt.tab_min", "ntt.tab_group"),
+                          (2, 4, 11, 64, 2)):
+            b64.config_set(f"{key}:{log_d}", v)
+        say("encode-r16", log_dim, lir, lbs)
+        out = b64.ReedSolomonCode(dom, log_dim, lir).encode_batch(b64.DeviceBuffer.from_host(msg), lbs)
+        assert np.array_equal(out.to_host(1 << log_d), orc.encode_batch(ref, msg, log_dim, lir, lbs, threads=8))
+    finally:
+        from binius64_amd._abi import _apply_tuning
+        b64.config_set("ntt.clear_sizes", 0)
+        _apply_tuning(b64.lib())
+
+
 def test_per_size_tuning_override(b64, orc):
     """A per-size schedule applies to its transform size only, and any schedule
     gives the oracle's transform."""
