@@ -282,6 +282,10 @@ struct b64_batch_plan {
     hipGraphExec_t exec = nullptr;
     bool use_graph = false;
     bool owns_buffers = true;
+    // A plan of one job without a graph runs the job's single-job launchers
+    // (job_enqueue): with nothing to batch, the per-size schedule is the
+    // faster one (a served job at low load is a batch of one).
+    bool single = false;
 
     ~b64_batch_plan() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -339,10 +343,14 @@ int b64_job_oracle(const b64_job_desc *job, uint32_t k, const void **d_codeword,
     return B64_OK;
 }
 
-// Plain path: the job's stages as single-job launches on `stream`, then its
-// roots read back.
-int b64_job_run(const b64_job_desc *job, void *stream) {
-    if (!job || !job->d_msg || !job->d_workspace || !job->challenges || !job->roots_out) return B64_ERR_INVALID_ARGS;
+}  // extern "C"
+
+// Plain path: the job's stages as single-job launches on `s` (the per-size
+// NTT schedule, register-resident passes included), then the copies of its
+// roots into roots_dst (host memory; pinned for the copies to stay
+// asynchronous).  Nothing waits.
+static int job_enqueue(const b64_job_desc *job, hipStream_t s, uint8_t *roots_dst) {
+    if (!job || !job->d_msg || !job->d_workspace || !job->challenges || !roots_dst) return B64_ERR_INVALID_ARGS;
     SizeInfo *si = nullptr;
     B64_TRY(size_info(job->log_msg_len, &si));
     B64_TRY(size_device(si));
@@ -351,7 +359,6 @@ int b64_job_run(const b64_job_desc *job, void *stream) {
     std::vector<Stage> stages;
     std::vector<const uint4 *> roots;
     B64_TRY(job_stages(*job, L, stages, roots));
-    hipStream_t s = as_stream(stream);
     const b64_fri_params &p = si->p;
     uint8_t *ws = (uint8_t *)job->d_workspace;
     // the same computation through the single-job launchers
@@ -378,7 +385,16 @@ int b64_job_run(const b64_job_desc *job, void *stream) {
         HIP_TRY(b64::launch_merkle_levels((uint4 *)(ws + L.tree[k]), n_leaves, job_iv(), s));
     }
     for (size_t r = 0; r < roots.size(); r++)
-        HIP_TRY(hipMemcpyAsync(job->roots_out + 32 * r, roots[r], 32, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(roots_dst + 32 * r, roots[r], 32, hipMemcpyDeviceToHost, s));
+    return B64_OK;
+}
+
+extern "C" {
+
+int b64_job_run(const b64_job_desc *job, void *stream) {
+    if (!job || !job->roots_out) return B64_ERR_INVALID_ARGS;
+    const hipStream_t s = as_stream(stream);
+    B64_TRY(job_enqueue(job, s, job->roots_out));
     HIP_TRY(hipStreamSynchronize(s));
     return B64_OK;
 }
@@ -397,6 +413,30 @@ int b64_batch_plan_build(const b64_job_desc *jobs, uint32_t n_jobs, int use_grap
     if (!plan) return B64_ERR_OUT_OF_MEMORY;
     std::unique_ptr<b64_batch_plan> guard(plan);
     plan->jobs.assign(jobs, jobs + n_jobs);
+    if (n_jobs == 1 && !use_graph) {
+        const b64_job_desc &jd = jobs[0];
+        if (!jd.d_msg || !jd.d_workspace || !jd.challenges || !jd.roots_out) return B64_ERR_INVALID_ARGS;
+        SizeInfo *si = nullptr;
+        B64_TRY(size_info(jd.log_msg_len, &si));
+        B64_TRY(size_device(si));
+        Layout L;
+        layout_of(si, L);
+        std::vector<Stage> st_list;
+        std::vector<const uint4 *> roots;
+        B64_TRY(job_stages(jd, L, st_list, roots));
+        plan->single = true;
+        plan->n_roots = (int)roots.size();
+        plan->roots_per_job.push_back(plan->n_roots);
+        if (st) {
+            if ((size_t)plan->n_roots > st->roots_cap) return B64_ERR_BUFFER_TOO_SMALL;
+            plan->owns_buffers = false;
+            plan->h_roots = st->h_roots;
+        } else {
+            HIP_TRY(hipHostMalloc((void **)&plan->h_roots, 32ull * plan->n_roots, hipHostMallocDefault));
+        }
+        *out = guard.release();
+        return B64_OK;
+    }
     // per-slot job lists
     std::map<int, std::vector<Stage>> slots;
     std::vector<const uint4 *> root_src;
@@ -499,6 +539,7 @@ extern "C" {
 int b64_batch_plan_enqueue(b64_batch_plan *plan, void *stream) {
     if (!plan) return B64_ERR_INVALID_ARGS;
     hipStream_t s = as_stream(stream);
+    if (plan->single) return job_enqueue(&plan->jobs[0], s, plan->h_roots);
     if (plan->use_graph)
         HIP_TRY(hipGraphLaunch(plan->exec, s));
     else

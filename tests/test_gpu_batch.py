@@ -76,3 +76,18 @@ def test_batch_matches_oracle_small(b64, orc):
     plan.run()
     for (job, msg, chal), n in zip(made, sizes):
         assert job.roots() == oracle_job(orc, n, msg, chal)["roots"]
+
+
+@pytest.mark.parametrize("n,use_graph", [(13, False), (20, False), (13, True), (20, True)])
+def test_single_job_plan(b64, orc, n, use_graph):
+    """A plan of one job: without a graph it runs the job's single-job launchers
+    (the per-size schedule, register-resident NTT passes at n = 20), with one
+    the multi-job kernels; both give the oracle's roots, on every replay."""
+    say("single-job plan", n, "graph", use_graph)
+    job, msg, chal = make_job(b64, orc, n, 70 + n)
+    want = oracle_job(orc, n, msg, chal)["roots"]
+    plan = b64.BatchPlan([job], use_graph=use_graph)
+    for rep in range(2):
+        job.roots_buf[:] = 0
+        plan.run()
+        assert job.roots() == want, rep
