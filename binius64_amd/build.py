@@ -36,9 +36,10 @@ def flags() -> list[str]:
     ]
 
 
-def build(verbose: bool = False) -> Path:
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *flags(), *[str(CSRC / s) for s in SOURCES], "-o", str(tmp)]
+OBJ_DIR = PKG / "_build"
+
+
+def _run(cmd: list[str], verbose: bool) -> None:
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -49,6 +50,24 @@ def build(verbose: bool = False) -> Path:
     noise = [l for l in res.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip()]
     if verbose and noise:
         sys.stderr.write("\n".join(noise) + "\n")
+
+
+def build(verbose: bool = False) -> Path:
+    """Compiles every source to an object in parallel (no device code crosses a
+    source file, so no relocatable device code is needed), then links."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    OBJ_DIR.mkdir(exist_ok=True)
+    compile_flags = [f for f in flags() if f != "-shared"]
+    objs = [OBJ_DIR / (Path(src).stem + ".o") for src in SOURCES]
+    jobs = int(os.environ.get("MAX_JOBS", "0")) or (os.cpu_count() or 4)
+    with ThreadPoolExecutor(max_workers=max(1, min(len(SOURCES), jobs))) as ex:
+        futs = [ex.submit(_run, [hipcc(), *compile_flags, "-c", str(CSRC / src), "-o", str(obj)], verbose)
+                for src, obj in zip(SOURCES, objs)]
+        for f in futs:
+            f.result()
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *[str(o) for o in objs], "-o", str(tmp)], verbose)
     os.replace(tmp, LIB)
     return LIB
 
