@@ -9,6 +9,9 @@
 // host reads back two field elements (the round evaluations), as the
 // Fiat-Shamir transcript needs them before the next challenge exists.
 #include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
 #include <string.h>
 
 #include <new>
@@ -74,8 +77,16 @@ __device__ __forceinline__ void block_xor2(uint4 &p, uint4 &q) {
 // the last block to finish do the reduction costs more than the launch it
 // saves: each block's device-scope release then writes back the L2 the round's
 // folded arrays are dirty in.)
+// Announces a round's totals, already stored in host-visible memory: the
+// stores are made visible system-wide before the round's sequence number is.
+__device__ __forceinline__ void publish_seq(uint32_t *flag, uint32_t seq) {
+    __threadfence_system();
+    *(volatile uint32_t *)flag = seq;
+}
+
 __global__ __launch_bounds__(SC_BLOCK) void k_sc_round(uint4 *__restrict__ a, uint4 *__restrict__ b, uint64_t half,
-                                                       int fold, uint4 r, uint4 *__restrict__ partial) {
+                                                       int fold, uint4 r, uint4 *__restrict__ partial,
+                                                       uint32_t *flag, uint32_t seq) {
     __shared__ __attribute__((aligned(256))) uint4 tab[WIN_ENTRIES];
     if (fold) {
         if (threadIdx.x < WIN_ROWS) win_build_row(tab, r, threadIdx.x);
@@ -110,13 +121,14 @@ __global__ __launch_bounds__(SC_BLOCK) void k_sc_round(uint4 *__restrict__ a, ui
     if (threadIdx.x == 0) {
         partial[2 * blockIdx.x] = y1;
         partial[2 * blockIdx.x + 1] = yi;
+        if (flag) publish_seq(flag, seq);  // a one-block round writes the totals itself
     }
 }
 
 // The round's totals: XOR of the n_blocks partial pairs, written to `result`,
 // host-visible pinned memory read after the stream synchronises.
 __global__ __launch_bounds__(SC_BLOCK) void k_sc_reduce(const uint4 *__restrict__ partial, int n_blocks,
-                                                        uint4 *__restrict__ result) {
+                                                        uint4 *__restrict__ result, uint32_t *flag, uint32_t seq) {
     uint4 y1 = make_uint4(0, 0, 0, 0), yi = make_uint4(0, 0, 0, 0);
     for (int k = threadIdx.x; k < n_blocks; k += SC_BLOCK) {
         y1 = gxor(y1, partial[2 * k]);
@@ -126,6 +138,7 @@ __global__ __launch_bounds__(SC_BLOCK) void k_sc_reduce(const uint4 *__restrict_
     if (threadIdx.x == 0) {
         result[0] = y1;
         result[1] = yi;
+        publish_seq(flag, seq);
     }
 }
 
@@ -242,6 +255,28 @@ void eq_host(const u128 *r, int n, u128 *out) {
 }  // namespace
 }  // namespace b64
 
+namespace b64 {
+
+// Waits for a round's totals: polls the sequence number the last kernel
+// publishes in host memory after them (visible a few microseconds after the
+// kernel ends, where a blocking stream wait takes tens).  A launch that failed
+// never publishes; past the polling bound the stream wait takes over and
+// reports the error.
+int wait_published(volatile uint32_t *flag, uint32_t seq, hipStream_t s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t i = 0; *flag != seq; i++) {
+        if ((i & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(100)) {
+            HIP_TRY(hipStreamSynchronize(s));
+            if (*flag != seq) return B64_ERR_HIP;
+            break;
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);  // the totals were stored before the number
+    return B64_OK;
+}
+
+}  // namespace b64
+
 // ===========================================================================
 // BivariateProductSumcheckProver
 // ===========================================================================
@@ -255,7 +290,11 @@ struct b64_sumcheck {
     uint4 *d_partial = nullptr;  // 2 * SC_MAX_BLOCKS partial sums
     uint4 *h_res = nullptr;      // pinned, 2 elements, written by the kernels
     uint4 *h_res_dev = nullptr;  // h_res as the device addresses it
+    uint32_t seq = 0;            // round sequence number, published after the totals
     hipStream_t s = nullptr;
+
+    volatile uint32_t *flag() const { return (volatile uint32_t *)(h_res + 2); }
+    uint32_t *flag_dev() const { return (uint32_t *)(h_res_dev + 2); }
 
     ~b64_sumcheck() {
         if (d_partial) (void)hipFreeAsync(d_partial, s);
@@ -287,6 +326,7 @@ int b64_sumcheck_create(void *d_a, void *d_b, uint32_t n_vars, const uint64_t su
         delete sc;
         return B64_ERR_OUT_OF_MEMORY;
     }
+    *sc->flag() = 0;
     *out = sc;
     return B64_OK;
 }
@@ -302,12 +342,17 @@ int b64_sumcheck_execute(b64_sumcheck *sc, uint64_t coeffs_out[6]) {
     const uint64_t half = 1ull << (sc->n_vars - 1);
     const int blocks = b64::sc_blocks(half);
     // a one-block round writes its totals straight to the pinned result
-    b64::k_sc_round<<<blocks, b64::SC_BLOCK, 0, sc->s>>>(sc->a, sc->b, half, sc->pending ? 1 : 0,
-                                                          to_u4(sc->r_pending),
-                                                          blocks == 1 ? sc->h_res_dev : sc->d_partial);
-    if (blocks > 1) b64::k_sc_reduce<<<1, b64::SC_BLOCK, 0, sc->s>>>(sc->d_partial, blocks, sc->h_res_dev);
+    const uint32_t seq = ++sc->seq;
+    if (blocks == 1) {
+        b64::k_sc_round<<<1, b64::SC_BLOCK, 0, sc->s>>>(sc->a, sc->b, half, sc->pending ? 1 : 0,
+                                                          to_u4(sc->r_pending), sc->h_res_dev, sc->flag_dev(), seq);
+    } else {
+        b64::k_sc_round<<<blocks, b64::SC_BLOCK, 0, sc->s>>>(sc->a, sc->b, half, sc->pending ? 1 : 0,
+                                                               to_u4(sc->r_pending), sc->d_partial, nullptr, 0);
+        b64::k_sc_reduce<<<1, b64::SC_BLOCK, 0, sc->s>>>(sc->d_partial, blocks, sc->h_res_dev, sc->flag_dev(), seq);
+    }
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(sc->s));
+    B64_TRY(b64::wait_published(sc->flag(), seq, sc->s));
     sc->pending = false;
     // RoundEvals2::interpolate (round_evals.rs:73-78, :126-140)
     const u128 y1 = from_u4(sc->h_res[0]), yinf = from_u4(sc->h_res[1]);
