@@ -162,67 +162,99 @@ __global__ void k_sc_final(const uint4 *__restrict__ a, const uint4 *__restrict_
 // entries, built by doubling as tensor_prod_eq_ind does, eq.rs:39-71).  Then
 // eq[x] = G0[x & 255] * c(x >> 8) with c(y) = prod_{g>=1} G_g[byte g-1 of y]:
 // each run of 256 outputs shares c(y), so the run is one window table of c(y)
-// applied to G0.  A block computes 256 values c(y) (one per lane), then its
-// waves take one run each per step.
+// applied to G0.  One small launch builds the group tables (k_eq_groups); then
+// every wave of k_eq_runs takes a range of runs with a table of its own, so no
+// wave waits for another.
 constexpr int EQ_MAX_VARS = 40;
 constexpr int EQ_GROUPS = (EQ_MAX_VARS + 7) / 8;
+constexpr int EQ_MAX_RUNS_PER_WAVE = 32;
+constexpr int EQ_TARGET_WAVES = 2048;  // 8 waves on each of 256 CUs
+constexpr int EQ_MAX_BLOCKS = 16384;
 struct EqPoint {
     uint4 r[EQ_MAX_VARS];
 };
 
-__global__ __launch_bounds__(SC_BLOCK) void k_eq_expand(EqPoint p, int n, uint4 *__restrict__ out) {
+// Orders one wave's LDS accesses across its lanes: a wave's LDS instructions
+// execute in issue order, so keeping the compiler from moving accesses across
+// this point is enough.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint4 readlane4(uint4 v, int lane) {
+    return make_uint4(__builtin_amdgcn_readlane(v.x, lane), __builtin_amdgcn_readlane(v.y, lane),
+                      __builtin_amdgcn_readlane(v.z, lane), __builtin_amdgcn_readlane(v.w, lane));
+}
+
+// One block: wave w doubles groups w, w + SC_WAVES, ... in LDS and writes them
+// to gtab[g * 256 ..].  With n <= 8 the single group is the whole tensor.
+__global__ __launch_bounds__(SC_BLOCK) void k_eq_groups(EqPoint p, int n, uint4 *__restrict__ gtab) {
     __shared__ uint4 g[EQ_GROUPS][256];
-    __shared__ __attribute__((aligned(256))) uint4 wtab[SC_WAVES][WIN_ENTRIES];
-    __shared__ uint4 cbuf[SC_BLOCK];
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     const int n_groups = n ? (n + 7) / 8 : 1;  // n = 0: the single entry 1
-    for (int gi = 0; gi < n_groups; gi++) {
+    for (int gi = w; gi < n_groups; gi += SC_WAVES) {
         const int len = (n - 8 * gi) < 8 ? (n - 8 * gi) : 8;
-        if (t == 0) g[gi][0] = make_uint4(1, 0, 0, 0);
-        __syncthreads();
+        uint4 *G = g[gi];
+        if (lane == 0) G[0] = make_uint4(1, 0, 0, 0);
+        wave_lds_sync();
         for (int k = 0; k < len; k++) {
-            if (t < (1 << k)) {
-                const uint4 prod = gf_mul(g[gi][t], p.r[8 * gi + k]);
-                g[gi][t + (1 << k)] = prod;
-                g[gi][t] = gxor(g[gi][t], prod);
+            const uint4 r = p.r[8 * gi + k];
+            for (int i = lane; i < (1 << k); i += 64) {
+                const uint4 prod = gf_mul(G[i], r);
+                G[i + (1 << k)] = prod;
+                G[i] = gxor(G[i], prod);
             }
-            __syncthreads();
+            wave_lds_sync();
         }
+        const int size = 1 << len;
+        for (int i = lane; i < size; i += 64) gtab[256 * gi + i] = G[i];
     }
-    if (n <= 8) {
-        if (blockIdx.x == 0 && t < (1 << n)) out[t] = g[0][t];
-        return;
-    }
-    const uint64_t n_runs = 1ull << (n - 8);
-    for (uint64_t y0 = (uint64_t)blockIdx.x * SC_BLOCK; y0 < n_runs; y0 += (uint64_t)gridDim.x * SC_BLOCK) {
-        {
-            const uint64_t y = y0 + t;
-            uint4 c = make_uint4(1, 0, 0, 0);
-            if (y < n_runs)
-                for (int gi = 1; gi < n_groups; gi++) c = gf_mul(c, g[gi][(y >> (8 * (gi - 1))) & 255]);
-            cbuf[t] = c;
-        }
-        __syncthreads();
-        for (int j = 0; j < 64; j++) {
-            const int slot = 64 * w + j;
-            const uint64_t y = y0 + slot;
-            if (lane < WIN_ROWS) win_build_row(wtab[w], cbuf[slot], lane);
-            __syncthreads();
-            if (y < n_runs) {
-                const uint32_t tb = lds_addr(wtab[w]);
-                uint4 *o = out + (y << 8);
+}
+
+// Runs [gw * rpw, (gw + 1) * rpw) for each wave gw; one wave per block.
+__global__ __launch_bounds__(64) void k_eq_runs(const uint4 *__restrict__ gtab, int n, int rpw,
+                                                uint4 *__restrict__ out) {
+    __shared__ __attribute__((aligned(256))) uint4 wtab[WIN_ENTRIES];
+    const int lane = threadIdx.x;
+    const int n_groups = (n + 7) / 8;
+    uint4 lo[4];
 #pragma unroll
-                for (int k = 0; k < 4; k++) o[lane + 64 * k] = win_mul_perm(tb, g[0][lane + 64 * k]);
-            }
-            __syncthreads();
+    for (int k = 0; k < 4; k++) lo[k] = gtab[lane + 64 * k];
+    const uint64_t n_runs = 1ull << (n - 8);
+    const uint64_t n_waves = n_runs / (uint64_t)rpw;
+    const uint32_t tb = lds_addr(wtab);
+    for (uint64_t gw = blockIdx.x; gw < n_waves; gw += gridDim.x) {
+        const uint64_t y0 = gw * (uint64_t)rpw;
+        // lane j (mod rpw) holds c(y0 + j)
+        const uint64_t y = y0 + (uint64_t)(lane & (rpw - 1));
+        uint4 c = gtab[256 + (y & 255)];
+        for (int gi = 2; gi < n_groups; gi++) c = gf_mul(c, gtab[256 * gi + ((y >> (8 * (gi - 1))) & 255)]);
+        for (int j = 0; j < rpw; j++) {
+            if (lane < WIN_ROWS) win_build_row(wtab, readlane4(c, j), lane);
+            wave_lds_sync();
+            uint4 *o = out + ((y0 + j) << 8);
+#pragma unroll
+            for (int k = 0; k < 4; k++) o[lane + 64 * k] = win_mul_perm(tb, lo[k]);
+            wave_lds_sync();  // every lane's lookups precede the next run's table
         }
     }
 }
 
-hipError_t launch_eq_expand(const EqPoint &p, int n, uint4 *out, hipStream_t s) {
-    uint64_t blocks = n <= 8 ? 1 : ((1ull << (n - 8)) + SC_BLOCK - 1) / SC_BLOCK;
-    if (blocks > 1024) blocks = 1024;
-    k_eq_expand<<<(unsigned)blocks, SC_BLOCK, 0, s>>>(p, n, out);
+// gtab: EQ_GROUPS * 256 elements of device scratch (unused for n <= 8).
+hipError_t launch_eq_expand(const EqPoint &p, int n, uint4 *gtab, uint4 *out, hipStream_t s) {
+    if (n <= 8) {
+        k_eq_groups<<<1, SC_BLOCK, 0, s>>>(p, n, out);
+        return hipGetLastError();
+    }
+    k_eq_groups<<<1, SC_BLOCK, 0, s>>>(p, n, gtab);
+    const uint64_t n_runs = 1ull << (n - 8);
+    int rpw = 1;
+    while (rpw < EQ_MAX_RUNS_PER_WAVE && n_runs / (uint64_t)(2 * rpw) >= (uint64_t)EQ_TARGET_WAVES) rpw *= 2;
+    uint64_t blocks = n_runs / (uint64_t)rpw;
+    if (blocks > EQ_MAX_BLOCKS) blocks = EQ_MAX_BLOCKS;
+    k_eq_runs<<<(unsigned)blocks, 64, 0, s>>>(gtab, n, rpw, out);
     return hipGetLastError();
 }
 
@@ -403,7 +435,15 @@ int b64_eq_ind_partial_eval(const uint64_t *point, uint32_t n, void *d_out, void
     b64::EqPoint p;
     memset(&p, 0, sizeof(p));
     for (uint32_t k = 0; k < n; k++) p.r[k] = to_u4(ld128(point + 2 * k));
-    HIP_TRY(b64::launch_eq_expand(p, (int)n, (uint4 *)d_out, as_stream(stream)));
+    const hipStream_t s = as_stream(stream);
+    uint4 *gtab = nullptr;
+    if (n > 8) {
+        b64::abi::keep_pool_memory();
+        HIP_TRY(hipMallocAsync((void **)&gtab, b64::EQ_GROUPS * 256 * sizeof(uint4), s));
+    }
+    const hipError_t e = b64::launch_eq_expand(p, (int)n, gtab, (uint4 *)d_out, s);
+    if (gtab) (void)hipFreeAsync(gtab, s);
+    HIP_TRY(e);
     return B64_OK;
 }
 

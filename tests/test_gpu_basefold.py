@@ -22,7 +22,7 @@ def say(*a):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("n", [0, 1, 3, 8, 9, 12, 16, 17, 20])
+@pytest.mark.parametrize("n", [0, 1, 3, 8, 9, 10, 11, 12, 13, 16, 17, 20, 21, 24])
 def test_eq_ind_partial_eval(b64, orc, n):
     say("eq", n)
     point = orc.random_elems(300 + n, n) if n else np.zeros((0, 2), dtype=np.uint64)
