@@ -269,6 +269,7 @@ struct b64_batch_plan {
         size_t args_off, start_off;  // into d_tables
         int n_jobs;
         uint32_t blocks;
+        int fold_nc;  // FoldCodeword: the n_chal all jobs share (1..4), else 0
     };
     std::vector<Launch> launches;
     std::vector<b64_job_desc> jobs;
@@ -300,7 +301,7 @@ struct b64_batch_plan {
     int enqueue(hipStream_t s) const {
         for (const Launch &l : launches)
             HIP_TRY(b64::launch_multi(l.kind, d_tables + l.args_off, (const uint32_t *)(d_tables + l.start_off),
-                                      l.n_jobs, l.blocks, s));
+                                      l.n_jobs, l.blocks, s, l.fold_nc));
         if (n_roots) {
             k_gather_roots<<<(n_roots + 63) / 64, 64, 0, s>>>(d_root_src, d_roots, n_roots);
             HIP_TRY(hipGetLastError());
@@ -470,11 +471,13 @@ int b64_batch_plan_build(const b64_job_desc *jobs, uint32_t n_jobs, int use_grap
         b64_batch_plan::Launch l;
         l.kind = v[0].kind;
         l.n_jobs = (int)v.size();
+        l.fold_nc = l.kind == b64::MultiKind::FoldCodeword ? v[0].fold.n_chal : 0;
         std::vector<uint32_t> start;
         uint32_t total = 0;
         std::vector<uint8_t> args;
         for (const Stage &x : v) {
             if (x.kind != l.kind) return B64_ERR_INVALID_ARGS;
+            if (x.kind == b64::MultiKind::FoldCodeword && x.fold.n_chal != l.fold_nc) l.fold_nc = 0;
             start.push_back(total);
             total += x.blocks;
             const void *a = nullptr;

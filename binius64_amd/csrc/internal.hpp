@@ -125,8 +125,9 @@ int fri_fold_round_enqueue(b64_fri_prover *fp, int *committed, const uint8_t **r
 int ntt_pass_plan(int log_d, int skip_early, int skip_late, bool msg_first, int *layer0, int *k, int *c, int *e);
 // One launch of `kind` over n_jobs jobs: d_jobs is a device array of the
 // kind's args, d_start[j] the first block of job j, `blocks` the total.
+// fold_nc: for FoldCodeword, the n_chal (1..4) every job shares, else 0.
 hipError_t launch_multi(MultiKind kind, const void *d_jobs, const uint32_t *d_start, int n_jobs, uint32_t blocks,
-                        hipStream_t s);
+                        hipStream_t s, int fold_nc = 0);
 // 128 B128 elements passed by value to a kernel (2 KiB of arguments).
 struct RsVec {
     uint4 v[128];

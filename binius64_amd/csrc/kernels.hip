@@ -936,7 +936,8 @@ __device__ __forceinline__ uint4 twiddle_of(const uint4 *__restrict__ W, int lay
 
 // fold_codeword (prover fri/fold.rs:297-318) with fold_chunk / fold_pair
 // (verifier fri/fold.rs:21-99): per coset, n_chal rounds of inverse butterfly
-// and line extrapolation x0 + (x1 - x0) * r.
+// and line extrapolation x0 + (x1 - x0) * r.  General form for n_chal 5..8
+// (the coset's values do not fit in registers).
 template <int MAXV>
 __device__ __forceinline__ void fold_codeword_core(const FoldCwArgs &a, uint64_t blk) {
     const uint64_t i = blk * BLOCK + threadIdx.x;
@@ -961,15 +962,123 @@ __device__ __forceinline__ void fold_codeword_core(const FoldCwArgs &a, uint64_t
     a.out[i] = v[0];
 }
 
+// The same fold for n_chal = NC <= 4, with the coset in registers.  Round r
+// works at layer log_len - 1 - r on blocks (i << s) | off, s = NC - 1 - r: the
+// twiddle is the XOR of the layer's basis elements selected by the block's bits
+// (twiddle_of), i.e. one sum over i's bits shared by the round plus, per off,
+// the basis elements below s.  Rounds and butterflies are unrolled by template
+// recursion: as loops they are too large for the unroller, and the coset array
+// then lands in scratch memory.  (Window tables for the challenge products were
+// tried: next to the general product in the same chain the compiler runs out
+// of registers.)
+template <int S, int OFF>
+__device__ __forceinline__ void fold_bfly_reg(uint4 *v, uint4 base, const uint4 *__restrict__ Wl, uint4 ch) {
+    if constexpr (OFF < (1 << S)) {
+        uint4 t = base;
+#pragma unroll
+        for (int b = 0; b < S; b++)
+            if ((OFF >> b) & 1) t = gxor(t, Wl[b]);
+        uint4 u = v[OFF << 1], w = v[(OFF << 1) | 1];
+        w = gxor(w, u);
+        u = gxor(u, gf_mul(w, t));
+        v[OFF] = gxor(u, gf_mul(gxor(w, u), ch));
+        fold_bfly_reg<S, OFF + 1>(v, base, Wl, ch);
+    }
+}
+
+template <int NC, int R>
+__device__ __forceinline__ void fold_rounds_reg(uint4 *v, const uint4 *__restrict__ W, int log_len, const uint4 *ch,
+                                                uint64_t i) {
+    if constexpr (R < NC) {
+        constexpr int S = NC - 1 - R;
+        const int layer = log_len - 1 - R;  // >= S, since log_len >= NC
+        const uint4 *Wl = W + layer * W_STRIDE;
+        uint4 base = make_uint4(0, 0, 0, 0);
+        for (int b = S; b < layer; b++) {
+            const uint32_t m = 0u - (uint32_t)((i >> (b - S)) & 1);
+            const uint4 wb = Wl[b];
+            base = make_uint4(base.x ^ (wb.x & m), base.y ^ (wb.y & m), base.z ^ (wb.z & m), base.w ^ (wb.w & m));
+        }
+        fold_bfly_reg<S, 0>(v, base, Wl, ch[R]);
+        fold_rounds_reg<NC, R + 1>(v, W, log_len, ch, i);
+    }
+}
+
+__device__ __forceinline__ uint32_t sreg(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint4 sreg4(uint4 v) { return make_uint4(sreg(v.x), sreg(v.y), sreg(v.z), sreg(v.w)); }
+template <typename T>
+__device__ __forceinline__ T *sreg_ptr(T *p) {
+    const uint64_t x = (uint64_t)p;
+    return (T *)(((uint64_t)sreg((uint32_t)(x >> 32)) << 32) | sreg((uint32_t)x));
+}
+
+// A job's fold, its arguments read from device memory (the same for the whole
+// block, so they are moved to scalar registers).
+template <int NC>
+__device__ __forceinline__ void fold_codeword_job(const FoldCwArgs &a, uint64_t i) {
+    uint4 ch[NC];
+#pragma unroll
+    for (int r = 0; r < NC; r++) ch[r] = sreg4(a.chal[r]);
+    const uint4 *cw = sreg_ptr(a.cw), *W = sreg_ptr(a.W);
+    uint4 *out = sreg_ptr(a.out);
+    const int log_len = (int)sreg((uint32_t)a.log_len);
+    uint4 v[1 << NC];
+#pragma unroll
+    for (int j = 0; j < (1 << NC); j++) v[j] = cw[(i << NC) + j];
+    fold_rounds_reg<NC, 0>(v, W, log_len, ch, i);
+    out[i] = v[0];
+}
+
+// n_chal 0..4 (the batch plans reject larger arities).
+__device__ __forceinline__ void fold_codeword_upto4(const FoldCwArgs &a, uint64_t blk) {
+    const uint64_t i = blk * BLOCK + threadIdx.x;
+    if (i >= a.n_out) return;
+    switch (a.n_chal) {
+        case 0: a.out[i] = a.cw[i]; break;
+        case 1: fold_codeword_job<1>(a, i); break;
+        case 2: fold_codeword_job<2>(a, i); break;
+        case 3: fold_codeword_job<3>(a, i); break;
+        case 4: fold_codeword_job<4>(a, i); break;
+        default: break;
+    }
+}
+
+template <int NC>
+__global__ __launch_bounds__(BLOCK) void k_fold_codeword_reg(FoldCwArgs a) {
+    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= a.n_out) return;
+    uint4 ch[NC];
+#pragma unroll
+    for (int r = 0; r < NC; r++) ch[r] = a.chal[r];
+    uint4 v[1 << NC];
+#pragma unroll
+    for (int j = 0; j < (1 << NC); j++) v[j] = a.cw[(i << NC) + j];
+    fold_rounds_reg<NC, 0>(v, a.W, a.log_len, ch, i);
+    a.out[i] = v[0];
+}
+
 template <int MAXV>
 __global__ __launch_bounds__(BLOCK) void k_fold_codeword(FoldCwArgs a) {
     fold_codeword_core<MAXV>(a, blockIdx.x);
 }
 
+// Jobs of mixed arity (the switch costs registers: one wave per SIMD).
 __global__ __launch_bounds__(BLOCK) void k_fold_codeword_multi(const FoldCwArgs *__restrict__ jobs,
                                                                const uint32_t *__restrict__ start, int n_jobs) {
     const int j = job_of_block2(start, n_jobs, blockIdx.x);
-    fold_codeword_core<16>(jobs[j], blockIdx.x - start[j]);
+    fold_codeword_upto4(jobs[j], blockIdx.x - start[j]);
+}
+
+// Jobs that all fold NC challenges.
+template <int NC>
+__global__ __launch_bounds__(BLOCK) void k_fold_codeword_multi_nc(const FoldCwArgs *__restrict__ jobs,
+                                                                  const uint32_t *__restrict__ start, int n_jobs) {
+    // the job index is block-uniform: say so, so that the job's fields load
+    // into scalar registers
+    const int j = __builtin_amdgcn_readfirstlane(job_of_block2(start, n_jobs, blockIdx.x));
+    const FoldCwArgs &a = jobs[j];
+    const uint64_t i = (uint64_t)(blockIdx.x - start[j]) * BLOCK + threadIdx.x;
+    if (i < a.n_out) fold_codeword_job<NC>(a, i);
 }
 
 hipError_t launch_fold_codeword(const DeviceDomain &dd, const uint4 *cw, uint4 *out, int log_len,
@@ -983,8 +1092,16 @@ hipError_t launch_fold_codeword(const DeviceDomain &dd, const uint4 *cw, uint4 *
     a.W = dd.W;
     if (n_chal > MAX_FOLD_CHAL) return hipErrorInvalidValue;
     for (int t = 0; t < n_chal; t++) a.chal[t] = chal[t];
-    if (n_chal <= 4)
+    if (n_chal == 0)
         k_fold_codeword<16><<<grid_for(a.n_out), BLOCK, 0, s>>>(a);
+    else if (n_chal == 1)
+        k_fold_codeword_reg<1><<<grid_for(a.n_out), BLOCK, 0, s>>>(a);
+    else if (n_chal == 2)
+        k_fold_codeword_reg<2><<<grid_for(a.n_out), BLOCK, 0, s>>>(a);
+    else if (n_chal == 3)
+        k_fold_codeword_reg<3><<<grid_for(a.n_out), BLOCK, 0, s>>>(a);
+    else if (n_chal == 4)
+        k_fold_codeword_reg<4><<<grid_for(a.n_out), BLOCK, 0, s>>>(a);
     else if (n_chal <= 8)
         k_fold_codeword<256><<<grid_for(a.n_out), BLOCK, 0, s>>>(a);
     else
@@ -1006,7 +1123,7 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass_tab_multi(const NttPassArgs 
 // jobs; tables live in device memory prepared by the caller.
 // ---------------------------------------------------------------------------
 hipError_t launch_multi(MultiKind kind, const void *d_jobs, const uint32_t *d_start, int n_jobs, uint32_t blocks,
-                        hipStream_t s) {
+                        hipStream_t s, int fold_nc) {
     if (blocks == 0 || n_jobs == 0) return hipSuccess;
     switch (kind) {
         case MultiKind::NttPass:
@@ -1031,9 +1148,17 @@ hipError_t launch_multi(MultiKind kind, const void *d_jobs, const uint32_t *d_st
         case MultiKind::Subtree:
             k_merkle_subtree_multi<<<blocks, BLOCK, 0, s>>>((const SubtreeArgs *)d_jobs, d_start, n_jobs);
             break;
-        case MultiKind::FoldCodeword:
-            k_fold_codeword_multi<<<blocks, BLOCK, 0, s>>>((const FoldCwArgs *)d_jobs, d_start, n_jobs);
+        case MultiKind::FoldCodeword: {
+            const FoldCwArgs *j = (const FoldCwArgs *)d_jobs;
+            switch (fold_nc) {
+                case 1: k_fold_codeword_multi_nc<1><<<blocks, BLOCK, 0, s>>>(j, d_start, n_jobs); break;
+                case 2: k_fold_codeword_multi_nc<2><<<blocks, BLOCK, 0, s>>>(j, d_start, n_jobs); break;
+                case 3: k_fold_codeword_multi_nc<3><<<blocks, BLOCK, 0, s>>>(j, d_start, n_jobs); break;
+                case 4: k_fold_codeword_multi_nc<4><<<blocks, BLOCK, 0, s>>>(j, d_start, n_jobs); break;
+                default: k_fold_codeword_multi<<<blocks, BLOCK, 0, s>>>(j, d_start, n_jobs); break;
+            }
             break;
+        }
     }
     return hipGetLastError();
 }

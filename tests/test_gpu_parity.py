@@ -267,7 +267,8 @@ def test_fold_interleaved(b64, orc, log_len, lbs):
     assert np.array_equal(out.to_host(1 << log_len), orc.fold_interleaved(cw, log_len, lbs, chal))
 
 
-@pytest.mark.parametrize("log_len,n_chal", [(1, 1), (5, 3), (13, 4), (17, 4), (9, 1), (12, 6)])
+@pytest.mark.parametrize("log_len,n_chal", [(1, 1), (2, 2), (4, 4), (6, 2), (5, 3), (13, 4), (17, 4), (21, 4),
+                                              (9, 1), (12, 6)])
 def test_fold_codeword(b64, orc, log_len, n_chal):
     dom, ref = b64.DomainContext.standard(log_len + 2), orc.Domain(dim=log_len + 2)
     cw = orc.random_elems(80, 1 << log_len)
