@@ -467,7 +467,7 @@ __global__ __launch_bounds__(R16_BLOCK) void k_ntt_pass_r16(NttPassArgs a, uint6
             for (int e = 0; e < 16; e++) x[e] = a.msg[run + rev_bits(e, 4)];
         } else {
 #pragma unroll
-            for (int e = 0; e < 16; e++) x[e] = a.data[base | ((uint64_t)e << lb)];
+            for (int e = 0; e < 16; e++) x[e] = (a.data + (hi << (lb + 4)) + ((uint64_t)e << lb))[(uint32_t)low];
         }
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -484,8 +484,13 @@ __global__ __launch_bounds__(R16_BLOCK) void k_ntt_pass_r16(NttPassArgs a, uint6
                 x[e | (1 << bit)] = v;
             }
         }
+        // The stores recompute their addresses from a fresh copy of the lane
+        // offset: kept from the loads, the 16 64-bit addresses pushed the kernel
+        // past 128 VGPRs into scratch.
+        uint32_t low_st = (uint32_t)low;
+        asm volatile("" : "+v"(low_st));
 #pragma unroll
-        for (int e = 0; e < 16; e++) a.data[base | ((uint64_t)e << lb)] = x[e];
+        for (int e = 0; e < 16; e++) (a.data + (hi << (lb + 4)) + ((uint64_t)e << lb))[low_st] = x[e];
     }
 }
 
