@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # passes of one forward NTT.  Keyed by log2 codeword length and the schedule
 # (variant, layers per pass, tile size); the counters cannot be read from
 # inside the timed run, so other schedules report null.
-TRAFFIC = {(25, 2, 4, 11): {"bytes": 1074217491, "valu_issue_per_cu_clk": 0.68}}
+TRAFFIC = {(25, 2, 4, 11): {"bytes": 1074217312, "valu_issue_per_cu_clk": 0.72}}
 
 
 def traffic_entry(log_cw, tuning):
