@@ -78,6 +78,19 @@ def test_batch_matches_oracle_small(b64, orc):
         assert job.roots() == oracle_job(orc, n, msg, chal)["roots"]
 
 
+@pytest.mark.parametrize("sizes", [[13, 13], [14, 14, 14], [16, 17, 16, 17], [19, 19]])
+def test_batch_uniform_arity_matches_oracle(b64, orc, sizes):
+    """Batched coset folds whose jobs all fold the same number of challenges
+    (arity 2, 3 and 4 at the first folded oracle, 4 at both for n = 19) run the
+    per-arity multi-job kernel; the mixed slots the general one."""
+    say("uniform-arity batch", sizes)
+    made = [make_job(b64, orc, n, 90 + i) for i, n in enumerate(sizes)]
+    plan = b64.BatchPlan([m[0] for m in made], use_graph=False)
+    plan.run()
+    for (job, msg, chal), n in zip(made, sizes):
+        assert job.roots() == oracle_job(orc, n, msg, chal)["roots"]
+
+
 @pytest.mark.parametrize("n,use_graph", [(13, False), (20, False), (13, True), (20, True)])
 def test_single_job_plan(b64, orc, n, use_graph):
     """A plan of one job: without a graph it runs the job's single-job launchers
