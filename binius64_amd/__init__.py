@@ -27,7 +27,7 @@ __all__ = [
     "B64Error", "DeviceBuffer", "DomainContext", "AdditiveNTT", "ReedSolomonCode",
     "BinaryMerkleTreeProver", "FRIParams", "CommitOutput", "commit_interleaved", "setup_commit_params",
     "ProverTranscript", "FRIFoldProver", "gf_mul", "gf_inv", "gf_mul_device", "gf_mul_const_device",
-    "pack_witness", "device_count", "config_set", "config_get", "LIB_PATH", "lib",
+    "pack_witness", "device_count", "config_set", "config_get", "ntt_plan", "LIB_PATH", "lib",
     "CommitJob", "BatchPlan", "Server", "fri_prove", "ring_switch_fold_1b_rows", "ring_switch_fold_elems",
     "Proof", "STD_CHALLENGER_NAME",
 ]
@@ -58,6 +58,22 @@ def device_count() -> int:
 def config_set(key: str, value: int) -> None:
     """Set a kernel tuning knob (see b64_config_set in include/b64hip.h)."""
     check(lib().b64_config_set(key.encode(), int(value)), f"config_set {key}")
+
+
+def ntt_plan(log_len: int, skip_early: int, skip_late: int, inverse: bool = False, encode: bool = False,
+             n_cus: int = 0) -> list:
+    """The pass schedule the library runs for one transform (b64_ntt_plan): per
+    pass a dict of layer0, n_layers, low_bits, extra_bits, kernel (name) and the
+    work items of its grid on a device of n_cus compute units (0 = current)."""
+    from ._abi import NTT_KERNEL_NAMES, NttPassInfoC
+    n = C.c_uint32()
+    check(lib().b64_ntt_plan(log_len, skip_early, skip_late, int(inverse), int(encode), n_cus, None, 0,
+                             C.byref(n)), "ntt_plan")
+    arr = (NttPassInfoC * max(1, n.value))()
+    check(lib().b64_ntt_plan(log_len, skip_early, skip_late, int(inverse), int(encode), n_cus, arr, n.value,
+                             C.byref(n)), "ntt_plan")
+    return [{"layer0": p.layer0, "n_layers": p.n_layers, "low_bits": p.low_bits, "extra_bits": p.extra_bits,
+             "kernel": NTT_KERNEL_NAMES[p.kernel], "workitems": p.workitems} for p in arr[:n.value]]
 
 
 def config_get(key: str) -> int:

@@ -31,6 +31,21 @@ class FriParamsC(C.Structure):
     ]
 
 
+class NttPassInfoC(C.Structure):
+    _fields_ = [
+        ("layer0", C.c_uint32),
+        ("n_layers", C.c_uint32),
+        ("low_bits", C.c_uint32),
+        ("extra_bits", C.c_uint32),
+        ("kernel", C.c_uint32),
+        ("workitems", C.c_uint64),
+    ]
+
+
+NTT_KERNEL_NAMES = {0: "k_ntt_pass", 1: "k_ntt_pass_tab", 2: "k_ntt_pass_tab", 3: "k_ntt_pass_tab_w4",
+                    4: "k_ntt_pass_r16"}
+
+
 class ServerConfigC(C.Structure):
     _fields_ = [
         ("hbm_budget_bytes", C.c_size_t),
@@ -89,6 +104,8 @@ SIGNATURES = {
     "b64_domain_subspace": (C.c_int, [vp, C.c_uint32, u64p]),
     "b64_domain_twiddle": (C.c_int, [vp, C.c_uint32, C.c_uint64, u64p]),
     "b64_ntt_forward": (C.c_int, [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp]),
+    "b64_ntt_plan": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_uint32,
+                               C.POINTER(NttPassInfoC), C.c_uint32, C.POINTER(C.c_uint32)]),
     "b64_ntt_inverse": (C.c_int, [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp]),
     "b64_rs_encode_batch": (C.c_int, [vp, C.c_uint32, C.c_uint32, vp, C.c_uint32, vp, vp]),
     "b64_merkle_commit": (C.c_int, [vp, C.c_uint64, C.c_uint64, vp, u8p, vp]),

@@ -319,7 +319,32 @@ static int ntt_common(const b64_domain *dc, void *d_data, uint32_t log_len, uint
     if (se + sl > log_len || log_len - sl > (uint32_t)d->dim || log_len > 40) return B64_ERR_INVALID_ARGS;
     b64::DeviceDomain dd;
     B64_TRY(d->device_view(dd));
-    HIP_TRY(b64::launch_ntt(dd, (uint4 *)d_data, (int)log_len, (int)se, (int)sl, inverse, as_stream(stream)));
+    const hipError_t e =
+        b64::launch_ntt(dd, (uint4 *)d_data, (int)log_len, (int)se, (int)sl, inverse, as_stream(stream));
+    if (e == hipErrorInvalidConfiguration) return B64_ERR_INVALID_ARGS;  // a pass grid too large
+    HIP_TRY(e);
+    return B64_OK;
+}
+
+int b64_ntt_plan(uint32_t log_len, uint32_t se, uint32_t sl, int inverse, int encode, uint32_t n_cus,
+                 b64_ntt_pass_info *passes, uint32_t cap, uint32_t *n_passes) {
+    if (!n_passes || (cap && !passes) || se + sl > log_len || log_len > 40 || (encode && inverse))
+        return B64_ERR_INVALID_ARGS;
+    int starts[64], ks[64], cs[64], es[64];
+    const int n = b64::ntt_pass_plan((int)log_len, (int)se, (int)sl, encode != 0, starts, ks, cs, es);
+    *n_passes = (uint32_t)n;
+    if ((uint32_t)n > cap) return cap ? B64_ERR_BUFFER_TOO_SMALL : B64_OK;
+    const b64::NttTuning &tu = b64::ntt_tuning_for((int)log_len);
+    const uint32_t cus = n_cus ? n_cus : b64::ntt_device_cus();
+    for (int p = 0; p < n; p++) {
+        b64_ntt_pass_info &o = passes[p];
+        o.layer0 = (uint32_t)starts[p];
+        o.n_layers = (uint32_t)ks[p];
+        o.low_bits = (uint32_t)cs[p];
+        o.extra_bits = (uint32_t)es[p];
+        o.kernel = (uint32_t)b64::ntt_pass_kernel(tu, (int)log_len, starts[p], ks[p], cs[p], es[p], inverse != 0);
+        o.workitems = b64::ntt_pass_workitems((int)o.kernel, (int)log_len, starts[p], ks[p], cs[p], es[p], cus);
+    }
     return B64_OK;
 }
 
@@ -347,8 +372,10 @@ int b64_rs_encode_batch(const b64_domain *dc, uint32_t log_dim, uint32_t lir, co
         return B64_OK;
     }
     // the first NTT pass gathers the bit-reversed, replicated message itself
-    HIP_TRY(b64::launch_ntt(dd, (uint4 *)d_out, (int)(m + lir), (int)lir, (int)log_batch, false, s,
-                            (const uint4 *)d_msg, (int)m));
+    const hipError_t e = b64::launch_ntt(dd, (uint4 *)d_out, (int)(m + lir), (int)lir, (int)log_batch, false, s,
+                                         (const uint4 *)d_msg, (int)m);
+    if (e == hipErrorInvalidConfiguration) return B64_ERR_INVALID_ARGS;  // a pass grid too large
+    HIP_TRY(e);
     return B64_OK;
 }
 
@@ -521,10 +548,13 @@ int b64_fri_params_choose(const b64_domain *d, uint32_t log_msg_len, uint32_t se
     uint32_t term = cap < lir ? lir : cap;
     if (term > rs_log_len) term = rs_log_len;
     const uint32_t q = (rs_log_len - term) / arity, rem = (rs_log_len - term) % arity;
+    // the reference's list has no length limit; one that does not fit the
+    // fixed-size array would be a different protocol, so it is refused
+    if (q + (rem != 0) > B64_MAX_FOLD_ARITIES) return B64_ERR_INVALID_ARGS;
     uint32_t ar[B64_MAX_FOLD_ARITIES];
     uint32_t n = 0;
-    for (uint32_t i = 0; i < q && n < B64_MAX_FOLD_ARITIES; i++) ar[n++] = arity;
-    if (rem && n < B64_MAX_FOLD_ARITIES) ar[n++] = rem;
+    for (uint32_t i = 0; i < q; i++) ar[n++] = arity;
+    if (rem) ar[n++] = rem;
     return b64_fri_params_new(log_dim, lir, lbs, ar, n, nq, out);
 }
 
@@ -1033,3 +1063,4 @@ int b64_fri_prover_finish_proof(b64_fri_prover *fp, b64_transcript *t) {
 }
 
 }  // extern "C"
+

@@ -123,6 +123,11 @@ int fri_fold_round_enqueue(b64_fri_prover *fp, int *committed, const uint8_t **r
 // A pass that gathers from the message (the first forward pass of an encode,
 // msg_first) keeps e = 0.
 int ntt_pass_plan(int log_d, int skip_early, int skip_late, bool msg_first, int *layer0, int *k, int *c, int *e);
+// Kernel (B64_NTT_KERNEL_*) of one pass of that schedule, and the work items
+// (blocks x threads) of its grid on a device of `cus` compute units.
+int ntt_pass_kernel(const NttTuning &tu, int log_d, int layer0, int k, int c, int e, bool inverse);
+uint64_t ntt_pass_workitems(int kernel, int log_d, int layer0, int k, int c, int e, uint32_t cus);
+uint32_t ntt_device_cus();
 // One launch of `kind` over n_jobs jobs: d_jobs is a device array of the
 // kind's args, d_start[j] the first block of job j, `blocks` the total.
 // fold_nc: for FoldCodeword, the n_chal (1..4) every job shares, else 0.

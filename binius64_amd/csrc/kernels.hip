@@ -436,6 +436,7 @@ static void launch_ntt_tab_w4(const NttPassArgs &a, uint32_t n_tiles, int tab_gr
 // bit-reversed message index).
 constexpr int R16_BLOCK = 1024;
 constexpr int R16_MIN_LB = 10;
+constexpr int R16_MAX_LB = 32;  // lane offsets within a hi group are 32-bit
 constexpr int R16_TABLES = 15;
 struct alignas(256) R16Smem {
     uint4 tabs[R16_TABLES * WIN_ENTRIES];
@@ -509,15 +510,20 @@ static uint32_t device_cus() {
 
 // One block per CU (the tables take 120 KiB); positions per block a power of
 // two that divides 2^lb, so a block never leaves its hi group.
-static void launch_ntt_r16(const NttPassArgs &a, hipStream_t s) {
-    const int lb = a.log_d - a.layer0 - 4;
-    const int total_log = a.log_d - 4;
+static int r16_per_log(int log_d, int layer0, uint32_t cus) {
+    const int lb = log_d - layer0 - 4;
+    const int total_log = log_d - 4;
     int cus_log = 0;
-    while ((1u << cus_log) < device_cus()) cus_log++;
+    while ((1u << cus_log) < cus) cus_log++;
     int per_log = total_log - cus_log;
     if (per_log < R16_MIN_LB) per_log = R16_MIN_LB;
     if (per_log > lb) per_log = lb;
-    const uint32_t blocks = 1u << (total_log - per_log);
+    return per_log;
+}
+
+static void launch_ntt_r16(const NttPassArgs &a, hipStream_t s) {
+    const int per_log = r16_per_log(a.log_d, a.layer0, device_cus());
+    const uint32_t blocks = 1u << (a.log_d - 4 - per_log);
     if (a.msg)
         k_ntt_pass_r16<true><<<blocks, R16_BLOCK, 0, s>>>(a, 1ull << per_log);
     else
@@ -525,16 +531,38 @@ static void launch_ntt_r16(const NttPassArgs &a, hipStream_t s) {
 }
 
 // The deepest layer of a pass has the fewest butterflies per twiddle, 2^c.
-template <bool INVERSE>
-static void launch_ntt_tab(const NttPassArgs &a, uint32_t n_tiles, const NttTuning &tu, hipStream_t s) {
-    if ((1 << a.c) >= a.tab_min) {
-        if (tu.waves == 4 && a.k + a.c + a.e <= NTT_W4_TILE_LOG)
-            launch_ntt_tab_w4<INVERSE>(a, n_tiles, tu.tab_group, s);
-        else
-            launch_ntt_tab_h<INVERSE, false>(a, n_tiles, tu.tab_group, s);
-    } else {
-        launch_ntt_tab_h<INVERSE, true>(a, n_tiles, tu.tab_group, s);
+// Kernel of one pass.  The register pass addresses a hi group's elements with
+// 32-bit lane offsets below 2^lb, so it takes lb <= R16_MAX_LB only.
+int ntt_pass_kernel(const NttTuning &tu, int log_d, int layer0, int k, int c, int e, bool inverse) {
+    const int lb = log_d - layer0 - k;
+    if (tu.variant == 2 && !inverse && k == 4 && e == 0 && lb >= R16_MIN_LB && lb <= R16_MAX_LB)
+        return B64_NTT_KERNEL_R16;
+    if (tu.variant >= 1) {
+        if ((1 << c) >= tu.tab_min)
+            return (tu.waves == 4 && k + c + e <= NTT_W4_TILE_LOG) ? B64_NTT_KERNEL_TAB_W4 : B64_NTT_KERNEL_TAB;
+        return B64_NTT_KERNEL_TAB_HYBRID;
     }
+    return B64_NTT_KERNEL_GENERAL;
+}
+
+// Work items (blocks x threads) of that pass's grid; HIP launches take fewer
+// than 2^32.
+uint64_t ntt_pass_workitems(int kernel, int log_d, int layer0, int k, int c, int e, uint32_t cus) {
+    if (kernel == B64_NTT_KERNEL_R16)
+        return (1ull << (log_d - 4 - r16_per_log(log_d, layer0, cus))) * (uint64_t)R16_BLOCK;
+    return (1ull << (log_d - k - c - e)) * (uint64_t)BLOCK;
+}
+
+uint32_t ntt_device_cus() { return device_cus(); }
+
+template <bool INVERSE>
+static void launch_ntt_tab(const NttPassArgs &a, uint32_t n_tiles, int kernel, const NttTuning &tu, hipStream_t s) {
+    if (kernel == B64_NTT_KERNEL_TAB_W4)
+        launch_ntt_tab_w4<INVERSE>(a, n_tiles, tu.tab_group, s);
+    else if (kernel == B64_NTT_KERNEL_TAB)
+        launch_ntt_tab_h<INVERSE, false>(a, n_tiles, tu.tab_group, s);
+    else
+        launch_ntt_tab_h<INVERSE, true>(a, n_tiles, tu.tab_group, s);
 }
 
 
@@ -580,6 +608,12 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
     int starts[64], ks[64], cs[64], es[64];
     const NttTuning &tu = ntt_tuning_for(log_d);
     const int n_passes = ntt_pass_plan(log_d, skip_early, skip_late, msg != nullptr, starts, ks, cs, es);
+    // refuse the whole transform before the first launch if any pass's grid is too large
+    for (int p = 0; p < n_passes; p++) {
+        const int kernel = ntt_pass_kernel(tu, log_d, starts[p], ks[p], cs[p], es[p], inverse);
+        if (ntt_pass_workitems(kernel, log_d, starts[p], ks[p], cs[p], es[p], device_cus()) >> 32)
+            return hipErrorInvalidConfiguration;
+    }
     for (int q = 0; q < n_passes; q++) {
         const int p = inverse ? (n_passes - 1 - q) : q;
         NttPassArgs a;
@@ -594,13 +628,14 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
         a.e = es[p];
         a.tab_min = tu.tab_min;
         const uint64_t n_tiles = 1ull << (log_d - a.k - a.c - a.e);
-        if (tu.variant == 2 && !inverse && a.k == 4 && a.e == 0 && log_d - a.layer0 - 4 >= R16_MIN_LB) {
+        const int kernel = ntt_pass_kernel(tu, log_d, a.layer0, a.k, a.c, a.e, inverse);
+        if (kernel == B64_NTT_KERNEL_R16) {
             launch_ntt_r16(a, s);
-        } else if (tu.variant >= 1) {
+        } else if (kernel != B64_NTT_KERNEL_GENERAL) {
             if (inverse)
-                launch_ntt_tab<true>(a, (unsigned)n_tiles, tu, s);
+                launch_ntt_tab<true>(a, (unsigned)n_tiles, kernel, tu, s);
             else
-                launch_ntt_tab<false>(a, (unsigned)n_tiles, tu, s);
+                launch_ntt_tab<false>(a, (unsigned)n_tiles, kernel, tu, s);
         } else {
             if (inverse)
                 k_ntt_pass<true><<<(unsigned)n_tiles, BLOCK, 0, s>>>(a);

@@ -138,6 +138,27 @@ int b64_ntt_forward(const b64_domain *d, void *d_data, uint32_t log_len, uint32_
 int b64_ntt_inverse(const b64_domain *d, void *d_data, uint32_t log_len, uint32_t skip_early,
                     uint32_t skip_late, void *stream);
 
+/* No counterpart in the reference: the pass schedule the library runs for a
+ * transform of 2^log_len elements over layers skip_early .. log_len - skip_late
+ * (encode != 0: the first pass gathers from the message, as in
+ * b64_rs_encode_batch), for the autotuner and the benches.  Per pass: its first
+ * layer, layer count, the low index bits held in a tile, the index bits above
+ * the flipped ones held in a tile, the kernel, and the work items of its grid on
+ * a device of n_cus compute units (0 = the current device).  A forward or
+ * inverse transform whose plan has a pass of 2^32 or more work items is refused
+ * with B64_ERR_INVALID_ARGS before anything is launched. */
+#define B64_NTT_KERNEL_GENERAL 0    /* LDS tile, general product in every layer */
+#define B64_NTT_KERNEL_TAB 1        /* LDS tile, window tables in every layer */
+#define B64_NTT_KERNEL_TAB_HYBRID 2 /* LDS tile, tables where a twiddle covers >= ntt.tab_min butterflies */
+#define B64_NTT_KERNEL_TAB_W4 3     /* as TAB, 2^10-element tiles, at most 128 VGPRs */
+#define B64_NTT_KERNEL_R16 4        /* 4 layers in registers, 15 resident tables */
+typedef struct {
+    uint32_t layer0, n_layers, low_bits, extra_bits, kernel;
+    uint64_t workitems;
+} b64_ntt_pass_info;
+int b64_ntt_plan(uint32_t log_len, uint32_t skip_early, uint32_t skip_late, int inverse, int encode, uint32_t n_cus,
+                 b64_ntt_pass_info *passes, uint32_t cap, uint32_t *n_passes);
+
 /* ---- ReedSolomonCode::encode_batch (verifier/math/src/reed_solomon.rs:128-201) -- */
 /* The code is ReedSolomonCode::with_domain_context_subspace(d, log_dim, log_inv_rate)
  * (reed_solomon.rs:54-65): SUBSPACE_DIMENSION_MISMATCH if log_dim+log_inv_rate
@@ -176,7 +197,8 @@ uint32_t b64_estimate_optimal_arity(uint32_t log_block_length, uint32_t digest_s
 int b64_fri_params_new(uint32_t log_dim, uint32_t log_inv_rate, uint32_t log_batch_size,
                        const uint32_t *fold_arities, uint32_t n_fold_arities, uint32_t n_test_queries,
                        b64_fri_params *out);
-/* FRIParams::choose_with_constant_fold_arity (:64-115) for an NTT over `d`. */
+/* FRIParams::choose_with_constant_fold_arity (:64-115) for an NTT over `d`.
+ * INVALID_ARGS when the arity list would exceed B64_MAX_FOLD_ARITIES entries. */
 int b64_fri_params_choose(const b64_domain *d, uint32_t log_msg_len, uint32_t security_bits,
                           uint32_t log_inv_rate, uint32_t arity, b64_fri_params *out);
 /* Verifier::setup + Prover::setup parameter chain (verifier/verifier/src/verify.rs:61-106,

@@ -120,3 +120,66 @@ def test_device_calls_report_no_device_or_run(b64):
     with pytest.raises(b64.B64Error) as e:
         b64.DeviceBuffer(64)
     assert e.value.code == 20
+
+
+KNOBS = ("variant", "max_k", "tile_log", "tab_min", "tab_group", "waves")
+
+
+@pytest.fixture
+def four_layer_register_schedule(b64):
+    """Per-size schedule 'variant 2, 4 layers per pass' for log_d 20..40 (the
+    register pass's schedule), restored afterwards."""
+    saved = {d: {k: b64.config_get(f"ntt.{k}:{d}") for k in KNOBS} for d in range(20, 41)}
+    for d in range(20, 41):
+        b64.config_set(f"ntt.variant:{d}", 2)
+        b64.config_set(f"ntt.max_k:{d}", 4)
+        b64.config_set(f"ntt.tile_log:{d}", 11)
+    yield
+    for d, knobs in saved.items():
+        for k, v in knobs.items():
+            b64.config_set(f"ntt.{k}:{d}", v)
+
+
+@pytest.mark.parametrize("log_d", list(range(20, 41)))
+def test_ntt_plan_register_pass_bounds(b64, four_layer_register_schedule, log_d):
+    """The register-resident pass addresses a hi group with 32-bit lane offsets,
+    so the planner gives it only passes whose flipped bits start at index bit
+    10..32 (checked on a 256-CU device)."""
+    for encode in (False, True):
+        plan = b64.ntt_plan(log_d, 1, 4, encode=encode, n_cus=256)
+        assert sum(p["n_layers"] for p in plan) == log_d - 5
+        at = 1
+        for p in plan:
+            assert p["layer0"] == at
+            at += p["n_layers"]
+            lb = log_d - p["layer0"] - p["n_layers"]
+            if p["kernel"] == "k_ntt_pass_r16":
+                assert p["n_layers"] == 4 and p["extra_bits"] == 0 and 10 <= lb <= 32
+        kinds = [p["kernel"] for p in plan]
+        lbs = [log_d - p["layer0"] - p["n_layers"] for p in plan]
+        assert all(k == "k_ntt_pass_r16" for k, lb, p in zip(kinds, lbs, plan)
+                   if 10 <= lb <= 32 and p["n_layers"] == 4 and p["extra_bits"] == 0)
+        if log_d >= 38:
+            # the first pass flips bits above index bit 32: not the register pass
+            assert lbs[0] > 32 and kinds[0] != "k_ntt_pass_r16"
+
+
+def test_ntt_plan_large_transform_refused_without_launch(b64):
+    """A transform whose pass grids exceed what HIP launches is an argument
+    error, reported before anything runs (on a device-less host the call stops
+    at the device check, so only the planner is exercised here)."""
+    plan = b64.ntt_plan(40, 0, 0, n_cus=256)
+    assert max(p["workitems"] for p in plan) >= 1 << 32
+    small = b64.ntt_plan(24, 1, 4, n_cus=256)
+    assert max(p["workitems"] for p in small) < 1 << 32
+
+
+def test_fri_params_choose_refuses_too_many_arities(b64):
+    dom = b64.DomainContext.standard(48)
+    # arity 1, rate 1/2: rs_log_len = log_msg_len, terminal dim 8 (232 queries),
+    # so log_msg_len - 8 fold rounds: 32 fit, 33 do not
+    p = b64.FRIParams.choose_with_constant_fold_arity(dom, 40, 96, 1, 1)
+    assert p.fold_arities == [1] * 32
+    with pytest.raises(b64.B64Error) as e:
+        b64.FRIParams.choose_with_constant_fold_arity(dom, 41, 96, 1, 1)
+    assert e.value.code == 1

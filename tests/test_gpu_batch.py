@@ -104,3 +104,29 @@ def test_single_job_plan(b64, orc, n, use_graph):
         job.roots_buf[:] = 0
         plan.run()
         assert job.roots() == want, rep
+
+
+def test_config4_mix_matches_oracle(b64, orc):
+    """The batching workload itself: 64 jobs, log_n ~ U{12..20} drawn with seed 4,
+    uniform random messages, FRI challenges seeded by job id -- one hipGraph
+    plan; every job's roots equal the oracle's FRIFoldProver roots
+    (prover/prover/src/fri/fold.rs:117-175 with those challenges) and the plain
+    per-job launches'."""
+    rng = np.random.default_rng(4)
+    sizes = [int(x) for x in rng.integers(12, 21, size=64)]
+    say("config-4 sizes", sizes)
+    jobs, inputs = [], []
+    for jid, n in enumerate(sizes):
+        msg = np.random.default_rng(1000 + jid).integers(0, 2 ** 63, size=(1 << n, 2), dtype=np.uint64)
+        chal = np.random.default_rng(jid).integers(0, 2 ** 63, size=(n, 2), dtype=np.uint64)
+        jobs.append(b64.CommitJob(n, msg, chal))
+        inputs.append((msg, chal))
+    plan = b64.BatchPlan(jobs, use_graph=True)
+    plan.run()
+    batched = [j.roots() for j in jobs]
+    for j in jobs:
+        j.roots_buf[:] = 0
+        j.run()
+    assert [j.roots() for j in jobs] == batched
+    for jid, (n, (msg, chal)) in enumerate(zip(sizes, inputs)):
+        assert batched[jid] == oracle_job(orc, n, msg, chal)["roots"], f"job {jid} (n={n})"

@@ -94,3 +94,47 @@ def test_concurrent_submitters(b64, orc):
         srv.close()
     for i, n in enumerate(sizes):
         assert results[i] == alone(b64, n, *inputs[i])
+
+
+def test_config5_stream_matches_oracle(b64, orc):
+    """The serving workload itself, first 48 requests of the seed-5 stream:
+    witnesses of 2^14..2^22 64-bit words (log-uniform), packed two words per
+    B128 element by pack_witness on the device (prover/prover/src/prove.rs:286-327,
+    n = 13..21), submitted at their Poisson arrival times (mean 200/s).  Every
+    job's roots equal the oracle's FRIFoldProver roots with the job's challenges
+    (prover/prover/src/fri/fold.rs:117-175) and the roots of the job run alone."""
+    import time
+
+    n_stream, n_jobs = 400, 48
+    rng = np.random.default_rng(5)
+    log_words = [int(w) for w in rng.integers(14, 23, size=n_stream)][:n_jobs]
+    arrivals = np.cumsum(rng.exponential(1.0 / 200.0, size=n_stream))[:n_jobs]
+    sizes = [w - 1 for w in log_words]
+    print("config-5 sizes", sizes, flush=True)
+    msgs, chals = [], []
+    for i, lw in enumerate(log_words):
+        words = np.random.default_rng(20_000 + i).integers(0, 2 ** 64, size=1 << lw, dtype=np.uint64)
+        packed = b64.pack_witness(b64.DeviceBuffer.from_host(words), 1 << lw, lw - 1).to_host(1 << (lw - 1))
+        assert np.array_equal(packed, orc.pack_witness(words)), f"pack_witness job {i}"
+        msgs.append(packed)
+        chals.append(np.random.default_rng(10_000 + i).integers(0, 2 ** 63, size=(lw - 1, 2), dtype=np.uint64))
+    srv = b64.Server(hbm_budget_bytes=8 << 30, max_batch=64, n_streams=3)
+    try:
+        tickets, t0 = [], time.perf_counter()
+        for i, n in enumerate(sizes):
+            dt = t0 + arrivals[i] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            tickets.append(srv.submit(n, msgs[i], chals[i]))
+        got = [srv.wait(t, b64.CommitJob.n_roots_for(n))[0] for t, n in zip(tickets, sizes)]
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert st["jobs_done"] == n_jobs
+    for i, n in enumerate(sizes):
+        p = orc.fri_params_setup(n, 1, 96)
+        dom = orc.Domain(orc.Domain(dim=n + 1).subspace(p["rs_log_len"]))
+        want = orc.fri_prove(dom, msgs[i], p["log_dim"], 1, p["log_batch_size"], p["fold_arities"], 1,
+                             challenges=chals[i], threads=8)["roots"]
+        assert got[i] == want, f"job {i} (n={n}) != oracle"
+        assert got[i] == alone(b64, n, msgs[i], chals[i]), f"job {i} (n={n}) != alone"
