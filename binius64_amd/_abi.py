@@ -143,6 +143,7 @@ SIGNATURES = {
     "b64_fri_prover_oracle": (C.c_int, [vp, C.c_uint32, C.POINTER(vp), u32p, C.POINTER(vp)]),
     "b64_pack_witness": (C.c_int, [vp, C.c_uint64, C.c_uint32, vp, vp]),
     "b64_ring_switch_fold_1b_rows": (C.c_int, [vp, vp, C.c_uint32, vp, vp]),
+    "b64_ring_switch_fold_1b_rows_noalloc": (C.c_int, [vp, vp, C.c_uint32, vp, vp]),
     "b64_ring_switch_fold_elems": (C.c_int, [vp, C.c_uint32, u64p, vp]),
     "b64_eq_ind_partial_eval": (C.c_int, [u64p, C.c_uint32, vp, vp]),
     "b64_sumcheck_create": (C.c_int, [vp, vp, C.c_uint32, u64p, vp, C.POINTER(vp)]),

@@ -1064,3 +1064,52 @@ int b64_fri_prover_finish_proof(b64_fri_prover *fp, b64_transcript *t) {
 
 }  // extern "C"
 
+// ===========================================================================
+// Ring-switch fold without allocation.  The partial sums of
+// k_rs_fold_1b_rows live in `scratch`, a file-scope device array; host code
+// reaches it only through the per-device address hipGetSymbolAddress returns
+// (the array's name on the host is a stand-in, not a device address).
+// ===========================================================================
+namespace {
+
+static __device__ __align__(256) uint4 scratch[(size_t)B64_RS_SCRATCH_BLOCKS * 128];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint4 *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// The current device's copy of `scratch`, looked up once per device with that
+// device current.
+int scratch_for_current_device(uint4 **out) {
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return B64_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return B64_ERR_HIP;
+        g_scratch_ptr[dev] = (uint4 *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return B64_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int b64_ring_switch_fold_1b_rows_noalloc(const void *d_mat, const void *d_vec, uint32_t log_len, void *d_out,
+                                         void *stream) {
+    if (!d_mat || !d_vec || !d_out || log_len > 40) return B64_ERR_INVALID_ARGS;
+    if (!device_ok()) return B64_ERR_NO_DEVICE;
+    const uint64_t n = 1ull << log_len;
+    if (b64::rs_fold_1b_rows_blocks(n) > B64_RS_SCRATCH_BLOCKS) return B64_ERR_BUFFER_TOO_SMALL;
+    uint4 *partial = nullptr;
+    B64_TRY(scratch_for_current_device(&partial));
+    HIP_TRY(b64::launch_rs_fold_1b_rows((const uint4 *)d_mat, (const uint4 *)d_vec, n, partial, (uint4 *)d_out,
+                                        as_stream(stream)));
+    return B64_OK;
+}
+
+}  // extern "C"

@@ -352,6 +352,20 @@ int b64_pack_witness(const void *d_words, uint64_t n_words, uint32_t log_elems, 
  * rows r < 2^log_len of d_vec[r] * bit b of d_mat[r], bit b of the LE u128. */
 int b64_ring_switch_fold_1b_rows(const void *d_mat, const void *d_vec, uint32_t log_len, void *d_out,
                                  void *stream);
+/* No counterpart in the reference: b64_ring_switch_fold_1b_rows for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops that
+ * forbid hipMalloc).  Same result; its per-block partial sums live in a file-scope
+ * device array of B64_RS_SCRATCH_BLOCKS * 128 elements (4 MiB), held by every GPU
+ * the library runs on until the process ends, and the call allocates nothing.
+ * Calls on one GPU share that array: two calls that overlap in time on one GPU
+ * (different streams, different host threads) race and must be ordered by the
+ * caller.  Every block writes all 128 of its partial sums before they are read,
+ * so no clearing is needed.  log_len > 40 is rejected (B64_ERR_INVALID_ARGS); a
+ * launch needing more blocks than the array holds is rejected with
+ * B64_ERR_BUFFER_TOO_SMALL (no log_len <= 40 does: blocks are capped at 2048). */
+#define B64_RS_SCRATCH_BLOCKS 2048
+int b64_ring_switch_fold_1b_rows_noalloc(const void *d_mat, const void *d_vec, uint32_t log_len, void *d_out,
+                                         void *stream);
 /* fold_b128_elems_inplace (ring_switch.rs:130-176), in place on 2^log_len elements:
  * x <- sum over b of bit b of x times vec[b]; vec is 128 host elements. */
 int b64_ring_switch_fold_elems(void *d_elems, uint32_t log_len, const uint64_t *vec, void *stream);
