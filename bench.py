@@ -40,25 +40,9 @@ def traffic_entry(log_cw, tuning):
     return TRAFFIC.get((log_cw, tuning["variant"], tuning["max_k"], tuning["tile_log"])) or {}
 
 
-def ntt_pass_kernels(log_d, skip_early, skip_late, tuning):
-    """Kernels of a forward NTT's passes, as the library plans them: the
-    layer range split evenly into passes of at most max_k layers; with
-    variant 2, 4-layer passes whose flipped bits start at index bit 10 or
-    above run in k_ntt_pass_r16, the others in k_ntt_pass_tab (k_ntt_pass for
-    variant 0)."""
-    first, end = skip_early, log_d - skip_late
-    n = end - first
-    max_k = tuning["max_k"]
-    n_passes = (n + max_k - 1) // max_k
-    names, at = [], first
-    for p in range(n_passes):
-        k = (end - at + (n_passes - p) - 1) // (n_passes - p)
-        lb = log_d - at - k
-        if tuning["variant"] == 2 and k == 4 and lb >= 10:
-            names.append("k_ntt_pass_r16")
-        else:
-            names.append("k_ntt_pass_tab" if tuning["variant"] >= 1 else "k_ntt_pass")
-        at += k
+def ntt_pass_kernels(b64, log_d, skip_early, skip_late):
+    """Kernels of a forward NTT's passes as the library plans them (b64_ntt_plan)."""
+    names = [p["kernel"] for p in b64.ntt_plan(log_d, skip_early, skip_late)]
     return " + ".join(f"{nm} x{names.count(nm)}" for nm in dict.fromkeys(names))
 
 
@@ -174,9 +158,8 @@ def main():
     n_layers = rs_log_len - lir  # layers lir .. log_cw - lbs
     tuning = {k: b64.config_get(f"ntt.{k}:{log_cw}")
               for k in ("variant", "max_k", "tile_log", "tab_min", "tab_group", "waves")}
-    max_k = tuning["max_k"]
-    n_passes = (n_layers + max_k - 1) // max_k  # the library's balanced pass split
-    kernel = ntt_pass_kernels(log_cw, lir, lbs, tuning)
+    n_passes = len(b64.ntt_plan(log_cw, lir, lbs))
+    kernel = ntt_pass_kernels(b64, log_cw, lir, lbs)
     stages = {
         "encode_ms": timed(lambda: b64.check(L.b64_rs_encode_batch(dom.ptr, log_dim, lir, msg.data_ptr(), lbs,
                                                                    cw.data_ptr(), sp))),

@@ -43,7 +43,7 @@ class NttPassInfoC(C.Structure):
 
 
 NTT_KERNEL_NAMES = {0: "k_ntt_pass", 1: "k_ntt_pass_tab", 2: "k_ntt_pass_tab", 3: "k_ntt_pass_tab_w4",
-                    4: "k_ntt_pass_r16"}
+                    4: "k_ntt_pass_r16", 5: "k_ntt_pass_t8", 6: "k_ntt_pass_r16w8", 7: "k_ntt_pass_t4u"}
 
 
 class ServerConfigC(C.Structure):
@@ -187,10 +187,12 @@ TUNING_PATH = Path(__file__).resolve().parent / "ntt_tuning.json"
 
 
 def _apply_tuning(L) -> None:
-    """Per-size NTT schedules measured by binius64_amd.autotune, if present."""
+    """Per-size NTT schedules measured by binius64_amd.autotune, if present:
+    single-job transforms and batch plans."""
     if TUNING_PATH.exists():
         from .autotune import apply_table, load_table
         apply_table(load_table(TUNING_PATH), L)
+        apply_table(load_table(TUNING_PATH, "batch_sizes"), L, prefix="batch")
 
 
 class B64Error(RuntimeError):

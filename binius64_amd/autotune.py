@@ -41,27 +41,40 @@ def default_candidates() -> List[Candidate]:
     c += [(1, 3, 11, 32, 2, 0), (1, 3, 11, 64, 1, 0), (1, 3, 10, 32, 2, 4)]
     # variant 2: 4-layer forward passes above index bit 10 in registers
     c += [(2, 4, 11, 128, 2, 0), (2, 4, 11, 64, 2, 0), (2, 4, 10, 128, 2, 0), (2, 4, 11, 64, 2, 4)]
+    # variants 3 and 4: 2 KiB-table register passes down to index bit 8, and the
+    # remaining forward passes on LDS tiles through 2 KiB (tab_group 2) or 1 KiB
+    # all-at-once (tab_group 1) tables
+    for v in (3, 4):
+        for tile in (9, 10, 11):
+            for tab_min in (8, 16, 32):
+                c.append((v, 4, tile, tab_min, 2, 0))
+            c.append((v, 4, tile, 16, 1, 0))
+    c += [(3, 5, 10, 16, 1, 0), (3, 3, 9, 8, 1, 0), (3, 5, 11, 16, 2, 0)]
     return c
 
 
-def apply_table(table: Dict[int, Candidate], lib=None) -> None:
-    """Install per-size overrides (log_d -> candidate) in the loaded library."""
+def apply_table(table: Dict[int, Candidate], lib=None, prefix: str = "ntt") -> None:
+    """Install per-size overrides (log_d -> candidate) in the loaded library:
+    prefix "ntt" for single-job transforms, "batch" for batch plans."""
     from ._abi import check
     from ._abi import lib as _lib
     L = lib if lib is not None else _lib()
-    check(L.b64_config_set(b"ntt.clear_sizes", 0), "config_set ntt.clear_sizes")
+    check(L.b64_config_set(f"{prefix}.clear_sizes".encode(), 0), f"config_set {prefix}.clear_sizes")
     for log_d, cand in table.items():
         for key, v in zip(KNOBS, cand):
+            key = prefix + key[3:]
             check(L.b64_config_set(f"{key}:{int(log_d)}".encode(), int(v)), f"config_set {key}:{log_d}")
 
 
-def load_table(path: Path = DEFAULT_PATH) -> Dict[int, Candidate]:
+def load_table(path: Path = DEFAULT_PATH, section: str = "sizes") -> Dict[int, Candidate]:
+    """The per-size table of one section ("sizes": single-job, "batch_sizes":
+    batch plans) of a tuning file."""
     data = json.loads(Path(path).read_text())
-    knobs = data.get("knobs", list(KNOBS))
+    knobs = [k.split(".", 1)[1] for k in data.get("knobs" if section == "sizes" else "batch_knobs", list(KNOBS))]
     out = {}
-    for k, v in data["sizes"].items():
+    for k, v in data.get(section, {}).items():
         cfg = dict(zip(knobs, v["config"]))
-        out[int(k)] = tuple(int(cfg.get(knob, 0)) for knob in KNOBS)
+        out[int(k)] = tuple(int(cfg.get(knob.split(".", 1)[1], 0)) for knob in KNOBS)
     return out
 
 
@@ -132,6 +145,12 @@ def main(argv=None):
     doc = {"device": "MI355X (gfx950)", "knobs": list(KNOBS),
            "what": "per transform size log_d = log_n + 1 of a commit job's encode: best NTT schedule",
            "sizes": {str(k): v for k, v in sorted(res.items())}}
+    out = Path(args.out)
+    if out.exists():  # keep the batch-plan table of the file being replaced
+        prev = json.loads(out.read_text())
+        for key in ("batch_knobs", "batch_sizes"):
+            if key in prev:
+                doc[key] = prev[key]
     Path(args.out).write_text(json.dumps(doc, indent=1) + "\n")
     print(json.dumps({"written": args.out, "sizes": len(res)}), flush=True)
 

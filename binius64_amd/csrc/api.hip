@@ -180,6 +180,22 @@ static bool parse_tuning_key(const char *key, char *field, size_t cap, int *log_
     return true;
 }
 
+// Splits "ntt.<knob>" / "batch.<knob>" into the table it addresses (single-job
+// or batch-plan schedules) and the knob name.
+static bool tuning_table(const char *field, bool *batch, const char **knob) {
+    if (!strncmp(field, "ntt.", 4)) {
+        *batch = false;
+        *knob = field + 4;
+        return true;
+    }
+    if (!strncmp(field, "batch.", 6)) {
+        *batch = true;
+        *knob = field + 6;
+        return true;
+    }
+    return false;
+}
+
 int b64_config_set(const char *key, int64_t value) {
     if (!key) return B64_ERR_INVALID_ARGS;
     if (!strcmp(key, "ntt.clear_sizes")) {
@@ -187,35 +203,43 @@ int b64_config_set(const char *key, int64_t value) {
         for (auto &t : b64::g_ntt_tuning_size) t.variant = -1;
         return B64_OK;
     }
+    if (!strcmp(key, "batch.clear_sizes")) {
+        (void)b64::batch_tuning_for(0);
+        for (auto &t : b64::g_batch_tuning_size) t.variant = -1;
+        return B64_OK;
+    }
     char field[32];
     int log_d;
-    if (!parse_tuning_key(key, field, sizeof field, &log_d)) return B64_ERR_INVALID_ARGS;
-    b64::NttTuning t = b64::ntt_tuning_for(log_d);
-    if (!strcmp(field, "ntt.variant")) {
-        if (value < 0 || value > 2) return B64_ERR_INVALID_ARGS;
+    bool batch;
+    const char *knob;
+    if (!parse_tuning_key(key, field, sizeof field, &log_d) || !tuning_table(field, &batch, &knob))
+        return B64_ERR_INVALID_ARGS;
+    b64::NttTuning t = batch ? b64::batch_tuning_for(log_d) : b64::ntt_tuning_for(log_d);
+    if (!strcmp(knob, "variant")) {
+        if (value < 0 || value > (batch ? 1 : 4)) return B64_ERR_INVALID_ARGS;
         t.variant = (int)value;
-    } else if (!strcmp(field, "ntt.max_k")) {
+    } else if (!strcmp(knob, "max_k")) {
         if (value < 1 || value > 7) return B64_ERR_INVALID_ARGS;
         t.max_k = (int)value;
-    } else if (!strcmp(field, "ntt.tile_log")) {
+    } else if (!strcmp(knob, "tile_log")) {
         if (value < 1 || value > 11) return B64_ERR_INVALID_ARGS;
         t.tile_log = (int)value;
-    } else if (!strcmp(field, "ntt.tab_min")) {
+    } else if (!strcmp(knob, "tab_min")) {
         if (value < 1 || value > (1 << 30)) return B64_ERR_INVALID_ARGS;
         t.tab_min = (int)value;
-    } else if (!strcmp(field, "ntt.tab_group")) {
+    } else if (!strcmp(knob, "tab_group")) {
         if (value != 1 && value != 2 && value != 4 && value != 8) return B64_ERR_INVALID_ARGS;
         t.tab_group = (int)value;
-    } else if (!strcmp(field, "ntt.waves")) {
+    } else if (!strcmp(knob, "waves")) {
         if (value != 0 && value != 4) return B64_ERR_INVALID_ARGS;
         t.waves = (int)value;
     } else {
         return B64_ERR_INVALID_ARGS;
     }
     if (log_d < 0)
-        b64::g_ntt_tuning = t;
+        (batch ? b64::g_batch_tuning : b64::g_ntt_tuning) = t;
     else
-        b64::g_ntt_tuning_size[log_d] = t;
+        (batch ? b64::g_batch_tuning_size : b64::g_ntt_tuning_size)[log_d] = t;
     return B64_OK;
 }
 
@@ -223,14 +247,17 @@ int b64_config_get(const char *key, int64_t *value) {
     if (!key || !value) return B64_ERR_INVALID_ARGS;
     char field[32];
     int log_d;
-    if (!parse_tuning_key(key, field, sizeof field, &log_d)) return B64_ERR_INVALID_ARGS;
-    const b64::NttTuning &t = b64::ntt_tuning_for(log_d);
-    if (!strcmp(field, "ntt.variant")) *value = t.variant;
-    else if (!strcmp(field, "ntt.max_k")) *value = t.max_k;
-    else if (!strcmp(field, "ntt.tile_log")) *value = t.tile_log;
-    else if (!strcmp(field, "ntt.tab_min")) *value = t.tab_min;
-    else if (!strcmp(field, "ntt.tab_group")) *value = t.tab_group;
-    else if (!strcmp(field, "ntt.waves")) *value = t.waves;
+    bool batch;
+    const char *knob;
+    if (!parse_tuning_key(key, field, sizeof field, &log_d) || !tuning_table(field, &batch, &knob))
+        return B64_ERR_INVALID_ARGS;
+    const b64::NttTuning &t = batch ? b64::batch_tuning_for(log_d) : b64::ntt_tuning_for(log_d);
+    if (!strcmp(knob, "variant")) *value = t.variant;
+    else if (!strcmp(knob, "max_k")) *value = t.max_k;
+    else if (!strcmp(knob, "tile_log")) *value = t.tile_log;
+    else if (!strcmp(knob, "tab_min")) *value = t.tab_min;
+    else if (!strcmp(knob, "tab_group")) *value = t.tab_group;
+    else if (!strcmp(knob, "waves")) *value = t.waves;
     else return B64_ERR_INVALID_ARGS;
     return B64_OK;
 }
@@ -331,10 +358,10 @@ int b64_ntt_plan(uint32_t log_len, uint32_t se, uint32_t sl, int inverse, int en
     if (!n_passes || (cap && !passes) || se + sl > log_len || log_len > 40 || (encode && inverse))
         return B64_ERR_INVALID_ARGS;
     int starts[64], ks[64], cs[64], es[64];
-    const int n = b64::ntt_pass_plan((int)log_len, (int)se, (int)sl, encode != 0, starts, ks, cs, es);
+    const b64::NttTuning &tu = b64::ntt_tuning_for((int)log_len);
+    const int n = b64::ntt_pass_plan(tu, (int)log_len, (int)se, (int)sl, encode != 0, starts, ks, cs, es);
     *n_passes = (uint32_t)n;
     if ((uint32_t)n > cap) return cap ? B64_ERR_BUFFER_TOO_SMALL : B64_OK;
-    const b64::NttTuning &tu = b64::ntt_tuning_for((int)log_len);
     const uint32_t cus = n_cus ? n_cus : b64::ntt_device_cus();
     for (int p = 0; p < n; p++) {
         b64_ntt_pass_info &o = passes[p];

@@ -151,7 +151,8 @@ int job_stages(const b64_job_desc &jd, const Layout &L, std::vector<Stage> &out,
     const int m = (int)jd.log_msg_len;
     // NTT passes, the first gathering the bit-reversed message
     int starts[64], ks[64], cs[64], es[64];
-    const int np = b64::ntt_pass_plan(log_d, 1, L.lbs, true, starts, ks, cs, es);
+    const b64::NttTuning &tu = b64::batch_tuning_for(log_d);
+    const int np = b64::ntt_pass_plan(tu, log_d, 1, L.lbs, true, starts, ks, cs, es);
     if (np > 16) return B64_ERR_INVALID_ARGS;
     for (int q = 0; q < np; q++) {
         Stage st;
@@ -168,7 +169,6 @@ int job_stages(const b64_job_desc &jd, const Layout &L, std::vector<Stage> &out,
         st.ntt.e = es[q];
         // the batched pass kernel is the table kernel; variant 0 maps to
         // "no layer qualifies for a table"
-        const b64::NttTuning &tu = b64::ntt_tuning_for(log_d);
         st.ntt.tab_min = tu.variant == 0 ? (1 << 30) : tu.tab_min;
         st.blocks = (uint32_t)(1ull << (log_d - ks[q] - cs[q] - es[q]));
         out.push_back(st);

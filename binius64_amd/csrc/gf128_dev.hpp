@@ -212,6 +212,134 @@ __device__ __forceinline__ uint4 win_mul_perm(uint32_t tab_base, uint4 v) {
     return acc;
 }
 
+// ---- 2 KiB window tables with pre-shifted rows -------------------------------
+// Row s (0..7) of the table of a constant t holds t * n * x^(4s) mod P for the 16
+// nibbles n, entry n at slot n ^ s (8 rows x 16 entries x 16 B = 2 KiB).  A 32-bit
+// word w of v is the sum of its nibbles times x^(4s), so w * t is the XOR of 8
+// row entries, and word q of v contributes that sum times x^(32q): word-aligned,
+// so the 32 lookups of a product accumulate into 7 unreduced words and one
+// reduction of the 96-bit high part finishes it (x^128 = 1 + x + x^2 + x^7 and
+// the folded high part stays below x^103, so a single fold suffices).  A table is
+// a quarter of the 32-row form's size and is built from one shift and three
+// doublings per row, so it pays for constants shared by as few as 16 products.
+constexpr int W8_ROWS = 8;
+constexpr int W8_ENTRIES = W8_ROWS * 16;  // uint4 entries per table (2 KiB)
+
+// v * x^k for 0 <= k < 32.
+__device__ __forceinline__ uint4 gf_mul_xk(uint4 v, uint32_t k) {
+    const uint32_t r = 32u - k;  // 1..32; alignbit takes r & 31, so k = 0 is selected below
+    const uint32_t o = v.w >> (r & 31u);  // the k bits shifted out (garbage for k = 0)
+    uint32_t w0 = v.x << k, w1 = __builtin_amdgcn_alignbit(v.y, v.x, r);
+    uint32_t w2 = __builtin_amdgcn_alignbit(v.z, v.y, r), w3 = __builtin_amdgcn_alignbit(v.w, v.z, r);
+    // o * (1 + x + x^2 + x^7): at most 38 bits
+    w0 ^= xor3(o, o << 1, o << 2) ^ (o << 7);
+    w1 ^= xor3(o >> 31, o >> 30, o >> 25);
+    return k ? make_uint4(w0, w1, w2, w3) : v;
+}
+
+// Builds entries 8 * half .. 8 * half + 7 of row `row` of t's table (16 lanes,
+// two per row, build a whole table).
+__device__ __forceinline__ void w8_build_half(uint4 *__restrict__ tab, uint4 t, int row, int half) {
+    const uint4 p0 = gf_mul_xk(t, 4u * (uint32_t)row);
+    const uint4 p1 = gf_mul_x(p0);
+    const uint4 p2 = gf_mul_x(p1);
+    const uint4 p3 = gf_mul_x(p2);
+    const uint4 z = make_uint4(0, 0, 0, 0);
+    const uint4 b = half ? p3 : z;  // entries 8..15 add the x^3 term
+    const uint4 e1 = gxor(b, p0), e2 = gxor(b, p1), e4 = gxor(b, p2);
+    const uint4 e3 = gxor(e1, p1), e5 = gxor(e4, p0), e6 = gxor(e4, p1), e7 = gxor(e5, p1);
+    uint4 *r = tab + row * 16 + 8 * half;
+    r[0 ^ row] = b;
+    r[1 ^ row] = e1;
+    r[2 ^ row] = e2;
+    r[3 ^ row] = e3;
+    r[4 ^ row] = e4;
+    r[5 ^ row] = e5;
+    r[6 ^ row] = e6;
+    r[7 ^ row] = e7;
+}
+
+// v * t through t's 2 KiB table at LDS address tab_base (256-byte aligned).
+__device__ __forceinline__ uint4 w8_mul(uint32_t tab_base, uint4 v) {
+    uint32_t acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint32_t w = vw[q] ^ 0x76543210u;  // nibble s -> slot nib ^ s of row s
+        const uint32_t lo = (w << 4) & 0xF0F0F0F0u;  // nibbles 0,2,4,6 * 16
+        const uint32_t hi = w & 0xF0F0F0F0u;         // nibbles 1,3,5,7 * 16
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t sel = 0x07060500u | (uint32_t)k;
+            const lds_cu32x4 *p0 = (const lds_cu32x4 *)(uintptr_t)__builtin_amdgcn_perm(tab_base, lo, sel);
+            const lds_cu32x4 *p1 = (const lds_cu32x4 *)(uintptr_t)__builtin_amdgcn_perm(tab_base, hi, sel);
+            const u32x4 e0 = p0[(2 * k) * 16], e1 = p1[(2 * k + 1) * 16];
+            acc[q + 0] = xor3(acc[q + 0], e0.x, e1.x);
+            acc[q + 1] = xor3(acc[q + 1], e0.y, e1.y);
+            acc[q + 2] = xor3(acc[q + 2], e0.z, e1.z);
+            acc[q + 3] = xor3(acc[q + 3], e0.w, e1.w);
+        }
+    }
+    // fold the high words h = acc[4..6] (below x^96): r = lo ^ h * (1 + x + x^2 + x^7)
+    const uint32_t h0 = acc[4], h1 = acc[5], h2 = acc[6];
+    const uint32_t r0 = xor3(xor3(acc[0], h0, h0 << 1), h0 << 2, h0 << 7);
+    const uint32_t r1 = xor3(xor3(acc[1], h1, __builtin_amdgcn_alignbit(h1, h0, 31)),
+                             __builtin_amdgcn_alignbit(h1, h0, 30), __builtin_amdgcn_alignbit(h1, h0, 25));
+    const uint32_t r2 = xor3(xor3(acc[2], h2, __builtin_amdgcn_alignbit(h2, h1, 31)),
+                             __builtin_amdgcn_alignbit(h2, h1, 30), __builtin_amdgcn_alignbit(h2, h1, 25));
+    const uint32_t r3 = xor3(xor3(acc[3], h2 >> 31, h2 >> 30), h2 >> 25, 0u);
+    return make_uint4(r0, r1, r2, r3);
+}
+
+// ---- 1 KiB window tables: the first four rows of the 2 KiB form -------------
+// Rows 0..3 (t * n * x^(4s), same layout, built by w8_build_half with row < 4).
+// Nibbles 0..3 of word q of v index rows 0..3 at word offset q, nibbles 4..7 the
+// same rows 16 bits further: two word-aligned accumulators, one 16-bit funnel
+// to merge them, and a full reduction of the 255-bit product.
+constexpr int W4_ROWS = 4;
+constexpr int W4_ENTRIES = W4_ROWS * 16;  // uint4 entries per table (1 KiB)
+
+__device__ __forceinline__ uint4 w4_mul(uint32_t tab_base, uint4 v) {
+    uint32_t A[7] = {0, 0, 0, 0, 0, 0, 0}, B[7] = {0, 0, 0, 0, 0, 0, 0};
+    const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint32_t w = vw[q] ^ 0x32103210u;  // nibble j -> slot nib ^ (j & 3)
+        const uint32_t lo = (w << 4) & 0xF0F0F0F0u;  // nibbles 0,2,4,6 * 16: rows 0,2,0,2
+        const uint32_t hi = w & 0xF0F0F0F0u;         // nibbles 1,3,5,7 * 16: rows 1,3,1,3
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t sel = 0x07060500u | (uint32_t)k;
+            const lds_cu32x4 *p0 = (const lds_cu32x4 *)(uintptr_t)__builtin_amdgcn_perm(tab_base, lo, sel);
+            const lds_cu32x4 *p1 = (const lds_cu32x4 *)(uintptr_t)__builtin_amdgcn_perm(tab_base, hi, sel);
+            const u32x4 e0 = p0[(2 * (k & 1)) * 16], e1 = p1[(2 * (k & 1) + 1) * 16];
+            uint32_t *acc = k < 2 ? A : B;
+            acc[q + 0] = xor3(acc[q + 0], e0.x, e1.x);
+            acc[q + 1] = xor3(acc[q + 1], e0.y, e1.y);
+            acc[q + 2] = xor3(acc[q + 2], e0.z, e1.z);
+            acc[q + 3] = xor3(acc[q + 3], e0.w, e1.w);
+        }
+    }
+    // u = A + B * x^16 (8 words, degree <= 254)
+    uint32_t u[8];
+    u[0] = A[0] ^ (B[0] << 16);
+#pragma unroll
+    for (int i = 1; i < 7; i++) u[i] = A[i] ^ __builtin_amdgcn_alignbit(B[i], B[i - 1], 16);
+    u[7] = B[6] >> 16;
+    // fold h = u[4..7] (below x^127): h * (1 + x + x^2 + x^7), whose bits at
+    // x^128 and up (g, below x^7) fold once more
+    const uint32_t h0 = u[4], h1 = u[5], h2 = u[6], h3 = u[7];
+    const uint32_t g = xor3(h3 >> 31, h3 >> 30, h3 >> 25);
+    const uint32_t r0 = xor3(xor3(u[0], h0, h0 << 1), xor3(h0 << 2, h0 << 7, g), xor3(g << 1, g << 2, g << 7));
+    const uint32_t r1 = xor3(xor3(u[1], h1, __builtin_amdgcn_alignbit(h1, h0, 31)),
+                             __builtin_amdgcn_alignbit(h1, h0, 30), __builtin_amdgcn_alignbit(h1, h0, 25));
+    const uint32_t r2 = xor3(xor3(u[2], h2, __builtin_amdgcn_alignbit(h2, h1, 31)),
+                             __builtin_amdgcn_alignbit(h2, h1, 30), __builtin_amdgcn_alignbit(h2, h1, 25));
+    const uint32_t r3 = xor3(xor3(u[3], h3, __builtin_amdgcn_alignbit(h3, h2, 31)),
+                             __builtin_amdgcn_alignbit(h3, h2, 30), __builtin_amdgcn_alignbit(h3, h2, 25));
+    return make_uint4(r0, r1, r2, r3);
+}
+
 // v * c using c's window table.
 __device__ __forceinline__ uint4 win_mul(const uint4 *__restrict__ tab, uint4 v) {
     uint4 acc = make_uint4(0, 0, 0, 0);

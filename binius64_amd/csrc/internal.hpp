@@ -84,10 +84,13 @@ enum class MultiKind { NttPass, Leaf16Fold, LeafHash, Subtree, FoldCodeword };
 
 struct NttTuning {
     int variant;   // 0 = general product, 1 = per-layer choice (window tables where amortised),
-                   // 2 = as 1, with forward 4-layer passes above index bit 10 in registers
+                   // 2 = as 1, with forward 4-layer passes above index bit 10 in registers,
+                   // 3 = as 2, forward passes at bits 8..9 in registers and the remaining
+                   //     forward passes on LDS tiles, both through 2 KiB tables,
+                   // 4 = as 3, every register pass through 2 KiB tables
     int max_k;     // layers per pass
     int tile_log;  // log2 elements per LDS tile
-    int tab_min;   // variant 1: butterflies per twiddle (in a tile) needed to build a table
+    int tab_min;   // variants >= 1: butterflies per twiddle (in a tile) needed to build a table
     int tab_group; // variant 1: window tables resident in LDS at once (1, 2, 4 or 8; 8 KiB each)
     int waves;     // tables-only passes: 4 = kernel limited to 128 VGPRs (4 waves per SIMD), 0 = default
 };
@@ -95,6 +98,12 @@ extern NttTuning g_ntt_tuning;  // defaults
 // Per transform size (log_d) overrides; variant < 0 marks an unset entry.
 extern NttTuning g_ntt_tuning_size[MAX_DIM + 1];
 const NttTuning &ntt_tuning_for(int log_d);
+// Schedules of the NTT passes in batch plans (the multi-job table kernel; only
+// max_k, tile_log, tab_min, tab_group and variant 0 = no tables are read), kept
+// apart from the single-job schedules ("batch.<knob>[:<log_d>]").
+extern NttTuning g_batch_tuning;
+extern NttTuning g_batch_tuning_size[MAX_DIM + 1];
+const NttTuning &batch_tuning_for(int log_d);
 
 // Kernel launchers.  All return hipError_t of the launch.
 hipError_t launch_gf_mul(const uint4 *a, const uint4 *b, uint4 *out, uint64_t n, hipStream_t s);
@@ -122,7 +131,8 @@ int fri_fold_round_enqueue(b64_fri_prover *fp, int *committed, const uint8_t **r
 // Pass schedule of a forward NTT (layer0, k, c, e per pass); returns the count.
 // A pass that gathers from the message (the first forward pass of an encode,
 // msg_first) keeps e = 0.
-int ntt_pass_plan(int log_d, int skip_early, int skip_late, bool msg_first, int *layer0, int *k, int *c, int *e);
+int ntt_pass_plan(const NttTuning &tu, int log_d, int skip_early, int skip_late, bool msg_first, int *layer0, int *k,
+                  int *c, int *e);
 // Kernel (B64_NTT_KERNEL_*) of one pass of that schedule, and the work items
 // (blocks x threads) of its grid on a device of `cus` compute units.
 int ntt_pass_kernel(const NttTuning &tu, int log_d, int layer0, int k, int c, int e, bool inverse);

@@ -99,6 +99,19 @@ const NttTuning &ntt_tuning_for(int log_d) {
     return g_ntt_tuning;
 }
 
+NttTuning g_batch_tuning = {1, 4, NTT_TILE_LOG, 128, 2, 0};
+NttTuning g_batch_tuning_size[MAX_DIM + 1] = {};
+
+const NttTuning &batch_tuning_for(int log_d) {
+    static bool init = false;
+    if (!init) {
+        for (auto &t : g_batch_tuning_size) t.variant = -1;
+        init = true;
+    }
+    if (log_d >= 0 && log_d <= MAX_DIM && g_batch_tuning_size[log_d].variant >= 0) return g_batch_tuning_size[log_d];
+    return g_batch_tuning;
+}
+
 
 __device__ __forceinline__ uint32_t rev_bits(uint32_t x, int b) { return b ? (__brev(x) >> (32 - b)) : 0u; }
 
@@ -224,7 +237,7 @@ __device__ __forceinline__ void ntt_store_tile(const uint4 *__restrict__ s, cons
 // a layer's twiddles and is summed once per layer, with the loads of a layer
 // issued together.  Needs k + e <= NTT_MAX_K.  Ends with a barrier.
 constexpr int NTT_TW = (1 << NTT_MAX_K) - 1;
-__device__ __forceinline__ int ntt_tw_base(int j, int e) { return ((1 << j) - 1) << e; }
+__host__ __device__ __forceinline__ int ntt_tw_base(int j, int e) { return ((1 << j) - 1) << e; }
 
 __device__ __forceinline__ void ntt_tile_twiddles(const NttPassArgs &a, uint64_t hi, uint4 *__restrict__ tw,
                                                   uint4 *__restrict__ thi) {
@@ -531,12 +544,222 @@ static void launch_ntt_r16(const NttPassArgs &a, hipStream_t s) {
 }
 
 // The deepest layer of a pass has the fewest butterflies per twiddle, 2^c.
+// ---- passes through 2 KiB tables (variants 3 and 4) ---------------------------
+// LDS-tile pass for the last layers of a transform, where a twiddle covers only
+// 16..128 butterflies: every twiddle of a layer in the tile gets a 2 KiB table
+// (w8_build_half, all threads of the block building at once, up to T8_GROUP
+// tables per round), then every butterfly of the layer runs through them.
+// Layers with fewer than tab_min butterflies per twiddle use the general product.
+constexpr int T8_GROUP = 16;  // tables per build round (32 KiB)
+template <int TL>
+struct alignas(256) NttT8Smem {
+    uint4 tabs[T8_GROUP * W8_ENTRIES];
+    uint4 s[1 << TL];
+    uint4 tw[NTT_TW];
+    uint4 thi[NTT_MAX_K];
+};
+
+// Variant with every table of the tile built in one round: 1 KiB tables (rows
+// 0..3, w4_mul) at the twiddle's index in tw, for tiles with at most T4U_TABLES
+// twiddles (4 layers with e <= 1).
+constexpr int T4U_TABLES = 30;
+template <int TL>
+struct alignas(256) NttT4USmem {
+    uint4 tabs[T4U_TABLES * W4_ENTRIES];
+    uint4 s[1 << TL];
+    uint4 tw[T4U_TABLES];
+    uint4 thi[NTT_MAX_K];
+};
+
+template <int TL>
+__global__ __launch_bounds__(BLOCK) void k_ntt_pass_t4u(NttPassArgs a) {
+    __shared__ NttT4USmem<TL> sm;
+    const NttTile T = ntt_tile_at(a, blockIdx.x);
+    ntt_load_tile(sm.s, a, T);
+    ntt_tile_twiddles(a, T.hi, sm.tw, sm.thi);  // ends with a barrier
+    uint4 *__restrict__ s = sm.s;
+    const uint32_t tabs = lds_addr(sm.tabs);
+    const int k = a.k, c = a.c, n_el = T.n_el;
+    const int n_tab = ntt_tw_base(k, a.e);
+    for (int r = threadIdx.x; r < 8 * n_tab; r += BLOCK)
+        w8_build_half(sm.tabs + (r >> 3) * W4_ENTRIES, sm.tw[r >> 3], (r >> 1) & 3, r & 1);
+    lds_barrier();
+    for (int j = 0; j < k; j++) {
+        const int tw0 = ntt_tw_base(j, a.e);
+        const int pos = c + (k - 1 - j);
+        const int per_blk = 1 << pos;
+        const int lowmask = per_blk - 1;
+        for (int P = threadIdx.x; P < (n_el >> 1); P += BLOCK) {
+            const int L0 = ((P >> pos) << (pos + 1)) | (P & lowmask);
+            const int L1 = L0 | per_blk;
+            uint4 u = s[L0], v = s[L1];
+            u = gxor(u, w4_mul(tabs + (uint32_t)((tw0 + (P >> pos)) * W4_ENTRIES * 16), v));
+            v = gxor(v, u);
+            s[L0] = u;
+            s[L1] = v;
+        }
+        lds_barrier();
+    }
+    ntt_store_tile(sm.s, a, T);
+}
+
+template <int TL>
+__global__ __launch_bounds__(BLOCK) void k_ntt_pass_t8(NttPassArgs a) {
+    __shared__ NttT8Smem<TL> sm;
+    const NttTile T = ntt_tile_at(a, blockIdx.x);
+    ntt_load_tile(sm.s, a, T);
+    ntt_tile_twiddles(a, T.hi, sm.tw, sm.thi);  // ends with a barrier
+    uint4 *__restrict__ s = sm.s;
+    const uint32_t tabs = lds_addr(sm.tabs);
+    const int k = a.k, c = a.c, n_el = T.n_el;
+    for (int j = 0; j < k; j++) {
+        const uint4 *__restrict__ tw = sm.tw + ntt_tw_base(j, a.e);
+        const int pos = c + (k - 1 - j);
+        const int per_blk = 1 << pos;
+        const int lowmask = per_blk - 1;
+        const int nblk = 1 << (j + a.e);
+        if (per_blk < a.tab_min) {
+            for (int P = threadIdx.x; P < (n_el >> 1); P += BLOCK) {
+                const int L0 = ((P >> pos) << (pos + 1)) | (P & lowmask);
+                const int L1 = L0 | per_blk;
+                uint4 u = s[L0], v = s[L1];
+                u = gxor(u, gf_mul(v, tw[P >> pos]));
+                v = gxor(v, u);
+                s[L0] = u;
+                s[L1] = v;
+            }
+            lds_barrier();
+            continue;
+        }
+        for (int g0 = 0; g0 < nblk; g0 += T8_GROUP) {
+            const int ng = nblk - g0 < T8_GROUP ? nblk - g0 : T8_GROUP;
+            for (int r = threadIdx.x; r < 16 * ng; r += BLOCK)
+                w8_build_half(sm.tabs + (r >> 4) * W8_ENTRIES, tw[g0 + (r >> 4)], (r >> 1) & 7, r & 1);
+            lds_barrier();
+            const int n_bf = ng * per_blk;
+            for (int q = threadIdx.x; q < n_bf; q += BLOCK) {
+                const int P = (g0 << pos) + q;
+                const int L0 = ((P >> pos) << (pos + 1)) | (P & lowmask);
+                const int L1 = L0 | per_blk;
+                uint4 u = s[L0], v = s[L1];
+                u = gxor(u, w8_mul(tabs + (uint32_t)((q >> pos) * W8_ENTRIES * 16), v));
+                v = gxor(v, u);
+                s[L0] = u;
+                s[L1] = v;
+            }
+            lds_barrier();
+        }
+    }
+    ntt_store_tile(sm.s, a, T);
+}
+
+// Four-layer register pass through 2 KiB tables: as k_ntt_pass_r16, with 256-thread
+// blocks and the 15 tables in 30 KiB, so hi groups of 256 positions (lb >= 8)
+// qualify and several blocks share a CU.
+constexpr int R8_BLOCK = 256;
+constexpr int R8_MIN_LB = 8;
+struct alignas(256) R8Smem {
+    uint4 tabs[R16_TABLES * W8_ENTRIES];
+    uint4 tw[NTT_TW];
+    uint4 thi[NTT_MAX_K];
+};
+
+template <bool MSG>
+__global__ __launch_bounds__(R8_BLOCK) void k_ntt_pass_r16w8(NttPassArgs a, uint64_t per_block) {
+    __shared__ R8Smem sm;
+    const int lb = a.log_d - a.layer0 - 4;
+    const uint64_t p0 = (uint64_t)blockIdx.x * per_block;
+    const uint64_t hi = p0 >> lb;
+    ntt_tile_twiddles(a, hi, sm.tw, sm.thi);  // a.k = 4, a.e = 0; ends with a barrier
+    for (int r = threadIdx.x; r < R16_TABLES * 16; r += R8_BLOCK)
+        w8_build_half(sm.tabs + (r >> 4) * W8_ENTRIES, sm.tw[r >> 4], (r >> 1) & 7, r & 1);
+    lds_barrier();
+    const uint32_t tabs = lds_addr(sm.tabs);
+    const uint64_t lowmask = (1ull << lb) - 1;
+    for (uint64_t p = p0 + threadIdx.x; p < p0 + per_block; p += R8_BLOCK) {
+        const uint64_t low = p & lowmask;
+        uint4 x[16];
+        if (MSG) {
+            const int mb = a.m - 4;
+            const uint64_t run = mb ? (__brevll(low) >> (64 - mb)) << 4 : 0;
+#pragma unroll
+            for (int e = 0; e < 16; e++) x[e] = a.msg[run + rev_bits(e, 4)];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 16; e++) x[e] = (a.data + (hi << (lb + 4)) + ((uint64_t)e << lb))[(uint32_t)low];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int bit = 3 - j;
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                if (e & (1 << bit)) continue;
+                const int blk = e >> (bit + 1);
+                const uint32_t tb = tabs + (uint32_t)(((1 << j) - 1 + blk) * W8_ENTRIES * 16);
+                uint4 u = x[e], v = x[e | (1 << bit)];
+                u = gxor(u, w8_mul(tb, v));
+                v = gxor(v, u);
+                x[e] = u;
+                x[e | (1 << bit)] = v;
+            }
+        }
+        uint32_t low_st = (uint32_t)low;
+        asm volatile("" : "+v"(low_st));
+#pragma unroll
+        for (int e = 0; e < 16; e++) (a.data + (hi << (lb + 4)) + ((uint64_t)e << lb))[low_st] = x[e];
+    }
+}
+
+// Positions per r16w8 block: a power of two within the hi group, at least a
+// block's worth, aiming at four blocks per CU.
+static int r8_per_log(int log_d, int layer0, uint32_t cus) {
+    const int lb = log_d - layer0 - 4;
+    const int total_log = log_d - 4;
+    int cus_log = 0;
+    while ((1u << cus_log) < cus) cus_log++;
+    int per_log = total_log - cus_log - 2;
+    if (per_log < R8_MIN_LB) per_log = R8_MIN_LB;
+    if (per_log > lb) per_log = lb;
+    return per_log;
+}
+
+static void launch_ntt_r16w8(const NttPassArgs &a, hipStream_t s) {
+    const int per_log = r8_per_log(a.log_d, a.layer0, device_cus());
+    const uint32_t blocks = 1u << (a.log_d - 4 - per_log);
+    if (a.msg)
+        k_ntt_pass_r16w8<true><<<blocks, R8_BLOCK, 0, s>>>(a, 1ull << per_log);
+    else
+        k_ntt_pass_r16w8<false><<<blocks, R8_BLOCK, 0, s>>>(a, 1ull << per_log);
+}
+
+static void launch_ntt_t4u(const NttPassArgs &a, uint32_t n_tiles, hipStream_t s) {
+    if (a.k + a.c + a.e <= 9)
+        k_ntt_pass_t4u<9><<<n_tiles, BLOCK, 0, s>>>(a);
+    else
+        k_ntt_pass_t4u<10><<<n_tiles, BLOCK, 0, s>>>(a);
+}
+
+static void launch_ntt_t8(const NttPassArgs &a, uint32_t n_tiles, hipStream_t s) {
+    const int tl = a.k + a.c + a.e;
+    if (tl <= 9)
+        k_ntt_pass_t8<9><<<n_tiles, BLOCK, 0, s>>>(a);
+    else if (tl == 10)
+        k_ntt_pass_t8<10><<<n_tiles, BLOCK, 0, s>>>(a);
+    else
+        k_ntt_pass_t8<11><<<n_tiles, BLOCK, 0, s>>>(a);
+}
+
 // Kernel of one pass.  The register pass addresses a hi group's elements with
 // 32-bit lane offsets below 2^lb, so it takes lb <= R16_MAX_LB only.
 int ntt_pass_kernel(const NttTuning &tu, int log_d, int layer0, int k, int c, int e, bool inverse) {
     const int lb = log_d - layer0 - k;
-    if (tu.variant == 2 && !inverse && k == 4 && e == 0 && lb >= R16_MIN_LB && lb <= R16_MAX_LB)
-        return B64_NTT_KERNEL_R16;
+    const bool reg4 = !inverse && k == 4 && e == 0 && lb <= R16_MAX_LB;
+    if (reg4 && (tu.variant == 2 || tu.variant == 3) && lb >= R16_MIN_LB) return B64_NTT_KERNEL_R16;
+    if (reg4 && tu.variant >= 3 && lb >= R8_MIN_LB) return B64_NTT_KERNEL_R16W8;
+    if (tu.variant >= 3 && !inverse && k + c + e <= 11) {
+        if (tu.tab_group == 1 && ntt_tw_base(k, e) <= T4U_TABLES && k + c + e <= 10) return B64_NTT_KERNEL_T4U;
+        return B64_NTT_KERNEL_T8;
+    }
     if (tu.variant >= 1) {
         if ((1 << c) >= tu.tab_min)
             return (tu.waves == 4 && k + c + e <= NTT_W4_TILE_LOG) ? B64_NTT_KERNEL_TAB_W4 : B64_NTT_KERNEL_TAB;
@@ -550,6 +773,8 @@ int ntt_pass_kernel(const NttTuning &tu, int log_d, int layer0, int k, int c, in
 uint64_t ntt_pass_workitems(int kernel, int log_d, int layer0, int k, int c, int e, uint32_t cus) {
     if (kernel == B64_NTT_KERNEL_R16)
         return (1ull << (log_d - 4 - r16_per_log(log_d, layer0, cus))) * (uint64_t)R16_BLOCK;
+    if (kernel == B64_NTT_KERNEL_R16W8)
+        return (1ull << (log_d - 4 - r8_per_log(log_d, layer0, cus))) * (uint64_t)R8_BLOCK;
     return (1ull << (log_d - k - c - e)) * (uint64_t)BLOCK;
 }
 
@@ -568,12 +793,11 @@ static void launch_ntt_tab(const NttPassArgs &a, uint32_t n_tiles, int kernel, c
 
 constexpr int NTT_MIN_TILES_LOG = 10;
 
-int ntt_pass_plan(int log_d, int skip_early, int skip_late, bool msg_first, int *layer0, int *kk, int *cc,
-                  int *ee) {
+int ntt_pass_plan(const NttTuning &tu, int log_d, int skip_early, int skip_late, bool msg_first, int *layer0,
+                  int *kk, int *cc, int *ee) {
     const int first = skip_early, end = log_d - skip_late;
     const int n_layers = end - first;
     if (n_layers <= 0) return 0;
-    const NttTuning &tu = ntt_tuning_for(log_d);
     int max_k = tu.max_k;
     if (max_k < 1 || max_k > NTT_MAX_K) max_k = NTT_MAX_K;
     int tile_log = tu.tile_log;
@@ -607,7 +831,7 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
     if (msg && (inverse || skip_early != log_d - msg_log)) return hipErrorInvalidValue;
     int starts[64], ks[64], cs[64], es[64];
     const NttTuning &tu = ntt_tuning_for(log_d);
-    const int n_passes = ntt_pass_plan(log_d, skip_early, skip_late, msg != nullptr, starts, ks, cs, es);
+    const int n_passes = ntt_pass_plan(tu, log_d, skip_early, skip_late, msg != nullptr, starts, ks, cs, es);
     // refuse the whole transform before the first launch if any pass's grid is too large
     for (int p = 0; p < n_passes; p++) {
         const int kernel = ntt_pass_kernel(tu, log_d, starts[p], ks[p], cs[p], es[p], inverse);
@@ -631,6 +855,12 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
         const int kernel = ntt_pass_kernel(tu, log_d, a.layer0, a.k, a.c, a.e, inverse);
         if (kernel == B64_NTT_KERNEL_R16) {
             launch_ntt_r16(a, s);
+        } else if (kernel == B64_NTT_KERNEL_R16W8) {
+            launch_ntt_r16w8(a, s);
+        } else if (kernel == B64_NTT_KERNEL_T8) {
+            launch_ntt_t8(a, (unsigned)n_tiles, s);
+        } else if (kernel == B64_NTT_KERNEL_T4U) {
+            launch_ntt_t4u(a, (unsigned)n_tiles, s);
         } else if (kernel != B64_NTT_KERNEL_GENERAL) {
             if (inverse)
                 launch_ntt_tab<true>(a, (unsigned)n_tiles, kernel, tu, s);
@@ -1167,7 +1397,7 @@ hipError_t launch_multi(MultiKind kind, const void *d_jobs, const uint32_t *d_st
     if (blocks == 0 || n_jobs == 0) return hipSuccess;
     switch (kind) {
         case MultiKind::NttPass:
-            switch (g_ntt_tuning.tab_group) {
+            switch (g_batch_tuning.tab_group) {
                 case 8:
                     k_ntt_pass_tab_multi<8><<<blocks, BLOCK, 0, s>>>((const NttPassArgs *)d_jobs, d_start, n_jobs);
                     break;

@@ -136,11 +136,35 @@ struct b64_server {
         if (d_pool) (void)hipFree(d_pool);
     }
 
+    // A batch that cannot complete (its launch failed, or the device reported an
+    // error): wait until nothing of it is in flight on its stream, return its
+    // memory and free the slot without marking its jobs done; waiters get the
+    // error through worker_error (called with mu held).
+    void fail_slot(Slot &s, int rc) {
+        (void)hipStreamSynchronize(s.stream);
+        if (s.plan) {
+            b64_batch_plan_destroy(s.plan);
+            s.plan = nullptr;
+        }
+        for (auto &j : s.jobs) pool->release(j->off, j->footprint);
+        s.jobs.clear();
+        s.busy = false;
+        if (worker_error == B64_OK) worker_error = rc;
+        cv_done.notify_all();
+    }
+
     // Retire finished batches (called with mu held).
     bool retire() {
         bool any = false;
         for (Slot &s : slots) {
-            if (!s.busy || hipEventQuery(s.done_ev) != hipSuccess) continue;
+            if (!s.busy) continue;
+            const hipError_t q = hipEventQuery(s.done_ev);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) {
+                fail_slot(s, B64_ERR_HIP);
+                any = true;
+                continue;
+            }
             if (s.plan) {
                 const int rc = b64_batch_plan_collect(s.plan);
                 if (rc != B64_OK) worker_error = rc;
@@ -206,14 +230,14 @@ struct b64_server {
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
             bool progressed = retire();
+            // after an error no more work is admitted; waiters see worker_error
+            if (worker_error != B64_OK) queue.clear();
             if (Slot *slot = admit()) {
                 lk.unlock();
                 const int rc = launch(slot);
                 lk.lock();
-                if (rc != B64_OK) {
-                    worker_error = rc;
-                    cv_done.notify_all();
-                }
+                // the done event was not recorded: the slot must not be retired as finished
+                if (rc != B64_OK) fail_slot(*slot, rc);
                 progressed = true;
             }
             bool inflight = false;

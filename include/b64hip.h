@@ -94,12 +94,18 @@ int b64_stream_synchronize(void *stream);
  * interface).  Keys: "ntt.variant" (0 = general multiply, 1 = per layer, LDS
  * window tables where a twiddle covers at least "ntt.tab_min" butterflies of a
  * tile, 2 = as 1 with the forward 4-layer passes whose flipped bits start at
- * index bit 10 or above run in registers), "ntt.max_k" (layers per NTT pass, 1..7), "ntt.tile_log" (log2 elements
+ * index bit 10 or above run in registers, 3 = as 2 with the forward 4-layer
+ * passes at index bits 8..9 in registers through 2 KiB tables and the other
+ * forward passes on LDS tiles through 2 KiB tables -- or, with "ntt.tab_group"
+ * 1 and at most 30 twiddles per tile, 1 KiB tables all built in one round --,
+ * 4 = as 3 with every register pass through 2 KiB tables), "ntt.max_k" (layers per NTT pass, 1..7), "ntt.tile_log" (log2 elements
  * per LDS tile), "ntt.tab_group" (window tables resident in LDS at once: 1, 2,
  * 4 or 8), "ntt.waves" (4 = tables-only passes use a kernel limited to 128 VGPRs,
  * 0 = default).  "ntt.<knob>:<log_d>" sets the knob for transforms of size
  * 2^log_d only (the autotuner's per-size table); "ntt.clear_sizes" drops every
- * per-size setting.  Returns INVALID_ARGS for an unknown key or value.  Settings
+ * per-size setting.  "batch.<knob>[:<log_d>]" and "batch.clear_sizes" do the same
+ * for the NTT passes of batch plans (multi-job LDS-tile kernel: variant 0 or 1,
+ * max_k, tile_log, tab_min, tab_group).  Returns INVALID_ARGS for an unknown key or value.  Settings
  * are process-wide; change them only while no call is running. */
 int b64_config_set(const char *key, int64_t value);
 int b64_config_get(const char *key, int64_t *value);
@@ -152,6 +158,9 @@ int b64_ntt_inverse(const b64_domain *d, void *d_data, uint32_t log_len, uint32_
 #define B64_NTT_KERNEL_TAB_HYBRID 2 /* LDS tile, tables where a twiddle covers >= ntt.tab_min butterflies */
 #define B64_NTT_KERNEL_TAB_W4 3     /* as TAB, 2^10-element tiles, at most 128 VGPRs */
 #define B64_NTT_KERNEL_R16 4        /* 4 layers in registers, 15 resident tables */
+#define B64_NTT_KERNEL_T8 5         /* LDS tile, 2 KiB tables per twiddle where one covers >= ntt.tab_min butterflies */
+#define B64_NTT_KERNEL_R16W8 6      /* 4 layers in registers, 15 resident 2 KiB tables, 256-thread blocks */
+#define B64_NTT_KERNEL_T4U 7        /* LDS tile, every twiddle's 1 KiB table built in one round */
 typedef struct {
     uint32_t layer0, n_layers, low_bits, extra_bits, kernel;
     uint64_t workitems;

@@ -183,3 +183,26 @@ def test_fri_params_choose_refuses_too_many_arities(b64):
     with pytest.raises(b64.B64Error) as e:
         b64.FRIParams.choose_with_constant_fold_arity(dom, 41, 96, 1, 1)
     assert e.value.code == 1
+
+
+def test_batch_schedule_keys_are_separate(b64):
+    """Batch plans keep their own NTT schedule table ("batch.<knob>"), applied
+    from the tuning file's batch section; setting it leaves the single-job
+    schedule alone."""
+    from binius64_amd._abi import TUNING_PATH
+    from binius64_amd.autotune import KNOBS, load_table
+    single, batch = load_table(TUNING_PATH), load_table(TUNING_PATH, "batch_sizes")
+    assert single and batch
+    for d, cand in batch.items():
+        assert tuple(b64.config_get(f"batch.{k.split('.')[1]}:{d}") for k in KNOBS) == cand
+        assert cand[0] in (0, 1)  # the multi-job kernel: general product or window tables
+    d = 20
+    before = b64.config_get(f"ntt.tab_group:{d}")
+    old = b64.config_get(f"batch.tab_group:{d}")
+    b64.config_set(f"batch.tab_group:{d}", 8 if old != 8 else 4)
+    try:
+        assert b64.config_get(f"ntt.tab_group:{d}") == before
+    finally:
+        b64.config_set(f"batch.tab_group:{d}", old)
+    with pytest.raises(b64.B64Error):
+        b64.config_set(f"batch.variant:{d}", 2)  # no register passes in the multi-job kernel

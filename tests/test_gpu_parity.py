@@ -82,7 +82,9 @@ def test_ntt_forward_windows(b64, orc, log_d):
     (1, 1, 1, 1, 2, 0), (1, 5, 9, 64, 2, 0), (0, 2, 4, 1, 2, 0), (1, 4, 11, 32, 2, 0), (1, 7, 11, 1, 4, 0),
     (1, 6, 11, 16, 8, 0), (1, 5, 10, 8, 8, 0), (1, 3, 11, 1, 4, 0), (1, 5, 10, 16, 2, 4), (1, 4, 9, 1, 4, 4),
     (1, 7, 11, 1, 2, 4), (1, 3, 10, 1, 8, 4), (1, 4, 11, 64, 1, 0), (1, 7, 11, 1, 1, 0),
-    (1, 3, 6, 4, 1, 4), (1, 5, 10, 16, 1, 0), (2, 4, 11, 128, 2, 0), (2, 4, 6, 8, 2, 0)])
+    (1, 3, 6, 4, 1, 4), (1, 5, 10, 16, 1, 0), (2, 4, 11, 128, 2, 0), (2, 4, 6, 8, 2, 0),
+    (3, 4, 9, 16, 2, 0), (3, 4, 11, 8, 2, 0), (4, 4, 10, 16, 2, 0), (3, 7, 11, 1, 2, 0), (4, 3, 9, 1 << 20, 2, 0),
+    (4, 5, 10, 2, 2, 0), (3, 4, 9, 16, 1, 0), (4, 4, 10, 16, 1, 0), (3, 3, 8, 1, 1, 0)])
 def test_ntt_tuning_variants(b64, orc, variant, max_k, tile_log, tab_min, tab_group, waves):
     """Every multiply variant and pass/tile shape gives the same transform."""
     old = {k: b64.config_get(k) for k in ("ntt.variant", "ntt.max_k", "ntt.tile_log", "ntt.tab_min", "ntt.tab_group",
@@ -110,7 +112,10 @@ def test_ntt_tuning_variants(b64, orc, variant, max_k, tile_log, tab_min, tab_gr
 
 @pytest.mark.parametrize("variant,max_k,tile_log,tab_min,tab_group", [(1, 4, 6, 8, 2), (1, 7, 11, 128, 2),
                                                                     (1, 4, 11, 64, 1), (1, 5, 10, 64, 2),
-                                                                    (2, 4, 11, 128, 2), (2, 4, 10, 64, 2)])
+                                                                    (2, 4, 11, 128, 2), (2, 4, 10, 64, 2),
+                                                                    (3, 4, 9, 16, 2), (4, 4, 9, 16, 2),
+                                                                    (3, 4, 11, 8, 2), (4, 4, 10, 32, 2),
+                                                                    (3, 4, 9, 16, 1), (3, 4, 10, 16, 1)])
 def test_ntt_schedules_log_d20(b64, orc, variant, max_k, tile_log, tab_min, tab_group):
     """Schedules at a size where the last passes take extra index bits into
     their tiles, and where variant 2 runs the upper 4-layer passes in registers
@@ -134,9 +139,10 @@ def test_ntt_schedules_log_d20(b64, orc, variant, max_k, tile_log, tab_min, tab_
         _apply_tuning(b64.lib())
 
 
+@pytest.mark.parametrize("variant", [2, 3, 4])
 @pytest.mark.parametrize("log_dim,lir,lbs", [(16, 2, 0), (15, 1, 2), (14, 3, 1)])
-def test_encode_register_passes(b64, orc, log_dim, lir, lbs):
-    """Variant 2 on encodes whose message-gathering pass runs in registers, with
+def test_encode_register_passes(b64, orc, log_dim, lir, lbs, variant):
+    """Variants 2..4 on encodes whose message-gathering pass runs in registers, with
     rates 1/2..1/8 (the replicas are the hi groups of that pass) and batches
     of 1, 2 and 4 polynomials; a nonstandard basis for the first case."""
     log_d = log_dim + lir + lbs
@@ -148,9 +154,9 @@ def test_encode_register_passes(b64, orc, log_dim, lir, lbs):
     msg = orc.random_elems(4242 + log_d, 1 << (log_dim + lbs))
     try:
         for key, v in zip(("ntt.variant", "ntt.max_k", "ntt.tile_log", "ntt.tab_min", "ntt.tab_group"),
-                          (2, 4, 11, 64, 2)):
+                          (variant, 4, 11, 64, 2)):
             b64.config_set(f"{key}:{log_d}", v)
-        say("encode-r16", log_dim, lir, lbs)
+        say("encode-r16", variant, log_dim, lir, lbs)
         out = b64.ReedSolomonCode(dom, log_dim, lir).encode_batch(b64.DeviceBuffer.from_host(msg), lbs)
         assert np.array_equal(out.to_host(1 << log_d), orc.encode_batch(ref, msg, log_dim, lir, lbs, threads=8))
     finally:
