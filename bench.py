@@ -9,8 +9,8 @@ challenges.  Default log_n = 24 (256 MiB message, 512 MiB codeword).
 Prints one JSON line.  `value` is message GB/s (16 * 2^log_n bytes / step time),
 the convention of the reference's pcs/ntt benches.  `roofline` covers the
 dominant kernel (the NTT pass), measured live with HIP events on the stream the
-kernels run on.  `cpu_baseline` times the CPU oracle (a restatement of the
-reference's rayon algorithm) on the host cores on a bounded sample.
+kernels run on.  `cpu_baseline` times the reference's x86-64 commit path
+(restated in oracle/b64_cpu_baseline.c) on the host threads, on the same message.
 """
 from __future__ import annotations
 
@@ -26,18 +26,23 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-# Per launch of the forward-NTT pass kernels, from rocprofv3 --pmc runs of
-# `bench.py --counters` (profiles/ntt_pass_log_n24_counters.md): HBM bytes
-# (FETCH_SIZE + WRITE_SIZE, each scaled by the same counter on a 512 MiB device
-# copy) and VALU issue rate (SQ_INSTS_VALU per CU per clock), averaged over the
-# passes of one forward NTT.  Keyed by log2 codeword length and the schedule
-# (variant, layers per pass, tile size); the counters cannot be read from
-# inside the timed run, so other schedules report null.
-TRAFFIC = {(25, 2, 4, 11): {"bytes": 1074217312, "valu_issue_per_cu_clk": 0.72}}
+# From rocprofv3 --pmc runs of `bench.py --counters`
+# (profiles/commit_step_log_n24_counters.md), keyed by log2 codeword length and
+# the schedule (variant, max_k, tile_log, tab_min, tab_group); the counters cannot
+# be read inside the timed run, so other schedules report null.
+#  - per launch of the forward-NTT pass kernels: HBM bytes (FETCH_SIZE + WRITE_SIZE,
+#    each scaled by the same counter on a 512 MiB device copy) and VALU issue rate
+#    (SQ_INSTS_VALU per CU per clock), averaged over the passes of one forward NTT;
+#  - per commit step: the VALU wave-instructions of all its kernels.
+TRAFFIC = {(25, 3, 4, 9, 16, 1): {"bytes": 1074350499, "valu_issue_per_cu_clk": 0.715,
+                                  "step_valu_wave_insts": 1116929344}}
+# MI355X VALU issue peak: one wave64 instruction per CU per clock, 256 CUs, 2.4 GHz
+VALU_PEAK_WAVE_INSTS_PER_S = 256 * 2.4e9
 
 
 def traffic_entry(log_cw, tuning):
-    return TRAFFIC.get((log_cw, tuning["variant"], tuning["max_k"], tuning["tile_log"])) or {}
+    key = (log_cw, tuning["variant"], tuning["max_k"], tuning["tile_log"], tuning["tab_min"], tuning["tab_group"])
+    return TRAFFIC.get(key) or {}
 
 
 def ntt_pass_kernels(b64, log_d, skip_early, skip_late):
@@ -58,7 +63,6 @@ def parse():
     ap.add_argument("--no-ring-switch", action="store_true", help="skip the ring-switch folds leg")
     ap.add_argument("--no-pcs", action="store_true", help="skip the PCS evaluation-proof leg")
     ap.add_argument("--no-vision", action="store_true", help="skip the Vision Merkle-commit leg")
-    ap.add_argument("--cpu-log-n", type=int, default=24, help="CPU baseline sample size")
     ap.add_argument("--ntt", default=None, metavar="V,K,T,M,G[,W]",
                     help="NTT schedule for every size in this run (replaces the autotuned table): "
                          "variant, max_k, tile_log, tab_min, tab_group[, waves]")
@@ -108,14 +112,18 @@ def main():
     pc = params.to_c()
 
     if args.counters:
-        b64.check(L.b64_rs_encode_batch(dom.ptr, log_dim, lir, msg.data_ptr(), lbs, cw.data_ptr(), sp), "encode")
+        # one whole commit step (its kernels in launch order), one forward NTT of
+        # the codeword, then the byte-counter calibration copy
+        b64.check(L.b64_fri_commit_fold(C.byref(pc), dom.ptr, msg.data_ptr(), cw.data_ptr(), tree.data_ptr(),
+                                        chal_p, folded.data_ptr(), None, sp), "commit_fold")
         torch.cuda.synchronize()
         b64.check(L.b64_ntt_forward(dom.ptr, cw.data_ptr(), log_cw, lir, lbs, sp), "ntt_forward")
         torch.cuda.synchronize()
         cal = torch.empty((1 << 26,), dtype=torch.int64, device=dev)
         cal.copy_(cw[: 1 << 26])
         torch.cuda.synchronize()
-        print(json.dumps({"counters_workload": {"log_n": n, "ntt_launches": "encode then ntt_forward",
+        print(json.dumps({"counters_workload": {"log_n": n, "launches": "commit step, then ntt_forward",
+                                                "ntt_passes": [p["kernel"] for p in b64.ntt_plan(log_cw, lir, lbs)],
                                                 "copy_bytes": 8 << 26}}), flush=True)
         return
 
@@ -183,6 +191,11 @@ def main():
                 "gf_mul_per_s": float(f"{muls / (stages['ntt_ms'] * 1e-3):.4g}")}
     step_bytes = 58 * (1 << n)  # algorithmic HBM bytes of the whole step
     path_achieved = step_bytes / (ms * 1e-3) / 1e9
+    step_valu = traffic_entry(log_cw, tuning).get("step_valu_wave_insts")
+    valu_roofline = {"bound": "valu", "wave_insts_per_step": step_valu,
+                     "peak_wave_insts_per_s": VALU_PEAK_WAVE_INSTS_PER_S,
+                     "floor_ms": round(step_valu / VALU_PEAK_WAVE_INSTS_PER_S * 1e3, 4) if step_valu else None,
+                     "frac": round(step_valu / VALU_PEAK_WAVE_INSTS_PER_S / (ms * 1e-3), 4) if step_valu else None}
 
     batch = None if args.no_batch else batch_bench(b64, torch, stream, sp)
     serve = None if args.no_serve else serve_bench(b64, torch, sp)
@@ -192,7 +205,13 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args.cpu_log_n)
+        import numpy as np
+        step()
+        torch.cuda.synchronize()
+        msg_host = msg.cpu().numpy().view(np.uint64).reshape(-1, 2)
+        gpu_root = tree.cpu().numpy().view(np.uint8)[-32:].tobytes()
+        gpu_folded = folded.cpu().numpy().view(np.uint64).reshape(-1, 2)
+        cpu = cpu_baseline(dom.subspace(rs_log_len), log_dim, lir, lbs, msg_host, chal, gpu_root, gpu_folded)
 
     out = {
         "metric": "RS-encode+commit GB/s (B128, log_n=24, rate 1/2)",
@@ -213,6 +232,7 @@ def main():
         "roofline": roofline,
         "path_hbm": {"algorithmic_bytes_per_step": step_bytes, "achieved_GBs": round(path_achieved, 2),
                      "frac": round(path_achieved / HBM_PEAK_GBS, 5)},
+        "valu_roofline": valu_roofline,
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "wall_s_timed_region": round(wall, 4),
         "cpu_baseline": cpu,
@@ -430,7 +450,7 @@ def ring_switch_bench(b64, torch, stream, sp, log_len: int = 24, reps: int = 10,
            "fold_elems_hbm_frac": round((32 << log_len) / (t2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     if with_cpu:
         from oracle import oracle as orc
-        cores = min(16, os.cpu_count() or 1)
+        cores = cpu_share()
         m_h = mat.cpu().numpy().view(np.uint64).reshape(-1, 2)
         v_h = vec.cpu().numpy().view(np.uint64).reshape(-1, 2)
         ts = []
@@ -504,7 +524,7 @@ def pcs_bench(b64, torch, sp, log_n: int = 24, reps: int = 3, with_cpu: bool = T
            "commit_ms": round(commit_ms, 3), "tape_bytes": len(tape), "deterministic": bool(same)}
     if with_cpu:
         from oracle import oracle as orc
-        cores = min(16, os.cpu_count() or 1)
+        cores = cpu_share()
         m_h = msg.cpu().numpy().view(np.uint64).reshape(-1, 2)
         cw_h = cw.cpu().numpy().view(np.uint64).reshape(-1, 2)
         tree_h = tree.cpu().numpy().view(np.uint8).reshape(n_nodes, 32)
@@ -559,7 +579,7 @@ def vision_bench(b64, torch, stream, sp, log_leaves: int = 17, reps: int = 3, wi
         from oracle import oracle as orc
         gold = _json.loads((ROOT / "tests" / "golden" / "reference_vectors.json").read_text())
         v4, v6 = orc.Vision(4, gold["vision4"]), orc.Vision(6, gold["vision6"])
-        cores = min(16, os.cpu_count() or 1)
+        cores = cpu_share()
         sample = data[: 2 * (16 << cpu_log_leaves)].cpu().numpy().view(np.uint64).reshape(-1, 2)
         t = time.perf_counter()
         orc.merkle_build_vision(v6, v4, sample, 16, threads=cores)
@@ -570,32 +590,55 @@ def vision_bench(b64, torch, stream, sp, log_leaves: int = 17, reps: int = 3, wi
     return res
 
 
-def cpu_baseline(log_n: int):
-    """The oracle's multithreaded restatement of NeighborsLastMultiThread +
-    BinaryMerkleTreeProver + fold_interleaved on the host cores, on a bounded
-    sample of the same workload (log_n elements instead of 2^24)."""
+def cpu_share() -> int:
+    """Host threads for the CPU legs: the CPUs this process may run on, capped at
+    the 16 a one-GPU box allots to its GPU's jobs."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def cpu_baseline(basis, log_dim: int, lir: int, lbs: int, msg_host, chal, gpu_tree_root: bytes, gpu_folded,
+                 reps: int = 5):
+    """The reference's x86-64 commit path (oracle/libb64cpu.so: NeighborsLastMultiThread
+    with 4-wide VPCLMULQDQ products over GenericPreExpanded twiddles, SHA-256 leaves
+    and Sha256Compression nodes with the SHA extensions, packed fold_interleaved, a
+    persistent pool) on the same message as the GPU step, with every host thread of
+    the box's share and with one thread; its root and folded oracle must equal the
+    GPU's."""
     import numpy as np
 
-    from oracle import oracle as orc
+    from oracle import cpu_baseline as cbl
 
-    cores = min(16, os.cpu_count() or 1)
-    p = orc.fri_params_setup(log_n, 1, 96)
-    std = orc.Domain(dim=log_n + 1)
-    dom = orc.Domain(std.subspace(p["rs_log_len"]))
-    msg = orc.random_elems(22, 1 << log_n)
-    chal = orc.random_elems(23, p["log_batch_size"])
-    lbs = p["log_batch_size"]
-    reps, times = 5, []
-    for _ in range(reps):
-        t = time.perf_counter()
-        cw = orc.encode_batch(dom, msg, p["log_dim"], 1, lbs, threads=cores)
-        orc.merkle_build(cw, 1 << lbs, threads=cores)
-        orc.fold_interleaved(cw, p["rs_log_len"], lbs, chal, threads=cores)
-        times.append(time.perf_counter() - t)
-    med = sorted(times)[reps // 2]
-    return {"value": round((16 << log_n) / med / 1e9, 4), "unit": "GB/s", "cores": cores, "kind": "port",
-            "sample": f"log_n={log_n} (2^{log_n} B128 message, same encode+merkle+fold step), median of {reps}",
-            "seconds": round(med, 3)}
+    cores = cpu_share()
+    n_cw = 1 << (log_dim + lir + lbs)
+    cw = np.zeros((n_cw, 2), dtype=np.uint64)
+    tree = np.zeros((2 * (n_cw >> lbs) - 1, 32), dtype=np.uint8)
+    folded = np.zeros((n_cw >> lbs, 2), dtype=np.uint64)
+    msg_bytes = 16 * msg_host.shape[0]
+
+    def timed(threads, n_reps):
+        base = cbl.Baseline(basis, threads)
+        base.commit_step(msg_host, log_dim, lir, lbs, chal, cw, tree, folded)  # warm-up (page faults, pool)
+        ts = []
+        for _ in range(n_reps):
+            t = time.perf_counter()
+            base.commit_step(msg_host, log_dim, lir, lbs, chal, cw, tree, folded)
+            ts.append(time.perf_counter() - t)
+        base.close()
+        return sorted(ts)[len(ts) // 2]
+
+    med = timed(cores, reps)
+    same = tree[-1].tobytes() == gpu_tree_root and np.array_equal(folded, gpu_folded)
+    single = timed(1, 1)
+    return {"value": round(msg_bytes / med / 1e9, 4), "unit": "GB/s", "cores": cores, "kind": "port",
+            "sample": f"the full step (2^{log_dim + lbs} B128 message, encode+merkle+fold) on the GPU step's "
+                      f"message, median of {reps} after a warm-up",
+            "seconds": round(med, 4), "host_cpus": os.cpu_count(),
+            "single_thread": {"value": round(msg_bytes / single / 1e9, 4), "unit": "GB/s", "seconds": round(single, 3)},
+            "isa": cbl.features(), "identical_to_gpu": bool(same)}
 
 
 if __name__ == "__main__":
