@@ -7,9 +7,12 @@ implementation.  If libb64hip.so is missing the import fails with a clear error
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "libb64hip.so"
+# B64_SANITIZED_LIBS names a folder of host-sanitizer builds (tests/sanitize/run.sh)
+LIB_PATH = Path(os.environ["B64_SANITIZED_LIBS"]) / "libb64hip.so" if os.environ.get("B64_SANITIZED_LIBS") \
+    else Path(__file__).resolve().parent / "libb64hip.so"
 
 u8p = C.POINTER(C.c_uint8)
 u32p = C.POINTER(C.c_uint32)

@@ -215,7 +215,9 @@ cb_ctx *cb_create(const uint64_t *basis, int dim, int nthreads) {
     c->expanded = (u128 **)calloc((size_t)dim, sizeof(u128 *));
     for (int layer = 0; layer < dim; layer++) {
         const u128 *row = ev + (size_t)(dim - 1 - layer) * dim;
-        u128 *e = (u128 *)aligned_alloc(64, sizeof(u128) * ((size_t)1 << layer) + 64);
+        /* aligned_alloc takes a size that is a multiple of the alignment */
+        const size_t bytes = (sizeof(u128) * ((size_t)1 << layer) + 63) & ~(size_t)63;
+        u128 *e = (u128 *)aligned_alloc(64, bytes);
         e[0] = 0;
         for (int b = 0; b < layer; b++)
             for (size_t j = 0; j < ((size_t)1 << b); j++) e[((size_t)1 << b) + j] = e[j] ^ row[1 + b];

@@ -1001,6 +1001,7 @@ void bo_transcript_free(bo_transcript *t) {
 }
 
 static void tape_append(bo_transcript *t, const uint8_t *b, size_t n) {
+    if (n == 0) return;
     if (t->tape_len + n > t->tape_cap) {
         size_t cap = t->tape_cap ? t->tape_cap : 1024;
         while (cap < t->tape_len + n) cap *= 2;
@@ -1066,7 +1067,7 @@ uint32_t bo_transcript_sample_bits(bo_transcript *t, int bits) {
 }
 
 size_t bo_transcript_tape(const bo_transcript *t, uint8_t *out, size_t cap) {
-    if (out && cap >= t->tape_len) memcpy(out, t->tape, t->tape_len);
+    if (out && t->tape_len && cap >= t->tape_len) memcpy(out, t->tape, t->tape_len);
     return t->tape_len;
 }
 

@@ -5,6 +5,7 @@ cpu_baseline leg times it, tests/ check it against the scalar oracle."""
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
@@ -16,7 +17,8 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        path = HERE / "libb64cpu.so"
+        san = os.environ.get("B64_SANITIZED_LIBS")  # sanitizer builds (tests/sanitize/run.sh)
+        path = Path(san) / "libb64cpu.so" if san else HERE / "libb64cpu.so"
         if not path.exists():
             raise RuntimeError(f"{path} missing: run `make -C oracle`")
         L = C.CDLL(str(path))

@@ -16,7 +16,9 @@ from pathlib import Path
 import numpy as np
 
 _HERE = Path(__file__).resolve().parent
-_LIB_PATH = _HERE / "libb64oracle.so"
+# B64_SANITIZED_LIBS names a folder of sanitizer builds (tests/sanitize/run.sh)
+_LIB_PATH = Path(os.environ["B64_SANITIZED_LIBS"]) / "libb64oracle.so" if os.environ.get("B64_SANITIZED_LIBS") \
+    else _HERE / "libb64oracle.so"
 _lib = None
 
 u64p = C.POINTER(C.c_uint64)
