@@ -134,12 +134,88 @@ def tune(log_ns: Iterable[int], candidates: Optional[Sequence[Candidate]] = None
     return out
 
 
+def batch_candidates() -> List[Candidate]:
+    """Batch-plan schedules: the multi-job table kernel (variants 0-1) or the
+    multi-job forms of the register and small-table passes (variants 3-4)."""
+    c: List[Candidate] = [(0, 7, 11, 1, 2, 0)]
+    for k, tile, tab_min in ((4, 11, 128), (4, 10, 128), (5, 11, 64), (7, 11, 128), (4, 11, 64)):
+        c.append((1, k, tile, tab_min, 2, 0))
+    for v in (3, 4):
+        for k, tile, tab_min, group in ((4, 9, 16, 1), (4, 10, 16, 1), (4, 11, 16, 2), (3, 9, 8, 1), (5, 10, 16, 1)):
+            c.append((v, k, tile, tab_min, group, 0))
+    return c
+
+
+class _BatchBench:
+    """A batch plan (hipGraph) of `n_jobs` commit jobs of message size 2^log_n."""
+
+    def __init__(self, b64, log_n: int, n_jobs: int):
+        import numpy as np
+        self.b64 = b64
+        rng = np.random.default_rng(log_n)
+        self.jobs = [b64.CommitJob(log_n, rng.integers(0, 2 ** 63, size=(1 << log_n, 2), dtype=np.uint64),
+                                   rng.integers(0, 2 ** 63, size=(log_n, 2), dtype=np.uint64))
+                     for _ in range(n_jobs)]
+
+    def seconds(self, reps: int) -> float:
+        plan = self.b64.BatchPlan(self.jobs, use_graph=True)  # planned under the current schedule
+        plan.run()
+        t = time.perf_counter()
+        for _ in range(reps):
+            plan.run()
+        return (time.perf_counter() - t) / reps
+
+
+def tune_batch(log_ns: Iterable[int], n_jobs: int = 8, candidates: Optional[Sequence[Candidate]] = None,
+               budget_s: float = 0.02, verbose: bool = False) -> Dict[int, dict]:
+    """Time every batch schedule on a plan of n_jobs jobs of each message size
+    log_n (NTT size log_d = log_n + 1); return {log_d: {"config", "ms", "all"}}."""
+    import binius64_amd as b64
+    from ._abi import check, lib
+
+    L = lib()
+    cands = list(candidates or batch_candidates())
+    out: Dict[int, dict] = {}
+    for log_n in log_ns:
+        log_d = log_n + 1
+        bench = _BatchBench(b64, log_n, n_jobs)
+        est = bench.seconds(1)
+        reps = max(3, min(100, int(budget_s / max(est, 1e-6))))
+        times = {}
+        for cand in cands:
+            for key, v in zip(KNOBS, cand):
+                check(L.b64_config_set(f"batch{key[3:]}:{log_d}".encode(), int(v)), key)
+            times[cand] = min(bench.seconds(reps) for _ in range(2))
+        best = min(times, key=times.get)
+        for key, v in zip(KNOBS, best):
+            check(L.b64_config_set(f"batch{key[3:]}:{log_d}".encode(), int(v)), key)
+        out[log_d] = {"config": list(best), "ms": round(1e3 * times[best], 5), "jobs": n_jobs,
+                      "all": {",".join(map(str, c)): round(1e3 * t, 5) for c, t in times.items()}}
+        if verbose:
+            print(json.dumps({"batch_log_n": log_n, "log_d": log_d, "best": best, "ms": out[log_d]["ms"]}), flush=True)
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--log-n", type=int, nargs="+", default=list(range(8, 25)))
+    ap.add_argument("--batch", action="store_true",
+                    help="tune the batch-plan schedules (batch_sizes section) instead of the single-job ones")
     ap.add_argument("--out", default=str(DEFAULT_PATH))
     args = ap.parse_args(argv)
     from ._abi import lib
+    if args.batch:
+        lib().b64_config_set(b"batch.clear_sizes", 0)
+        res = tune_batch(args.log_n, verbose=True)
+        out = Path(args.out)
+        doc = json.loads(out.read_text()) if out.exists() else {"device": "MI355X (gfx950)", "knobs": list(KNOBS)}
+        doc["batch_knobs"] = ["batch" + k[3:] for k in KNOBS]
+        doc["batch_what"] = ("per NTT size log_d = log_n + 1: best schedule for a batch plan of 8 jobs of that "
+                             "size (hipGraph replay)")
+        doc["batch_sizes"] = {str(k): v for k, v in sorted(res.items())}
+        out.write_text(json.dumps(doc, indent=1) + "\n")
+        print(json.dumps({"written": args.out, "batch_sizes": len(res)}), flush=True)
+        return
     lib().b64_config_set(b"ntt.clear_sizes", 0)
     res = tune(args.log_n, verbose=True)
     doc = {"device": "MI355X (gfx950)", "knobs": list(KNOBS),

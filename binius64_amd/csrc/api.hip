@@ -216,7 +216,7 @@ int b64_config_set(const char *key, int64_t value) {
         return B64_ERR_INVALID_ARGS;
     b64::NttTuning t = batch ? b64::batch_tuning_for(log_d) : b64::ntt_tuning_for(log_d);
     if (!strcmp(knob, "variant")) {
-        if (value < 0 || value > (batch ? 1 : 4)) return B64_ERR_INVALID_ARGS;
+        if (value < 0 || value > 4 || (batch && value == 2)) return B64_ERR_INVALID_ARGS;
         t.variant = (int)value;
     } else if (!strcmp(knob, "max_k")) {
         if (value < 1 || value > 7) return B64_ERR_INVALID_ARGS;

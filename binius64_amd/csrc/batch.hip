@@ -125,6 +125,7 @@ struct Stage {
     b64::MultiKind kind;
     union {
         b64::NttPassArgs ntt;
+        b64::NttJobArgs nttj;
         b64::Leaf16Args leaf16;
         b64::LeafArgs leaf;
         b64::SubtreeArgs sub;
@@ -157,20 +158,28 @@ int job_stages(const b64_job_desc &jd, const Layout &L, std::vector<Stage> &out,
     for (int q = 0; q < np; q++) {
         Stage st;
         st.slot = q;
-        st.kind = b64::MultiKind::NttPass;
-        st.ntt.data = cw;
-        st.ntt.W = si->dd.W;
-        st.ntt.msg = q == 0 ? (const uint4 *)jd.d_msg : nullptr;
-        st.ntt.m = m;
-        st.ntt.log_d = log_d;
-        st.ntt.layer0 = starts[q];
-        st.ntt.k = ks[q];
-        st.ntt.c = cs[q];
-        st.ntt.e = es[q];
-        // the batched pass kernel is the table kernel; variant 0 maps to
-        // "no layer qualifies for a table"
-        st.ntt.tab_min = tu.variant == 0 ? (1 << 30) : tu.tab_min;
-        st.blocks = (uint32_t)(1ull << (log_d - ks[q] - cs[q] - es[q]));
+        b64::NttPassArgs a;
+        a.data = cw;
+        a.W = si->dd.W;
+        a.msg = q == 0 ? (const uint4 *)jd.d_msg : nullptr;
+        a.m = m;
+        a.log_d = log_d;
+        a.layer0 = starts[q];
+        a.k = ks[q];
+        a.c = cs[q];
+        a.e = es[q];
+        uint64_t per_block = 0;
+        b64::ntt_multi_pass(tu, log_d, starts[q], ks[q], cs[q], es[q], q == 0, &st.kind, &per_block, &st.blocks);
+        if (st.kind == b64::MultiKind::NttPass) {
+            // the multi-job table kernel (32-row tables); variant 0 maps to "no
+            // layer qualifies for a table", variants >= 3 keep its own threshold
+            a.tab_min = tu.variant == 0 ? (1 << 30) : tu.variant >= 3 ? 128 : tu.tab_min;
+            st.ntt = a;
+        } else {
+            a.tab_min = tu.tab_min;
+            st.nttj.a = a;
+            st.nttj.per_block = per_block;
+        }
         out.push_back(st);
     }
     // leaves of the codeword + interleaved fold into oracle 0
@@ -438,8 +447,9 @@ int b64_batch_plan_build(const b64_job_desc *jobs, uint32_t n_jobs, int use_grap
         *out = guard.release();
         return B64_OK;
     }
-    // per-slot job lists
-    std::map<int, std::vector<Stage>> slots;
+    // per-slot job lists; an NTT slot splits by kernel (jobs of different sizes
+    // may run a pass in different kernels)
+    std::map<std::pair<int, int>, std::vector<Stage>> slots;
     std::vector<const uint4 *> root_src;
     for (uint32_t j = 0; j < n_jobs; j++) {
         const b64_job_desc &jd = jobs[j];
@@ -452,7 +462,7 @@ int b64_batch_plan_build(const b64_job_desc *jobs, uint32_t n_jobs, int use_grap
         std::vector<Stage> st;
         std::vector<const uint4 *> roots;
         B64_TRY(job_stages(jd, L, st, roots));
-        for (const Stage &x : st) slots[x.slot].push_back(x);
+        for (const Stage &x : st) slots[{x.slot, (int)x.kind}].push_back(x);
         plan->roots_per_job.push_back((int)roots.size());
         root_src.insert(root_src.end(), roots.begin(), roots.end());
     }
@@ -484,6 +494,12 @@ int b64_batch_plan_build(const b64_job_desc *jobs, uint32_t n_jobs, int use_grap
             size_t sz = 0;
             switch (x.kind) {
                 case b64::MultiKind::NttPass: a = &x.ntt; sz = sizeof(x.ntt); break;
+                case b64::MultiKind::NttR16:
+                case b64::MultiKind::NttR16Msg:
+                case b64::MultiKind::NttR16W8:
+                case b64::MultiKind::NttR16W8Msg:
+                case b64::MultiKind::NttT4U9:
+                case b64::MultiKind::NttT4U10: a = &x.nttj; sz = sizeof(x.nttj); break;
                 case b64::MultiKind::Leaf16Fold: a = &x.leaf16; sz = sizeof(x.leaf16); break;
                 case b64::MultiKind::LeafHash: a = &x.leaf; sz = sizeof(x.leaf); break;
                 case b64::MultiKind::Subtree: a = &x.sub; sz = sizeof(x.sub); break;

@@ -80,7 +80,16 @@ struct FoldCwArgs {
     const uint4 *W;
     uint4 chal[MAX_FOLD_CHAL];
 };
-enum class MultiKind { NttPass, Leaf16Fold, LeafHash, Subtree, FoldCodeword };
+// A pass of one job in a multi-job register or small-table pass launch.
+struct NttJobArgs {
+    NttPassArgs a;
+    uint64_t per_block;  // positions per block (register passes), else 0
+};
+enum class MultiKind {
+    NttPass,  // multi-job LDS-tile kernel with 32-row tables (NttPassArgs)
+    NttR16, NttR16Msg, NttR16W8, NttR16W8Msg, NttT4U9, NttT4U10,  // (NttJobArgs)
+    Leaf16Fold, LeafHash, Subtree, FoldCodeword
+};
 
 struct NttTuning {
     int variant;   // 0 = general product, 1 = per-layer choice (window tables where amortised),
@@ -138,6 +147,9 @@ int ntt_pass_plan(const NttTuning &tu, int log_d, int skip_early, int skip_late,
 int ntt_pass_kernel(const NttTuning &tu, int log_d, int layer0, int k, int c, int e, bool inverse);
 uint64_t ntt_pass_workitems(int kernel, int log_d, int layer0, int k, int c, int e, uint32_t cus);
 uint32_t ntt_device_cus();
+// Batch-plan kernel of one job's pass (see kernels.hip); returns the B64_NTT_KERNEL_* it runs.
+int ntt_multi_pass(const NttTuning &tu, int log_d, int layer0, int k, int c, int e, bool msg, MultiKind *kind,
+                   uint64_t *per_block, uint32_t *blocks);
 // One launch of `kind` over n_jobs jobs: d_jobs is a device array of the
 // kind's args, d_start[j] the first block of job j, `blocks` the total.
 // fold_nc: for FoldCodeword, the n_chal (1..4) every job shares, else 0.

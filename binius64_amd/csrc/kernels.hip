@@ -457,11 +457,12 @@ struct alignas(256) R16Smem {
     uint4 thi[NTT_MAX_K];
 };
 
+// Block `blk` of the pass (single-job launches: blockIdx.x; multi-job: the
+// block's index within its job).
 template <bool MSG>
-__global__ __launch_bounds__(R16_BLOCK) void k_ntt_pass_r16(NttPassArgs a, uint64_t per_block) {
-    __shared__ R16Smem sm;
+__device__ __forceinline__ void ntt_r16_core(const NttPassArgs &a, uint64_t per_block, uint64_t blk, R16Smem &sm) {
     const int lb = a.log_d - a.layer0 - 4;
-    const uint64_t p0 = (uint64_t)blockIdx.x * per_block;
+    const uint64_t p0 = blk * per_block;
     const uint64_t hi = p0 >> lb;
     ntt_tile_twiddles(a, hi, sm.tw, sm.thi);  // a.k = 4, a.e = 0; ends with a barrier
     for (int r = threadIdx.x; r < R16_TABLES * WIN_ROWS; r += R16_BLOCK)
@@ -506,6 +507,12 @@ __global__ __launch_bounds__(R16_BLOCK) void k_ntt_pass_r16(NttPassArgs a, uint6
 #pragma unroll
         for (int e = 0; e < 16; e++) (a.data + (hi << (lb + 4)) + ((uint64_t)e << lb))[low_st] = x[e];
     }
+}
+
+template <bool MSG>
+__global__ __launch_bounds__(R16_BLOCK) void k_ntt_pass_r16(NttPassArgs a, uint64_t per_block) {
+    __shared__ R16Smem sm;
+    ntt_r16_core<MSG>(a, per_block, blockIdx.x, sm);
 }
 
 static uint32_t device_cus() {
@@ -572,9 +579,8 @@ struct alignas(256) NttT4USmem {
 };
 
 template <int TL>
-__global__ __launch_bounds__(BLOCK) void k_ntt_pass_t4u(NttPassArgs a) {
-    __shared__ NttT4USmem<TL> sm;
-    const NttTile T = ntt_tile_at(a, blockIdx.x);
+__device__ __forceinline__ void ntt_t4u_core(const NttPassArgs &a, uint64_t tile, NttT4USmem<TL> &sm) {
+    const NttTile T = ntt_tile_at(a, tile);
     ntt_load_tile(sm.s, a, T);
     ntt_tile_twiddles(a, T.hi, sm.tw, sm.thi);  // ends with a barrier
     uint4 *__restrict__ s = sm.s;
@@ -601,6 +607,12 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass_t4u(NttPassArgs a) {
         lds_barrier();
     }
     ntt_store_tile(sm.s, a, T);
+}
+
+template <int TL>
+__global__ __launch_bounds__(BLOCK) void k_ntt_pass_t4u(NttPassArgs a) {
+    __shared__ NttT4USmem<TL> sm;
+    ntt_t4u_core<TL>(a, blockIdx.x, sm);
 }
 
 template <int TL>
@@ -665,10 +677,9 @@ struct alignas(256) R8Smem {
 };
 
 template <bool MSG>
-__global__ __launch_bounds__(R8_BLOCK) void k_ntt_pass_r16w8(NttPassArgs a, uint64_t per_block) {
-    __shared__ R8Smem sm;
+__device__ __forceinline__ void ntt_r16w8_core(const NttPassArgs &a, uint64_t per_block, uint64_t blk, R8Smem &sm) {
     const int lb = a.log_d - a.layer0 - 4;
-    const uint64_t p0 = (uint64_t)blockIdx.x * per_block;
+    const uint64_t p0 = blk * per_block;
     const uint64_t hi = p0 >> lb;
     ntt_tile_twiddles(a, hi, sm.tw, sm.thi);  // a.k = 4, a.e = 0; ends with a barrier
     for (int r = threadIdx.x; r < R16_TABLES * 16; r += R8_BLOCK)
@@ -708,6 +719,12 @@ __global__ __launch_bounds__(R8_BLOCK) void k_ntt_pass_r16w8(NttPassArgs a, uint
 #pragma unroll
         for (int e = 0; e < 16; e++) (a.data + (hi << (lb + 4)) + ((uint64_t)e << lb))[low_st] = x[e];
     }
+}
+
+template <bool MSG>
+__global__ __launch_bounds__(R8_BLOCK) void k_ntt_pass_r16w8(NttPassArgs a, uint64_t per_block) {
+    __shared__ R8Smem sm;
+    ntt_r16w8_core<MSG>(a, per_block, blockIdx.x, sm);
 }
 
 // Positions per r16w8 block: a power of two within the hi group, at least a
@@ -1388,6 +1405,70 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass_tab_multi(const NttPassArgs 
     ntt_tab_tile<false, G, true>(a, blockIdx.x - start[j], sm);
 }
 
+// The register and small-table passes over many jobs: each job brings its own
+// pass arguments and positions per block (NttJobArgs).
+template <bool MSG>
+__global__ __launch_bounds__(R16_BLOCK) void k_ntt_pass_r16_multi(const NttJobArgs *__restrict__ jobs,
+                                                                 const uint32_t *__restrict__ start, int n_jobs) {
+    __shared__ R16Smem sm;
+    const int j = job_of_block2(start, n_jobs, blockIdx.x);
+    const NttJobArgs ja = jobs[j];
+    ntt_r16_core<MSG>(ja.a, ja.per_block, blockIdx.x - start[j], sm);
+}
+
+template <bool MSG>
+__global__ __launch_bounds__(R8_BLOCK) void k_ntt_pass_r16w8_multi(const NttJobArgs *__restrict__ jobs,
+                                                                  const uint32_t *__restrict__ start, int n_jobs) {
+    __shared__ R8Smem sm;
+    const int j = job_of_block2(start, n_jobs, blockIdx.x);
+    const NttJobArgs ja = jobs[j];
+    ntt_r16w8_core<MSG>(ja.a, ja.per_block, blockIdx.x - start[j], sm);
+}
+
+template <int TL>
+__global__ __launch_bounds__(BLOCK) void k_ntt_pass_t4u_multi(const NttJobArgs *__restrict__ jobs,
+                                                             const uint32_t *__restrict__ start, int n_jobs) {
+    __shared__ NttT4USmem<TL> sm;
+    const int j = job_of_block2(start, n_jobs, blockIdx.x);
+    const NttJobArgs ja = jobs[j];
+    ntt_t4u_core<TL>(ja.a, blockIdx.x - start[j], sm);
+}
+
+// Multi-job kind, positions per block and block count of one job's pass under a
+// batch schedule: with batch variant >= 3 the pass runs the kernel a single-job
+// launch would (register passes, the one-round small-table tile), in its
+// multi-job form; other passes, and variants 0-1, run the multi-job table kernel.
+int ntt_multi_pass(const NttTuning &tu, int log_d, int layer0, int k, int c, int e, bool msg, MultiKind *kind,
+                   uint64_t *per_block, uint32_t *blocks) {
+    *per_block = 0;
+    if (tu.variant >= 3) {
+        const int kernel = ntt_pass_kernel(tu, log_d, layer0, k, c, e, false);
+        const uint32_t cus = device_cus();
+        if (kernel == B64_NTT_KERNEL_R16) {
+            const int per_log = r16_per_log(log_d, layer0, cus);
+            *kind = msg ? MultiKind::NttR16Msg : MultiKind::NttR16;
+            *per_block = 1ull << per_log;
+            *blocks = 1u << (log_d - 4 - per_log);
+            return B64_NTT_KERNEL_R16;
+        }
+        if (kernel == B64_NTT_KERNEL_R16W8) {
+            const int per_log = r8_per_log(log_d, layer0, cus);
+            *kind = msg ? MultiKind::NttR16W8Msg : MultiKind::NttR16W8;
+            *per_block = 1ull << per_log;
+            *blocks = 1u << (log_d - 4 - per_log);
+            return B64_NTT_KERNEL_R16W8;
+        }
+        if (kernel == B64_NTT_KERNEL_T4U) {
+            *kind = k + c + e <= 9 ? MultiKind::NttT4U9 : MultiKind::NttT4U10;
+            *blocks = 1u << (log_d - k - c - e);
+            return B64_NTT_KERNEL_T4U;
+        }
+    }
+    *kind = MultiKind::NttPass;
+    *blocks = 1u << (log_d - k - c - e);
+    return B64_NTT_KERNEL_TAB_HYBRID;
+}
+
 // ---------------------------------------------------------------------------
 // Multi-job launchers (batched path): one launch covers the same stage of many
 // jobs; tables live in device memory prepared by the caller.
@@ -1408,6 +1489,24 @@ hipError_t launch_multi(MultiKind kind, const void *d_jobs, const uint32_t *d_st
                     k_ntt_pass_tab_multi<2><<<blocks, BLOCK, 0, s>>>((const NttPassArgs *)d_jobs, d_start, n_jobs);
                     break;
             }
+            break;
+        case MultiKind::NttR16:
+            k_ntt_pass_r16_multi<false><<<blocks, R16_BLOCK, 0, s>>>((const NttJobArgs *)d_jobs, d_start, n_jobs);
+            break;
+        case MultiKind::NttR16Msg:
+            k_ntt_pass_r16_multi<true><<<blocks, R16_BLOCK, 0, s>>>((const NttJobArgs *)d_jobs, d_start, n_jobs);
+            break;
+        case MultiKind::NttR16W8:
+            k_ntt_pass_r16w8_multi<false><<<blocks, R8_BLOCK, 0, s>>>((const NttJobArgs *)d_jobs, d_start, n_jobs);
+            break;
+        case MultiKind::NttR16W8Msg:
+            k_ntt_pass_r16w8_multi<true><<<blocks, R8_BLOCK, 0, s>>>((const NttJobArgs *)d_jobs, d_start, n_jobs);
+            break;
+        case MultiKind::NttT4U9:
+            k_ntt_pass_t4u_multi<9><<<blocks, BLOCK, 0, s>>>((const NttJobArgs *)d_jobs, d_start, n_jobs);
+            break;
+        case MultiKind::NttT4U10:
+            k_ntt_pass_t4u_multi<10><<<blocks, BLOCK, 0, s>>>((const NttJobArgs *)d_jobs, d_start, n_jobs);
             break;
         case MultiKind::Leaf16Fold:
             k_leaf16_fold_multi<<<blocks, BLOCK, 0, s>>>((const Leaf16Args *)d_jobs, d_start, n_jobs);

@@ -104,8 +104,10 @@ int b64_stream_synchronize(void *stream);
  * 0 = default).  "ntt.<knob>:<log_d>" sets the knob for transforms of size
  * 2^log_d only (the autotuner's per-size table); "ntt.clear_sizes" drops every
  * per-size setting.  "batch.<knob>[:<log_d>]" and "batch.clear_sizes" do the same
- * for the NTT passes of batch plans (multi-job LDS-tile kernel: variant 0 or 1,
- * max_k, tile_log, tab_min, tab_group).  Returns INVALID_ARGS for an unknown key or value.  Settings
+ * for the NTT passes of batch plans: variants 0-1 run every pass in the multi-job
+ * LDS-tile kernel (max_k, tile_log, tab_min, tab_group), variants 3-4 run each
+ * pass in the multi-job form of the kernel a single-job launch would pick (register
+ * passes, the one-round small-table tile), the rest in the table kernel.  Returns INVALID_ARGS for an unknown key or value.  Settings
  * are process-wide; change them only while no call is running. */
 int b64_config_set(const char *key, int64_t value);
 int b64_config_get(const char *key, int64_t *value);

@@ -130,3 +130,31 @@ def test_config4_mix_matches_oracle(b64, orc):
     assert [j.roots() for j in jobs] == batched
     for jid, (n, (msg, chal)) in enumerate(zip(sizes, inputs)):
         assert batched[jid] == oracle_job(orc, n, msg, chal)["roots"], f"job {jid} (n={n})"
+
+
+@pytest.mark.parametrize("cand", [(1, 4, 11, 128, 2), (3, 4, 9, 16, 1), (3, 4, 10, 16, 2), (4, 4, 9, 16, 1),
+                                  (3, 3, 9, 8, 1), (3, 5, 10, 16, 1), (0, 7, 11, 1, 2)])
+def test_batch_schedules_match_oracle(b64, orc, cand):
+    """Batch-plan NTT schedules: variants 3-4 run each job's passes in the
+    multi-job register / small-table kernels (per-job kernel choice, several
+    launches per NTT slot), variants 0-1 in the multi-job table kernel; every
+    schedule gives the oracle's roots for uneven sizes, with and without a graph."""
+    from binius64_amd._abi import _apply_tuning
+    sizes = [8, 12, 20, 13, 17, 14, 19, 16, 9, 18]
+    say("batch schedule", cand, sizes)
+    made = [make_job(b64, orc, n, 300 + i) for i, n in enumerate(sizes)]
+    want = [oracle_job(orc, n, msg, chal)["roots"] for (job, msg, chal), n in zip(made, sizes)]
+    try:
+        b64.config_set("batch.clear_sizes", 0)
+        for key, v in zip(("variant", "max_k", "tile_log", "tab_min", "tab_group"), cand):
+            b64.config_set(f"batch.{key}", v)
+        for use_graph in (False, True):
+            for job, _, _ in made:
+                job.roots_buf[:] = 0
+            plan = b64.BatchPlan([m[0] for m in made], use_graph=use_graph)
+            plan.run()
+            assert [m[0].roots() for m in made] == want, use_graph
+    finally:
+        for key, v in zip(("variant", "max_k", "tile_log", "tab_min", "tab_group"), (1, 4, 11, 128, 2)):
+            b64.config_set(f"batch.{key}", v)
+        _apply_tuning(b64.lib())
