@@ -998,26 +998,29 @@ __device__ __forceinline__ void leaf16_fold_core(const Leaf16Args &a, uint64_t b
     sha256_compress(st, w);
     store_digest_be(a.digests + 2 * leaf, st);
     if (do_fold) {
+        // permute-addressed lookups: one VALU op per lookup address (the SHA rounds
+        // keep this kernel VALU-bound)
+        const uint32_t tb = lds_addr(tabs);
         int len = 16;
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             len >>= 1;
 #pragma unroll
             for (int q = 0; q < len; q++)
-                v[q] = gxor(v[2 * q], win_mul(tabs + t * WIN_ENTRIES, gxor(v[2 * q], v[2 * q + 1])));
+                v[q] = gxor(v[2 * q], win_mul_perm(tb + (uint32_t)(t * WIN_ENTRIES * 16), gxor(v[2 * q], v[2 * q + 1])));
         }
         a.folded[leaf] = v[0];
     }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_leaf16_fold(Leaf16Args a) {
-    __shared__ uint4 tabs[4 * WIN_ENTRIES];
+    __shared__ __attribute__((aligned(256))) uint4 tabs[4 * WIN_ENTRIES];
     leaf16_fold_core(a, blockIdx.x, tabs);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_leaf16_fold_multi(const Leaf16Args *__restrict__ jobs,
                                                              const uint32_t *__restrict__ start, int n_jobs) {
-    __shared__ uint4 tabs[4 * WIN_ENTRIES];
+    __shared__ __attribute__((aligned(256))) uint4 tabs[4 * WIN_ENTRIES];
     const int j = job_of_block2(start, n_jobs, blockIdx.x);
     leaf16_fold_core(jobs[j], blockIdx.x - start[j], tabs);
 }
