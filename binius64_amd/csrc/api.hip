@@ -58,6 +58,8 @@ struct b64_domain {
     std::vector<u128> evals;  // evals[i*dim + k], k < dim - i
     std::vector<uint4> W;     // dim * W_STRIDE
     uint4 *d_W = nullptr;
+    uint4 *d_X = nullptr;     // pre-expanded twiddles of layers < x_layers (2^x_layers entries)
+    int x_layers = 0;
     std::mutex mu;
 
     u128 eval(int i, int k) const { return evals[(size_t)i * dim + k]; }
@@ -72,11 +74,28 @@ struct b64_domain {
         std::lock_guard<std::mutex> g(mu);
         if (!d_W) {
             if (!device_ok()) return B64_ERR_NO_DEVICE;
-            HIP_TRY(hipMalloc(&d_W, W.size() * sizeof(uint4)));
-            HIP_TRY(hipMemcpy(d_W, W.data(), W.size() * sizeof(uint4), hipMemcpyHostToDevice));
+            // the basis, and the pre-expanded low layers read by the LDS-tile
+            // passes over the last layers (a twiddle per 16..128 butterflies there)
+            const int xl = dim < b64::X_MAX_LAYERS ? dim : b64::X_MAX_LAYERS;
+            uint4 *w = nullptr, *x = nullptr;
+            hipError_t e = hipMalloc(&w, W.size() * sizeof(uint4));
+            if (e == hipSuccess) e = hipMalloc(&x, ((size_t)1 << xl) * sizeof(uint4));
+            if (e == hipSuccess) e = hipMemcpy(w, W.data(), W.size() * sizeof(uint4), hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = b64::launch_expand_twiddles(w, x, xl, nullptr);
+            if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+            if (e != hipSuccess) {
+                if (w) (void)hipFree(w);
+                if (x) (void)hipFree(x);
+                return b64::abi::hip_status(e);
+            }
+            d_W = w;
+            d_X = x;
+            x_layers = xl;
         }
         dd.dim = dim;
         dd.W = d_W;
+        dd.X = d_X;
+        dd.x_layers = x_layers;
         return B64_OK;
     }
 };
@@ -318,6 +337,7 @@ int b64_domain_create_standard(uint32_t dim, b64_domain **out) {
 void b64_domain_destroy(b64_domain *d) {
     if (!d) return;
     if (d->d_W) (void)hipFree(d->d_W);
+    if (d->d_X) (void)hipFree(d->d_X);
     delete d;
 }
 

@@ -161,6 +161,8 @@ int job_stages(const b64_job_desc &jd, const Layout &L, std::vector<Stage> &out,
         b64::NttPassArgs a;
         a.data = cw;
         a.W = si->dd.W;
+        a.X = si->dd.X;
+        a.x_layers = si->dd.x_layers;
         a.msg = q == 0 ? (const uint4 *)jd.d_msg : nullptr;
         a.m = m;
         a.log_d = log_d;

@@ -259,6 +259,28 @@ __device__ __forceinline__ void w8_build_half(uint4 *__restrict__ tab, uint4 t, 
     r[7 ^ row] = e7;
 }
 
+// Builds row `row` of t's table (all 16 entries, one lane per row).
+__device__ __forceinline__ void w8_build_row(uint4 *__restrict__ tab, uint4 t, int row) {
+    const uint4 p0 = gf_mul_xk(t, 4u * (uint32_t)row);
+    const uint4 p1 = gf_mul_x(p0);
+    const uint4 p2 = gf_mul_x(p1);
+    const uint4 p3 = gf_mul_x(p2);
+    uint4 e[16];
+    e[0] = make_uint4(0, 0, 0, 0);
+    e[1] = p0;
+    e[2] = p1;
+    e[3] = gxor(p0, p1);
+    e[4] = p2;
+#pragma unroll
+    for (int n = 5; n < 8; n++) e[n] = gxor(e[n - 4], p2);
+    e[8] = p3;
+#pragma unroll
+    for (int n = 9; n < 16; n++) e[n] = gxor(e[n - 8], p3);
+    uint4 *r = tab + row * 16;
+#pragma unroll
+    for (int n = 0; n < 16; n++) r[n ^ row] = e[n];
+}
+
 // v * t through t's 2 KiB table at LDS address tab_base (256-byte aligned).
 __device__ __forceinline__ uint4 w8_mul(uint32_t tab_base, uint4 v) {
     uint32_t acc[7] = {0, 0, 0, 0, 0, 0, 0};

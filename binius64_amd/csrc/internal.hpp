@@ -19,9 +19,15 @@ struct u128h {
 // Per-domain device data: W[layer][i] = evals[dim-1-layer][1+i] for i < layer,
 // the F2-basis of the layer's twiddles (ntt/domain_context.rs:117-128,149-151):
 // twiddle(layer, block) = XOR over set bits i of block of W[layer][i].
+// X (when not null) holds the same twiddles pre-expanded for layers below
+// x_layers (the domain's GenericPreExpanded form, domain_context.rs:117-151):
+// X[2^layer + block] = twiddle(layer, block).
+constexpr int X_MAX_LAYERS = 21;  // at most 2^21 entries (32 MiB) per domain
 struct DeviceDomain {
     int dim;
     const uint4 *W;  // dim * W_STRIDE entries
+    const uint4 *X = nullptr;
+    int x_layers = 0;
 };
 
 // Sha256Compression IV (LE words of SHA-256("BINIUS SHA-256 COMPRESS")).
@@ -53,6 +59,9 @@ struct NttPassArgs {
     // tile (independent butterfly groups), when the c low bits cannot fill it
     int e;
     int tab_min;  // layers whose twiddle covers >= tab_min butterflies use window tables
+    // pre-expanded twiddles of the layers below x_layers (DeviceDomain::X), or null
+    const uint4 *X = nullptr;
+    int x_layers = 0;
 };
 struct LeafArgs {
     const uint4 *elems;
@@ -118,6 +127,8 @@ const NttTuning &batch_tuning_for(int log_d);
 hipError_t launch_gf_mul(const uint4 *a, const uint4 *b, uint4 *out, uint64_t n, hipStream_t s);
 hipError_t launch_gf_mul_const(const uint4 *a, uint4 c, uint4 *out, uint64_t n, hipStream_t s);
 hipError_t launch_bitrev_replicate(const uint4 *msg, uint4 *out, int m, int lir, hipStream_t s);
+// X[2^layer + block] = twiddle(layer, block) for layer < x_layers (X[0] = 0).
+hipError_t launch_expand_twiddles(const uint4 *W, uint4 *X, int x_layers, hipStream_t s);
 hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_early, int skip_late,
                       bool inverse, hipStream_t s, const uint4 *msg = nullptr, int msg_log = 0);
 hipError_t launch_leaf_hash(const uint4 *elems, uint64_t n_leaves, int log_batch, uint64_t batch,

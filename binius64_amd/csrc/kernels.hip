@@ -66,6 +66,31 @@ hipError_t launch_bitrev_replicate(const uint4 *msg, uint4 *out, int m, int lir,
     return hipGetLastError();
 }
 
+// Pre-expanded twiddles (the GenericPreExpanded domain, ntt/domain_context.rs:
+// 117-151): entry 2^layer + block is the XOR of W[layer][i] over the set bits i
+// of block.
+__global__ __launch_bounds__(BLOCK) void k_expand_twiddles(const uint4 *__restrict__ W, uint4 *__restrict__ X,
+                                                           uint32_t n) {
+    const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= n) return;
+    uint4 acc = make_uint4(0, 0, 0, 0);
+    if (idx) {
+        const int layer = 31 - __clz(idx);
+        const uint32_t block = idx - (1u << layer);
+        const uint4 *Wl = W + layer * W_STRIDE;
+        for (int i = 0; i < layer; i++)
+            if ((block >> i) & 1) acc = gxor(acc, Wl[i]);
+    }
+    X[idx] = acc;
+}
+
+hipError_t launch_expand_twiddles(const uint4 *W, uint4 *X, int x_layers, hipStream_t s) {
+    if (x_layers < 0 || x_layers > X_MAX_LAYERS) return hipErrorInvalidValue;
+    const uint32_t n = 1u << x_layers;
+    k_expand_twiddles<<<grid_for(n), BLOCK, 0, s>>>(W, X, n);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Additive NTT (ntt/reference.rs:17-76): LDS-tiled passes of up to NTT_MAX_K layers.
 //
@@ -148,9 +173,11 @@ __device__ __forceinline__ NttTile ntt_tile_at(const NttPassArgs &a, uint64_t ti
 // A tile in flight: thread x holds elements x, x + BLOCK, ... of the load order,
 // as one register vector (the compiler keeps it in VGPRs where a uint4 array
 // passed between functions can end up in scratch memory).
-typedef uint32_t u32x32 __attribute__((ext_vector_type(4 * NTT_PER_THREAD)));
-struct NttTileRegs {
-    u32x32 v;
+template <int NPT>
+struct NttTileRegsN {
+    typedef uint32_t vec __attribute__((ext_vector_type(4 * NPT)));
+    static constexpr int n = NPT;
+    vec v;
     __device__ __forceinline__ void set(int i, uint4 e) {
         v[4 * i] = e.x;
         v[4 * i + 1] = e.y;
@@ -161,17 +188,19 @@ struct NttTileRegs {
         return make_uint4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
     }
 };
+typedef NttTileRegsN<NTT_PER_THREAD> NttTileRegs;
 
 // Issues the tile's loads.  Element L = (bf << c) | low of the tile holds
 // index base | bf << lb | low.  In message mode the loads run in message order
 // so they are runs of 2^k consecutive message elements: with
 // i = bf << (m-k) | mid << c | low, rev_m(i) = rev_k(bf) | rev(mid) << k | rev_c(low) << (m-c).
-__device__ __forceinline__ void ntt_fetch_tile(NttTileRegs &R, const NttPassArgs &a, const NttTile &t) {
+template <int NPT>
+__device__ __forceinline__ void ntt_fetch_tile(NttTileRegsN<NPT> &R, const NttPassArgs &a, const NttTile &t) {
     const int k = a.k, c = a.c;
     if (a.msg == nullptr) {
         const int cmask = (1 << c) - 1;
 #pragma unroll
-        for (int i = 0; i < NTT_PER_THREAD; i++) {
+        for (int i = 0; i < NPT; i++) {
             // the test is uniform across the block; in tiles smaller than a
             // block, lanes past the end re-read a valid element (not stored)
             const int L = (threadIdx.x + i * BLOCK) & (t.n_el - 1);
@@ -182,7 +211,7 @@ __device__ __forceinline__ void ntt_fetch_tile(NttTileRegs &R, const NttPassArgs
         const uint64_t mbase = mid_bits ? ((__brevll(t.mid) >> (64 - mid_bits)) << k) : 0;
         const int kmask = (1 << k) - 1;
 #pragma unroll
-        for (int i = 0; i < NTT_PER_THREAD; i++) {
+        for (int i = 0; i < NPT; i++) {
             const int q = (threadIdx.x + i * BLOCK) & (t.n_el - 1);
             if (i * BLOCK < t.n_el) R.set(i, a.msg[mbase + (q & kmask) + ((uint64_t)(q >> k) << (a.m - c))]);
         }
@@ -190,19 +219,20 @@ __device__ __forceinline__ void ntt_fetch_tile(NttTileRegs &R, const NttPassArgs
 }
 
 // Writes a fetched tile into LDS in tile order.
-__device__ __forceinline__ void ntt_put_tile(uint4 *__restrict__ s, const NttTileRegs &R, const NttPassArgs &a,
+template <int NPT>
+__device__ __forceinline__ void ntt_put_tile(uint4 *__restrict__ s, const NttTileRegsN<NPT> &R, const NttPassArgs &a,
                                              int n_el) {
     const int k = a.k, c = a.c;
     if (a.msg == nullptr) {
 #pragma unroll
-        for (int i = 0; i < NTT_PER_THREAD; i++) {
+        for (int i = 0; i < NPT; i++) {
             const int L = threadIdx.x + i * BLOCK;
             if (L < n_el) s[L] = R.get(i);
         }
     } else {
         const int kmask = (1 << k) - 1;
 #pragma unroll
-        for (int i = 0; i < NTT_PER_THREAD; i++) {
+        for (int i = 0; i < NPT; i++) {
             const int q = threadIdx.x + i * BLOCK;
             if (q < n_el) s[(rev_bits(q & kmask, k) << c) | rev_bits(q >> k, c)] = R.get(i);
         }
@@ -217,14 +247,15 @@ __device__ __forceinline__ void ntt_load_tile(uint4 *__restrict__ s, const NttPa
     ntt_put_tile(s, R, a, t.n_el);
 }
 
+template <int NPT = NTT_PER_THREAD>
 __device__ __forceinline__ void ntt_store_tile(const uint4 *__restrict__ s, const NttPassArgs &a, const NttTile &t) {
     const int cmask = (1 << a.c) - 1;
-    NttTileRegs R;
+    NttTileRegsN<NPT> R;
 #pragma unroll
-    for (int i = 0; i < NTT_PER_THREAD; i++)
+    for (int i = 0; i < NPT; i++)
         if (i * BLOCK < t.n_el) R.set(i, s[(threadIdx.x + i * BLOCK) & (t.n_el - 1)]);
 #pragma unroll
-    for (int i = 0; i < NTT_PER_THREAD; i++) {
+    for (int i = 0; i < NPT; i++) {
         const int L = threadIdx.x + i * BLOCK;
         if (L < t.n_el) a.data[t.base | ((uint64_t)(L >> a.c) << t.lb) | (uint64_t)(L & cmask)] = R.get(i);
     }
@@ -578,18 +609,37 @@ struct alignas(256) NttT4USmem {
     uint4 thi[NTT_MAX_K];
 };
 
+// With pre-expanded twiddles the tile's twiddle loads go out first, its element
+// loads behind them, and the tables are built (a row per lane) while the
+// element loads are in flight.
 template <int TL>
 __device__ __forceinline__ void ntt_t4u_core(const NttPassArgs &a, uint64_t tile, NttT4USmem<TL> &sm) {
+    constexpr int NPT = (1 << TL) / BLOCK;
     const NttTile T = ntt_tile_at(a, tile);
-    ntt_load_tile(sm.s, a, T);
-    ntt_tile_twiddles(a, T.hi, sm.tw, sm.thi);  // ends with a barrier
+    const int k = a.k, c = a.c, e = a.e, n_el = T.n_el;
+    const int n_tab = ntt_tw_base(k, e);
+    const bool pre = a.X != nullptr && a.layer0 + k <= a.x_layers;  // uniform
+    uint4 twv = make_uint4(0, 0, 0, 0);
+    if (pre && (int)threadIdx.x < n_tab) {
+        const int t = threadIdx.x;
+        const int j = 31 - __clz((t >> e) + 1);  // layer of tw[t]
+        const uint64_t blk = (T.hi << (j + e)) | (uint64_t)(t - ntt_tw_base(j, e));
+        twv = a.X[(1ull << (a.layer0 + j)) + blk];
+    }
+    NttTileRegsN<NPT> R;
+    ntt_fetch_tile(R, a, T);
+    if (pre) {
+        if ((int)threadIdx.x < n_tab) sm.tw[threadIdx.x] = twv;
+        lds_barrier();
+    } else {
+        ntt_tile_twiddles(a, T.hi, sm.tw, sm.thi);  // ends with a barrier
+    }
+    for (int r = threadIdx.x; r < W4_ROWS * n_tab; r += BLOCK)
+        w8_build_row(sm.tabs + (r / W4_ROWS) * W4_ENTRIES, sm.tw[r / W4_ROWS], r % W4_ROWS);
+    ntt_put_tile(sm.s, R, a, n_el);
+    lds_barrier();
     uint4 *__restrict__ s = sm.s;
     const uint32_t tabs = lds_addr(sm.tabs);
-    const int k = a.k, c = a.c, n_el = T.n_el;
-    const int n_tab = ntt_tw_base(k, a.e);
-    for (int r = threadIdx.x; r < 8 * n_tab; r += BLOCK)
-        w8_build_half(sm.tabs + (r >> 3) * W4_ENTRIES, sm.tw[r >> 3], (r >> 1) & 3, r & 1);
-    lds_barrier();
     for (int j = 0; j < k; j++) {
         const int tw0 = ntt_tw_base(j, a.e);
         const int pos = c + (k - 1 - j);
@@ -606,7 +656,7 @@ __device__ __forceinline__ void ntt_t4u_core(const NttPassArgs &a, uint64_t tile
         }
         lds_barrier();
     }
-    ntt_store_tile(sm.s, a, T);
+    ntt_store_tile<NPT>(sm.s, a, T);
 }
 
 template <int TL>
@@ -860,6 +910,8 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
         NttPassArgs a;
         a.data = data;
         a.W = dd.W;
+        a.X = dd.X;
+        a.x_layers = dd.x_layers;
         a.msg = (q == 0 && !inverse) ? msg : nullptr;
         a.m = msg_log;
         a.log_d = log_d;
