@@ -182,18 +182,40 @@ __device__ __forceinline__ uint4 gxor3v(uint4 a, u32x4 b, u32x4 c) {
     return make_uint4(xor3(a.x, b.x, c.x), xor3(a.y, b.y, c.y), xor3(a.z, b.z, c.z), xor3(a.w, b.w, c.w));
 }
 
+// The lookups go out in groups of G (2, 4 or 8) before their XORs, so a wave
+// keeps G table reads in flight; G trades LDS latency against VGPRs.
+// (a & b) ^ c
+__device__ __forceinline__ uint32_t and_xor(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x6A);
+}
+
+// The nibbles of w ^ swz, pre-scaled by 16, in two masked words: lo holds
+// nibbles 0,2,4,6 and hi nibbles 1,3,5,7 (one byte each).  The swizzle is
+// folded into the masking (three VALU ops per word).
+__device__ __forceinline__ void nib_split(uint32_t w, uint32_t swz, uint32_t &lo, uint32_t &hi) {
+    lo = and_xor(w << 4, 0xF0F0F0F0u, (swz << 4) & 0xF0F0F0F0u);
+    hi = and_xor(w, 0xF0F0F0F0u, swz & 0xF0F0F0F0u);
+}
+
+template <int G = 8>
 __device__ __forceinline__ uint4 win_word_perm(uint32_t base, uint32_t w, int row0, uint4 acc) {
-    w ^= win_swz(row0);
-    const uint32_t lo = (w << 4) & 0xF0F0F0F0u;  // nibbles 0,2,4,6 * 16
-    const uint32_t hi = w & 0xF0F0F0F0u;         // nibbles 1,3,5,7 * 16
+    uint32_t lo, hi;  // nibbles 0,2,4,6 and 1,3,5,7 (after the row swizzle) * 16
+    nib_split(w, win_swz(row0), lo, hi);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const uint32_t sel = 0x07060500u | (uint32_t)k;  // byte k of src1, bytes 1..3 of base
-        const uint32_t a0 = __builtin_amdgcn_perm(base, lo, sel);
-        const uint32_t a1 = __builtin_amdgcn_perm(base, hi, sel);
-        const lds_cu32x4 *p0 = (const lds_cu32x4 *)(uintptr_t)a0;
-        const lds_cu32x4 *p1 = (const lds_cu32x4 *)(uintptr_t)a1;
-        acc = gxor3v(acc, p0[(row0 + 2 * k) * 16], p1[(row0 + 2 * k + 1) * 16]);
+    for (int g = 0; g < 8; g += G) {
+        u32x4 e[G];
+#pragma unroll
+        for (int k = g / 2; k < (g + G) / 2; k++) {
+            const uint32_t sel = 0x07060500u | (uint32_t)k;  // byte k of src1, bytes 1..3 of base
+            const uint32_t a0 = __builtin_amdgcn_perm(base, lo, sel);
+            const uint32_t a1 = __builtin_amdgcn_perm(base, hi, sel);
+            const lds_cu32x4 *p0 = (const lds_cu32x4 *)(uintptr_t)a0;
+            const lds_cu32x4 *p1 = (const lds_cu32x4 *)(uintptr_t)a1;
+            e[2 * k - g] = p0[(row0 + 2 * k) * 16];
+            e[2 * k + 1 - g] = p1[(row0 + 2 * k + 1) * 16];
+        }
+#pragma unroll
+        for (int k = 0; k < G / 2; k++) acc = gxor3v(acc, e[2 * k], e[2 * k + 1]);
     }
     return acc;
 }
@@ -203,12 +225,14 @@ __device__ __forceinline__ uint32_t lds_addr(const uint4 *p) {
     return (uint32_t)(uintptr_t)(const lds_cuint4 *)p;
 }
 
-__device__ __forceinline__ uint4 win_mul_perm(uint32_t tab_base, uint4 v) {
-    uint4 acc = make_uint4(0, 0, 0, 0);
-    acc = win_word_perm(tab_base, v.x, 0, acc);
-    acc = win_word_perm(tab_base, v.y, 8, acc);
-    acc = win_word_perm(tab_base, v.z, 16, acc);
-    acc = win_word_perm(tab_base, v.w, 24, acc);
+// acc0 + v * c (acc0 = 0 for the plain product; an NTT butterfly passes u).
+template <int G = 8>
+__device__ __forceinline__ uint4 win_mul_perm(uint32_t tab_base, uint4 v, uint4 acc0 = make_uint4(0, 0, 0, 0)) {
+    uint4 acc = acc0;
+    acc = win_word_perm<G>(tab_base, v.x, 0, acc);
+    acc = win_word_perm<G>(tab_base, v.y, 8, acc);
+    acc = win_word_perm<G>(tab_base, v.z, 16, acc);
+    acc = win_word_perm<G>(tab_base, v.w, 24, acc);
     return acc;
 }
 
@@ -281,25 +305,34 @@ __device__ __forceinline__ void w8_build_row(uint4 *__restrict__ tab, uint4 t, i
     for (int n = 0; n < 16; n++) r[n ^ row] = e[n];
 }
 
-// v * t through t's 2 KiB table at LDS address tab_base (256-byte aligned).
-__device__ __forceinline__ uint4 w8_mul(uint32_t tab_base, uint4 v) {
-    uint32_t acc[7] = {0, 0, 0, 0, 0, 0, 0};
+// v * t through t's 2 KiB table at LDS address tab_base (256-byte aligned); the
+// lookups go out in groups of G (as win_word_perm).
+template <int G = 8>
+__device__ __forceinline__ uint4 w8_mul(uint32_t tab_base, uint4 v, uint4 acc0 = make_uint4(0, 0, 0, 0)) {
+    uint32_t acc[7] = {acc0.x, acc0.y, acc0.z, acc0.w, 0, 0, 0};
     const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        const uint32_t w = vw[q] ^ 0x76543210u;  // nibble s -> slot nib ^ s of row s
-        const uint32_t lo = (w << 4) & 0xF0F0F0F0u;  // nibbles 0,2,4,6 * 16
-        const uint32_t hi = w & 0xF0F0F0F0u;         // nibbles 1,3,5,7 * 16
+        uint32_t lo, hi;  // nibble s -> slot nib ^ s of row s
+        nib_split(vw[q], 0x76543210u, lo, hi);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const uint32_t sel = 0x07060500u | (uint32_t)k;
-            const lds_cu32x4 *p0 = (const lds_cu32x4 *)(uintptr_t)__builtin_amdgcn_perm(tab_base, lo, sel);
-            const lds_cu32x4 *p1 = (const lds_cu32x4 *)(uintptr_t)__builtin_amdgcn_perm(tab_base, hi, sel);
-            const u32x4 e0 = p0[(2 * k) * 16], e1 = p1[(2 * k + 1) * 16];
-            acc[q + 0] = xor3(acc[q + 0], e0.x, e1.x);
-            acc[q + 1] = xor3(acc[q + 1], e0.y, e1.y);
-            acc[q + 2] = xor3(acc[q + 2], e0.z, e1.z);
-            acc[q + 3] = xor3(acc[q + 3], e0.w, e1.w);
+        for (int g = 0; g < 8; g += G) {
+            u32x4 e[G];
+#pragma unroll
+            for (int k = g / 2; k < (g + G) / 2; k++) {
+                const uint32_t sel = 0x07060500u | (uint32_t)k;
+                const lds_cu32x4 *p0 = (const lds_cu32x4 *)(uintptr_t)__builtin_amdgcn_perm(tab_base, lo, sel);
+                const lds_cu32x4 *p1 = (const lds_cu32x4 *)(uintptr_t)__builtin_amdgcn_perm(tab_base, hi, sel);
+                e[2 * k - g] = p0[(2 * k) * 16];
+                e[2 * k + 1 - g] = p1[(2 * k + 1) * 16];
+            }
+#pragma unroll
+            for (int k = 0; k < G / 2; k++) {
+                acc[q + 0] = xor3(acc[q + 0], e[2 * k].x, e[2 * k + 1].x);
+                acc[q + 1] = xor3(acc[q + 1], e[2 * k].y, e[2 * k + 1].y);
+                acc[q + 2] = xor3(acc[q + 2], e[2 * k].z, e[2 * k + 1].z);
+                acc[q + 3] = xor3(acc[q + 3], e[2 * k].w, e[2 * k + 1].w);
+            }
         }
     }
     // fold the high words h = acc[4..6] (below x^96): r = lo ^ h * (1 + x + x^2 + x^7)
@@ -321,25 +354,30 @@ __device__ __forceinline__ uint4 w8_mul(uint32_t tab_base, uint4 v) {
 constexpr int W4_ROWS = 4;
 constexpr int W4_ENTRIES = W4_ROWS * 16;  // uint4 entries per table (1 KiB)
 
-__device__ __forceinline__ uint4 w4_mul(uint32_t tab_base, uint4 v) {
-    uint32_t A[7] = {0, 0, 0, 0, 0, 0, 0}, B[7] = {0, 0, 0, 0, 0, 0, 0};
+__device__ __forceinline__ uint4 w4_mul(uint32_t tab_base, uint4 v, uint4 acc0 = make_uint4(0, 0, 0, 0)) {
+    uint32_t A[7] = {acc0.x, acc0.y, acc0.z, acc0.w, 0, 0, 0}, B[7] = {0, 0, 0, 0, 0, 0, 0};
     const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        const uint32_t w = vw[q] ^ 0x32103210u;  // nibble j -> slot nib ^ (j & 3)
-        const uint32_t lo = (w << 4) & 0xF0F0F0F0u;  // nibbles 0,2,4,6 * 16: rows 0,2,0,2
-        const uint32_t hi = w & 0xF0F0F0F0u;         // nibbles 1,3,5,7 * 16: rows 1,3,1,3
+        // nibble j -> slot nib ^ (j & 3); lo: nibbles 0,2,4,6 (rows 0,2,0,2), hi: 1,3,5,7 (rows 1,3,1,3)
+        uint32_t lo, hi;
+        nib_split(vw[q], 0x32103210u, lo, hi);
+        u32x4 e[8];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const uint32_t sel = 0x07060500u | (uint32_t)k;
             const lds_cu32x4 *p0 = (const lds_cu32x4 *)(uintptr_t)__builtin_amdgcn_perm(tab_base, lo, sel);
             const lds_cu32x4 *p1 = (const lds_cu32x4 *)(uintptr_t)__builtin_amdgcn_perm(tab_base, hi, sel);
-            const u32x4 e0 = p0[(2 * (k & 1)) * 16], e1 = p1[(2 * (k & 1) + 1) * 16];
+            e[2 * k] = p0[(2 * (k & 1)) * 16];
+            e[2 * k + 1] = p1[(2 * (k & 1) + 1) * 16];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
             uint32_t *acc = k < 2 ? A : B;
-            acc[q + 0] = xor3(acc[q + 0], e0.x, e1.x);
-            acc[q + 1] = xor3(acc[q + 1], e0.y, e1.y);
-            acc[q + 2] = xor3(acc[q + 2], e0.z, e1.z);
-            acc[q + 3] = xor3(acc[q + 3], e0.w, e1.w);
+            acc[q + 0] = xor3(acc[q + 0], e[2 * k].x, e[2 * k + 1].x);
+            acc[q + 1] = xor3(acc[q + 1], e[2 * k].y, e[2 * k + 1].y);
+            acc[q + 2] = xor3(acc[q + 2], e[2 * k].z, e[2 * k + 1].z);
+            acc[q + 3] = xor3(acc[q + 3], e[2 * k].w, e[2 * k + 1].w);
         }
     }
     // u = A + B * x^16 (8 words, degree <= 254)

@@ -408,11 +408,11 @@ __device__ __forceinline__ void ntt_tab_layers(const NttPassArgs &a, int n_el, S
                 const uint4 *tab = tabs + (q >> pos) * WIN_ENTRIES;
                 uint4 u = s[L0], v = s[L1];
                 if (!INVERSE) {
-                    u = gxor(u, win_mul_perm(lds_addr(tab), v));
+                    u = win_mul_perm(lds_addr(tab), v, u);
                     v = gxor(v, u);
                 } else {
                     v = gxor(v, u);
-                    u = gxor(u, win_mul_perm(lds_addr(tab), v));
+                    u = win_mul_perm(lds_addr(tab), v, u);
                 }
                 s[L0] = u;
                 s[L1] = v;
@@ -482,11 +482,26 @@ constexpr int R16_BLOCK = 1024;
 constexpr int R16_MIN_LB = 10;
 constexpr int R16_MAX_LB = 32;  // lane offsets within a hi group are 32-bit
 constexpr int R16_TABLES = 15;
+constexpr int R16_LOOKUPS = 4;  // table reads in flight per wave (VGPR budget: 128)
 struct alignas(256) R16Smem {
     uint4 tabs[R16_TABLES * WIN_ENTRIES];
     uint4 tw[NTT_TW];
     uint4 thi[NTT_MAX_K];
 };
+
+// Element e (0..15, the value of index bits lb .. lb+3) of lane offset `low` in
+// hi group `hi`.  For the message-gathering pass, codeword index bits
+// lb .. lb+3 are message bits 3 .. 0 (reversed), so a lane's 16 elements are
+// one 16-element run of the message.
+template <bool MSG>
+__device__ __forceinline__ uint4 r16_load(const NttPassArgs &a, uint64_t hi, int lb, uint64_t low, int e) {
+    if (MSG) {
+        const int mb = a.m - 4;
+        const uint64_t run = mb ? (__brevll(low) >> (64 - mb)) << 4 : 0;
+        return a.msg[run + rev_bits(e, 4)];
+    }
+    return (a.data + (hi << (lb + 4)) + ((uint64_t)e << lb))[(uint32_t)low];
+}
 
 // Block `blk` of the pass (single-job launches: blockIdx.x; multi-job: the
 // block's index within its job).
@@ -501,22 +516,22 @@ __device__ __forceinline__ void ntt_r16_core(const NttPassArgs &a, uint64_t per_
     lds_barrier();
     const uint32_t tabs = lds_addr(sm.tabs);
     const uint64_t lowmask = (1ull << lb) - 1;
-    for (uint64_t p = p0 + threadIdx.x; p < p0 + per_block; p += R16_BLOCK) {
-        const uint64_t low = p & lowmask;
-        const uint64_t base = (hi << (lb + 4)) | low;
-        uint4 x[16];
-        if (MSG) {
-            // codeword index bits lb .. lb+3 are message bits 3 .. 0 (reversed)
-            const int mb = a.m - 4;
-            const uint64_t run = mb ? (__brevll(low) >> (64 - mb)) << 4 : 0;
+    // Every thread makes per_block / R16_BLOCK passes of the loop (per_block is a
+    // multiple of R16_BLOCK), so `more` is uniform across the block.  The loads of
+    // the next position go out during the last layer, pair by pair as the
+    // current position's elements are stored, so each wave's HBM traffic
+    // overlaps its own arithmetic instead of running as a separate phase.
+    // lane offsets within the hi group: low0 + i for i = threadIdx.x, +R16_BLOCK, ...
+    const uint32_t low0 = (uint32_t)(p0 & lowmask), nper = (uint32_t)per_block;
+    uint32_t i = threadIdx.x;
+    uint4 x[16];
 #pragma unroll
-            for (int e = 0; e < 16; e++) x[e] = a.msg[run + rev_bits(e, 4)];
-        } else {
+    for (int e = 0; e < 16; e++) x[e] = r16_load<MSG>(a, hi, lb, low0 + i, e);
+    for (;;) {
+        const uint32_t in = i + R16_BLOCK;
+        const bool more = in < nper;
 #pragma unroll
-            for (int e = 0; e < 16; e++) x[e] = (a.data + (hi << (lb + 4)) + ((uint64_t)e << lb))[(uint32_t)low];
-        }
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < 3; j++) {
             const int bit = 3 - j;  // layer j flips element bit 3 - j
 #pragma unroll
             for (int e = 0; e < 16; e++) {
@@ -524,19 +539,30 @@ __device__ __forceinline__ void ntt_r16_core(const NttPassArgs &a, uint64_t per_
                 const int blk = e >> (bit + 1);
                 const uint32_t tb = tabs + (uint32_t)(((1 << j) - 1 + blk) * WIN_ENTRIES * 16);
                 uint4 u = x[e], v = x[e | (1 << bit)];
-                u = gxor(u, win_mul_perm(tb, v));
+                u = gxor(u, win_mul_perm<R16_LOOKUPS>(tb, v));
                 v = gxor(v, u);
                 x[e] = u;
                 x[e | (1 << bit)] = v;
             }
         }
-        // The stores recompute their addresses from a fresh copy of the lane
-        // offset: kept from the loads, the 16 64-bit addresses pushed the kernel
-        // past 128 VGPRs into scratch.
-        uint32_t low_st = (uint32_t)low;
-        asm volatile("" : "+v"(low_st));
+        // layer 3 (element bit 0): each finished pair is stored, and its
+        // registers take the next position's pair (after the last position, a
+        // re-read of the pair just stored, unused: no branch in the loop body)
+        const uint32_t low = low0 + i, lown = more ? low0 + in : low;
 #pragma unroll
-        for (int e = 0; e < 16; e++) (a.data + (hi << (lb + 4)) + ((uint64_t)e << lb))[low_st] = x[e];
+        for (int e = 0; e < 16; e += 2) {
+            const uint32_t tb = tabs + (uint32_t)((7 + (e >> 1)) * WIN_ENTRIES * 16);
+            uint4 u = x[e], v = x[e + 1];
+            u = gxor(u, win_mul_perm<R16_LOOKUPS>(tb, v));
+            v = gxor(v, u);
+            uint4 *row = a.data + (hi << (lb + 4));
+            row[((uint64_t)e << lb) + low] = u;
+            row[((uint64_t)(e + 1) << lb) + low] = v;
+            x[e] = r16_load<MSG>(a, hi, lb, lown, e);
+            x[e + 1] = r16_load<MSG>(a, hi, lb, lown, e + 1);
+        }
+        if (!more) break;
+        i = in;
     }
 }
 
@@ -649,7 +675,7 @@ __device__ __forceinline__ void ntt_t4u_core(const NttPassArgs &a, uint64_t tile
             const int L0 = ((P >> pos) << (pos + 1)) | (P & lowmask);
             const int L1 = L0 | per_blk;
             uint4 u = s[L0], v = s[L1];
-            u = gxor(u, w4_mul(tabs + (uint32_t)((tw0 + (P >> pos)) * W4_ENTRIES * 16), v));
+            u = w4_mul(tabs + (uint32_t)((tw0 + (P >> pos)) * W4_ENTRIES * 16), v, u);
             v = gxor(v, u);
             s[L0] = u;
             s[L1] = v;
@@ -704,7 +730,7 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass_t8(NttPassArgs a) {
                 const int L0 = ((P >> pos) << (pos + 1)) | (P & lowmask);
                 const int L1 = L0 | per_blk;
                 uint4 u = s[L0], v = s[L1];
-                u = gxor(u, w8_mul(tabs + (uint32_t)((q >> pos) * W8_ENTRIES * 16), v));
+                u = w8_mul(tabs + (uint32_t)((q >> pos) * W8_ENTRIES * 16), v, u);
                 v = gxor(v, u);
                 s[L0] = u;
                 s[L1] = v;
@@ -720,6 +746,7 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass_t8(NttPassArgs a) {
 // qualify and several blocks share a CU.
 constexpr int R8_BLOCK = 256;
 constexpr int R8_MIN_LB = 8;
+constexpr int R8_LOOKUPS = 4;  // table reads in flight per wave (VGPR budget: 128)
 struct alignas(256) R8Smem {
     uint4 tabs[R16_TABLES * W8_ENTRIES];
     uint4 tw[NTT_TW];
@@ -737,20 +764,18 @@ __device__ __forceinline__ void ntt_r16w8_core(const NttPassArgs &a, uint64_t pe
     lds_barrier();
     const uint32_t tabs = lds_addr(sm.tabs);
     const uint64_t lowmask = (1ull << lb) - 1;
-    for (uint64_t p = p0 + threadIdx.x; p < p0 + per_block; p += R8_BLOCK) {
-        const uint64_t low = p & lowmask;
-        uint4 x[16];
-        if (MSG) {
-            const int mb = a.m - 4;
-            const uint64_t run = mb ? (__brevll(low) >> (64 - mb)) << 4 : 0;
+    // the loop of ntt_r16_core: the next position's loads go out during the last
+    // layer (per_block is a multiple of R8_BLOCK, so `more` is uniform)
+    const uint32_t low0 = (uint32_t)(p0 & lowmask), nper = (uint32_t)per_block;
+    uint32_t i = threadIdx.x;
+    uint4 x[16];
 #pragma unroll
-            for (int e = 0; e < 16; e++) x[e] = a.msg[run + rev_bits(e, 4)];
-        } else {
+    for (int e = 0; e < 16; e++) x[e] = r16_load<MSG>(a, hi, lb, low0 + i, e);
+    for (;;) {
+        const uint32_t in = i + R8_BLOCK;
+        const bool more = in < nper;
 #pragma unroll
-            for (int e = 0; e < 16; e++) x[e] = (a.data + (hi << (lb + 4)) + ((uint64_t)e << lb))[(uint32_t)low];
-        }
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < 3; j++) {
             const int bit = 3 - j;
 #pragma unroll
             for (int e = 0; e < 16; e++) {
@@ -758,16 +783,27 @@ __device__ __forceinline__ void ntt_r16w8_core(const NttPassArgs &a, uint64_t pe
                 const int blk = e >> (bit + 1);
                 const uint32_t tb = tabs + (uint32_t)(((1 << j) - 1 + blk) * W8_ENTRIES * 16);
                 uint4 u = x[e], v = x[e | (1 << bit)];
-                u = gxor(u, w8_mul(tb, v));
+                u = gxor(u, w8_mul<R8_LOOKUPS>(tb, v));
                 v = gxor(v, u);
                 x[e] = u;
                 x[e | (1 << bit)] = v;
             }
         }
-        uint32_t low_st = (uint32_t)low;
-        asm volatile("" : "+v"(low_st));
+        const uint32_t low = low0 + i, lown = more ? low0 + in : low;
 #pragma unroll
-        for (int e = 0; e < 16; e++) (a.data + (hi << (lb + 4)) + ((uint64_t)e << lb))[low_st] = x[e];
+        for (int e = 0; e < 16; e += 2) {
+            const uint32_t tb = tabs + (uint32_t)((7 + (e >> 1)) * W8_ENTRIES * 16);
+            uint4 u = x[e], v = x[e + 1];
+            u = gxor(u, w8_mul<R8_LOOKUPS>(tb, v));
+            v = gxor(v, u);
+            uint4 *row = a.data + (hi << (lb + 4));
+            row[((uint64_t)e << lb) + low] = u;
+            row[((uint64_t)(e + 1) << lb) + low] = v;
+            x[e] = r16_load<MSG>(a, hi, lb, lown, e);
+            x[e + 1] = r16_load<MSG>(a, hi, lb, lown, e + 1);
+        }
+        if (!more) break;
+        i = in;
     }
 }
 
@@ -1059,7 +1095,7 @@ __device__ __forceinline__ void leaf16_fold_core(const Leaf16Args &a, uint64_t b
             len >>= 1;
 #pragma unroll
             for (int q = 0; q < len; q++)
-                v[q] = gxor(v[2 * q], win_mul_perm(tb + (uint32_t)(t * WIN_ENTRIES * 16), gxor(v[2 * q], v[2 * q + 1])));
+                v[q] = win_mul_perm(tb + (uint32_t)(t * WIN_ENTRIES * 16), gxor(v[2 * q], v[2 * q + 1]), v[2 * q]);
         }
         a.folded[leaf] = v[0];
     }

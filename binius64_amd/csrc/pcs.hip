@@ -100,10 +100,10 @@ __global__ __launch_bounds__(SC_BLOCK) void k_sc_round(uint4 *__restrict__ a, ui
         if (fold) {
             const uint4 x0 = a[i], x1 = a[i + half], x2 = a[i + len], x3 = a[i + len + half];
             const uint4 z0 = b[i], z1 = b[i + half], z2 = b[i + len], z3 = b[i + len + half];
-            a0 = gxor(x0, win_mul_perm(tb, gxor(x0, x2)));
-            a1 = gxor(x1, win_mul_perm(tb, gxor(x1, x3)));
-            b0 = gxor(z0, win_mul_perm(tb, gxor(z0, z2)));
-            b1 = gxor(z1, win_mul_perm(tb, gxor(z1, z3)));
+            a0 = win_mul_perm(tb, gxor(x0, x2), x0);
+            a1 = win_mul_perm(tb, gxor(x1, x3), x1);
+            b0 = win_mul_perm(tb, gxor(z0, z2), z0);
+            b1 = win_mul_perm(tb, gxor(z1, z3), z1);
             a[i] = a0;
             a[i + half] = a1;
             b[i] = b0;
