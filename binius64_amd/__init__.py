@@ -26,7 +26,7 @@ from ._abi import B64Error, FriParamsC, LIB_PATH, check, lib
 __all__ = [
     "B64Error", "DeviceBuffer", "DomainContext", "AdditiveNTT", "ReedSolomonCode",
     "BinaryMerkleTreeProver", "FRIParams", "CommitOutput", "commit_interleaved", "setup_commit_params",
-    "ProverTranscript", "FRIFoldProver", "gf_mul", "gf_inv", "gf_mul_device", "gf_mul_const_device",
+    "ProverTranscript", "FRIFoldProver", "ProveJob", "batch_prove", "gf_mul", "gf_inv", "gf_mul_device", "gf_mul_const_device",
     "pack_witness", "device_count", "config_set", "config_get", "ntt_plan", "LIB_PATH", "lib",
     "CommitJob", "BatchPlan", "Server", "fri_prove", "ring_switch_fold_1b_rows", "ring_switch_fold_elems",
     "Proof", "STD_CHALLENGER_NAME",
@@ -727,6 +727,42 @@ class BatchPlan:
 
     def __del__(self):
         _release(self, "b64_batch_plan_destroy")
+
+
+class ProveJob:
+    """One message to commit and prove with FRI under its own transcript
+    (b64_prove_job): what commit_interleaved, the root written to the transcript,
+    FRIFoldProver's rounds with sampled challenges and finish_proof do for one
+    message, with the Verifier::setup parameters of its size."""
+
+    def __init__(self, log_msg_len: int, message, transcript: Optional[ProverTranscript] = None):
+        from ._abi import ProveJobC
+        self.log_msg_len = log_msg_len
+        self.msg = message if isinstance(message, DeviceBuffer) else DeviceBuffer.from_host(_elems(message))
+        ws = lib().b64_job_workspace_size(log_msg_len)
+        if ws == 0:
+            raise B64Error(1, "job_workspace_size")
+        self.workspace = DeviceBuffer(ws)
+        self.transcript = transcript if transcript is not None else ProverTranscript()
+        self.n_roots = lib().b64_job_n_roots(log_msg_len)
+        self.roots_buf = np.zeros(32 * self.n_roots, dtype=np.uint8)
+        self.desc = ProveJobC(log_msg_len, self.msg.ptr, self.workspace.ptr, self.transcript.ptr,
+                              _p8(self.roots_buf))
+
+    def roots(self) -> List[bytes]:
+        b = self.roots_buf.tobytes()
+        return [b[32 * i:32 * i + 32] for i in range(self.n_roots)]
+
+    def tape(self) -> bytes:
+        return self.transcript.finalize()
+
+
+def batch_prove(jobs: List[ProveJob], stream=None) -> None:
+    """b64_batch_prove: every job's commitment and FRI proof, one launch per
+    stage across the jobs of each round."""
+    from ._abi import ProveJobC
+    arr = (ProveJobC * len(jobs))(*[j.desc for j in jobs])
+    check(lib().b64_batch_prove(arr, len(jobs), _stream(stream)), "batch_prove")
 
 
 class Server:

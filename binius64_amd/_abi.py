@@ -67,8 +67,19 @@ class JobDescC(C.Structure):
     ]
 
 
+class ProveJobC(C.Structure):
+    _fields_ = [
+        ("log_msg_len", C.c_uint32),
+        ("d_msg", C.c_void_p),
+        ("d_workspace", C.c_void_p),
+        ("transcript", C.c_void_p),
+        ("roots_out", C.POINTER(C.c_uint8)),
+    ]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
+    "b64_batch_prove": (C.c_int, [C.POINTER(ProveJobC), C.c_uint32, vp]),
     "b64_job_workspace_size": (C.c_size_t, [C.c_uint32]),
     "b64_job_n_roots": (C.c_uint32, [C.c_uint32]),
     "b64_job_oracle": (C.c_int, [C.POINTER(JobDescC), C.c_uint32, C.POINTER(vp), u32p]),

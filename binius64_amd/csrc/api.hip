@@ -896,6 +896,94 @@ int fetch(void *dst, const void *d_src, size_t bytes, hipStream_t s) {
 
 }  // namespace
 
+int b64::FriQuery::prepare(const b64_fri_params &params, const FriOracleView &first, const FriOracleView *committed,
+                           size_t n_committed) {
+    p = params;
+    // terminate codeword: the last committed oracle, or the codeword itself
+    term = n_committed ? committed[n_committed - 1] : first;
+    // optimal layer depths (verifier fri/common.rs:154-168, scheme.rs:70-72)
+    const int cap = log2_ceil_u64(p.n_test_queries);
+    depths.clear();
+    int log_n_cosets = (int)(p.log_dim + p.log_inv_rate + p.log_batch_size);
+    for (uint32_t i = 0; i <= p.n_fold_arities; i++) {
+        log_n_cosets -= (int)(i == 0 ? p.log_batch_size : p.fold_arities[i - 1]);
+        depths.push_back(cap < log_n_cosets ? cap : log_n_cosets);
+    }
+    // the first oracle and every committed oracle but the terminal one
+    opened.clear();
+    opened.push_back(first);
+    for (size_t i = 0; i + 1 < n_committed; i++) opened.push_back(committed[i]);
+    if (opened.size() > depths.size()) opened.resize(depths.size());
+    return B64_OK;
+}
+
+int b64::FriQuery::fetch(hipStream_t s) {
+    term_buf.resize(16ull << term.log_len);
+    HIP_TRY(hipMemcpyAsync(term_buf.data(), term.cw, term_buf.size(), hipMemcpyDeviceToHost, s));
+    layer_buf.assign(opened.size(), {});
+    for (size_t i = 0; i < opened.size(); i++) {
+        const FriOracleView &o = opened[i];
+        const uint64_t total = (2ull << o.tree_log_len) - 1;
+        const uint64_t start = total + 1 - (2ull << depths[i]);
+        layer_buf[i].resize(32ull << depths[i]);
+        HIP_TRY(hipMemcpyAsync(layer_buf[i].data(), (const uint8_t *)o.tree + 32 * start, layer_buf[i].size(),
+                               hipMemcpyDeviceToHost, s));
+    }
+    return B64_OK;
+}
+
+int b64::FriQuery::transcribe(b64_transcript *t) {
+    b64_transcript_decommit(t, term_buf.data(), term_buf.size());
+    for (const auto &layer : layer_buf) b64_transcript_decommit(t, layer.data(), layer.size());
+    // queries: decommitments are not observed, so every index can be sampled
+    // first.  Each opening (prove_coset_opening, query.rs:112-134: the coset's
+    // values, then the branch digests from the leaf up) is listed as 16-byte
+    // granules in tape order.
+    const uint32_t nq = p.n_test_queries;
+    const uint32_t rs_log_len = p.log_dim + p.log_inv_rate;
+    std::vector<uint64_t> idx(nq);
+    for (uint32_t q = 0; q < nq; q++) idx[q] = b64_transcript_sample_bits(t, rs_log_len);
+    src.clear();
+    for (uint32_t q = 0; q < nq; q++) {
+        uint64_t cur = idx[q];
+        for (size_t i = 0; i < opened.size(); i++) {
+            const FriOracleView &o = opened[i];
+            const int log_coset = (int)(i == 0 ? p.log_batch_size : p.fold_arities[i - 1]);
+            if (i > 0) cur >>= log_coset;
+            if (cur >> o.tree_log_len) return B64_ERR_INDEX_OUT_OF_RANGE;
+            const uint64_t vals = (uint64_t)(uintptr_t)o.cw + ((cur << log_coset) << 4);
+            for (uint64_t k = 0; k < (1ull << log_coset); k++) src.push_back(vals + 16 * k);
+            const int br_len = o.tree_log_len - depths[i];
+            for (int j = 0; j < br_len; j++) {
+                // sibling at level j of BinaryMerkleTree::branch (binary_merkle_tree.rs:154-169)
+                const uint64_t node = (((1ull << j) - 1) << (o.tree_log_len + 1 - j)) | ((cur >> j) ^ 1);
+                const uint64_t at = (uint64_t)(uintptr_t)o.tree + 32 * node;
+                src.push_back(at);
+                src.push_back(at + 16);
+            }
+        }
+    }
+    return B64_OK;
+}
+
+void b64::FriQuery::openings(b64_transcript *t, const uint8_t *granules) const {
+    if (!src.empty()) b64_transcript_decommit(t, granules, src.size() * 16);
+}
+
+int b64::gather_granules(const uint64_t *src, size_t n, uint8_t *out, hipStream_t s) {
+    if (n == 0) return B64_OK;
+    uint8_t *d_buf = nullptr;
+    HIP_TRY(hipMallocAsync((void **)&d_buf, n * (sizeof(uint64_t) + 16), s));
+    int rc = B64_OK;
+    if (hipMemcpyAsync(d_buf, src, n * sizeof(uint64_t), hipMemcpyHostToDevice, s) != hipSuccess ||
+        b64::launch_gather16((const uint64_t *)d_buf, (uint4 *)(d_buf + n * sizeof(uint64_t)), n, s) != hipSuccess ||
+        hipMemcpyAsync(out, d_buf + n * sizeof(uint64_t), n * 16, hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = B64_ERR_HIP;
+    (void)hipFreeAsync(d_buf, s);
+    HIP_TRY(hipStreamSynchronize(s));
+    return rc;
+}
+
 extern "C" {
 
 int b64_fri_prover_create(const b64_fri_params *p, const b64_domain *d, const void *d_codeword, const void *d_tree,
@@ -1033,79 +1121,17 @@ int b64_fri_prover_oracle(const b64_fri_prover *fp, uint32_t k, const void **d_c
 int b64_fri_prover_finish_proof(b64_fri_prover *fp, b64_transcript *t) {
     if (!fp || !t) return B64_ERR_INVALID_ARGS;
     if (fp->curr_round != fp->n_rounds()) return B64_ERR_EARLY_PROVER_FINISH;  // finalize (:190-192)
-    const b64_fri_params &p = fp->p;
-    const hipStream_t s = fp->s;
-    // terminate codeword: the last committed oracle, or the codeword itself
-    const b64_fri_prover::Oracle &term = fp->committed.empty() ? fp->first : fp->committed.back();
-    std::vector<uint8_t> buf(16ull << term.log_len);
-    B64_TRY(fetch(buf.data(), term.cw, buf.size(), s));
-    HIP_TRY(hipStreamSynchronize(s));
-    b64_transcript_decommit(t, buf.data(), buf.size());
-
-    // optimal layer depths (verifier fri/common.rs:154-168, scheme.rs:70-72)
-    const int cap = log2_ceil_u64(p.n_test_queries);
-    std::vector<int> depths;
-    int log_n_cosets = (int)(p.log_dim + p.log_inv_rate + p.log_batch_size);
-    for (uint32_t i = 0; i <= p.n_fold_arities; i++) {
-        log_n_cosets -= (int)(i == 0 ? p.log_batch_size : p.fold_arities[i - 1]);
-        depths.push_back(cap < log_n_cosets ? cap : log_n_cosets);
-    }
-    // the first oracle and every committed oracle but the terminal one
-    std::vector<const b64_fri_prover::Oracle *> opened;
-    opened.push_back(&fp->first);
-    for (size_t i = 0; i + 1 < fp->committed.size(); i++) opened.push_back(&fp->committed[i]);
-    const size_t n_open = opened.size() < depths.size() ? opened.size() : depths.size();
-    for (size_t i = 0; i < n_open; i++) {
-        const b64_fri_prover::Oracle &o = *opened[i];
-        const int depth = depths[i];
-        const uint64_t total = (2ull << o.tree_log_len) - 1;
-        const uint64_t start = total + 1 - (2ull << depth);
-        std::vector<uint8_t> layer(32ull << depth);
-        B64_TRY(fetch(layer.data(), (const uint8_t *)o.tree + 32 * start, layer.size(), s));
-        HIP_TRY(hipStreamSynchronize(s));
-        b64_transcript_decommit(t, layer.data(), layer.size());
-    }
-    // queries: decommitments are not observed, so every index can be sampled
-    // first.  Each opening (prove_coset_opening, query.rs:112-134: the coset's
-    // values, then the branch digests from the leaf up) is listed as 16-byte
-    // granules in tape order and gathered on the device in one launch.
-    const uint32_t nq = p.n_test_queries;
-    std::vector<uint64_t> idx(nq);
-    for (uint32_t q = 0; q < nq; q++) idx[q] = b64_transcript_sample_bits(t, fp->rs_log_len());
-    std::vector<uint64_t> src;
-    for (uint32_t q = 0; q < nq; q++) {
-        uint64_t cur = idx[q];
-        for (size_t i = 0; i < n_open; i++) {
-            const b64_fri_prover::Oracle &o = *opened[i];
-            const int log_coset = (int)(i == 0 ? p.log_batch_size : p.fold_arities[i - 1]);
-            if (i > 0) cur >>= log_coset;
-            if (cur >> o.tree_log_len) return B64_ERR_INDEX_OUT_OF_RANGE;
-            const uint64_t vals = (uint64_t)(uintptr_t)o.cw + ((cur << log_coset) << 4);
-            for (uint64_t k = 0; k < (1ull << log_coset); k++) src.push_back(vals + 16 * k);
-            const int br_len = o.tree_log_len - depths[i];
-            for (int j = 0; j < br_len; j++) {
-                // sibling at level j of BinaryMerkleTree::branch (binary_merkle_tree.rs:154-169)
-                const uint64_t node = (((1ull << j) - 1) << (o.tree_log_len + 1 - j)) | ((cur >> j) ^ 1);
-                const uint64_t at = (uint64_t)(uintptr_t)o.tree + 32 * node;
-                src.push_back(at);
-                src.push_back(at + 16);
-            }
-        }
-    }
-    if (src.empty()) return B64_OK;
-    const size_t n = src.size();
-    uint8_t *d_buf = nullptr;
-    HIP_TRY(hipMallocAsync((void **)&d_buf, n * (sizeof(uint64_t) + 16), s));
-    std::vector<uint8_t> out(n * 16);
-    int rc = B64_OK;
-    if (hipMemcpyAsync(d_buf, src.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, s) != hipSuccess ||
-        b64::launch_gather16((const uint64_t *)d_buf, (uint4 *)(d_buf + n * sizeof(uint64_t)), n, s) != hipSuccess ||
-        hipMemcpyAsync(out.data(), d_buf + n * sizeof(uint64_t), n * 16, hipMemcpyDeviceToHost, s) != hipSuccess)
-        rc = B64_ERR_HIP;
-    (void)hipFreeAsync(d_buf, s);
-    HIP_TRY(hipStreamSynchronize(s));
-    if (rc != B64_OK) return rc;
-    b64_transcript_decommit(t, out.data(), out.size());
+    std::vector<b64::FriOracleView> committed;
+    for (const auto &o : fp->committed) committed.push_back({o.cw, o.tree, o.log_len, o.tree_log_len});
+    const b64::FriOracleView first = {fp->first.cw, fp->first.tree, fp->first.log_len, fp->first.tree_log_len};
+    b64::FriQuery q;
+    B64_TRY(q.prepare(fp->p, first, committed.data(), committed.size()));
+    B64_TRY(q.fetch(fp->s));
+    HIP_TRY(hipStreamSynchronize(fp->s));
+    B64_TRY(q.transcribe(t));
+    std::vector<uint8_t> out(q.src.size() * 16);
+    B64_TRY(b64::gather_granules(q.src.data(), q.src.size(), out.data(), fp->s));
+    q.openings(t, out.data());
     return B64_OK;
 }
 

@@ -130,6 +130,7 @@ struct Stage {
         b64::LeafArgs leaf;
         b64::SubtreeArgs sub;
         b64::FoldCwArgs fold;
+        b64::FoldIlArgs foldil;
     };
     uint32_t blocks;
     Stage() {}
@@ -142,15 +143,10 @@ constexpr int SLOT_LEAF16 = 16, SLOT_TREE0 = 17, SLOT_ORACLE = 32;
 
 inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + 255) / 256); }
 
-int job_stages(const b64_job_desc &jd, const Layout &L, std::vector<Stage> &out, std::vector<const uint4 *> &roots) {
+// The forward NTT passes of an encode (the first gathering the bit-reversed message).
+int ntt_stages(const Layout &L, const void *d_msg, uint32_t m, uint4 *cw, std::vector<Stage> &out) {
     const SizeInfo *si = L.si;
-    const b64_fri_params &p = si->p;
-    uint8_t *ws = (uint8_t *)jd.d_workspace;
-    uint4 *cw = (uint4 *)(ws + L.cw);
-    uint4 *tree0 = (uint4 *)(ws + L.tree0);
     const int log_d = L.rs + L.lbs;
-    const int m = (int)jd.log_msg_len;
-    // NTT passes, the first gathering the bit-reversed message
     int starts[64], ks[64], cs[64], es[64];
     const b64::NttTuning &tu = b64::batch_tuning_for(log_d);
     const int np = b64::ntt_pass_plan(tu, log_d, 1, L.lbs, true, starts, ks, cs, es);
@@ -163,8 +159,8 @@ int job_stages(const b64_job_desc &jd, const Layout &L, std::vector<Stage> &out,
         a.W = si->dd.W;
         a.X = si->dd.X;
         a.x_layers = si->dd.x_layers;
-        a.msg = q == 0 ? (const uint4 *)jd.d_msg : nullptr;
-        a.m = m;
+        a.msg = q == 0 ? (const uint4 *)d_msg : nullptr;
+        a.m = (int)m;
         a.log_d = log_d;
         a.layer0 = starts[q];
         a.k = ks[q];
@@ -184,82 +180,104 @@ int job_stages(const b64_job_desc &jd, const Layout &L, std::vector<Stage> &out,
         }
         out.push_back(st);
     }
-    // leaves of the codeword + interleaved fold into oracle 0
-    uint4 *o0 = (uint4 *)(ws + L.oracle[0]);
-    {
+    return B64_OK;
+}
+
+// 16-element leaves of the codeword (with the interleaved fold into oracle 0
+// when chal is given) at SLOT_LEAF16.
+void leaf16_stage(const Layout &L, uint8_t *ws, const uint64_t *chal, std::vector<Stage> &out) {
+    Stage st;
+    st.slot = SLOT_LEAF16;
+    st.kind = b64::MultiKind::Leaf16Fold;
+    st.leaf16.cw = (const uint4 *)(ws + L.cw);
+    st.leaf16.n_leaves = 1ull << L.rs;
+    st.leaf16.digests = (uint4 *)(ws + L.tree0);
+    st.leaf16.folded = chal ? (uint4 *)(ws + L.oracle[0]) : nullptr;
+    for (int t = 0; t < 4; t++) st.leaf16.chal[t] = chal ? to_u4(chal + 2 * t) : make_uint4(0, 0, 0, 0);
+    st.blocks = blocks_for(st.leaf16.n_leaves);
+    out.push_back(st);
+}
+
+// The tree levels above 2^k leaves' digests, from slot0 on; its root is listed.
+void tree_stages(uint4 *tree, uint64_t n_leaves, int slot0, std::vector<Stage> &out,
+                 std::vector<const uint4 *> &roots) {
+    uint64_t offs[16], ns[16];
+    int lv[16];
+    const int steps = b64::merkle_reduce_steps(n_leaves, offs, ns, lv);
+    for (int i = 0; i < steps; i++) {
         Stage st;
-        st.slot = SLOT_LEAF16;
-        st.kind = b64::MultiKind::Leaf16Fold;
-        st.leaf16.cw = cw;
-        st.leaf16.n_leaves = 1ull << L.rs;
-        st.leaf16.digests = tree0;
-        st.leaf16.folded = o0;
-        for (int t = 0; t < 4; t++) st.leaf16.chal[t] = to_u4(jd.challenges + 2 * t);
-        st.blocks = blocks_for(st.leaf16.n_leaves);
+        st.slot = slot0 + i;
+        st.kind = b64::MultiKind::Subtree;
+        st.sub.tree = tree;
+        st.sub.in_off = offs[i];
+        st.sub.n_in = ns[i];
+        st.sub.levels = lv[i];
+        st.sub.iv = job_iv();
+        st.blocks = (uint32_t)(ns[i] / b64::merkle_chunk(ns[i]));
         out.push_back(st);
     }
-    auto tree_steps = [&](uint4 *tree, uint64_t n_leaves, int slot0) {
-        uint64_t offs[16], ns[16];
-        int lv[16];
-        const int steps = b64::merkle_reduce_steps(n_leaves, offs, ns, lv);
-        for (int i = 0; i < steps; i++) {
-            Stage st;
-            st.slot = slot0 + i;
-            st.kind = b64::MultiKind::Subtree;
-            st.sub.tree = tree;
-            st.sub.in_off = offs[i];
-            st.sub.n_in = ns[i];
-            st.sub.levels = lv[i];
-            st.sub.iv = job_iv();
-            st.blocks = (uint32_t)(ns[i] / b64::merkle_chunk(ns[i]));
-            out.push_back(st);
-        }
-        roots.push_back(tree + 2 * (2 * n_leaves - 2));
-    };
-    tree_steps(tree0, 1ull << L.rs, SLOT_TREE0);
+    roots.push_back(tree + 2 * (2 * n_leaves - 2));
+}
+
+// Folded oracle k: for k > 0 fold_codeword of oracle k-1 with chal (the
+// arity's challenges), then its leaves (cosets) and tree.
+int oracle_stages(const Layout &L, uint8_t *ws, int k, const uint64_t *chal, std::vector<Stage> &out,
+                  std::vector<const uint4 *> &roots) {
+    const b64_fri_params &p = L.si->p;
+    const int base = SLOT_ORACLE + 16 * k;
+    uint4 *ok = (uint4 *)(ws + L.oracle[k]);
+    if (k > 0) {
+        const int a = (int)p.fold_arities[k - 1];
+        if (a > 4) return B64_ERR_INVALID_ARGS;
+        Stage st;
+        st.slot = base;
+        st.kind = b64::MultiKind::FoldCodeword;
+        st.fold.cw = (const uint4 *)(ws + L.oracle[k - 1]);
+        st.fold.out = ok;
+        st.fold.n_out = 1ull << L.olen[k];
+        st.fold.log_len = L.olen[k - 1];
+        st.fold.n_chal = a;
+        st.fold.W = L.si->dd.W;
+        for (int t = 0; t < a; t++) st.fold.chal[t] = to_u4(chal + 2 * t);
+        st.blocks = blocks_for(st.fold.n_out);
+        out.push_back(st);
+    }
+    uint4 *tk = (uint4 *)(ws + L.tree[k]);
+    const uint64_t n_leaves = 1ull << (L.olen[k] - L.ocoset[k]);
+    Stage st;
+    st.slot = L.ocoset[k] == 4 ? base + 1 : base + 2;  // one kernel kind per slot
+    if (L.ocoset[k] == 4) {
+        st.kind = b64::MultiKind::Leaf16Fold;
+        st.leaf16.cw = ok;
+        st.leaf16.n_leaves = n_leaves;
+        st.leaf16.digests = tk;
+        st.leaf16.folded = nullptr;
+        for (int t = 0; t < 4; t++) st.leaf16.chal[t] = make_uint4(0, 0, 0, 0);
+    } else {
+        st.kind = b64::MultiKind::LeafHash;
+        st.leaf.elems = ok;
+        st.leaf.n_leaves = n_leaves;
+        st.leaf.batch = 1ull << L.ocoset[k];
+        st.leaf.digests = tk;
+    }
+    st.blocks = blocks_for(n_leaves);
+    out.push_back(st);
+    tree_stages(tk, n_leaves, base + 3, out, roots);
+    return B64_OK;
+}
+
+int job_stages(const b64_job_desc &jd, const Layout &L, std::vector<Stage> &out, std::vector<const uint4 *> &roots) {
+    const b64_fri_params &p = L.si->p;
+    uint8_t *ws = (uint8_t *)jd.d_workspace;
+    B64_TRY(ntt_stages(L, jd.d_msg, jd.log_msg_len, (uint4 *)(ws + L.cw), out));
+    // leaves of the codeword + interleaved fold into oracle 0, then tree 0
+    leaf16_stage(L, ws, jd.challenges, out);
+    tree_stages((uint4 *)(ws + L.tree0), 1ull << L.rs, SLOT_TREE0, out, roots);
     // folded oracles: fold (k > 0), leaves, tree
     int chal_at = L.lbs;
     for (int k = 0; k < L.n_oracles; k++) {
-        const int base = SLOT_ORACLE + 16 * k;
-        uint4 *ok = (uint4 *)(ws + L.oracle[k]);
-        if (k > 0) {
-            const int a = (int)p.fold_arities[k - 1];
-            if (a > 4) return B64_ERR_INVALID_ARGS;
-            Stage st;
-            st.slot = base;
-            st.kind = b64::MultiKind::FoldCodeword;
-            st.fold.cw = (const uint4 *)(ws + L.oracle[k - 1]);
-            st.fold.out = ok;
-            st.fold.n_out = 1ull << L.olen[k];
-            st.fold.log_len = L.olen[k - 1];
-            st.fold.n_chal = a;
-            st.fold.W = si->dd.W;
-            for (int t = 0; t < a; t++) st.fold.chal[t] = to_u4(jd.challenges + 2 * (chal_at + t));
-            chal_at += a;
-            st.blocks = blocks_for(st.fold.n_out);
-            out.push_back(st);
-        }
-        uint4 *tk = (uint4 *)(ws + L.tree[k]);
-        const uint64_t n_leaves = 1ull << (L.olen[k] - L.ocoset[k]);
-        Stage st;
-        st.slot = L.ocoset[k] == 4 ? base + 1 : base + 2;  // one kernel kind per slot
-        if (L.ocoset[k] == 4) {
-            st.kind = b64::MultiKind::Leaf16Fold;
-            st.leaf16.cw = ok;
-            st.leaf16.n_leaves = n_leaves;
-            st.leaf16.digests = tk;
-            st.leaf16.folded = nullptr;
-            for (int t = 0; t < 4; t++) st.leaf16.chal[t] = make_uint4(0, 0, 0, 0);
-        } else {
-            st.kind = b64::MultiKind::LeafHash;
-            st.leaf.elems = ok;
-            st.leaf.n_leaves = n_leaves;
-            st.leaf.batch = 1ull << L.ocoset[k];
-            st.leaf.digests = tk;
-        }
-        st.blocks = blocks_for(n_leaves);
-        out.push_back(st);
-        tree_steps(tk, n_leaves, base + 3);
+        B64_TRY(oracle_stages(L, ws, k, jd.challenges + 2 * chal_at, out, roots));
+        if (k > 0) chal_at += (int)p.fold_arities[k - 1];
     }
     return B64_OK;
 }
@@ -274,14 +292,81 @@ __global__ void k_gather_roots(const uint4 *const *__restrict__ src, uint4 *__re
 
 }  // namespace
 
+namespace {
+
+// One multi-job launch: a stage slot over every job that has it.
+struct Launch {
+    b64::MultiKind kind;
+    size_t args_off, start_off;  // into the device tables
+    int n_jobs;
+    uint32_t blocks;
+    int fold_nc;  // FoldCodeword: the n_chal all jobs share (1..4), else 0
+};
+
+typedef std::map<std::pair<int, int>, std::vector<Stage>> SlotMap;
+
+size_t put_bytes(std::vector<uint8_t> &host, const void *p, size_t n) {
+    const size_t off = align_up(host.size());
+    host.resize(off + n);
+    memcpy(host.data() + off, p, n);
+    return off;
+}
+
+// Per slot (in slot order) an args array and a block-start array, appended to host.
+int build_launches(const SlotMap &slots, std::vector<uint8_t> &host, std::vector<Launch> &launches) {
+    for (auto &kv : slots) {
+        const std::vector<Stage> &v = kv.second;
+        // stages in one slot share a kind
+        Launch l;
+        l.kind = v[0].kind;
+        l.n_jobs = (int)v.size();
+        l.fold_nc = l.kind == b64::MultiKind::FoldCodeword ? v[0].fold.n_chal : 0;
+        std::vector<uint32_t> start;
+        uint32_t total = 0;
+        std::vector<uint8_t> args;
+        for (const Stage &x : v) {
+            if (x.kind != l.kind) return B64_ERR_INVALID_ARGS;
+            if (x.kind == b64::MultiKind::FoldCodeword && x.fold.n_chal != l.fold_nc) l.fold_nc = 0;
+            start.push_back(total);
+            total += x.blocks;
+            const void *a = nullptr;
+            size_t sz = 0;
+            switch (x.kind) {
+                case b64::MultiKind::NttPass: a = &x.ntt; sz = sizeof(x.ntt); break;
+                case b64::MultiKind::NttR16:
+                case b64::MultiKind::NttR16Msg:
+                case b64::MultiKind::NttR16W8:
+                case b64::MultiKind::NttR16W8Msg:
+                case b64::MultiKind::NttT4U9:
+                case b64::MultiKind::NttT4U10: a = &x.nttj; sz = sizeof(x.nttj); break;
+                case b64::MultiKind::Leaf16Fold: a = &x.leaf16; sz = sizeof(x.leaf16); break;
+                case b64::MultiKind::LeafHash: a = &x.leaf; sz = sizeof(x.leaf); break;
+                case b64::MultiKind::Subtree: a = &x.sub; sz = sizeof(x.sub); break;
+                case b64::MultiKind::FoldCodeword: a = &x.fold; sz = sizeof(x.fold); break;
+                case b64::MultiKind::FoldInterleaved: a = &x.foldil; sz = sizeof(x.foldil); break;
+            }
+            const size_t o = args.size();
+            args.resize(o + sz);
+            memcpy(args.data() + o, a, sz);
+        }
+        l.blocks = total;
+        l.args_off = put_bytes(host, args.data(), args.size());
+        l.start_off = put_bytes(host, start.data(), start.size() * sizeof(uint32_t));
+        launches.push_back(l);
+    }
+    return B64_OK;
+}
+
+int enqueue_launches(const std::vector<Launch> &launches, const uint8_t *d_tables, hipStream_t s) {
+    for (const Launch &l : launches)
+        HIP_TRY(b64::launch_multi(l.kind, d_tables + l.args_off, (const uint32_t *)(d_tables + l.start_off), l.n_jobs,
+                                  l.blocks, s, l.fold_nc));
+    return B64_OK;
+}
+
+}  // namespace
+
 struct b64_batch_plan {
-    struct Launch {
-        b64::MultiKind kind;
-        size_t args_off, start_off;  // into d_tables
-        int n_jobs;
-        uint32_t blocks;
-        int fold_nc;  // FoldCodeword: the n_chal all jobs share (1..4), else 0
-    };
     std::vector<Launch> launches;
     std::vector<b64_job_desc> jobs;
     std::vector<int> roots_per_job;
@@ -310,9 +395,7 @@ struct b64_batch_plan {
     }
 
     int enqueue(hipStream_t s) const {
-        for (const Launch &l : launches)
-            HIP_TRY(b64::launch_multi(l.kind, d_tables + l.args_off, (const uint32_t *)(d_tables + l.start_off),
-                                      l.n_jobs, l.blocks, s, l.fold_nc));
+        B64_TRY(enqueue_launches(launches, d_tables, s));
         if (n_roots) {
             k_gather_roots<<<(n_roots + 63) / 64, 64, 0, s>>>(d_root_src, d_roots, n_roots);
             HIP_TRY(hipGetLastError());
@@ -451,7 +534,7 @@ int b64_batch_plan_build(const b64_job_desc *jobs, uint32_t n_jobs, int use_grap
     }
     // per-slot job lists; an NTT slot splits by kernel (jobs of different sizes
     // may run a pass in different kernels)
-    std::map<std::pair<int, int>, std::vector<Stage>> slots;
+    SlotMap slots;
     std::vector<const uint4 *> root_src;
     for (uint32_t j = 0; j < n_jobs; j++) {
         const b64_job_desc &jd = jobs[j];
@@ -471,52 +554,8 @@ int b64_batch_plan_build(const b64_job_desc *jobs, uint32_t n_jobs, int use_grap
     plan->n_roots = (int)root_src.size();
     // device tables: per launch an args array and a block-start array
     std::vector<uint8_t> host;
-    auto put = [&](const void *p, size_t n) {
-        const size_t off = align_up(host.size());
-        host.resize(off + n);
-        memcpy(host.data() + off, p, n);
-        return off;
-    };
-    for (auto &kv : slots) {
-        const std::vector<Stage> &v = kv.second;
-        // stages in one slot share a kind
-        b64_batch_plan::Launch l;
-        l.kind = v[0].kind;
-        l.n_jobs = (int)v.size();
-        l.fold_nc = l.kind == b64::MultiKind::FoldCodeword ? v[0].fold.n_chal : 0;
-        std::vector<uint32_t> start;
-        uint32_t total = 0;
-        std::vector<uint8_t> args;
-        for (const Stage &x : v) {
-            if (x.kind != l.kind) return B64_ERR_INVALID_ARGS;
-            if (x.kind == b64::MultiKind::FoldCodeword && x.fold.n_chal != l.fold_nc) l.fold_nc = 0;
-            start.push_back(total);
-            total += x.blocks;
-            const void *a = nullptr;
-            size_t sz = 0;
-            switch (x.kind) {
-                case b64::MultiKind::NttPass: a = &x.ntt; sz = sizeof(x.ntt); break;
-                case b64::MultiKind::NttR16:
-                case b64::MultiKind::NttR16Msg:
-                case b64::MultiKind::NttR16W8:
-                case b64::MultiKind::NttR16W8Msg:
-                case b64::MultiKind::NttT4U9:
-                case b64::MultiKind::NttT4U10: a = &x.nttj; sz = sizeof(x.nttj); break;
-                case b64::MultiKind::Leaf16Fold: a = &x.leaf16; sz = sizeof(x.leaf16); break;
-                case b64::MultiKind::LeafHash: a = &x.leaf; sz = sizeof(x.leaf); break;
-                case b64::MultiKind::Subtree: a = &x.sub; sz = sizeof(x.sub); break;
-                case b64::MultiKind::FoldCodeword: a = &x.fold; sz = sizeof(x.fold); break;
-            }
-            const size_t o = args.size();
-            args.resize(o + sz);
-            memcpy(args.data() + o, a, sz);
-        }
-        l.blocks = total;
-        l.args_off = put(args.data(), args.size());
-        l.start_off = put(start.data(), start.size() * sizeof(uint32_t));
-        plan->launches.push_back(l);
-    }
-    const size_t root_src_off = put(root_src.data(), root_src.size() * sizeof(void *));
+    B64_TRY(build_launches(slots, host, plan->launches));
+    const size_t root_src_off = put_bytes(host, root_src.data(), root_src.size() * sizeof(void *));
     if (st) {
         // caller-owned buffers, uploaded in stream order (no allocation, no device sync)
         if (host.size() > st->tables_cap || (size_t)plan->n_roots > st->roots_cap) return B64_ERR_BUFFER_TOO_SMALL;
@@ -591,3 +630,223 @@ uint32_t b64_batch_plan_n_launches(const b64_batch_plan *plan) { return plan ? (
 void b64_batch_plan_destroy(b64_batch_plan *plan) { delete plan; }
 
 }  // extern "C"
+
+// ===========================================================================
+// Batched proofs: each job's FRI commit phase driven by its own transcript.
+// Round 0 commits every codeword (NTT passes, 16-element leaves, tree 0); each
+// later round folds and commits the next oracle of every job that has one due,
+// after the host has written the previous roots to the jobs' transcripts and
+// sampled their challenges.  The query phases then share one gather launch.
+// ===========================================================================
+namespace {
+
+struct ProveJob {
+    b64_prove_job jd;
+    Layout L;
+    uint32_t curr_round = 0, next_commit = 0;
+    bool has_next = true;
+    int k = 0;           // next oracle to commit
+    int n_roots = 0;
+    bool due = false;    // oracle k's fold challenges are in `unprocessed`
+    std::vector<uint64_t> unprocessed;  // challenges as (lo, hi) pairs
+    uint32_t n_rounds() const { return L.si->p.log_dim + L.si->p.log_batch_size; }
+
+    // FRIFoldProver rounds (fold.rs:107-175): sample and receive challenges
+    // until a commitment is due or every round has its challenge.
+    void advance() {
+        for (;;) {
+            if (has_next && next_commit == curr_round) {
+                due = true;
+                return;
+            }
+            if (curr_round >= n_rounds()) return;
+            uint8_t c[16];
+            b64_transcript_sample(jd.transcript, c, 16);
+            uint64_t w[2];
+            memcpy(w, c, 16);
+            unprocessed.push_back(w[0]);
+            unprocessed.push_back(w[1]);
+            curr_round++;
+        }
+    }
+    // A root written by the round that committed oracle k - 1 (or the codeword).
+    void take_root(const uint8_t *root) {
+        b64_transcript_message(jd.transcript, root, 32);
+        if (jd.roots_out) memcpy(jd.roots_out + 32 * n_roots, root, 32);
+        n_roots++;
+    }
+};
+
+// Device tables and root buffers of the rounds, grown as needed (between rounds,
+// after the host has waited for the previous one).
+struct RoundBuffers {
+    uint8_t *d_tables = nullptr, *h_stage = nullptr, *h_roots = nullptr;
+    uint4 *d_roots = nullptr;
+    size_t tables_cap = 0, roots_cap = 0;
+    ~RoundBuffers() {
+        if (d_tables) (void)hipFree(d_tables);
+        if (h_stage) (void)hipHostFree(h_stage);
+        if (d_roots) (void)hipFree(d_roots);
+        if (h_roots) (void)hipHostFree(h_roots);
+    }
+    int reserve(size_t tables, size_t roots) {
+        if (tables > tables_cap) {
+            if (d_tables) HIP_TRY(hipFree(d_tables));
+            if (h_stage) HIP_TRY(hipHostFree(h_stage));
+            d_tables = nullptr;
+            h_stage = nullptr;
+            tables_cap = 0;
+            HIP_TRY(hipMalloc(&d_tables, tables));
+            HIP_TRY(hipHostMalloc((void **)&h_stage, tables, hipHostMallocDefault));
+            tables_cap = tables;
+        }
+        if (roots > roots_cap) {
+            if (d_roots) HIP_TRY(hipFree(d_roots));
+            if (h_roots) HIP_TRY(hipHostFree(h_roots));
+            d_roots = nullptr;
+            h_roots = nullptr;
+            roots_cap = 0;
+            HIP_TRY(hipMalloc(&d_roots, 32 * roots));
+            HIP_TRY(hipHostMalloc((void **)&h_roots, 32 * roots, hipHostMallocDefault));
+            roots_cap = roots;
+        }
+        return B64_OK;
+    }
+};
+
+// One round: the stages of every job grouped into multi-job launches, then the
+// listed roots copied to buf.h_roots; waits for the stream.
+int run_round(const SlotMap &slots, const std::vector<const uint4 *> &root_src, RoundBuffers &buf, hipStream_t s) {
+    std::vector<uint8_t> host;
+    std::vector<Launch> launches;
+    B64_TRY(build_launches(slots, host, launches));
+    const size_t src_off = put_bytes(host, root_src.data(), root_src.size() * sizeof(void *));
+    B64_TRY(buf.reserve(host.size() ? host.size() : 16, root_src.size() ? root_src.size() : 1));
+    memcpy(buf.h_stage, host.data(), host.size());
+    HIP_TRY(hipMemcpyAsync(buf.d_tables, buf.h_stage, host.size(), hipMemcpyHostToDevice, s));
+    B64_TRY(enqueue_launches(launches, buf.d_tables, s));
+    const int n = (int)root_src.size();
+    if (n) {
+        k_gather_roots<<<(n + 63) / 64, 64, 0, s>>>((const uint4 *const *)(buf.d_tables + src_off), buf.d_roots, n);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(buf.h_roots, buf.d_roots, 32ull * n, hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return B64_OK;
+}
+
+}  // namespace
+
+extern "C" int b64_batch_prove(const b64_prove_job *jobs, uint32_t n_jobs, void *stream) {
+    if (!jobs || !n_jobs) return B64_ERR_INVALID_ARGS;
+    const hipStream_t s = as_stream(stream);
+    std::vector<ProveJob> pj(n_jobs);
+    for (uint32_t j = 0; j < n_jobs; j++) {
+        const b64_prove_job &jd = jobs[j];
+        if (!jd.d_msg || !jd.d_workspace || !jd.transcript) return B64_ERR_INVALID_ARGS;
+        SizeInfo *si = nullptr;
+        B64_TRY(size_info(jd.log_msg_len, &si));
+        B64_TRY(size_device(si));
+        pj[j].jd = jd;
+        layout_of(si, pj[j].L);
+        pj[j].next_commit = si->p.log_batch_size;  // FRIFoldProver::new (fold.rs:70)
+    }
+    RoundBuffers buf;
+    // round 0: commit_interleaved of every message
+    {
+        SlotMap slots;
+        std::vector<const uint4 *> roots;
+        for (ProveJob &J : pj) {
+            uint8_t *ws = (uint8_t *)J.jd.d_workspace;
+            std::vector<Stage> st;
+            B64_TRY(ntt_stages(J.L, J.jd.d_msg, J.jd.log_msg_len, (uint4 *)(ws + J.L.cw), st));
+            leaf16_stage(J.L, ws, nullptr, st);
+            tree_stages((uint4 *)(ws + J.L.tree0), 1ull << J.L.rs, SLOT_TREE0, st, roots);
+            for (const Stage &x : st) slots[{x.slot, (int)x.kind}].push_back(x);
+        }
+        B64_TRY(run_round(slots, roots, buf, s));
+        for (uint32_t j = 0; j < n_jobs; j++) {
+            pj[j].take_root(buf.h_roots + 32 * j);
+            pj[j].advance();
+        }
+    }
+    // fold rounds: oracle k of every job with a commitment due
+    for (;;) {
+        SlotMap slots;
+        std::vector<const uint4 *> roots;
+        std::vector<ProveJob *> due;
+        for (ProveJob &J : pj) {
+            if (!J.due) continue;
+            const Layout &L = J.L;
+            const b64_fri_params &p = L.si->p;
+            uint8_t *ws = (uint8_t *)J.jd.d_workspace;
+            if (J.k >= L.n_oracles) return B64_ERR_INVALID_ARGS;
+            const size_t n_ch = J.unprocessed.size() / 2;
+            std::vector<Stage> st;
+            if (J.k == 0) {
+                // fold_interleaved (fold.rs:260-282) of the codeword into oracle 0
+                if (n_ch != (size_t)L.lbs || L.lbs != 4) return B64_ERR_INVALID_ARGS;
+                Stage f;
+                f.slot = SLOT_ORACLE - 1;
+                f.kind = b64::MultiKind::FoldInterleaved;
+                f.foldil.cw = (const uint4 *)(ws + L.cw);
+                f.foldil.out = (uint4 *)(ws + L.oracle[0]);
+                f.foldil.n_out = 1ull << L.rs;
+                for (int t = 0; t < 4; t++) f.foldil.chal[t] = to_u4(J.unprocessed.data() + 2 * t);
+                f.blocks = blocks_for(f.foldil.n_out);
+                st.push_back(f);
+            } else if (n_ch != p.fold_arities[J.k - 1]) {
+                return B64_ERR_INVALID_ARGS;
+            }
+            B64_TRY(oracle_stages(L, ws, J.k, J.unprocessed.data(), st, roots));
+            for (const Stage &x : st) slots[{x.slot, (int)x.kind}].push_back(x);
+            due.push_back(&J);
+        }
+        if (due.empty()) break;
+        B64_TRY(run_round(slots, roots, buf, s));
+        for (size_t i = 0; i < due.size(); i++) {
+            ProveJob &J = *due[i];
+            const b64_fri_params &p = J.L.si->p;
+            J.unprocessed.clear();
+            J.due = false;
+            if ((uint32_t)J.k < p.n_fold_arities)
+                J.next_commit += p.fold_arities[J.k];
+            else
+                J.has_next = false;
+            J.k++;
+            J.take_root(buf.h_roots + 32 * i);
+            J.advance();
+        }
+    }
+    // finish_proof of every job: one wait for the terminal codewords and layers,
+    // one gather launch for every opening
+    std::vector<b64::FriQuery> q(n_jobs);
+    for (uint32_t j = 0; j < n_jobs; j++) {
+        const ProveJob &J = pj[j];
+        const Layout &L = J.L;
+        if (J.curr_round != J.n_rounds() || J.k != L.n_oracles) return B64_ERR_EARLY_PROVER_FINISH;
+        const uint8_t *ws = (const uint8_t *)J.jd.d_workspace;
+        const b64::FriOracleView first = {(const uint4 *)(ws + L.cw), (const uint4 *)(ws + L.tree0), L.rs + L.lbs,
+                                          L.rs};
+        std::vector<b64::FriOracleView> committed;
+        for (int k = 0; k < L.n_oracles; k++)
+            committed.push_back({(const uint4 *)(ws + L.oracle[k]), (const uint4 *)(ws + L.tree[k]), L.olen[k],
+                                 L.olen[k] - L.ocoset[k]});
+        B64_TRY(q[j].prepare(L.si->p, first, committed.data(), committed.size()));
+        B64_TRY(q[j].fetch(s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    std::vector<uint64_t> src;
+    for (uint32_t j = 0; j < n_jobs; j++) {
+        B64_TRY(q[j].transcribe(pj[j].jd.transcript));
+        src.insert(src.end(), q[j].src.begin(), q[j].src.end());
+    }
+    std::vector<uint8_t> out(src.size() * 16);
+    B64_TRY(b64::gather_granules(src.data(), src.size(), out.data(), s));
+    size_t at = 0;
+    for (uint32_t j = 0; j < n_jobs; j++) {
+        q[j].openings(pj[j].jd.transcript, out.data() + 16 * at);
+        at += q[j].src.size();
+    }
+    return B64_OK;
+}

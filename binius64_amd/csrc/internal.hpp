@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <vector>
 #include <stddef.h>
 
 #include "../../include/b64hip.h"
@@ -89,6 +91,13 @@ struct FoldCwArgs {
     const uint4 *W;
     uint4 chal[MAX_FOLD_CHAL];
 };
+// One job's fold_interleaved of a batch-2^4 codeword (16 elements per output).
+struct FoldIlArgs {
+    const uint4 *cw;
+    uint4 *out;
+    uint64_t n_out;
+    uint4 chal[4];
+};
 // A pass of one job in a multi-job register or small-table pass launch.
 struct NttJobArgs {
     NttPassArgs a;
@@ -97,7 +106,8 @@ struct NttJobArgs {
 enum class MultiKind {
     NttPass,  // multi-job LDS-tile kernel with 32-row tables (NttPassArgs)
     NttR16, NttR16Msg, NttR16W8, NttR16W8Msg, NttT4U9, NttT4U10,  // (NttJobArgs)
-    Leaf16Fold, LeafHash, Subtree, FoldCodeword
+    Leaf16Fold, LeafHash, Subtree, FoldCodeword,
+    FoldInterleaved  // fold_interleaved of batch 2^4 (FoldIlArgs)
 };
 
 struct NttTuning {
@@ -179,6 +189,39 @@ hipError_t launch_rs_fold_1b_rows(const uint4 *mat, const uint4 *vec, uint64_t n
 hipError_t launch_gather16(const uint64_t *src, uint4 *dst, uint64_t n, hipStream_t s);
 hipError_t launch_pack_witness(const uint64_t *words, uint64_t n_words, uint4 *out, uint64_t n_elems,
                                hipStream_t s);
+
+// A committed FRI oracle on the device: codeword 2^log_len elements, Merkle tree
+// over 2^tree_log_len cosets.
+struct FriOracleView {
+    const uint4 *cw;
+    const uint4 *tree;
+    int log_len, tree_log_len;
+};
+
+// The query phase of FRIFoldProver::finish_proof (prover fri/fold.rs:221-244,
+// query.rs:51-134), in steps so that many proofs can share device round trips:
+//   prepare  -- the oracles (the interleaved codeword, then every committed one);
+//   fetch    -- enqueue the copies of the terminal codeword and the opened layers;
+//   transcribe (after those copies) -- decommit them, sample the query indices and
+//               list every opening as 16-byte granules (device addresses, src);
+//   openings -- decommit the gathered granules (src.size() * 16 bytes).
+struct FriQuery {
+    b64_fri_params p;
+    FriOracleView term;
+    std::vector<FriOracleView> opened;
+    std::vector<int> depths;
+    std::vector<uint8_t> term_buf;
+    std::vector<std::vector<uint8_t>> layer_buf;
+    std::vector<uint64_t> src;
+    int prepare(const b64_fri_params &params, const FriOracleView &first, const FriOracleView *committed,
+                size_t n_committed);
+    int fetch(hipStream_t s);
+    int transcribe(b64_transcript *t);
+    void openings(b64_transcript *t, const uint8_t *granules) const;
+};
+// out[i] = the 16 bytes at device address src[i], for i < n: one launch, one copy,
+// then a stream synchronize.
+int gather_granules(const uint64_t *src, size_t n, uint8_t *out, hipStream_t s);
 
 // Caller-owned buffers for a batch plan (the serving loop reuses one per stream
 // slot, so building a plan neither allocates nor synchronizes the device).

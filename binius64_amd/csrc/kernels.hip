@@ -1263,13 +1263,13 @@ struct FoldChallenges {
 // with r_t, the eq tensor's bit order, multilinear/eq.rs:39-71); the result is
 // the same field element as the inner product.
 template <int LOGB>
-__global__ __launch_bounds__(BLOCK) void k_fold_interleaved(const uint4 *__restrict__ cw, uint4 *__restrict__ out,
-                                                            uint64_t n_out, FoldChallenges ch) {
-    __shared__ uint4 tabs[(LOGB > 0 ? LOGB : 1) * WIN_ENTRIES];
+__device__ __forceinline__ void fold_interleaved_core(const uint4 *__restrict__ cw, uint4 *__restrict__ out,
+                                                      uint64_t n_out, const uint4 *chal, uint64_t blk,
+                                                      uint4 *__restrict__ tabs) {
     for (int r = threadIdx.x; r < LOGB * WIN_ROWS; r += BLOCK)
-        win_build_row(tabs + (r >> 5) * WIN_ENTRIES, ch.r[r >> 5], r & 31);
+        win_build_row(tabs + (r >> 5) * WIN_ENTRIES, chal[r >> 5], r & 31);
     __syncthreads();
-    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t i = blk * BLOCK + threadIdx.x;
     if (i >= n_out) return;
     constexpr int B = 1 << LOGB;
     uint4 v[B];
@@ -1283,6 +1283,13 @@ __global__ __launch_bounds__(BLOCK) void k_fold_interleaved(const uint4 *__restr
         for (int q = 0; q < len; q++) v[q] = gxor(v[2 * q], win_mul(tabs + t * WIN_ENTRIES, gxor(v[2 * q], v[2 * q + 1])));
     }
     out[i] = v[0];
+}
+
+template <int LOGB>
+__global__ __launch_bounds__(BLOCK) void k_fold_interleaved(const uint4 *__restrict__ cw, uint4 *__restrict__ out,
+                                                            uint64_t n_out, FoldChallenges ch) {
+    __shared__ uint4 tabs[(LOGB > 0 ? LOGB : 1) * WIN_ENTRIES];
+    fold_interleaved_core<LOGB>(cw, out, n_out, ch.r, blockIdx.x, tabs);
 }
 
 hipError_t launch_fold_interleaved(const uint4 *cw, uint4 *out, uint64_t n_out, int log_batch,
@@ -1496,6 +1503,15 @@ __global__ __launch_bounds__(BLOCK) void k_ntt_pass_tab_multi(const NttPassArgs 
     ntt_tab_tile<false, G, true>(a, blockIdx.x - start[j], sm);
 }
 
+// fold_interleaved (batch 2^4) over many jobs.
+__global__ __launch_bounds__(BLOCK) void k_fold_interleaved_multi(const FoldIlArgs *__restrict__ jobs,
+                                                                 const uint32_t *__restrict__ start, int n_jobs) {
+    __shared__ uint4 tabs[4 * WIN_ENTRIES];
+    const int j = job_of_block2(start, n_jobs, blockIdx.x);
+    const FoldIlArgs &a = jobs[j];
+    fold_interleaved_core<4>(a.cw, a.out, a.n_out, a.chal, blockIdx.x - start[j], tabs);
+}
+
 // The register and small-table passes over many jobs: each job brings its own
 // pass arguments and positions per block (NttJobArgs).
 template <bool MSG>
@@ -1619,6 +1635,9 @@ hipError_t launch_multi(MultiKind kind, const void *d_jobs, const uint32_t *d_st
             }
             break;
         }
+        case MultiKind::FoldInterleaved:
+            k_fold_interleaved_multi<<<blocks, BLOCK, 0, s>>>((const FoldIlArgs *)d_jobs, d_start, n_jobs);
+            break;
     }
     return hipGetLastError();
 }
