@@ -788,6 +788,18 @@ class Server:
         self._held[t.value] = m
         return t.value
 
+    def submit_proof(self, log_msg_len: int, message: np.ndarray, transcript: ProverTranscript) -> int:
+        """Queue a proof job (b64_server_submit_proof): its commitment and FRI proof
+        are written to `transcript`, which is held (with the message) until
+        poll/wait reports the job done."""
+        m = np.ascontiguousarray(_elems(message))
+        assert m.shape[0] == 1 << log_msg_len
+        t = C.c_uint64()
+        check(lib().b64_server_submit_proof(self.ptr, log_msg_len, _p64(m), transcript.ptr, C.byref(t)),
+              "server_submit_proof")
+        self._held[t.value] = (m, transcript)
+        return t.value
+
     def poll(self, ticket: int, n_roots: int):
         roots = np.zeros(32 * n_roots, dtype=np.uint8)
         done, lat = C.c_int(0), C.c_double(0)

@@ -90,6 +90,7 @@ SIGNATURES = {
     "b64_batch_plan_collect": (C.c_int, [vp]),
     "b64_server_create": (C.c_int, [C.POINTER(ServerConfigC), C.POINTER(vp)]),
     "b64_server_submit": (C.c_int, [vp, C.c_uint32, u64p, u64p, C.POINTER(C.c_uint64)]),
+    "b64_server_submit_proof": (C.c_int, [vp, C.c_uint32, u64p, vp, C.POINTER(C.c_uint64)]),
     "b64_server_poll": (C.c_int, [vp, C.c_uint64, u8p, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "b64_server_wait": (C.c_int, [vp, C.c_uint64, u8p, C.POINTER(C.c_double)]),
     "b64_server_stats": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_size_t)]),

@@ -917,24 +917,33 @@ int b64::FriQuery::prepare(const b64_fri_params &params, const FriOracleView &fi
     return B64_OK;
 }
 
-int b64::FriQuery::fetch(hipStream_t s) {
-    term_buf.resize(16ull << term.log_len);
-    HIP_TRY(hipMemcpyAsync(term_buf.data(), term.cw, term_buf.size(), hipMemcpyDeviceToHost, s));
-    layer_buf.assign(opened.size(), {});
+size_t b64::FriQuery::fetch_bytes() const {
+    size_t n = 16ull << term.log_len;
+    for (size_t i = 0; i < opened.size(); i++) n += 32ull << depths[i];
+    return n;
+}
+
+int b64::FriQuery::fetch(hipStream_t s, uint8_t *dst) const {
+    HIP_TRY(hipMemcpyAsync(dst, term.cw, 16ull << term.log_len, hipMemcpyDeviceToHost, s));
+    dst += 16ull << term.log_len;
     for (size_t i = 0; i < opened.size(); i++) {
         const FriOracleView &o = opened[i];
         const uint64_t total = (2ull << o.tree_log_len) - 1;
         const uint64_t start = total + 1 - (2ull << depths[i]);
-        layer_buf[i].resize(32ull << depths[i]);
-        HIP_TRY(hipMemcpyAsync(layer_buf[i].data(), (const uint8_t *)o.tree + 32 * start, layer_buf[i].size(),
-                               hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(dst, (const uint8_t *)o.tree + 32 * start, 32ull << depths[i], hipMemcpyDeviceToHost,
+                               s));
+        dst += 32ull << depths[i];
     }
     return B64_OK;
 }
 
-int b64::FriQuery::transcribe(b64_transcript *t) {
-    b64_transcript_decommit(t, term_buf.data(), term_buf.size());
-    for (const auto &layer : layer_buf) b64_transcript_decommit(t, layer.data(), layer.size());
+int b64::FriQuery::transcribe(b64_transcript *t, const uint8_t *fetched) {
+    b64_transcript_decommit(t, fetched, 16ull << term.log_len);
+    fetched += 16ull << term.log_len;
+    for (size_t i = 0; i < opened.size(); i++) {
+        b64_transcript_decommit(t, fetched, 32ull << depths[i]);
+        fetched += 32ull << depths[i];
+    }
     // queries: decommitments are not observed, so every index can be sampled
     // first.  Each opening (prove_coset_opening, query.rs:112-134: the coset's
     // values, then the branch digests from the leaf up) is listed as 16-byte
@@ -1126,9 +1135,10 @@ int b64_fri_prover_finish_proof(b64_fri_prover *fp, b64_transcript *t) {
     const b64::FriOracleView first = {fp->first.cw, fp->first.tree, fp->first.log_len, fp->first.tree_log_len};
     b64::FriQuery q;
     B64_TRY(q.prepare(fp->p, first, committed.data(), committed.size()));
-    B64_TRY(q.fetch(fp->s));
+    std::vector<uint8_t> fetched(q.fetch_bytes());
+    B64_TRY(q.fetch(fp->s, fetched.data()));
     HIP_TRY(hipStreamSynchronize(fp->s));
-    B64_TRY(q.transcribe(t));
+    B64_TRY(q.transcribe(t, fetched.data()));
     std::vector<uint8_t> out(q.src.size() * 16);
     B64_TRY(b64::gather_granules(q.src.data(), q.src.size(), out.data(), fp->s));
     q.openings(t, out.data());

@@ -633,10 +633,13 @@ void b64_batch_plan_destroy(b64_batch_plan *plan) { delete plan; }
 
 // ===========================================================================
 // Batched proofs: each job's FRI commit phase driven by its own transcript.
-// Round 0 commits every codeword (NTT passes, 16-element leaves, tree 0); each
-// later round folds and commits the next oracle of every job that has one due,
+// Phase 0 commits every codeword (NTT passes, 16-element leaves, tree 0); each
+// fold phase folds and commits the next oracle of every job that has one due,
 // after the host has written the previous roots to the jobs' transcripts and
-// sampled their challenges.  The query phases then share one gather launch.
+// sampled their challenges; then the query phases of all jobs share one fetch
+// of terminal codewords and layers and one gather launch of the openings.
+// Every phase is enqueued without waiting; the driver (b64_batch_prove, or the
+// server's worker) waits for it and calls step() for the host part.
 // ===========================================================================
 namespace {
 
@@ -649,6 +652,8 @@ struct ProveJob {
     int n_roots = 0;
     bool due = false;    // oracle k's fold challenges are in `unprocessed`
     std::vector<uint64_t> unprocessed;  // challenges as (lo, hi) pairs
+    b64::FriQuery q;
+    size_t fetch_at = 0, src_at = 0;    // offsets of its query data in the phase buffers
     uint32_t n_rounds() const { return L.si->p.log_dim + L.si->p.log_batch_size; }
 
     // FRIFoldProver rounds (fold.rs:107-175): sample and receive challenges
@@ -669,7 +674,7 @@ struct ProveJob {
             curr_round++;
         }
     }
-    // A root written by the round that committed oracle k - 1 (or the codeword).
+    // The root of the codeword or of the oracle just committed, written to the transcript.
     void take_root(const uint8_t *root) {
         b64_transcript_message(jd.transcript, root, 32);
         if (jd.roots_out) memcpy(jd.roots_out + 32 * n_roots, root, 32);
@@ -677,176 +682,253 @@ struct ProveJob {
     }
 };
 
-// Device tables and root buffers of the rounds, grown as needed (between rounds,
-// after the host has waited for the previous one).
-struct RoundBuffers {
-    uint8_t *d_tables = nullptr, *h_stage = nullptr, *h_roots = nullptr;
-    uint4 *d_roots = nullptr;
-    size_t tables_cap = 0, roots_cap = 0;
-    ~RoundBuffers() {
-        if (d_tables) (void)hipFree(d_tables);
-        if (h_stage) (void)hipHostFree(h_stage);
-        if (d_roots) (void)hipFree(d_roots);
-        if (h_roots) (void)hipHostFree(h_roots);
-    }
-    int reserve(size_t tables, size_t roots) {
-        if (tables > tables_cap) {
-            if (d_tables) HIP_TRY(hipFree(d_tables));
-            if (h_stage) HIP_TRY(hipHostFree(h_stage));
-            d_tables = nullptr;
-            h_stage = nullptr;
-            tables_cap = 0;
-            HIP_TRY(hipMalloc(&d_tables, tables));
-            HIP_TRY(hipHostMalloc((void **)&h_stage, tables, hipHostMallocDefault));
-            tables_cap = tables;
-        }
-        if (roots > roots_cap) {
-            if (d_roots) HIP_TRY(hipFree(d_roots));
-            if (h_roots) HIP_TRY(hipHostFree(h_roots));
-            d_roots = nullptr;
-            h_roots = nullptr;
-            roots_cap = 0;
-            HIP_TRY(hipMalloc(&d_roots, 32 * roots));
-            HIP_TRY(hipHostMalloc((void **)&h_roots, 32 * roots, hipHostMallocDefault));
-            roots_cap = roots;
-        }
-        return B64_OK;
-    }
-};
+int grow_device(uint8_t **d, size_t *cap, size_t need) {
+    if (need <= *cap) return B64_OK;
+    if (*d) HIP_TRY(hipFree(*d));
+    *d = nullptr;
+    *cap = 0;
+    HIP_TRY(hipMalloc(d, need));
+    *cap = need;
+    return B64_OK;
+}
 
-// One round: the stages of every job grouped into multi-job launches, then the
-// listed roots copied to buf.h_roots; waits for the stream.
-int run_round(const SlotMap &slots, const std::vector<const uint4 *> &root_src, RoundBuffers &buf, hipStream_t s) {
-    std::vector<uint8_t> host;
-    std::vector<Launch> launches;
-    B64_TRY(build_launches(slots, host, launches));
-    const size_t src_off = put_bytes(host, root_src.data(), root_src.size() * sizeof(void *));
-    B64_TRY(buf.reserve(host.size() ? host.size() : 16, root_src.size() ? root_src.size() : 1));
-    memcpy(buf.h_stage, host.data(), host.size());
-    HIP_TRY(hipMemcpyAsync(buf.d_tables, buf.h_stage, host.size(), hipMemcpyHostToDevice, s));
-    B64_TRY(enqueue_launches(launches, buf.d_tables, s));
-    const int n = (int)root_src.size();
-    if (n) {
-        k_gather_roots<<<(n + 63) / 64, 64, 0, s>>>((const uint4 *const *)(buf.d_tables + src_off), buf.d_roots, n);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(buf.h_roots, buf.d_roots, 32ull * n, hipMemcpyDeviceToHost, s));
-    }
-    HIP_TRY(hipStreamSynchronize(s));
+int grow_pinned(uint8_t **h, size_t *cap, size_t need) {
+    if (need <= *cap) return B64_OK;
+    if (*h) HIP_TRY(hipHostFree(*h));
+    *h = nullptr;
+    *cap = 0;
+    HIP_TRY(hipHostMalloc((void **)h, need, hipHostMallocDefault));
+    *cap = need;
     return B64_OK;
 }
 
 }  // namespace
 
-extern "C" int b64_batch_prove(const b64_prove_job *jobs, uint32_t n_jobs, void *stream) {
-    if (!jobs || !n_jobs) return B64_ERR_INVALID_ARGS;
-    const hipStream_t s = as_stream(stream);
-    std::vector<ProveJob> pj(n_jobs);
+// Buffers of a batch's phases; they only grow, so a caller that keeps one set
+// (the server, per stream slot) stops allocating once it has seen its largest
+// batch.  Only touched between phases, after the previous phase has completed.
+struct b64::ProveBuffers {
+    uint8_t *d_tables = nullptr, *h_stage = nullptr;  // stage tables (and gather sources)
+    uint8_t *d_out = nullptr, *h_out = nullptr;       // roots, or gathered granules
+    uint8_t *h_fetch = nullptr;                       // terminal codewords and layers
+    size_t d_tables_cap = 0, h_stage_cap = 0, d_out_cap = 0, h_out_cap = 0, h_fetch_cap = 0;
+    ~ProveBuffers() {
+        if (d_tables) (void)hipFree(d_tables);
+        if (d_out) (void)hipFree(d_out);
+        if (h_stage) (void)hipHostFree(h_stage);
+        if (h_out) (void)hipHostFree(h_out);
+        if (h_fetch) (void)hipHostFree(h_fetch);
+    }
+};
+
+struct b64::ProveBatch {
+    enum class Phase { Rounds, Fetch, Gather, Done };
+    std::vector<ProveJob> pj;
+    b64::ProveBuffers *buf = nullptr;
+    Phase phase = Phase::Rounds;
+    bool first = true;               // the next round is phase 0 (the codewords' commitments)
+    std::vector<ProveJob *> in_round;  // jobs whose root the enqueued round writes, in root order
+    std::vector<uint8_t> host;
+    std::vector<Launch> launches;
+    size_t n_src = 0;
+
+    // Stages of the next round into host tables; false when no job has one due.
+    int build_round(bool *any) {
+        SlotMap slots;
+        std::vector<const uint4 *> roots;
+        in_round.clear();
+        for (ProveJob &J : pj) {
+            const Layout &L = J.L;
+            uint8_t *ws = (uint8_t *)J.jd.d_workspace;
+            std::vector<Stage> st;
+            if (first) {
+                B64_TRY(ntt_stages(L, J.jd.d_msg, J.jd.log_msg_len, (uint4 *)(ws + L.cw), st));
+                leaf16_stage(L, ws, nullptr, st);
+                tree_stages((uint4 *)(ws + L.tree0), 1ull << L.rs, SLOT_TREE0, st, roots);
+            } else {
+                if (!J.due) continue;
+                const b64_fri_params &p = L.si->p;
+                if (J.k >= L.n_oracles) return B64_ERR_INVALID_ARGS;
+                const size_t n_ch = J.unprocessed.size() / 2;
+                if (J.k == 0) {
+                    // fold_interleaved (fold.rs:260-282) of the codeword into oracle 0
+                    if (n_ch != (size_t)L.lbs || L.lbs != 4) return B64_ERR_INVALID_ARGS;
+                    Stage f;
+                    f.slot = SLOT_ORACLE - 1;
+                    f.kind = b64::MultiKind::FoldInterleaved;
+                    f.foldil.cw = (const uint4 *)(ws + L.cw);
+                    f.foldil.out = (uint4 *)(ws + L.oracle[0]);
+                    f.foldil.n_out = 1ull << L.rs;
+                    for (int t = 0; t < 4; t++) f.foldil.chal[t] = to_u4(J.unprocessed.data() + 2 * t);
+                    f.blocks = blocks_for(f.foldil.n_out);
+                    st.push_back(f);
+                } else if (n_ch != p.fold_arities[J.k - 1]) {
+                    return B64_ERR_INVALID_ARGS;
+                }
+                B64_TRY(oracle_stages(L, ws, J.k, J.unprocessed.data(), st, roots));
+            }
+            for (const Stage &x : st) slots[{x.slot, (int)x.kind}].push_back(x);
+            in_round.push_back(&J);
+        }
+        *any = !in_round.empty();
+        if (!*any) return B64_OK;
+        host.clear();
+        launches.clear();
+        B64_TRY(build_launches(slots, host, launches));
+        roots_off = put_bytes(host, roots.data(), roots.size() * sizeof(void *));
+        return B64_OK;
+    }
+    size_t roots_off = 0;
+
+    int enqueue(hipStream_t s) {
+        switch (phase) {
+            case Phase::Rounds: {
+                const size_t n = in_round.size();
+                B64_TRY(grow_device(&buf->d_tables, &buf->d_tables_cap, host.size()));
+                B64_TRY(grow_pinned(&buf->h_stage, &buf->h_stage_cap, host.size()));
+                B64_TRY(grow_device(&buf->d_out, &buf->d_out_cap, 32 * n));
+                B64_TRY(grow_pinned(&buf->h_out, &buf->h_out_cap, 32 * n));
+                memcpy(buf->h_stage, host.data(), host.size());
+                HIP_TRY(hipMemcpyAsync(buf->d_tables, buf->h_stage, host.size(), hipMemcpyHostToDevice, s));
+                B64_TRY(enqueue_launches(launches, buf->d_tables, s));
+                k_gather_roots<<<(unsigned)((n + 63) / 64), 64, 0, s>>>(
+                    (const uint4 *const *)(buf->d_tables + roots_off), (uint4 *)buf->d_out, (int)n);
+                HIP_TRY(hipGetLastError());
+                HIP_TRY(hipMemcpyAsync(buf->h_out, buf->d_out, 32 * n, hipMemcpyDeviceToHost, s));
+                return B64_OK;
+            }
+            case Phase::Fetch: {
+                size_t total = 0;
+                for (ProveJob &J : pj) {
+                    J.fetch_at = total;
+                    total += J.q.fetch_bytes();
+                }
+                B64_TRY(grow_pinned(&buf->h_fetch, &buf->h_fetch_cap, total));
+                for (ProveJob &J : pj) B64_TRY(J.q.fetch(s, buf->h_fetch + J.fetch_at));
+                return B64_OK;
+            }
+            case Phase::Gather: {
+                const size_t src_bytes = n_src * sizeof(uint64_t);
+                B64_TRY(grow_device(&buf->d_tables, &buf->d_tables_cap, src_bytes));
+                B64_TRY(grow_pinned(&buf->h_stage, &buf->h_stage_cap, src_bytes));
+                B64_TRY(grow_device(&buf->d_out, &buf->d_out_cap, 16 * n_src));
+                B64_TRY(grow_pinned(&buf->h_out, &buf->h_out_cap, 16 * n_src));
+                for (ProveJob &J : pj)
+                    memcpy(buf->h_stage + J.src_at * sizeof(uint64_t), J.q.src.data(), J.q.src.size() * sizeof(uint64_t));
+                HIP_TRY(hipMemcpyAsync(buf->d_tables, buf->h_stage, src_bytes, hipMemcpyHostToDevice, s));
+                HIP_TRY(b64::launch_gather16((const uint64_t *)buf->d_tables, (uint4 *)buf->d_out, n_src, s));
+                HIP_TRY(hipMemcpyAsync(buf->h_out, buf->d_out, 16 * n_src, hipMemcpyDeviceToHost, s));
+                return B64_OK;
+            }
+            case Phase::Done: return B64_OK;
+        }
+        return B64_OK;
+    }
+
+    // The host part after the enqueued phase has completed; prepares the next
+    // phase (to be enqueued) or sets *done.
+    int step(bool *done) {
+        *done = false;
+        switch (phase) {
+            case Phase::Rounds: {
+                for (size_t i = 0; i < in_round.size(); i++) {
+                    ProveJob &J = *in_round[i];
+                    if (!first) {
+                        const b64_fri_params &p = J.L.si->p;
+                        J.unprocessed.clear();
+                        J.due = false;
+                        if ((uint32_t)J.k < p.n_fold_arities)
+                            J.next_commit += p.fold_arities[J.k];
+                        else
+                            J.has_next = false;
+                        J.k++;
+                    }
+                    J.take_root(buf->h_out + 32 * i);
+                    J.advance();
+                }
+                first = false;
+                bool any = false;
+                B64_TRY(build_round(&any));
+                if (any) return B64_OK;
+                // every commitment made: the query phases (finish_proof)
+                for (ProveJob &J : pj) {
+                    const Layout &L = J.L;
+                    if (J.curr_round != J.n_rounds() || J.k != L.n_oracles) return B64_ERR_EARLY_PROVER_FINISH;
+                    const uint8_t *ws = (const uint8_t *)J.jd.d_workspace;
+                    const b64::FriOracleView cw = {(const uint4 *)(ws + L.cw), (const uint4 *)(ws + L.tree0),
+                                                   L.rs + L.lbs, L.rs};
+                    std::vector<b64::FriOracleView> committed;
+                    for (int k = 0; k < L.n_oracles; k++)
+                        committed.push_back({(const uint4 *)(ws + L.oracle[k]), (const uint4 *)(ws + L.tree[k]),
+                                             L.olen[k], L.olen[k] - L.ocoset[k]});
+                    B64_TRY(J.q.prepare(L.si->p, cw, committed.data(), committed.size()));
+                }
+                phase = Phase::Fetch;
+                return B64_OK;
+            }
+            case Phase::Fetch: {
+                n_src = 0;
+                for (ProveJob &J : pj) {
+                    B64_TRY(J.q.transcribe(J.jd.transcript, buf->h_fetch + J.fetch_at));
+                    J.src_at = n_src;
+                    n_src += J.q.src.size();
+                }
+                phase = n_src ? Phase::Gather : Phase::Done;
+                *done = phase == Phase::Done;
+                return B64_OK;
+            }
+            case Phase::Gather: {
+                for (ProveJob &J : pj) J.q.openings(J.jd.transcript, buf->h_out + 16 * J.src_at);
+                phase = Phase::Done;
+                *done = true;
+                return B64_OK;
+            }
+            case Phase::Done: *done = true; return B64_OK;
+        }
+        return B64_OK;
+    }
+};
+
+int b64::prove_batch_create(const b64_prove_job *jobs, uint32_t n_jobs, b64::ProveBuffers *buf,
+                            b64::ProveBatch **out) {
+    if (!jobs || !n_jobs || !buf || !out) return B64_ERR_INVALID_ARGS;
+    std::unique_ptr<b64::ProveBatch> pb(new (std::nothrow) b64::ProveBatch());
+    if (!pb) return B64_ERR_OUT_OF_MEMORY;
+    pb->buf = buf;
+    pb->pj.resize(n_jobs);
     for (uint32_t j = 0; j < n_jobs; j++) {
         const b64_prove_job &jd = jobs[j];
         if (!jd.d_msg || !jd.d_workspace || !jd.transcript) return B64_ERR_INVALID_ARGS;
         SizeInfo *si = nullptr;
         B64_TRY(size_info(jd.log_msg_len, &si));
         B64_TRY(size_device(si));
-        pj[j].jd = jd;
-        layout_of(si, pj[j].L);
-        pj[j].next_commit = si->p.log_batch_size;  // FRIFoldProver::new (fold.rs:70)
+        pb->pj[j].jd = jd;
+        layout_of(si, pb->pj[j].L);
+        pb->pj[j].next_commit = si->p.log_batch_size;  // FRIFoldProver::new (fold.rs:70)
     }
-    RoundBuffers buf;
-    // round 0: commit_interleaved of every message
-    {
-        SlotMap slots;
-        std::vector<const uint4 *> roots;
-        for (ProveJob &J : pj) {
-            uint8_t *ws = (uint8_t *)J.jd.d_workspace;
-            std::vector<Stage> st;
-            B64_TRY(ntt_stages(J.L, J.jd.d_msg, J.jd.log_msg_len, (uint4 *)(ws + J.L.cw), st));
-            leaf16_stage(J.L, ws, nullptr, st);
-            tree_stages((uint4 *)(ws + J.L.tree0), 1ull << J.L.rs, SLOT_TREE0, st, roots);
-            for (const Stage &x : st) slots[{x.slot, (int)x.kind}].push_back(x);
-        }
-        B64_TRY(run_round(slots, roots, buf, s));
-        for (uint32_t j = 0; j < n_jobs; j++) {
-            pj[j].take_root(buf.h_roots + 32 * j);
-            pj[j].advance();
-        }
-    }
-    // fold rounds: oracle k of every job with a commitment due
-    for (;;) {
-        SlotMap slots;
-        std::vector<const uint4 *> roots;
-        std::vector<ProveJob *> due;
-        for (ProveJob &J : pj) {
-            if (!J.due) continue;
-            const Layout &L = J.L;
-            const b64_fri_params &p = L.si->p;
-            uint8_t *ws = (uint8_t *)J.jd.d_workspace;
-            if (J.k >= L.n_oracles) return B64_ERR_INVALID_ARGS;
-            const size_t n_ch = J.unprocessed.size() / 2;
-            std::vector<Stage> st;
-            if (J.k == 0) {
-                // fold_interleaved (fold.rs:260-282) of the codeword into oracle 0
-                if (n_ch != (size_t)L.lbs || L.lbs != 4) return B64_ERR_INVALID_ARGS;
-                Stage f;
-                f.slot = SLOT_ORACLE - 1;
-                f.kind = b64::MultiKind::FoldInterleaved;
-                f.foldil.cw = (const uint4 *)(ws + L.cw);
-                f.foldil.out = (uint4 *)(ws + L.oracle[0]);
-                f.foldil.n_out = 1ull << L.rs;
-                for (int t = 0; t < 4; t++) f.foldil.chal[t] = to_u4(J.unprocessed.data() + 2 * t);
-                f.blocks = blocks_for(f.foldil.n_out);
-                st.push_back(f);
-            } else if (n_ch != p.fold_arities[J.k - 1]) {
-                return B64_ERR_INVALID_ARGS;
-            }
-            B64_TRY(oracle_stages(L, ws, J.k, J.unprocessed.data(), st, roots));
-            for (const Stage &x : st) slots[{x.slot, (int)x.kind}].push_back(x);
-            due.push_back(&J);
-        }
-        if (due.empty()) break;
-        B64_TRY(run_round(slots, roots, buf, s));
-        for (size_t i = 0; i < due.size(); i++) {
-            ProveJob &J = *due[i];
-            const b64_fri_params &p = J.L.si->p;
-            J.unprocessed.clear();
-            J.due = false;
-            if ((uint32_t)J.k < p.n_fold_arities)
-                J.next_commit += p.fold_arities[J.k];
-            else
-                J.has_next = false;
-            J.k++;
-            J.take_root(buf.h_roots + 32 * i);
-            J.advance();
-        }
-    }
-    // finish_proof of every job: one wait for the terminal codewords and layers,
-    // one gather launch for every opening
-    std::vector<b64::FriQuery> q(n_jobs);
-    for (uint32_t j = 0; j < n_jobs; j++) {
-        const ProveJob &J = pj[j];
-        const Layout &L = J.L;
-        if (J.curr_round != J.n_rounds() || J.k != L.n_oracles) return B64_ERR_EARLY_PROVER_FINISH;
-        const uint8_t *ws = (const uint8_t *)J.jd.d_workspace;
-        const b64::FriOracleView first = {(const uint4 *)(ws + L.cw), (const uint4 *)(ws + L.tree0), L.rs + L.lbs,
-                                          L.rs};
-        std::vector<b64::FriOracleView> committed;
-        for (int k = 0; k < L.n_oracles; k++)
-            committed.push_back({(const uint4 *)(ws + L.oracle[k]), (const uint4 *)(ws + L.tree[k]), L.olen[k],
-                                 L.olen[k] - L.ocoset[k]});
-        B64_TRY(q[j].prepare(L.si->p, first, committed.data(), committed.size()));
-        B64_TRY(q[j].fetch(s));
-    }
-    HIP_TRY(hipStreamSynchronize(s));
-    std::vector<uint64_t> src;
-    for (uint32_t j = 0; j < n_jobs; j++) {
-        B64_TRY(q[j].transcribe(pj[j].jd.transcript));
-        src.insert(src.end(), q[j].src.begin(), q[j].src.end());
-    }
-    std::vector<uint8_t> out(src.size() * 16);
-    B64_TRY(b64::gather_granules(src.data(), src.size(), out.data(), s));
-    size_t at = 0;
-    for (uint32_t j = 0; j < n_jobs; j++) {
-        q[j].openings(pj[j].jd.transcript, out.data() + 16 * at);
-        at += q[j].src.size();
+    bool any = false;
+    B64_TRY(pb->build_round(&any));
+    *out = pb.release();
+    return B64_OK;
+}
+
+int b64::prove_batch_enqueue(b64::ProveBatch *pb, hipStream_t s) { return pb->enqueue(s); }
+int b64::prove_batch_step(b64::ProveBatch *pb, bool *done) { return pb->step(done); }
+void b64::prove_batch_destroy(b64::ProveBatch *pb) { delete pb; }
+b64::ProveBuffers *b64::prove_buffers_create() { return new (std::nothrow) b64::ProveBuffers(); }
+void b64::prove_buffers_destroy(b64::ProveBuffers *b) { delete b; }
+
+extern "C" int b64_batch_prove(const b64_prove_job *jobs, uint32_t n_jobs, void *stream) {
+    if (!jobs || !n_jobs) return B64_ERR_INVALID_ARGS;
+    const hipStream_t s = as_stream(stream);
+    b64::ProveBuffers buf;
+    b64::ProveBatch *raw = nullptr;
+    B64_TRY(b64::prove_batch_create(jobs, n_jobs, &buf, &raw));
+    std::unique_ptr<b64::ProveBatch> pb(raw);
+    for (bool done = false; !done;) {
+        B64_TRY(pb->enqueue(s));
+        HIP_TRY(hipStreamSynchronize(s));
+        B64_TRY(pb->step(&done));
     }
     return B64_OK;
 }

@@ -200,28 +200,41 @@ struct FriOracleView {
 
 // The query phase of FRIFoldProver::finish_proof (prover fri/fold.rs:221-244,
 // query.rs:51-134), in steps so that many proofs can share device round trips:
-//   prepare  -- the oracles (the interleaved codeword, then every committed one);
-//   fetch    -- enqueue the copies of the terminal codeword and the opened layers;
-//   transcribe (after those copies) -- decommit them, sample the query indices and
-//               list every opening as 16-byte granules (device addresses, src);
-//   openings -- decommit the gathered granules (src.size() * 16 bytes).
+//   prepare    -- the oracles (the interleaved codeword, then every committed one);
+//   fetch      -- enqueue the copies of the terminal codeword and the opened
+//                 layers into host memory (fetch_bytes() bytes);
+//   transcribe (after those copies) -- decommit them, sample the query indices
+//                 and list every opening as 16-byte granules (device addresses, src);
+//   openings   -- decommit the gathered granules (src.size() * 16 bytes).
 struct FriQuery {
     b64_fri_params p;
     FriOracleView term;
     std::vector<FriOracleView> opened;
     std::vector<int> depths;
-    std::vector<uint8_t> term_buf;
-    std::vector<std::vector<uint8_t>> layer_buf;
     std::vector<uint64_t> src;
     int prepare(const b64_fri_params &params, const FriOracleView &first, const FriOracleView *committed,
                 size_t n_committed);
-    int fetch(hipStream_t s);
-    int transcribe(b64_transcript *t);
+    size_t fetch_bytes() const;
+    int fetch(hipStream_t s, uint8_t *dst) const;
+    int transcribe(b64_transcript *t, const uint8_t *fetched);
     void openings(b64_transcript *t, const uint8_t *granules) const;
 };
 // out[i] = the 16 bytes at device address src[i], for i < n: one launch, one copy,
 // then a stream synchronize.
 int gather_granules(const uint64_t *src, size_t n, uint8_t *out, hipStream_t s);
+
+// Batched proofs (b64_batch_prove) as a resumable sequence of phases: enqueue a
+// phase (nothing waits), wait for the stream by any means, then step() does the
+// host part (roots into transcripts, challenges, next phase's tables) and
+// reports when every proof is complete.  Buffers persist across batches.
+struct ProveBuffers;
+struct ProveBatch;
+ProveBuffers *prove_buffers_create();
+void prove_buffers_destroy(ProveBuffers *b);
+int prove_batch_create(const b64_prove_job *jobs, uint32_t n_jobs, ProveBuffers *buf, ProveBatch **out);
+int prove_batch_enqueue(ProveBatch *pb, hipStream_t s);
+int prove_batch_step(ProveBatch *pb, bool *done);
+void prove_batch_destroy(ProveBatch *pb);
 
 // Caller-owned buffers for a batch plan (the serving loop reuses one per stream
 // slot, so building a plan neither allocates nor synchronizes the device).

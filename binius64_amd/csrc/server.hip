@@ -10,6 +10,13 @@
 // the device start as soon as memory and a slot free up, and their kernels
 // overlap with the running batches.  Each slot owns its batch-plan buffers, so
 // the loop neither allocates device memory nor synchronizes the device.
+//
+// Proof jobs (b64_server_submit_proof) are batched with other proof jobs: such a
+// batch runs the phases of b64_batch_prove on its slot, and whenever a phase
+// completes the worker does its host part (roots into the jobs' transcripts,
+// challenges sampled, the next phase enqueued) before the slot moves on, so the
+// rounds of proof batches interleave with everything else in flight.  Their
+// phase buffers grow to the largest batch a slot has seen and are then reused.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -23,6 +30,7 @@
 #include <thread>
 #include <vector>
 
+#include "abi_util.hpp"
 #include "internal.hpp"
 
 namespace {
@@ -80,6 +88,7 @@ struct Job {
     uint64_t ticket = 0;
     uint32_t n = 0;
     const uint64_t *msg = nullptr;  // caller-owned until the job is done
+    b64_transcript *transcript = nullptr;  // proof jobs: caller-owned until the job is done
     std::vector<uint64_t> chal;
     std::vector<uint8_t> roots;
     size_t footprint = 0, off = 0, ws_off = 0;
@@ -96,6 +105,9 @@ struct Slot {
     hipEvent_t done_ev = nullptr;
     b64::BatchStorage st{};
     b64_batch_plan *plan = nullptr;
+    b64::ProveBuffers *pbuf = nullptr;
+    b64::ProveBatch *prove = nullptr;  // a proof batch in progress
+    bool ready = false;                // its enqueued phase has completed: host part due
     std::vector<std::shared_ptr<Job>> jobs;
     bool busy = false;
 };
@@ -126,6 +138,8 @@ struct b64_server {
         for (Slot &s : slots) {
             if (s.stream) (void)hipStreamSynchronize(s.stream);
             if (s.plan) b64_batch_plan_destroy(s.plan);
+            if (s.prove) b64::prove_batch_destroy(s.prove);
+            if (s.pbuf) b64::prove_buffers_destroy(s.pbuf);
             if (s.done_ev) (void)hipEventDestroy(s.done_ev);
             if (s.stream) (void)hipStreamDestroy(s.stream);
             if (s.st.d_tables) (void)hipFree(s.st.d_tables);
@@ -146,6 +160,11 @@ struct b64_server {
             b64_batch_plan_destroy(s.plan);
             s.plan = nullptr;
         }
+        if (s.prove) {
+            b64::prove_batch_destroy(s.prove);
+            s.prove = nullptr;
+        }
+        s.ready = false;
         for (auto &j : s.jobs) pool->release(j->off, j->footprint);
         s.jobs.clear();
         s.busy = false;
@@ -153,15 +172,34 @@ struct b64_server {
         cv_done.notify_all();
     }
 
-    // Retire finished batches (called with mu held).
+    // Jobs of a slot delivered: memory back to the pool (called with mu held).
+    void complete(Slot &s) {
+        const auto now = Clock::now();
+        for (auto &j : s.jobs) {
+            pool->release(j->off, j->footprint);
+            j->t_done = now;
+            j->done = true;
+            n_done++;
+        }
+        s.jobs.clear();
+        s.busy = false;
+    }
+
+    // Retire finished batches; a proof batch whose phase completed is marked
+    // ready for its host part instead (called with mu held).
     bool retire() {
         bool any = false;
         for (Slot &s : slots) {
-            if (!s.busy) continue;
+            if (!s.busy || s.ready) continue;
             const hipError_t q = hipEventQuery(s.done_ev);
             if (q == hipErrorNotReady) continue;
             if (q != hipSuccess) {
                 fail_slot(s, B64_ERR_HIP);
+                any = true;
+                continue;
+            }
+            if (s.prove) {
+                s.ready = true;
                 any = true;
                 continue;
             }
@@ -171,15 +209,7 @@ struct b64_server {
                 b64_batch_plan_destroy(s.plan);
                 s.plan = nullptr;
             }
-            const auto now = Clock::now();
-            for (auto &j : s.jobs) {
-                pool->release(j->off, j->footprint);
-                j->t_done = now;
-                j->done = true;
-                n_done++;
-            }
-            s.jobs.clear();
-            s.busy = false;
+            complete(s);
             any = true;
         }
         if (any) cv_done.notify_all();
@@ -196,7 +226,9 @@ struct b64_server {
                 break;
             }
         if (!slot || queue.empty()) return nullptr;
-        while (!queue.empty() && slot->jobs.size() < cfg.max_batch) {
+        // a batch holds commit jobs or proof jobs, in arrival order
+        const bool proofs = queue.front()->transcript != nullptr;
+        while (!queue.empty() && slot->jobs.size() < cfg.max_batch && (queue.front()->transcript != nullptr) == proofs) {
             auto &j = queue.front();
             if (!pool->alloc(j->footprint, &j->off)) break;
             j->ws_off = j->off + pool_round(16ull << j->n);
@@ -212,6 +244,20 @@ struct b64_server {
     // Uploads the messages and enqueues the batch (called without mu: only the
     // worker touches a busy slot's jobs and buffers).
     int launch(Slot *slot) {
+        if (slot->jobs[0]->transcript) {
+            std::vector<b64_prove_job> descs(slot->jobs.size());
+            for (size_t i = 0; i < slot->jobs.size(); i++) {
+                Job &j = *slot->jobs[i];
+                void *d_msg = d_pool + j.off;
+                if (hipMemcpyAsync(d_msg, j.msg, 16ull << j.n, hipMemcpyHostToDevice, slot->stream) != hipSuccess)
+                    return B64_ERR_HIP;
+                descs[i] = {j.n, d_msg, d_pool + j.ws_off, j.transcript, j.roots.data()};
+            }
+            B64_TRY(b64::prove_batch_create(descs.data(), (uint32_t)descs.size(), slot->pbuf, &slot->prove));
+            B64_TRY(b64::prove_batch_enqueue(slot->prove, slot->stream));
+            if (hipEventRecord(slot->done_ev, slot->stream) != hipSuccess) return B64_ERR_HIP;
+            return B64_OK;
+        }
         std::vector<b64_job_desc> descs(slot->jobs.size());
         for (size_t i = 0; i < slot->jobs.size(); i++) {
             Job &j = *slot->jobs[i];
@@ -226,6 +272,16 @@ struct b64_server {
         return rc;
     }
 
+    // The host part of a proof batch's completed phase, then its next phase
+    // enqueued (called without mu, like launch).
+    int advance(Slot *slot, bool *done) {
+        B64_TRY(b64::prove_batch_step(slot->prove, done));
+        if (*done) return B64_OK;
+        B64_TRY(b64::prove_batch_enqueue(slot->prove, slot->stream));
+        if (hipEventRecord(slot->done_ev, slot->stream) != hipSuccess) return B64_ERR_HIP;
+        return B64_OK;
+    }
+
     void run() {
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
@@ -238,6 +294,23 @@ struct b64_server {
                 lk.lock();
                 // the done event was not recorded: the slot must not be retired as finished
                 if (rc != B64_OK) fail_slot(*slot, rc);
+                progressed = true;
+            }
+            for (Slot &s : slots) {
+                if (!s.busy || !s.ready) continue;
+                s.ready = false;
+                bool done = false;
+                lk.unlock();
+                const int rc = advance(&s, &done);
+                lk.lock();
+                if (rc != B64_OK) {
+                    fail_slot(s, rc);
+                } else if (done) {
+                    b64::prove_batch_destroy(s.prove);
+                    s.prove = nullptr;
+                    complete(s);
+                    cv_done.notify_all();
+                }
                 progressed = true;
             }
             bool inflight = false;
@@ -277,6 +350,8 @@ int b64_server_create(const b64_server_config *cfg, b64_server **out) {
             return B64_ERR_OUT_OF_MEMORY;
         if (hipHostMalloc((void **)&s.st.h_roots, 32 * ROOTS_CAP, hipHostMallocDefault) != hipSuccess)
             return B64_ERR_OUT_OF_MEMORY;
+        s.pbuf = b64::prove_buffers_create();
+        if (!s.pbuf) return B64_ERR_OUT_OF_MEMORY;
     }
     b64_server *p = srv.get();
     p->worker = std::thread([p] { p->run(); });
@@ -284,15 +359,15 @@ int b64_server_create(const b64_server_config *cfg, b64_server **out) {
     return B64_OK;
 }
 
-int b64_server_submit(b64_server *srv, uint32_t log_msg_len, const uint64_t *msg, const uint64_t *challenges,
-                      uint64_t *ticket) {
-    if (!srv || !msg || !challenges || !ticket) return B64_ERR_INVALID_ARGS;
+static int submit(b64_server *srv, uint32_t log_msg_len, const uint64_t *msg, const uint64_t *challenges,
+                  b64_transcript *transcript, uint64_t *ticket) {
     const size_t ws = b64_job_workspace_size(log_msg_len);
     if (ws == 0) return B64_ERR_INVALID_ARGS;
     auto j = std::make_shared<Job>();
     j->n = log_msg_len;
     j->msg = msg;
-    j->chal.assign(challenges, challenges + 2ull * log_msg_len);
+    j->transcript = transcript;
+    if (challenges) j->chal.assign(challenges, challenges + 2ull * log_msg_len);
     j->roots.assign(32ull * b64_job_n_roots(log_msg_len), 0);
     j->footprint = pool_round(16ull << log_msg_len) + pool_round(ws);
     if (j->footprint > srv->pool->size()) return B64_ERR_ADMISSION_REJECTED;
@@ -307,6 +382,18 @@ int b64_server_submit(b64_server *srv, uint32_t log_msg_len, const uint64_t *msg
     }
     srv->cv_work.notify_one();
     return B64_OK;
+}
+
+int b64_server_submit(b64_server *srv, uint32_t log_msg_len, const uint64_t *msg, const uint64_t *challenges,
+                      uint64_t *ticket) {
+    if (!srv || !msg || !challenges || !ticket) return B64_ERR_INVALID_ARGS;
+    return submit(srv, log_msg_len, msg, challenges, nullptr, ticket);
+}
+
+int b64_server_submit_proof(b64_server *srv, uint32_t log_msg_len, const uint64_t *msg, b64_transcript *transcript,
+                            uint64_t *ticket) {
+    if (!srv || !msg || !transcript || !ticket) return B64_ERR_INVALID_ARGS;
+    return submit(srv, log_msg_len, msg, nullptr, transcript, ticket);
 }
 
 static int finish(b64_server *srv, std::shared_ptr<Job> j, uint8_t *roots_out, double *latency_s) {

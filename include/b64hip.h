@@ -364,6 +364,13 @@ int b64_server_create(const b64_server_config *cfg, b64_server **out);
  * challenges are copied.  ADMISSION_REJECTED if the job alone exceeds the budget. */
 int b64_server_submit(b64_server *srv, uint32_t log_msg_len, const uint64_t *msg, const uint64_t *challenges,
                       uint64_t *ticket);
+/* Queues a proof job: the job of b64_batch_prove (commitment and FRI proof under
+ * `transcript`, which must stay valid and untouched until poll/wait reports the
+ * job done; the proof is then on its tape).  Proof jobs are batched with other
+ * proof jobs; each of their rounds waits for the host to sample challenges, and
+ * the slot's other work and the other slots keep running meanwhile. */
+int b64_server_submit_proof(b64_server *srv, uint32_t log_msg_len, const uint64_t *msg, b64_transcript *transcript,
+                            uint64_t *ticket);
 /* *done = 1 and roots (b64_job_n_roots * 32 bytes) written once the job has
  * finished; latency_s = submit-to-completion time measured by the server. */
 int b64_server_poll(b64_server *srv, uint64_t ticket, uint8_t *roots_out, int *done, double *latency_s);

@@ -138,3 +138,49 @@ def test_config5_stream_matches_oracle(b64, orc):
                              challenges=chals[i], threads=8)["roots"]
         assert got[i] == want, f"job {i} (n={n}) != oracle"
         assert got[i] == alone(b64, n, msgs[i], chals[i]), f"job {i} (n={n}) != alone"
+
+
+def test_served_proofs_match_oracle(b64, orc):
+    """Proof jobs served from the seed-5 stream (n = 13..21 from pack_witness
+    sizes), mixed with commit jobs and submitted at their Poisson arrival times:
+    every proof job's transcript tape equals the oracle's FRI prover tape for its
+    message (prover/prover/src/fri/fold.rs:117-175, 221-244: challenges sampled
+    from the transcript) and the tape of the same job proved alone; the commit
+    jobs in between still give their own roots."""
+    import time
+
+    rng = np.random.default_rng(5)
+    sizes = [int(w) - 1 for w in rng.integers(14, 23, size=400)][:24]
+    arrivals = np.cumsum(rng.exponential(1.0 / 200.0, size=400))[:24]
+    print("served proof sizes", sizes, flush=True)
+    msgs = [np.random.default_rng(30_000 + i).integers(0, 2 ** 63, size=(1 << n, 2), dtype=np.uint64)
+            for i, n in enumerate(sizes)]
+    srv = b64.Server(hbm_budget_bytes=8 << 30, max_batch=16, n_streams=3)
+    transcripts = [b64.ProverTranscript() for _ in sizes]
+    try:
+        tickets, commits, t0 = [], [], time.perf_counter()
+        for i, n in enumerate(sizes):
+            dt = t0 + arrivals[i] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            tickets.append(srv.submit_proof(n, msgs[i], transcripts[i]))
+            if i % 3 == 0:  # a commit job between the proofs
+                m, ch = job_inputs(orc, 10 + i % 5, 700 + i)
+                commits.append((10 + i % 5, m, ch, srv.submit(10 + i % 5, m, ch)))
+        got = [srv.wait(t, b64.CommitJob.n_roots_for(n))[0] for t, n in zip(tickets, sizes)]
+        got_commits = [srv.wait(t, b64.CommitJob.n_roots_for(n))[0] for n, _, _, t in commits]
+    finally:
+        srv.close()
+    for i, n in enumerate(sizes):
+        p = orc.fri_params_setup(n, 1, 96)
+        dom = orc.Domain(orc.Domain(dim=n + 1).subspace(p["rs_log_len"]))
+        want = orc.fri_prove(dom, msgs[i], p["log_dim"], 1, p["log_batch_size"], p["fold_arities"],
+                             p["n_test_queries"], threads=8)
+        assert transcripts[i].finalize() == want["tape"], f"proof job {i} (n={n}) tape != oracle"
+        assert got[i] == want["roots"], f"proof job {i} (n={n}) roots != oracle"
+    for i in (0, 11, 23):
+        alone_job = b64.ProveJob(sizes[i], msgs[i])
+        b64.batch_prove([alone_job])
+        assert alone_job.tape() == transcripts[i].finalize(), f"proof job {i} != alone"
+    for (n, m, ch, _), r in zip(commits, got_commits):
+        assert r == alone(b64, n, m, ch)
