@@ -60,6 +60,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-proofs", action="store_true", help="skip the batched and served FRI proofs leg")
     ap.add_argument("--no-ring-switch", action="store_true", help="skip the ring-switch folds leg")
     ap.add_argument("--no-pcs", action="store_true", help="skip the PCS evaluation-proof leg")
     ap.add_argument("--no-vision", action="store_true", help="skip the Vision Merkle-commit leg")
@@ -199,6 +200,7 @@ def main():
 
     batch = None if args.no_batch else batch_bench(b64, torch, stream, sp)
     serve = None if args.no_serve else serve_bench(b64, torch, sp)
+    proofs = None if args.no_proofs else proofs_bench(b64, torch, sp)
     ring = None if args.no_ring_switch else ring_switch_bench(b64, torch, stream, sp, with_cpu=not args.no_cpu)
     pcs = None if args.no_pcs else pcs_bench(b64, torch, sp, log_n=n, with_cpu=not args.no_cpu)
     vision = None if args.no_vision else vision_bench(b64, torch, stream, sp, with_cpu=not args.no_cpu)
@@ -238,6 +240,7 @@ def main():
         "cpu_baseline": cpu,
         "batch": batch,
         "serve": serve,
+        "proofs": proofs,
         "ring_switch": ring,
         "pcs_prove": pcs,
         "vision_merkle": vision,
@@ -431,6 +434,159 @@ def serve_bench(b64, torch, sp, n_jobs: int = 400, rate: float = 200.0, detail: 
             "batches_at_rate": st_poisson["batches"], "batches_total": st["batches"],
             "peak_pool_bytes": st["peak_pool_bytes"], "roots_identical": bool(same and same_plain),
             "plain_latency": "measured: the same arrivals, one job at a time in arrival order"}
+
+
+def proofs_bench(b64, torch, sp, n_batch: int = 64, reps: int = 3, n_serve: int = 400, rate: float = 200.0):
+    """Whole FRI proofs (commitment, fold rounds with challenges sampled from each
+    job's transcript, query phase) of the batching and serving workloads.
+
+    Batch: the 64-job mix (log_n ~ U{12..20}, seed 4), b64_batch_prove over all
+    jobs vs the same prover one job at a time.  Serve: the seed-5 Poisson stream
+    (n = 13..21, mean `rate`/s) as proof jobs through the server vs one job at a
+    time under the same arrivals; p50/p99 submit-to-proof latency, and saturated
+    throughput.  Every tape is compared with the one-job tape of the same job."""
+    import threading
+
+    import numpy as np
+
+    from binius64_amd._abi import ProveJobC
+
+    L = b64.lib()
+    new_t = lambda: b64.ProverTranscript()
+
+    def prove(jobs_in, transcripts):
+        descs = (ProveJobC * len(jobs_in))(*[ProveJobC(n, m.ptr, ws.ptr, t.ptr, None)
+                                             for (n, m, ws), t in zip(jobs_in, transcripts)])
+        b64.check(L.b64_batch_prove(descs, len(jobs_in), sp), "batch_prove")
+
+    # ---- batch ------------------------------------------------------------------
+    rng = np.random.default_rng(4)
+    sizes = [int(x) for x in rng.integers(12, 21, size=n_batch)]
+    jobs = []
+    for jid, n in enumerate(sizes):
+        msg = np.random.default_rng(1000 + jid).integers(0, 2 ** 63, size=(1 << n, 2), dtype=np.uint64)
+        jobs.append((n, b64.DeviceBuffer.from_host(msg), b64.DeviceBuffer(b64.CommitJob.workspace_size(n))))
+    ts = [new_t() for _ in jobs]
+    for j, t in zip(jobs, ts):  # warm every size; these tapes are the reference
+        prove([j], [t])
+    want = [t.finalize() for t in ts]
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ts = [new_t() for _ in jobs]
+        for j, t in zip(jobs, ts):
+            prove([j], [t])
+    t_plain = (time.perf_counter() - t0) / reps
+    same = [t.finalize() for t in ts] == want
+    prove(jobs, [new_t() for _ in jobs])
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ts = [new_t() for _ in jobs]
+        prove(jobs, ts)
+    t_batch = (time.perf_counter() - t0) / reps
+    same = same and [t.finalize() for t in ts] == want
+    batch = {"jobs": n_batch, "log_n_range": [12, 20], "sizes_seed": 4,
+             "plain_proofs_per_s": round(n_batch / t_plain, 1), "batched_proofs_per_s": round(n_batch / t_batch, 1),
+             "speedup": round(t_plain / t_batch, 3), "plain_ms": round(1e3 * t_plain, 3),
+             "batched_ms": round(1e3 * t_batch, 3), "tapes_identical": bool(same)}
+
+    # ---- serve ------------------------------------------------------------------
+    rng = np.random.default_rng(5)
+    ssizes = [int(w) - 1 for w in rng.integers(14, 23, size=n_serve)]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_serve))
+    pool_elems = 1 << 22
+    g = torch.Generator().manual_seed(5)
+    pool_t = torch.randint(-(2 ** 63), 2 ** 63 - 1, (2 * pool_elems,), dtype=torch.int64, generator=g).pin_memory()
+    pool = pool_t.numpy().view(np.uint64).reshape(-1, 2)
+    offs = [int(rng.integers(0, pool_elems - (1 << n) + 1)) for n in ssizes]
+    msgs = [pool[o:o + (1 << n)] for o, n in zip(offs, ssizes)]
+    n_roots = [b64.CommitJob.n_roots_for(n) for n in ssizes]
+    nmax = max(ssizes)
+    d_msg = b64.DeviceBuffer(16 << nmax)
+    d_ws = b64.DeviceBuffer(b64.CommitJob.workspace_size(nmax))
+
+    def plain_one(i, t):
+        b64.check(L.b64_memcpy_htod(d_msg.ptr, msgs[i].ctypes.data, msgs[i].nbytes, sp), "htod")
+        prove([(ssizes[i], d_msg, d_ws)], [t])
+
+    for n in sorted(set(ssizes)):  # warm every size
+        plain_one(ssizes.index(n), new_t())
+    t0 = time.perf_counter()
+    tp = [new_t() for _ in range(n_serve)]
+    for i in range(n_serve):
+        plain_one(i, tp[i])
+    t_plain_s = time.perf_counter() - t0
+    want_s = [t.finalize() for t in tp]
+
+    import queue
+    fifo, lat_plain = queue.Queue(), np.zeros(n_serve)
+
+    def plain_worker():
+        for _ in range(n_serve):
+            i, t_arr = fifo.get()
+            plain_one(i, new_t())
+            lat_plain[i] = time.perf_counter() - t_arr
+
+    worker = threading.Thread(target=plain_worker)
+    worker.start()
+    t0 = time.perf_counter()
+    for i in range(n_serve):
+        dt = t0 + arrivals[i] - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        fifo.put((i, time.perf_counter()))
+    worker.join()
+
+    srv = b64.Server(hbm_budget_bytes=8 << 30, max_batch=64, n_streams=3)
+    same_s = True
+    try:
+        for n in sorted(set(ssizes)):  # warm-up (untimed)
+            i = ssizes.index(n)
+            for tk in [srv.submit_proof(n, msgs[i], new_t()) for _ in range(3)]:
+                srv.wait(tk, n_roots[i])
+        tickets, tr = [0] * n_serve, [new_t() for _ in range(n_serve)]
+        ready = [threading.Event() for _ in range(n_serve)]
+
+        def submitter():
+            t0 = time.perf_counter()
+            for i in range(n_serve):
+                dt = t0 + arrivals[i] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                tickets[i] = srv.submit_proof(ssizes[i], msgs[i], tr[i])
+                ready[i].set()
+
+        t0 = time.perf_counter()
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat = np.zeros(n_serve)
+        for i in range(n_serve):
+            ready[i].wait()
+            _, lat[i] = srv.wait(tickets[i], n_roots[i])
+        t_poisson = time.perf_counter() - t0
+        th.join()
+        same_s = [t.finalize() for t in tr] == want_s
+        t0 = time.perf_counter()
+        tr = [new_t() for _ in range(n_serve)]
+        tk = [srv.submit_proof(ssizes[i], msgs[i], tr[i]) for i in range(n_serve)]
+        for i in range(n_serve):
+            srv.wait(tk[i], n_roots[i])
+        t_sat = time.perf_counter() - t0
+        same_s = same_s and [t.finalize() for t in tr] == want_s
+    finally:
+        srv.close()
+
+    def pct(a, q):
+        return round(1e3 * float(np.percentile(a, q)), 3)
+
+    serve = {"jobs": n_serve, "arrival_rate_per_s": rate, "seed": 5, "log_n_range": [13, 21],
+             "served_p50_ms": pct(lat, 50), "served_p99_ms": pct(lat, 99),
+             "served_proofs_per_s_at_rate": round(n_serve / t_poisson, 1),
+             "plain_p50_ms": pct(lat_plain, 50), "plain_p99_ms": pct(lat_plain, 99),
+             "saturated_served_proofs_per_s": round(n_serve / t_sat, 1),
+             "plain_proofs_per_s": round(n_serve / t_plain_s, 1),
+             "saturated_speedup": round(t_plain_s / t_sat, 3), "tapes_identical": bool(same_s),
+             "plain_latency": "measured: the same arrivals, one proof at a time in arrival order"}
+    return {"batch": batch, "serve": serve}
 
 
 def ring_switch_bench(b64, torch, stream, sp, log_len: int = 24, reps: int = 10, with_cpu: bool = True):

@@ -721,32 +721,44 @@ struct b64::ProveBuffers {
     }
 };
 
+// Each phase carries, for every job, the next step it is ready for: the
+// round that commits its next oracle (jobs grouped into multi-job launches),
+// the fetch of its terminal codeword and layers once its rounds are over, or
+// the gather of its openings once those are transcribed.  A job is complete
+// two phases after its last round, whatever the other jobs still have to do.
 struct b64::ProveBatch {
-    enum class Phase { Rounds, Fetch, Gather, Done };
+    enum class St { Rounds, Fetch, Gather, Done };
     std::vector<ProveJob> pj;
+    std::vector<St> st;
     b64::ProveBuffers *buf = nullptr;
-    Phase phase = Phase::Rounds;
-    bool first = true;               // the next round is phase 0 (the codewords' commitments)
-    std::vector<ProveJob *> in_round;  // jobs whose root the enqueued round writes, in root order
-    std::vector<uint8_t> host;
+    bool first = true;                  // the next round is phase 0 (the codewords' commitments)
+    std::vector<uint32_t> in_round, in_fetch, in_gather;  // jobs in the enqueued phase
+    std::vector<uint8_t> host;          // tables of the round, then the gather sources
     std::vector<Launch> launches;
-    size_t n_src = 0;
+    size_t roots_off = 0, src_off = 0, n_src = 0;
 
-    // Stages of the next round into host tables; false when no job has one due.
-    int build_round(bool *any) {
+    // The next phase from each job's state.
+    int plan() {
         SlotMap slots;
         std::vector<const uint4 *> roots;
         in_round.clear();
-        for (ProveJob &J : pj) {
+        in_fetch.clear();
+        in_gather.clear();
+        for (uint32_t j = 0; j < pj.size(); j++) {
+            ProveJob &J = pj[j];
             const Layout &L = J.L;
             uint8_t *ws = (uint8_t *)J.jd.d_workspace;
-            std::vector<Stage> st;
+            if (st[j] == St::Gather) {
+                in_gather.push_back(j);
+                continue;
+            }
+            if (st[j] != St::Rounds) continue;
+            std::vector<Stage> stg;
             if (first) {
-                B64_TRY(ntt_stages(L, J.jd.d_msg, J.jd.log_msg_len, (uint4 *)(ws + L.cw), st));
-                leaf16_stage(L, ws, nullptr, st);
-                tree_stages((uint4 *)(ws + L.tree0), 1ull << L.rs, SLOT_TREE0, st, roots);
-            } else {
-                if (!J.due) continue;
+                B64_TRY(ntt_stages(L, J.jd.d_msg, J.jd.log_msg_len, (uint4 *)(ws + L.cw), stg));
+                leaf16_stage(L, ws, nullptr, stg);
+                tree_stages((uint4 *)(ws + L.tree0), 1ull << L.rs, SLOT_TREE0, stg, roots);
+            } else if (J.due) {
                 const b64_fri_params &p = L.si->p;
                 if (J.k >= L.n_oracles) return B64_ERR_INVALID_ARGS;
                 const size_t n_ch = J.unprocessed.size() / 2;
@@ -761,130 +773,113 @@ struct b64::ProveBatch {
                     f.foldil.n_out = 1ull << L.rs;
                     for (int t = 0; t < 4; t++) f.foldil.chal[t] = to_u4(J.unprocessed.data() + 2 * t);
                     f.blocks = blocks_for(f.foldil.n_out);
-                    st.push_back(f);
+                    stg.push_back(f);
                 } else if (n_ch != p.fold_arities[J.k - 1]) {
                     return B64_ERR_INVALID_ARGS;
                 }
-                B64_TRY(oracle_stages(L, ws, J.k, J.unprocessed.data(), st, roots));
+                B64_TRY(oracle_stages(L, ws, J.k, J.unprocessed.data(), stg, roots));
+            } else {
+                // every commitment made: the query phase (finish_proof)
+                if (J.curr_round != J.n_rounds() || J.k != L.n_oracles) return B64_ERR_EARLY_PROVER_FINISH;
+                const b64::FriOracleView cw = {(const uint4 *)(ws + L.cw), (const uint4 *)(ws + L.tree0),
+                                               L.rs + L.lbs, L.rs};
+                std::vector<b64::FriOracleView> committed;
+                for (int k = 0; k < L.n_oracles; k++)
+                    committed.push_back({(const uint4 *)(ws + L.oracle[k]), (const uint4 *)(ws + L.tree[k]),
+                                         L.olen[k], L.olen[k] - L.ocoset[k]});
+                B64_TRY(J.q.prepare(L.si->p, cw, committed.data(), committed.size()));
+                st[j] = St::Fetch;
+                in_fetch.push_back(j);
+                continue;
             }
-            for (const Stage &x : st) slots[{x.slot, (int)x.kind}].push_back(x);
-            in_round.push_back(&J);
+            for (const Stage &x : stg) slots[{x.slot, (int)x.kind}].push_back(x);
+            in_round.push_back(j);
         }
-        *any = !in_round.empty();
-        if (!*any) return B64_OK;
+        first = false;
         host.clear();
         launches.clear();
         B64_TRY(build_launches(slots, host, launches));
         roots_off = put_bytes(host, roots.data(), roots.size() * sizeof(void *));
+        n_src = 0;
+        for (uint32_t j : in_gather) {
+            pj[j].src_at = n_src;
+            n_src += pj[j].q.src.size();
+        }
+        src_off = align_up(host.size());
+        host.resize(src_off + n_src * sizeof(uint64_t));
+        for (uint32_t j : in_gather)
+            memcpy(host.data() + src_off + pj[j].src_at * sizeof(uint64_t), pj[j].q.src.data(),
+                   pj[j].q.src.size() * sizeof(uint64_t));
         return B64_OK;
     }
-    size_t roots_off = 0;
+
+    // Output layout of a phase: the round's roots, then the gathered granules.
+    size_t granules_off() const { return align_up(32 * in_round.size()); }
 
     int enqueue(hipStream_t s) {
-        switch (phase) {
-            case Phase::Rounds: {
-                const size_t n = in_round.size();
-                B64_TRY(grow_device(&buf->d_tables, &buf->d_tables_cap, host.size()));
-                B64_TRY(grow_pinned(&buf->h_stage, &buf->h_stage_cap, host.size()));
-                B64_TRY(grow_device(&buf->d_out, &buf->d_out_cap, 32 * n));
-                B64_TRY(grow_pinned(&buf->h_out, &buf->h_out_cap, 32 * n));
-                memcpy(buf->h_stage, host.data(), host.size());
-                HIP_TRY(hipMemcpyAsync(buf->d_tables, buf->h_stage, host.size(), hipMemcpyHostToDevice, s));
-                B64_TRY(enqueue_launches(launches, buf->d_tables, s));
-                k_gather_roots<<<(unsigned)((n + 63) / 64), 64, 0, s>>>(
-                    (const uint4 *const *)(buf->d_tables + roots_off), (uint4 *)buf->d_out, (int)n);
-                HIP_TRY(hipGetLastError());
-                HIP_TRY(hipMemcpyAsync(buf->h_out, buf->d_out, 32 * n, hipMemcpyDeviceToHost, s));
-                return B64_OK;
+        const size_t n_roots = in_round.size(), out_bytes = granules_off() + 16 * n_src;
+        if (!host.empty()) {
+            B64_TRY(grow_device(&buf->d_tables, &buf->d_tables_cap, host.size()));
+            B64_TRY(grow_pinned(&buf->h_stage, &buf->h_stage_cap, host.size()));
+            memcpy(buf->h_stage, host.data(), host.size());
+            HIP_TRY(hipMemcpyAsync(buf->d_tables, buf->h_stage, host.size(), hipMemcpyHostToDevice, s));
+        }
+        B64_TRY(grow_device(&buf->d_out, &buf->d_out_cap, out_bytes));
+        B64_TRY(grow_pinned(&buf->h_out, &buf->h_out_cap, out_bytes));
+        if (n_roots) {
+            B64_TRY(enqueue_launches(launches, buf->d_tables, s));
+            k_gather_roots<<<(unsigned)((n_roots + 63) / 64), 64, 0, s>>>(
+                (const uint4 *const *)(buf->d_tables + roots_off), (uint4 *)buf->d_out, (int)n_roots);
+            HIP_TRY(hipGetLastError());
+        }
+        if (n_src)
+            HIP_TRY(b64::launch_gather16((const uint64_t *)(buf->d_tables + src_off),
+                                         (uint4 *)(buf->d_out + granules_off()), n_src, s));
+        if (out_bytes) HIP_TRY(hipMemcpyAsync(buf->h_out, buf->d_out, out_bytes, hipMemcpyDeviceToHost, s));
+        if (!in_fetch.empty()) {
+            size_t total = 0;
+            for (uint32_t j : in_fetch) {
+                pj[j].fetch_at = total;
+                total += pj[j].q.fetch_bytes();
             }
-            case Phase::Fetch: {
-                size_t total = 0;
-                for (ProveJob &J : pj) {
-                    J.fetch_at = total;
-                    total += J.q.fetch_bytes();
-                }
-                B64_TRY(grow_pinned(&buf->h_fetch, &buf->h_fetch_cap, total));
-                for (ProveJob &J : pj) B64_TRY(J.q.fetch(s, buf->h_fetch + J.fetch_at));
-                return B64_OK;
-            }
-            case Phase::Gather: {
-                const size_t src_bytes = n_src * sizeof(uint64_t);
-                B64_TRY(grow_device(&buf->d_tables, &buf->d_tables_cap, src_bytes));
-                B64_TRY(grow_pinned(&buf->h_stage, &buf->h_stage_cap, src_bytes));
-                B64_TRY(grow_device(&buf->d_out, &buf->d_out_cap, 16 * n_src));
-                B64_TRY(grow_pinned(&buf->h_out, &buf->h_out_cap, 16 * n_src));
-                for (ProveJob &J : pj)
-                    memcpy(buf->h_stage + J.src_at * sizeof(uint64_t), J.q.src.data(), J.q.src.size() * sizeof(uint64_t));
-                HIP_TRY(hipMemcpyAsync(buf->d_tables, buf->h_stage, src_bytes, hipMemcpyHostToDevice, s));
-                HIP_TRY(b64::launch_gather16((const uint64_t *)buf->d_tables, (uint4 *)buf->d_out, n_src, s));
-                HIP_TRY(hipMemcpyAsync(buf->h_out, buf->d_out, 16 * n_src, hipMemcpyDeviceToHost, s));
-                return B64_OK;
-            }
-            case Phase::Done: return B64_OK;
+            B64_TRY(grow_pinned(&buf->h_fetch, &buf->h_fetch_cap, total));
+            for (uint32_t j : in_fetch) B64_TRY(pj[j].q.fetch(s, buf->h_fetch + pj[j].fetch_at));
         }
         return B64_OK;
     }
 
-    // The host part after the enqueued phase has completed; prepares the next
-    // phase (to be enqueued) or sets *done.
-    int step(bool *done) {
-        *done = false;
-        switch (phase) {
-            case Phase::Rounds: {
-                for (size_t i = 0; i < in_round.size(); i++) {
-                    ProveJob &J = *in_round[i];
-                    if (!first) {
-                        const b64_fri_params &p = J.L.si->p;
-                        J.unprocessed.clear();
-                        J.due = false;
-                        if ((uint32_t)J.k < p.n_fold_arities)
-                            J.next_commit += p.fold_arities[J.k];
-                        else
-                            J.has_next = false;
-                        J.k++;
-                    }
-                    J.take_root(buf->h_out + 32 * i);
-                    J.advance();
-                }
-                first = false;
-                bool any = false;
-                B64_TRY(build_round(&any));
-                if (any) return B64_OK;
-                // every commitment made: the query phases (finish_proof)
-                for (ProveJob &J : pj) {
-                    const Layout &L = J.L;
-                    if (J.curr_round != J.n_rounds() || J.k != L.n_oracles) return B64_ERR_EARLY_PROVER_FINISH;
-                    const uint8_t *ws = (const uint8_t *)J.jd.d_workspace;
-                    const b64::FriOracleView cw = {(const uint4 *)(ws + L.cw), (const uint4 *)(ws + L.tree0),
-                                                   L.rs + L.lbs, L.rs};
-                    std::vector<b64::FriOracleView> committed;
-                    for (int k = 0; k < L.n_oracles; k++)
-                        committed.push_back({(const uint4 *)(ws + L.oracle[k]), (const uint4 *)(ws + L.tree[k]),
-                                             L.olen[k], L.olen[k] - L.ocoset[k]});
-                    B64_TRY(J.q.prepare(L.si->p, cw, committed.data(), committed.size()));
-                }
-                phase = Phase::Fetch;
-                return B64_OK;
+    // The host part after the enqueued phase has completed, then the next phase
+    // planned.  Jobs whose proofs became complete are appended to `finished`;
+    // *done once every job is complete.
+    int step(bool *done, std::vector<uint32_t> *finished) {
+        for (size_t i = 0; i < in_round.size(); i++) {
+            ProveJob &J = pj[in_round[i]];
+            if (J.due) {
+                const b64_fri_params &p = J.L.si->p;
+                J.unprocessed.clear();
+                J.due = false;
+                if ((uint32_t)J.k < p.n_fold_arities)
+                    J.next_commit += p.fold_arities[J.k];
+                else
+                    J.has_next = false;
+                J.k++;
             }
-            case Phase::Fetch: {
-                n_src = 0;
-                for (ProveJob &J : pj) {
-                    B64_TRY(J.q.transcribe(J.jd.transcript, buf->h_fetch + J.fetch_at));
-                    J.src_at = n_src;
-                    n_src += J.q.src.size();
-                }
-                phase = n_src ? Phase::Gather : Phase::Done;
-                *done = phase == Phase::Done;
-                return B64_OK;
-            }
-            case Phase::Gather: {
-                for (ProveJob &J : pj) J.q.openings(J.jd.transcript, buf->h_out + 16 * J.src_at);
-                phase = Phase::Done;
-                *done = true;
-                return B64_OK;
-            }
-            case Phase::Done: *done = true; return B64_OK;
+            J.take_root(buf->h_out + 32 * i);
+            J.advance();
         }
+        for (uint32_t j : in_gather) {
+            pj[j].q.openings(pj[j].jd.transcript, buf->h_out + granules_off() + 16 * pj[j].src_at);
+            st[j] = St::Done;
+            if (finished) finished->push_back(j);
+        }
+        for (uint32_t j : in_fetch) {
+            B64_TRY(pj[j].q.transcribe(pj[j].jd.transcript, buf->h_fetch + pj[j].fetch_at));
+            st[j] = pj[j].q.src.empty() ? St::Done : St::Gather;
+            if (st[j] == St::Done && finished) finished->push_back(j);
+        }
+        B64_TRY(plan());
+        *done = true;
+        for (St x : st) *done = *done && x == St::Done;
         return B64_OK;
     }
 };
@@ -906,14 +901,16 @@ int b64::prove_batch_create(const b64_prove_job *jobs, uint32_t n_jobs, b64::Pro
         layout_of(si, pb->pj[j].L);
         pb->pj[j].next_commit = si->p.log_batch_size;  // FRIFoldProver::new (fold.rs:70)
     }
-    bool any = false;
-    B64_TRY(pb->build_round(&any));
+    pb->st.assign(n_jobs, b64::ProveBatch::St::Rounds);
+    B64_TRY(pb->plan());
     *out = pb.release();
     return B64_OK;
 }
 
 int b64::prove_batch_enqueue(b64::ProveBatch *pb, hipStream_t s) { return pb->enqueue(s); }
-int b64::prove_batch_step(b64::ProveBatch *pb, bool *done) { return pb->step(done); }
+int b64::prove_batch_step(b64::ProveBatch *pb, bool *done, std::vector<uint32_t> *finished) {
+    return pb->step(done, finished);
+}
 void b64::prove_batch_destroy(b64::ProveBatch *pb) { delete pb; }
 b64::ProveBuffers *b64::prove_buffers_create() { return new (std::nothrow) b64::ProveBuffers(); }
 void b64::prove_buffers_destroy(b64::ProveBuffers *b) { delete b; }
@@ -928,7 +925,7 @@ extern "C" int b64_batch_prove(const b64_prove_job *jobs, uint32_t n_jobs, void 
     for (bool done = false; !done;) {
         B64_TRY(pb->enqueue(s));
         HIP_TRY(hipStreamSynchronize(s));
-        B64_TRY(pb->step(&done));
+        B64_TRY(pb->step(&done, nullptr));
     }
     return B64_OK;
 }

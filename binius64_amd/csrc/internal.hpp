@@ -225,15 +225,17 @@ int gather_granules(const uint64_t *src, size_t n, uint8_t *out, hipStream_t s);
 
 // Batched proofs (b64_batch_prove) as a resumable sequence of phases: enqueue a
 // phase (nothing waits), wait for the stream by any means, then step() does the
-// host part (roots into transcripts, challenges, next phase's tables) and
-// reports when every proof is complete.  Buffers persist across batches.
+// host part (roots into transcripts, challenges, the next phase's tables),
+// appends the jobs whose proofs it completed to `finished` (indices into the
+// batch, may be null) and reports when every proof is complete.  Buffers
+// persist across batches.
 struct ProveBuffers;
 struct ProveBatch;
 ProveBuffers *prove_buffers_create();
 void prove_buffers_destroy(ProveBuffers *b);
 int prove_batch_create(const b64_prove_job *jobs, uint32_t n_jobs, ProveBuffers *buf, ProveBatch **out);
 int prove_batch_enqueue(ProveBatch *pb, hipStream_t s);
-int prove_batch_step(ProveBatch *pb, bool *done);
+int prove_batch_step(ProveBatch *pb, bool *done, std::vector<uint32_t> *finished);
 void prove_batch_destroy(ProveBatch *pb);
 
 // Caller-owned buffers for a batch plan (the serving loop reuses one per stream

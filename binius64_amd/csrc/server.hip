@@ -165,22 +165,27 @@ struct b64_server {
             s.prove = nullptr;
         }
         s.ready = false;
-        for (auto &j : s.jobs) pool->release(j->off, j->footprint);
+        for (auto &j : s.jobs)
+            if (!j->done) pool->release(j->off, j->footprint);
         s.jobs.clear();
         s.busy = false;
         if (worker_error == B64_OK) worker_error = rc;
         cv_done.notify_all();
     }
 
-    // Jobs of a slot delivered: memory back to the pool (called with mu held).
+    // A job delivered: its memory back to the pool (called with mu held).
+    void deliver(Job &j, Clock::time_point now) {
+        pool->release(j.off, j.footprint);
+        j.t_done = now;
+        j.done = true;
+        n_done++;
+    }
+
+    // Every job of a slot delivered (called with mu held).
     void complete(Slot &s) {
         const auto now = Clock::now();
-        for (auto &j : s.jobs) {
-            pool->release(j->off, j->footprint);
-            j->t_done = now;
-            j->done = true;
-            n_done++;
-        }
+        for (auto &j : s.jobs)
+            if (!j->done) deliver(*j, now);
         s.jobs.clear();
         s.busy = false;
     }
@@ -274,8 +279,8 @@ struct b64_server {
 
     // The host part of a proof batch's completed phase, then its next phase
     // enqueued (called without mu, like launch).
-    int advance(Slot *slot, bool *done) {
-        B64_TRY(b64::prove_batch_step(slot->prove, done));
+    int advance(Slot *slot, bool *done, std::vector<uint32_t> *finished) {
+        B64_TRY(b64::prove_batch_step(slot->prove, done, finished));
         if (*done) return B64_OK;
         B64_TRY(b64::prove_batch_enqueue(slot->prove, slot->stream));
         if (hipEventRecord(slot->done_ev, slot->stream) != hipSuccess) return B64_ERR_HIP;
@@ -300,17 +305,22 @@ struct b64_server {
                 if (!s.busy || !s.ready) continue;
                 s.ready = false;
                 bool done = false;
+                std::vector<uint32_t> finished;
                 lk.unlock();
-                const int rc = advance(&s, &done);
+                const int rc = advance(&s, &done, &finished);
                 lk.lock();
+                // proofs completed in this step are delivered at once; the rest of
+                // the batch carries on
+                const auto now = Clock::now();
+                for (uint32_t i : finished) deliver(*s.jobs[i], now);
                 if (rc != B64_OK) {
                     fail_slot(s, rc);
                 } else if (done) {
                     b64::prove_batch_destroy(s.prove);
                     s.prove = nullptr;
                     complete(s);
-                    cv_done.notify_all();
                 }
+                if (!finished.empty() || done || rc != B64_OK) cv_done.notify_all();
                 progressed = true;
             }
             bool inflight = false;
