@@ -327,15 +327,16 @@ uint32_t b64_batch_plan_n_launches(const b64_batch_plan *plan);
 void b64_batch_plan_destroy(b64_batch_plan *plan);
 
 /* ---- batched proofs: FRI driven by each job's own transcript ------------------ */
-/* No single counterpart in the reference: for every job, the sequence a prover
- * runs per commitment (prover/prover/src/fri/commit.rs:25-75 commit_interleaved,
- * the caller's transcript.message().write(root) as in protocols/basefold.rs:95-99,
- * then FRIFoldProver::execute_fold_round with each challenge sampled from the
- * transcript, fri/fold.rs:117-175, and finish_proof, :221-244), with the
- * Verifier::setup parameters of the job's size (as b64_job_desc).  Each round's
- * device work runs as one launch per stage across the jobs that have it; the
- * host samples every job's challenges between rounds (one device round trip
- * per round).  Each transcript ends byte-identical to the one-job prover's. */
+/* No single counterpart in the reference: for every job, the sequence of
+ * prover/prover/src/fri/tests.rs:57-88 (commit_interleaved, fri/commit.rs:25-75;
+ * the root written to the transcript; FRIFoldProver::execute_fold_round with
+ * each challenge sampled from the transcript, fri/fold.rs:117-175; finish_proof,
+ * :221-244), with the Verifier::setup parameters of the job's size (as
+ * b64_job_desc).  Each round's device work runs as one launch per stage across
+ * the jobs that have it; the host samples every job's challenges between
+ * rounds (one device round trip per round), and a job's query phase starts as
+ * soon as its own rounds are over.  Each transcript ends byte-identical to the
+ * one-job prover's. */
 typedef struct b64_prove_job {
     uint32_t log_msg_len;        /* 8 .. 40 (batch 2^4) */
     const void *d_msg;           /* device, 2^log_msg_len elements */
