@@ -34,8 +34,8 @@ HBM_PEAK_GBS = 8000.0
 #    each scaled by the same counter on a 512 MiB device copy) and VALU issue rate
 #    (SQ_INSTS_VALU per CU per clock), averaged over the passes of one forward NTT;
 #  - per commit step: the VALU wave-instructions of all its kernels.
-TRAFFIC = {(25, 3, 4, 9, 16, 1): {"bytes": 1074350499, "valu_issue_per_cu_clk": 0.715,
-                                  "step_valu_wave_insts": 1116929344}}
+TRAFFIC = {(25, 3, 4, 9, 16, 1): {"bytes": 1082724185, "valu_issue_per_cu_clk": 0.719,
+                                  "step_valu_wave_insts": 1055436096}}
 # MI355X VALU issue peak: one wave64 instruction per CU per clock, 256 CUs, 2.4 GHz
 VALU_PEAK_WAVE_INSTS_PER_S = 256 * 2.4e9
 
