@@ -38,6 +38,10 @@ TRAFFIC = {(25, 3, 4, 9, 16, 1): {"bytes": 1082724185, "valu_issue_per_cu_clk": 
                                   "step_valu_wave_insts": 1055436096}}
 # MI355X VALU issue peak: one wave64 instruction per CU per clock, 256 CUs, 2.4 GHz
 VALU_PEAK_WAVE_INSTS_PER_S = 256 * 2.4e9
+# LDS: a wave-wide ds_read_b128 occupies a CU's LDS for 4 clocks (256 B/clock;
+# SQ_LDS_IDX_ACTIVE / SQ_INSTS_LDS = 4.0 in profiles/commit_step_log_n24_counters.md)
+LDS_CLK_PER_LOOKUP = 4
+LDS_CU_CLK_PER_S = 256 * 2.4e9
 
 
 def traffic_entry(log_cw, tuning):
@@ -190,6 +194,15 @@ def main():
                 "avg_launch_ms": round(ntt_pass_ms, 4),
                 "algorithmic_bytes_per_launch": pass_bytes,
                 "gf_mul_per_s": float(f"{muls / (stages['ntt_ms'] * 1e-3):.4g}")}
+    # LDS bound of the NTT passes: with schedule variants 3-4 every product goes
+    # through a 4-bit window table, 32 lookups per product, each a wave-wide
+    # ds_read_b128 that holds a CU's LDS for 4 clocks (256 bytes per clock)
+    lds_roofline = None
+    if tuning["variant"] >= 3:
+        lookups = muls * 32 // 64 // n_passes  # wave-wide lookups per launch
+        floor_ms = lookups * LDS_CLK_PER_LOOKUP / LDS_CU_CLK_PER_S * 1e3
+        lds_roofline = {"bound": "lds", "wave_lookups_per_launch": lookups, "clk_per_lookup": LDS_CLK_PER_LOOKUP,
+                        "floor_ms": round(floor_ms, 4), "frac": round(floor_ms / ntt_pass_ms, 4)}
     step_bytes = 58 * (1 << n)  # algorithmic HBM bytes of the whole step
     path_achieved = step_bytes / (ms * 1e-3) / 1e9
     step_valu = traffic_entry(log_cw, tuning).get("step_valu_wave_insts")
@@ -235,6 +248,7 @@ def main():
         "path_hbm": {"algorithmic_bytes_per_step": step_bytes, "achieved_GBs": round(path_achieved, 2),
                      "frac": round(path_achieved / HBM_PEAK_GBS, 5)},
         "valu_roofline": valu_roofline,
+        "lds_roofline": lds_roofline,
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "wall_s_timed_region": round(wall, 4),
         "cpu_baseline": cpu,
