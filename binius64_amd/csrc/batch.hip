@@ -682,12 +682,14 @@ struct ProveJob {
     }
 };
 
-int grow_device(uint8_t **d, size_t *cap, size_t need) {
+// Device buffers grow in stream order (no device-wide synchronisation, so a
+// serving loop's other streams keep running).
+int grow_device(uint8_t **d, size_t *cap, size_t need, hipStream_t s) {
     if (need <= *cap) return B64_OK;
-    if (*d) HIP_TRY(hipFree(*d));
+    if (*d) HIP_TRY(hipFreeAsync(*d, s));
     *d = nullptr;
     *cap = 0;
-    HIP_TRY(hipMalloc(d, need));
+    HIP_TRY(hipMallocAsync((void **)d, need, s));
     *cap = need;
     return B64_OK;
 }
@@ -819,12 +821,12 @@ struct b64::ProveBatch {
     int enqueue(hipStream_t s) {
         const size_t n_roots = in_round.size(), out_bytes = granules_off() + 16 * n_src;
         if (!host.empty()) {
-            B64_TRY(grow_device(&buf->d_tables, &buf->d_tables_cap, host.size()));
+            B64_TRY(grow_device(&buf->d_tables, &buf->d_tables_cap, host.size(), s));
             B64_TRY(grow_pinned(&buf->h_stage, &buf->h_stage_cap, host.size()));
             memcpy(buf->h_stage, host.data(), host.size());
             HIP_TRY(hipMemcpyAsync(buf->d_tables, buf->h_stage, host.size(), hipMemcpyHostToDevice, s));
         }
-        B64_TRY(grow_device(&buf->d_out, &buf->d_out_cap, out_bytes));
+        B64_TRY(grow_device(&buf->d_out, &buf->d_out_cap, out_bytes, s));
         B64_TRY(grow_pinned(&buf->h_out, &buf->h_out_cap, out_bytes));
         if (n_roots) {
             B64_TRY(enqueue_launches(launches, buf->d_tables, s));
