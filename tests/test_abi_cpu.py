@@ -206,3 +206,21 @@ def test_batch_schedule_keys_are_separate(b64):
         b64.config_set(f"batch.tab_group:{d}", old)
     with pytest.raises(b64.B64Error):
         b64.config_set(f"batch.variant:{d}", 2)  # batch plans have variants 0, 1, 3, 4
+
+
+def test_batched_proof_entry_points_reject_bad_arguments(b64):
+    """b64_batch_prove / b64_server_submit_proof check their arguments before any
+    device call (no GPU here)."""
+    import ctypes as C
+    from binius64_amd._abi import ProveJobC
+    L = b64.lib()
+    assert L.b64_batch_prove(None, 0, None) == 1
+    t = b64.ProverTranscript()
+    bad = (ProveJobC * 1)(ProveJobC(12, None, None, t.ptr, None))  # no message, no workspace
+    assert L.b64_batch_prove(bad, 1, None) == 1
+    no_t = (ProveJobC * 1)(ProveJobC(12, 16, 16, None, None))  # no transcript
+    assert L.b64_batch_prove(no_t, 1, None) == 1
+    ticket = C.c_uint64()
+    msg = np.zeros((1 << 12, 2), dtype=np.uint64)
+    assert L.b64_server_submit_proof(None, 12, msg.ctypes.data_as(C.POINTER(C.c_uint64)), t.ptr,
+                                     C.byref(ticket)) == 1
