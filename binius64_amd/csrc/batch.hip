@@ -684,23 +684,28 @@ struct ProveJob {
 
 // Device buffers grow in stream order (no device-wide synchronisation, so a
 // serving loop's other streams keep running).
+// Capacities at least double, so a buffer is reallocated O(log size) times.
+inline size_t grown(size_t cap, size_t need) { return need > 2 * cap ? need : 2 * cap; }
+
 int grow_device(uint8_t **d, size_t *cap, size_t need, hipStream_t s) {
     if (need <= *cap) return B64_OK;
+    const size_t n = grown(*cap, need);
     if (*d) HIP_TRY(hipFreeAsync(*d, s));
     *d = nullptr;
     *cap = 0;
-    HIP_TRY(hipMallocAsync((void **)d, need, s));
-    *cap = need;
+    HIP_TRY(hipMallocAsync((void **)d, n, s));
+    *cap = n;
     return B64_OK;
 }
 
 int grow_pinned(uint8_t **h, size_t *cap, size_t need) {
     if (need <= *cap) return B64_OK;
+    const size_t n = grown(*cap, need);
     if (*h) HIP_TRY(hipHostFree(*h));
     *h = nullptr;
     *cap = 0;
-    HIP_TRY(hipHostMalloc((void **)h, need, hipHostMallocDefault));
-    *cap = need;
+    HIP_TRY(hipHostMalloc((void **)h, n, hipHostMallocDefault));
+    *cap = n;
     return B64_OK;
 }
 
@@ -914,20 +919,70 @@ int b64::prove_batch_step(b64::ProveBatch *pb, bool *done, std::vector<uint32_t>
     return pb->step(done, finished);
 }
 void b64::prove_batch_destroy(b64::ProveBatch *pb) { delete pb; }
-b64::ProveBuffers *b64::prove_buffers_create() { return new (std::nothrow) b64::ProveBuffers(); }
+b64::ProveBuffers *b64::prove_buffers_create(size_t reserve, hipStream_t s) {
+    std::unique_ptr<b64::ProveBuffers> b(new (std::nothrow) b64::ProveBuffers());
+    if (!b) return nullptr;
+    if (reserve &&
+        (grow_device(&b->d_tables, &b->d_tables_cap, reserve, s) != B64_OK ||
+         grow_pinned(&b->h_stage, &b->h_stage_cap, reserve) != B64_OK ||
+         grow_device(&b->d_out, &b->d_out_cap, 2 * reserve, s) != B64_OK ||
+         grow_pinned(&b->h_out, &b->h_out_cap, 2 * reserve) != B64_OK ||
+         grow_pinned(&b->h_fetch, &b->h_fetch_cap, reserve) != B64_OK))
+        return nullptr;
+    return b.release();
+}
 void b64::prove_buffers_destroy(b64::ProveBuffers *b) { delete b; }
+
+namespace {
+
+// Phase buffers of b64_batch_prove, kept per device for later calls (pinned host
+// allocations cost far more than a small batch's work); a call takes a set
+// from the pool and returns it when it ends, so concurrent calls get their own.
+constexpr int POOL_DEVICES = 64;
+std::mutex g_prove_pool_mu;
+std::vector<b64::ProveBuffers *> g_prove_pool[POOL_DEVICES];
+
+struct PooledBuffers {
+    int dev = -1;
+    b64::ProveBuffers *buf = nullptr;
+    int acquire() {
+        HIP_TRY(hipGetDevice(&dev));
+        if (dev < 0 || dev >= POOL_DEVICES) return B64_ERR_NO_DEVICE;
+        {
+            std::lock_guard<std::mutex> g(g_prove_pool_mu);
+            if (!g_prove_pool[dev].empty()) {
+                buf = g_prove_pool[dev].back();
+                g_prove_pool[dev].pop_back();
+                return B64_OK;
+            }
+        }
+        buf = b64::prove_buffers_create(0, nullptr);
+        return buf ? B64_OK : B64_ERR_OUT_OF_MEMORY;
+    }
+    ~PooledBuffers() {
+        if (!buf) return;
+        std::lock_guard<std::mutex> g(g_prove_pool_mu);
+        g_prove_pool[dev].push_back(buf);
+    }
+};
+
+}  // namespace
 
 extern "C" int b64_batch_prove(const b64_prove_job *jobs, uint32_t n_jobs, void *stream) {
     if (!jobs || !n_jobs) return B64_ERR_INVALID_ARGS;
     const hipStream_t s = as_stream(stream);
-    b64::ProveBuffers buf;
+    PooledBuffers pool;
+    B64_TRY(pool.acquire());
     b64::ProveBatch *raw = nullptr;
-    B64_TRY(b64::prove_batch_create(jobs, n_jobs, &buf, &raw));
+    B64_TRY(b64::prove_batch_create(jobs, n_jobs, pool.buf, &raw));
     std::unique_ptr<b64::ProveBatch> pb(raw);
-    for (bool done = false; !done;) {
-        B64_TRY(pb->enqueue(s));
-        HIP_TRY(hipStreamSynchronize(s));
-        B64_TRY(pb->step(&done, nullptr));
+    int rc = B64_OK;
+    for (bool done = false; !done && rc == B64_OK;) {
+        rc = pb->enqueue(s);
+        if (rc == B64_OK && hipStreamSynchronize(s) != hipSuccess) rc = B64_ERR_HIP;
+        if (rc == B64_OK) rc = pb->step(&done, nullptr);
     }
-    return B64_OK;
+    // the buffers go back to the pool only once nothing of this call is in flight
+    if (rc != B64_OK) (void)hipStreamSynchronize(s);
+    return rc;
 }

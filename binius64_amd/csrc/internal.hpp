@@ -231,7 +231,9 @@ int gather_granules(const uint64_t *src, size_t n, uint8_t *out, hipStream_t s);
 // persist across batches.
 struct ProveBuffers;
 struct ProveBatch;
-ProveBuffers *prove_buffers_create();
+// reserve: bytes of stage tables and fetched data (twice that of results) to
+// allocate up front, in stream order on s for the device part (0: none).
+ProveBuffers *prove_buffers_create(size_t reserve, hipStream_t s);
 void prove_buffers_destroy(ProveBuffers *b);
 int prove_batch_create(const b64_prove_job *jobs, uint32_t n_jobs, ProveBuffers *buf, ProveBatch **out);
 int prove_batch_enqueue(ProveBatch *pb, hipStream_t s);

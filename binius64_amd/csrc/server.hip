@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -99,6 +100,7 @@ struct Job {
 constexpr uint32_t MAX_BATCH = 256;
 constexpr size_t TABLE_BYTES = 4u << 20;  // per slot: plan tables for MAX_BATCH jobs
 constexpr size_t ROOTS_CAP = MAX_BATCH * (B64_MAX_FOLD_ARITIES + 2);
+constexpr size_t PROVE_RESERVE = 4u << 20;  // per slot: proof-phase tables (results: twice)
 
 struct Slot {
     hipStream_t stream = nullptr;
@@ -121,6 +123,7 @@ struct b64_server {
     std::vector<Slot> slots;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
+    std::atomic<uint64_t> n_submitted{0};  // bumped by every submit: the polling worker sees new work
     std::deque<std::shared_ptr<Job>> queue;
     std::map<uint64_t, std::shared_ptr<Job>> jobs;
     uint64_t next_ticket = 1, n_done = 0, n_batches = 0;
@@ -287,6 +290,25 @@ struct b64_server {
         return B64_OK;
     }
 
+    // With batches on the device, wait until one of them completes or a job is
+    // submitted, polling (a timed condition-variable wait sleeps tens of
+    // microseconds past its deadline, which a proof batch would pay once per
+    // phase).  Gives the lock back every 200 us.  Only the worker reads or
+    // writes the slots' state, so the loop reads it without the lock.
+    void poll_inflight(std::unique_lock<std::mutex> &lk) {
+        const uint64_t seen = n_submitted.load(std::memory_order_acquire);
+        lk.unlock();
+        const auto t_end = Clock::now() + std::chrono::microseconds(200);
+        for (;;) {
+            bool ready = false;
+            for (Slot &s : slots)
+                if (s.busy && !s.ready && hipEventQuery(s.done_ev) != hipErrorNotReady) ready = true;
+            if (ready || n_submitted.load(std::memory_order_acquire) != seen || Clock::now() > t_end) break;
+            std::this_thread::yield();
+        }
+        lk.lock();
+    }
+
     void run() {
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
@@ -328,7 +350,7 @@ struct b64_server {
             if (stop && !inflight && queue.empty()) break;
             if (progressed) continue;
             if (inflight)
-                cv_work.wait_for(lk, std::chrono::microseconds(20));
+                poll_inflight(lk);
             else
                 cv_work.wait(lk, [&] { return stop || !queue.empty(); });
         }
@@ -360,7 +382,9 @@ int b64_server_create(const b64_server_config *cfg, b64_server **out) {
             return B64_ERR_OUT_OF_MEMORY;
         if (hipHostMalloc((void **)&s.st.h_roots, 32 * ROOTS_CAP, hipHostMallocDefault) != hipSuccess)
             return B64_ERR_OUT_OF_MEMORY;
-        s.pbuf = b64::prove_buffers_create();
+        // proof batches' phase buffers: sized for a few dozen jobs' query phases
+        // up front, so growth stays rare once the server runs
+        s.pbuf = b64::prove_buffers_create(PROVE_RESERVE, s.stream);
         if (!s.pbuf) return B64_ERR_OUT_OF_MEMORY;
     }
     b64_server *p = srv.get();
@@ -390,6 +414,7 @@ static int submit(b64_server *srv, uint32_t log_msg_len, const uint64_t *msg, co
         srv->queue.push_back(j);
         *ticket = j->ticket;
     }
+    srv->n_submitted.fetch_add(1, std::memory_order_release);
     srv->cv_work.notify_one();
     return B64_OK;
 }
