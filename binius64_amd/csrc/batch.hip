@@ -970,6 +970,8 @@ struct PooledBuffers {
 
 extern "C" int b64_batch_prove(const b64_prove_job *jobs, uint32_t n_jobs, void *stream) {
     if (!jobs || !n_jobs) return B64_ERR_INVALID_ARGS;
+    for (uint32_t j = 0; j < n_jobs; j++)
+        if (!jobs[j].d_msg || !jobs[j].d_workspace || !jobs[j].transcript) return B64_ERR_INVALID_ARGS;
     const hipStream_t s = as_stream(stream);
     PooledBuffers pool;
     B64_TRY(pool.acquire());
