@@ -184,3 +184,35 @@ def test_served_proofs_match_oracle(b64, orc):
         assert alone_job.tape() == transcripts[i].finalize(), f"proof job {i} != alone"
     for (n, m, ch, _), r in zip(commits, got_commits):
         assert r == alone(b64, n, m, ch)
+
+
+def test_tight_budget_proofs_and_commits(b64, orc):
+    """Proof jobs and commit jobs under a budget that holds about one and a half
+    of the largest job: admission waits for memory freed by proofs delivered one
+    by one; every tape equals the same job proved alone, every commit its roots."""
+    sizes = [16, 14, 16, 12, 15, 16, 13, 16]
+    ws = max(b64.CommitJob.workspace_size(n) + (16 << n) for n in sizes)
+    budget = int(ws * 1.5) + (1 << 20)
+    srv = b64.Server(hbm_budget_bytes=budget, max_batch=8, n_streams=2)
+    msgs = [orc.random_elems(800 + i, 1 << n) for i, n in enumerate(sizes)]
+    ts = [b64.ProverTranscript() for _ in sizes]
+    try:
+        tickets, commits = [], []
+        for i, n in enumerate(sizes):
+            tickets.append(srv.submit_proof(n, msgs[i], ts[i]))
+            if i % 2:
+                m, c = job_inputs(orc, 11, 900 + i)
+                commits.append((m, c, srv.submit(11, m, c)))
+        got = [srv.wait(t, b64.CommitJob.n_roots_for(n))[0] for t, n in zip(tickets, sizes)]
+        got_c = [srv.wait(t, b64.CommitJob.n_roots_for(11))[0] for _, _, t in commits]
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert st["peak_pool_bytes"] <= budget
+    for i, n in enumerate(sizes):
+        job = b64.ProveJob(n, msgs[i])
+        b64.batch_prove([job])
+        assert ts[i].finalize() == job.tape(), f"proof job {i} (n={n})"
+        assert got[i] == job.roots()
+    for (m, c, _), r in zip(commits, got_c):
+        assert r == alone(b64, 11, m, c)
