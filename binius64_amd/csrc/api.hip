@@ -887,15 +887,6 @@ struct b64_fri_prover {
     uint32_t rs_log_len() const { return p.log_dim + p.log_inv_rate; }
 };
 
-namespace {
-
-int fetch(void *dst, const void *d_src, size_t bytes, hipStream_t s) {
-    HIP_TRY(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, s));
-    return B64_OK;
-}
-
-}  // namespace
-
 int b64::FriQuery::prepare(const b64_fri_params &params, const FriOracleView &first, const FriOracleView *committed,
                            size_t n_committed) {
     p = params;
@@ -917,80 +908,51 @@ int b64::FriQuery::prepare(const b64_fri_params &params, const FriOracleView &fi
     return B64_OK;
 }
 
-size_t b64::FriQuery::fetch_bytes() const {
-    size_t n = 16ull << term.log_len;
-    for (size_t i = 0; i < opened.size(); i++) n += 32ull << depths[i];
-    return n;
-}
-
-int b64::FriQuery::fetch(hipStream_t s, uint8_t *dst) const {
-    HIP_TRY(hipMemcpyAsync(dst, term.cw, 16ull << term.log_len, hipMemcpyDeviceToHost, s));
-    dst += 16ull << term.log_len;
-    for (size_t i = 0; i < opened.size(); i++) {
-        const FriOracleView &o = opened[i];
-        const uint64_t total = (2ull << o.tree_log_len) - 1;
-        const uint64_t start = total + 1 - (2ull << depths[i]);
-        HIP_TRY(hipMemcpyAsync(dst, (const uint8_t *)o.tree + 32 * start, 32ull << depths[i], hipMemcpyDeviceToHost,
-                               s));
-        dst += 32ull << depths[i];
-    }
-    return B64_OK;
-}
-
-int b64::FriQuery::transcribe(b64_transcript *t, const uint8_t *fetched) {
-    b64_transcript_decommit(t, fetched, 16ull << term.log_len);
-    fetched += 16ull << term.log_len;
-    for (size_t i = 0; i < opened.size(); i++) {
-        b64_transcript_decommit(t, fetched, 32ull << depths[i]);
-        fetched += 32ull << depths[i];
-    }
-    // queries: decommitments are not observed, so every index can be sampled
-    // first.  Each opening (prove_coset_opening, query.rs:112-134: the coset's
-    // values, then the branch digests from the leaf up) is listed as 16-byte
-    // granules in tape order.
-    const uint32_t nq = p.n_test_queries;
+int b64::FriQuery::sample(b64_transcript *t) {
+    // queries (fold.rs:230-243): each index is sampled, then its opening written;
+    // decommitments are not observed, so the indices can all come first
     const uint32_t rs_log_len = p.log_dim + p.log_inv_rate;
-    std::vector<uint64_t> idx(nq);
-    for (uint32_t q = 0; q < nq; q++) idx[q] = b64_transcript_sample_bits(t, rs_log_len);
-    src.clear();
-    for (uint32_t q = 0; q < nq; q++) {
-        uint64_t cur = idx[q];
+    idx.resize(p.n_test_queries);
+    for (uint32_t q = 0; q < p.n_test_queries; q++) idx[q] = b64_transcript_sample_bits(t, rs_log_len);
+    for (uint64_t v : idx) {
+        uint64_t cur = v;
         for (size_t i = 0; i < opened.size(); i++) {
-            const FriOracleView &o = opened[i];
-            const int log_coset = (int)(i == 0 ? p.log_batch_size : p.fold_arities[i - 1]);
-            if (i > 0) cur >>= log_coset;
-            if (cur >> o.tree_log_len) return B64_ERR_INDEX_OUT_OF_RANGE;
-            const uint64_t vals = (uint64_t)(uintptr_t)o.cw + ((cur << log_coset) << 4);
-            for (uint64_t k = 0; k < (1ull << log_coset); k++) src.push_back(vals + 16 * k);
-            const int br_len = o.tree_log_len - depths[i];
-            for (int j = 0; j < br_len; j++) {
-                // sibling at level j of BinaryMerkleTree::branch (binary_merkle_tree.rs:154-169)
-                const uint64_t node = (((1ull << j) - 1) << (o.tree_log_len + 1 - j)) | ((cur >> j) ^ 1);
-                const uint64_t at = (uint64_t)(uintptr_t)o.tree + 32 * node;
-                src.push_back(at);
-                src.push_back(at + 16);
-            }
+            if (i > 0) cur >>= p.fold_arities[i - 1];
+            if (cur >> opened[i].tree_log_len) return B64_ERR_INDEX_OUT_OF_RANGE;
         }
     }
     return B64_OK;
 }
 
-void b64::FriQuery::openings(b64_transcript *t, const uint8_t *granules) const {
-    if (!src.empty()) b64_transcript_decommit(t, granules, src.size() * 16);
-}
-
-int b64::gather_granules(const uint64_t *src, size_t n, uint8_t *out, hipStream_t s) {
-    if (n == 0) return B64_OK;
-    uint8_t *d_buf = nullptr;
-    HIP_TRY(hipMallocAsync((void **)&d_buf, n * (sizeof(uint64_t) + 16), s));
-    int rc = B64_OK;
-    if (hipMemcpyAsync(d_buf, src, n * sizeof(uint64_t), hipMemcpyHostToDevice, s) != hipSuccess ||
-        b64::launch_gather16((const uint64_t *)d_buf, (uint4 *)(d_buf + n * sizeof(uint64_t)), n, s) != hipSuccess ||
-        hipMemcpyAsync(out, d_buf + n * sizeof(uint64_t), n * 16, hipMemcpyDeviceToHost, s) != hipSuccess)
-        rc = B64_ERR_HIP;
-    (void)hipFreeAsync(d_buf, s);
-    HIP_TRY(hipStreamSynchronize(s));
-    return rc;
+int b64::FriQuery::args(QueryArgs *a) const {
+    *a = QueryArgs();
+    if (opened.size() > (size_t)MAX_OPENED || term.log_len > 40) return B64_ERR_INVALID_ARGS;
+    a->term = term.cw;
+    a->term_gran = 1ull << term.log_len;
+    a->nq = (uint32_t)idx.size();
+    a->n_opened = (int)opened.size();
+    uint64_t per_query = 0;
+    for (size_t i = 0; i < opened.size(); i++) {
+        const FriOracleView &v = opened[i];
+        QueryOracle &o = a->o[i];
+        o.cw = v.cw;
+        o.tree = v.tree;
+        o.tree_log_len = v.tree_log_len;
+        o.log_coset = (int)(i == 0 ? p.log_batch_size : p.fold_arities[i - 1]);
+        o.depth = depths[i];
+        // the kernel reads cw[< 2^log_len] and tree nodes < 2^(tree_log_len+1) only under these
+        if (v.tree_log_len + o.log_coset != v.log_len || o.depth < 0 || o.depth > v.tree_log_len ||
+            v.log_len > 40)
+            return B64_ERR_INVALID_ARGS;
+        o.n_gran = (1u << o.log_coset) + 2u * (uint32_t)(v.tree_log_len - o.depth);
+        a->layers_gran += 2ull << o.depth;
+        per_query += o.n_gran;
+    }
+    if (per_query * a->nq >= (1ull << 31)) return B64_ERR_INVALID_ARGS;  // 32-bit offsets in the kernel
+    a->per_query = (uint32_t)per_query;
+    a->n_gran = a->term_gran + a->layers_gran + per_query * a->nq;
+    if (a->n_gran >= (1ull << 39)) return B64_ERR_INVALID_ARGS;  // 2^31 blocks at most
+    return B64_OK;
 }
 
 extern "C" {
@@ -1098,7 +1060,8 @@ int b64::fri_fold_round_enqueue(b64_fri_prover *fp, int *committed, const uint8_
         fp->next_commit += fp->p.fold_arities[k];
     else
         fp->has_next = false;
-    B64_TRY(fetch(fp->h_root, (const uint8_t *)o.tree + 32 * ((2ull << o.tree_log_len) - 2), 32, fp->s));
+    HIP_TRY(hipMemcpyAsync(fp->h_root, (const uint8_t *)o.tree + 32 * ((2ull << o.tree_log_len) - 2), 32,
+                           hipMemcpyDeviceToHost, fp->s));
     if (root_host) *root_host = fp->h_root;
     *committed = 1;
     return B64_OK;
@@ -1135,13 +1098,29 @@ int b64_fri_prover_finish_proof(b64_fri_prover *fp, b64_transcript *t) {
     const b64::FriOracleView first = {fp->first.cw, fp->first.tree, fp->first.log_len, fp->first.tree_log_len};
     b64::FriQuery q;
     B64_TRY(q.prepare(fp->p, first, committed.data(), committed.size()));
-    std::vector<uint8_t> fetched(q.fetch_bytes());
-    B64_TRY(q.fetch(fp->s, fetched.data()));
+    B64_TRY(q.sample(t));
+    b64::QueryArgs a;
+    B64_TRY(q.args(&a));
+    // one device buffer: [args][first block][indices][tape]
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t idx_off = up(sizeof(a) + sizeof(uint32_t)), out_off = up(idx_off + 8 * q.idx.size());
+    std::vector<uint8_t> host(idx_off + 8 * q.idx.size());
+    memcpy(host.data(), &a, sizeof(a));
+    memset(host.data() + sizeof(a), 0, sizeof(uint32_t));
+    if (!q.idx.empty()) memcpy(host.data() + idx_off, q.idx.data(), 8 * q.idx.size());
+    std::vector<uint8_t> out(16 * a.n_gran);
+    uint8_t *d = nullptr;
+    HIP_TRY(hipMallocAsync((void **)&d, out_off + out.size(), fp->s));
+    int rc = B64_OK;
+    if (hipMemcpyAsync(d, host.data(), host.size(), hipMemcpyHostToDevice, fp->s) != hipSuccess ||
+        b64::launch_fri_query((const b64::QueryArgs *)d, (const uint32_t *)(d + sizeof(a)), 1, b64::fri_query_blocks(a),
+                              (const uint64_t *)(d + idx_off), (uint4 *)(d + out_off), fp->s) != hipSuccess ||
+        hipMemcpyAsync(out.data(), d + out_off, out.size(), hipMemcpyDeviceToHost, fp->s) != hipSuccess)
+        rc = B64_ERR_HIP;
+    (void)hipFreeAsync(d, fp->s);
     HIP_TRY(hipStreamSynchronize(fp->s));
-    B64_TRY(q.transcribe(t, fetched.data()));
-    std::vector<uint8_t> out(q.src.size() * 16);
-    B64_TRY(b64::gather_granules(q.src.data(), q.src.size(), out.data(), fp->s));
-    q.openings(t, out.data());
+    B64_TRY(rc);
+    b64_transcript_decommit(t, out.data(), out.size());
     return B64_OK;
 }
 

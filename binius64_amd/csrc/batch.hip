@@ -636,8 +636,9 @@ void b64_batch_plan_destroy(b64_batch_plan *plan) { delete plan; }
 // Phase 0 commits every codeword (NTT passes, 16-element leaves, tree 0); each
 // fold phase folds and commits the next oracle of every job that has one due,
 // after the host has written the previous roots to the jobs' transcripts and
-// sampled their challenges; then the query phases of all jobs share one fetch
-// of terminal codewords and layers and one gather launch of the openings.
+// sampled their challenges; a job whose rounds are over samples its query
+// indices and writes its whole query-phase tape in one launch shared with the
+// other jobs of that phase.
 // Every phase is enqueued without waiting; the driver (b64_batch_prove, or the
 // server's worker) waits for it and calls step() for the host part.
 // ===========================================================================
@@ -653,7 +654,7 @@ struct ProveJob {
     bool due = false;    // oracle k's fold challenges are in `unprocessed`
     std::vector<uint64_t> unprocessed;  // challenges as (lo, hi) pairs
     b64::FriQuery q;
-    size_t fetch_at = 0, src_at = 0;    // offsets of its query data in the phase buffers
+    b64::QueryArgs qa;  // its query phase (out_at: granule offset in the phase's output)
     uint32_t n_rounds() const { return L.si->p.log_dim + L.si->p.log_batch_size; }
 
     // FRIFoldProver rounds (fold.rs:107-175): sample and receive challenges
@@ -716,49 +717,55 @@ int grow_pinned(uint8_t **h, size_t *cap, size_t need) {
 // batch.  Only touched between phases, after the previous phase has completed.
 struct b64::ProveBuffers {
     uint8_t *d_tables = nullptr, *h_stage = nullptr;  // stage tables (and gather sources)
-    uint8_t *d_out = nullptr, *h_out = nullptr;       // roots, or gathered granules
-    uint8_t *h_fetch = nullptr;                       // terminal codewords and layers
-    size_t d_tables_cap = 0, h_stage_cap = 0, d_out_cap = 0, h_out_cap = 0, h_fetch_cap = 0;
+    uint8_t *d_out = nullptr;                         // roots, then query tapes
+    uint8_t *h_out[2] = {};                           // alternate phases' copies of d_out
+    size_t d_tables_cap = 0, h_stage_cap = 0, d_out_cap = 0, h_out_cap[2] = {};
     ~ProveBuffers() {
         if (d_tables) (void)hipFree(d_tables);
         if (d_out) (void)hipFree(d_out);
         if (h_stage) (void)hipHostFree(h_stage);
-        if (h_out) (void)hipHostFree(h_out);
-        if (h_fetch) (void)hipHostFree(h_fetch);
+        for (uint8_t *h : h_out)
+            if (h) (void)hipHostFree(h);
     }
 };
 
 // Each phase carries, for every job, the next step it is ready for: the
 // round that commits its next oracle (jobs grouped into multi-job launches),
-// the fetch of its terminal codeword and layers once its rounds are over, or
-// the gather of its openings once those are transcribed.  A job is complete
-// two phases after its last round, whatever the other jobs still have to do.
+// or, once its rounds are over, its query phase.  A job is complete one phase
+// after its last round, whatever the other jobs still have to do.
 struct b64::ProveBatch {
-    enum class St { Rounds, Fetch, Gather, Done };
+    enum class St { Rounds, Query, Done };
     std::vector<ProveJob> pj;
     std::vector<St> st;
     b64::ProveBuffers *buf = nullptr;
     bool first = true;                  // the next round is phase 0 (the codewords' commitments)
-    std::vector<uint32_t> in_round, in_fetch, in_gather;  // jobs in the enqueued phase
-    std::vector<uint8_t> host;          // tables of the round, then the gather sources
+    std::vector<uint32_t> in_round, in_query;  // jobs in the enqueued phase
+    std::vector<uint8_t> host;          // tables of the phase's launches
     std::vector<Launch> launches;
-    size_t roots_off = 0, src_off = 0, n_src = 0;
+    size_t roots_off = 0;
+    size_t q_args_off = 0, q_start_off = 0, q_idx_off = 0;  // the query launch's tables
+    uint32_t q_blocks = 0;
+    uint64_t n_gran = 0;                // granules of the phase's query tapes
+    int ob = 0;                         // h_out buffer of the enqueued phase
+    // query tapes of the completed phase, not yet decommitted: (job, byte offset in h_out[pending_ob])
+    std::vector<std::pair<uint32_t, size_t>> pending;
+    int pending_ob = 0;
 
     // The next phase from each job's state.
     int plan() {
         SlotMap slots;
         std::vector<const uint4 *> roots;
+        std::vector<b64::QueryArgs> qargs;
+        std::vector<uint32_t> qstart;
+        std::vector<uint64_t> qidx;
         in_round.clear();
-        in_fetch.clear();
-        in_gather.clear();
+        in_query.clear();
+        n_gran = 0;
+        q_blocks = 0;
         for (uint32_t j = 0; j < pj.size(); j++) {
             ProveJob &J = pj[j];
             const Layout &L = J.L;
             uint8_t *ws = (uint8_t *)J.jd.d_workspace;
-            if (st[j] == St::Gather) {
-                in_gather.push_back(j);
-                continue;
-            }
             if (st[j] != St::Rounds) continue;
             std::vector<Stage> stg;
             if (first) {
@@ -795,8 +802,19 @@ struct b64::ProveBatch {
                     committed.push_back({(const uint4 *)(ws + L.oracle[k]), (const uint4 *)(ws + L.tree[k]),
                                          L.olen[k], L.olen[k] - L.ocoset[k]});
                 B64_TRY(J.q.prepare(L.si->p, cw, committed.data(), committed.size()));
-                st[j] = St::Fetch;
-                in_fetch.push_back(j);
+                B64_TRY(J.q.sample(J.jd.transcript));
+                B64_TRY(J.q.args(&J.qa));
+                J.qa.idx_at = qidx.size();
+                J.qa.out_at = n_gran;
+                const uint32_t nb = b64::fri_query_blocks(J.qa);
+                if ((uint64_t)q_blocks + nb >= (1ull << 31)) return B64_ERR_INVALID_ARGS;
+                qstart.push_back(q_blocks);
+                q_blocks += nb;
+                n_gran += J.qa.n_gran;
+                qidx.insert(qidx.end(), J.q.idx.begin(), J.q.idx.end());
+                qargs.push_back(J.qa);
+                st[j] = St::Query;
+                in_query.push_back(j);
                 continue;
             }
             for (const Stage &x : stg) slots[{x.slot, (int)x.kind}].push_back(x);
@@ -807,24 +825,19 @@ struct b64::ProveBatch {
         launches.clear();
         B64_TRY(build_launches(slots, host, launches));
         roots_off = put_bytes(host, roots.data(), roots.size() * sizeof(void *));
-        n_src = 0;
-        for (uint32_t j : in_gather) {
-            pj[j].src_at = n_src;
-            n_src += pj[j].q.src.size();
+        if (!in_query.empty()) {
+            q_args_off = put_bytes(host, qargs.data(), qargs.size() * sizeof(b64::QueryArgs));
+            q_start_off = put_bytes(host, qstart.data(), qstart.size() * sizeof(uint32_t));
+            q_idx_off = put_bytes(host, qidx.data(), qidx.size() * sizeof(uint64_t));
         }
-        src_off = align_up(host.size());
-        host.resize(src_off + n_src * sizeof(uint64_t));
-        for (uint32_t j : in_gather)
-            memcpy(host.data() + src_off + pj[j].src_at * sizeof(uint64_t), pj[j].q.src.data(),
-                   pj[j].q.src.size() * sizeof(uint64_t));
         return B64_OK;
     }
 
-    // Output layout of a phase: the round's roots, then the gathered granules.
+    // Output layout of a phase: the round's roots, then the query tapes.
     size_t granules_off() const { return align_up(32 * in_round.size()); }
 
     int enqueue(hipStream_t s) {
-        const size_t n_roots = in_round.size(), out_bytes = granules_off() + 16 * n_src;
+        const size_t n_roots = in_round.size(), out_bytes = granules_off() + 16 * n_gran;
         if (!host.empty()) {
             B64_TRY(grow_device(&buf->d_tables, &buf->d_tables_cap, host.size(), s));
             B64_TRY(grow_pinned(&buf->h_stage, &buf->h_stage_cap, host.size()));
@@ -832,33 +845,29 @@ struct b64::ProveBatch {
             HIP_TRY(hipMemcpyAsync(buf->d_tables, buf->h_stage, host.size(), hipMemcpyHostToDevice, s));
         }
         B64_TRY(grow_device(&buf->d_out, &buf->d_out_cap, out_bytes, s));
-        B64_TRY(grow_pinned(&buf->h_out, &buf->h_out_cap, out_bytes));
+        B64_TRY(grow_pinned(&buf->h_out[ob], &buf->h_out_cap[ob], out_bytes));
         if (n_roots) {
             B64_TRY(enqueue_launches(launches, buf->d_tables, s));
             k_gather_roots<<<(unsigned)((n_roots + 63) / 64), 64, 0, s>>>(
                 (const uint4 *const *)(buf->d_tables + roots_off), (uint4 *)buf->d_out, (int)n_roots);
             HIP_TRY(hipGetLastError());
         }
-        if (n_src)
-            HIP_TRY(b64::launch_gather16((const uint64_t *)(buf->d_tables + src_off),
-                                         (uint4 *)(buf->d_out + granules_off()), n_src, s));
-        if (out_bytes) HIP_TRY(hipMemcpyAsync(buf->h_out, buf->d_out, out_bytes, hipMemcpyDeviceToHost, s));
-        if (!in_fetch.empty()) {
-            size_t total = 0;
-            for (uint32_t j : in_fetch) {
-                pj[j].fetch_at = total;
-                total += pj[j].q.fetch_bytes();
-            }
-            B64_TRY(grow_pinned(&buf->h_fetch, &buf->h_fetch_cap, total));
-            for (uint32_t j : in_fetch) B64_TRY(pj[j].q.fetch(s, buf->h_fetch + pj[j].fetch_at));
-        }
+        if (!in_query.empty())
+            HIP_TRY(b64::launch_fri_query((const b64::QueryArgs *)(buf->d_tables + q_args_off),
+                                          (const uint32_t *)(buf->d_tables + q_start_off), (int)in_query.size(),
+                                          q_blocks, (const uint64_t *)(buf->d_tables + q_idx_off),
+                                          (uint4 *)(buf->d_out + granules_off()), s));
+        if (out_bytes) HIP_TRY(hipMemcpyAsync(buf->h_out[ob], buf->d_out, out_bytes, hipMemcpyDeviceToHost, s));
         return B64_OK;
     }
 
-    // The host part after the enqueued phase has completed, then the next phase
-    // planned.  Jobs whose proofs became complete are appended to `finished`;
-    // *done once every job is complete.
-    int step(bool *done, std::vector<uint32_t> *finished) {
+    // The host part after the enqueued phase has completed: roots into the
+    // transcripts and challenges sampled, then the next phase planned (*more
+    // when there is one).  The phase's query tapes stay pending in its output
+    // buffer, which the next phase does not write, until drain().
+    int step(bool *more) {
+        drain(nullptr);  // a caller that skipped it
+        const uint8_t *h = buf->h_out[ob];
         for (size_t i = 0; i < in_round.size(); i++) {
             ProveJob &J = pj[in_round[i]];
             if (J.due) {
@@ -871,23 +880,27 @@ struct b64::ProveBatch {
                     J.has_next = false;
                 J.k++;
             }
-            J.take_root(buf->h_out + 32 * i);
+            J.take_root(h + 32 * i);
             J.advance();
         }
-        for (uint32_t j : in_gather) {
-            pj[j].q.openings(pj[j].jd.transcript, buf->h_out + granules_off() + 16 * pj[j].src_at);
+        for (uint32_t j : in_query) pending.push_back({j, granules_off() + 16 * pj[j].qa.out_at});
+        pending_ob = ob;
+        B64_TRY(plan());
+        ob ^= 1;
+        *more = !in_round.empty() || !in_query.empty();
+        return B64_OK;
+    }
+
+    // Decommits the pending query tapes: those jobs' proofs are complete
+    // (appended to `finished`, indices into the batch, if not null).
+    void drain(std::vector<uint32_t> *finished) {
+        for (const auto &pq : pending) {
+            const uint32_t j = pq.first;
+            b64_transcript_decommit(pj[j].jd.transcript, buf->h_out[pending_ob] + pq.second, 16 * pj[j].qa.n_gran);
             st[j] = St::Done;
             if (finished) finished->push_back(j);
         }
-        for (uint32_t j : in_fetch) {
-            B64_TRY(pj[j].q.transcribe(pj[j].jd.transcript, buf->h_fetch + pj[j].fetch_at));
-            st[j] = pj[j].q.src.empty() ? St::Done : St::Gather;
-            if (st[j] == St::Done && finished) finished->push_back(j);
-        }
-        B64_TRY(plan());
-        *done = true;
-        for (St x : st) *done = *done && x == St::Done;
-        return B64_OK;
+        pending.clear();
     }
 };
 
@@ -915,9 +928,8 @@ int b64::prove_batch_create(const b64_prove_job *jobs, uint32_t n_jobs, b64::Pro
 }
 
 int b64::prove_batch_enqueue(b64::ProveBatch *pb, hipStream_t s) { return pb->enqueue(s); }
-int b64::prove_batch_step(b64::ProveBatch *pb, bool *done, std::vector<uint32_t> *finished) {
-    return pb->step(done, finished);
-}
+int b64::prove_batch_step(b64::ProveBatch *pb, bool *more) { return pb->step(more); }
+void b64::prove_batch_drain(b64::ProveBatch *pb, std::vector<uint32_t> *finished) { pb->drain(finished); }
 void b64::prove_batch_destroy(b64::ProveBatch *pb) { delete pb; }
 b64::ProveBuffers *b64::prove_buffers_create(size_t reserve, hipStream_t s) {
     std::unique_ptr<b64::ProveBuffers> b(new (std::nothrow) b64::ProveBuffers());
@@ -926,8 +938,8 @@ b64::ProveBuffers *b64::prove_buffers_create(size_t reserve, hipStream_t s) {
         (grow_device(&b->d_tables, &b->d_tables_cap, reserve, s) != B64_OK ||
          grow_pinned(&b->h_stage, &b->h_stage_cap, reserve) != B64_OK ||
          grow_device(&b->d_out, &b->d_out_cap, 2 * reserve, s) != B64_OK ||
-         grow_pinned(&b->h_out, &b->h_out_cap, 2 * reserve) != B64_OK ||
-         grow_pinned(&b->h_fetch, &b->h_fetch_cap, reserve) != B64_OK))
+         grow_pinned(&b->h_out[0], &b->h_out_cap[0], 2 * reserve) != B64_OK ||
+         grow_pinned(&b->h_out[1], &b->h_out_cap[1], 2 * reserve) != B64_OK))
         return nullptr;
     return b.release();
 }
@@ -978,11 +990,15 @@ extern "C" int b64_batch_prove(const b64_prove_job *jobs, uint32_t n_jobs, void 
     b64::ProveBatch *raw = nullptr;
     B64_TRY(b64::prove_batch_create(jobs, n_jobs, pool.buf, &raw));
     std::unique_ptr<b64::ProveBatch> pb(raw);
-    int rc = B64_OK;
-    for (bool done = false; !done && rc == B64_OK;) {
-        rc = pb->enqueue(s);
-        if (rc == B64_OK && hipStreamSynchronize(s) != hipSuccess) rc = B64_ERR_HIP;
-        if (rc == B64_OK) rc = pb->step(&done, nullptr);
+    int rc = pb->enqueue(s);
+    if (rc == B64_OK && hipStreamSynchronize(s) != hipSuccess) rc = B64_ERR_HIP;
+    while (rc == B64_OK) {
+        bool more = false;
+        rc = pb->step(&more);
+        if (rc == B64_OK && more) rc = pb->enqueue(s);
+        pb->drain(nullptr);  // while the device runs the next phase
+        if (rc != B64_OK || !more) break;
+        if (hipStreamSynchronize(s) != hipSuccess) rc = B64_ERR_HIP;
     }
     // the buffers go back to the pool only once nothing of this call is in flight
     if (rc != B64_OK) (void)hipStreamSynchronize(s);

@@ -185,8 +185,6 @@ hipError_t launch_rs_fold_elems(uint4 *elems, uint64_t n, const RsVec &v, hipStr
 int rs_fold_1b_rows_blocks(uint64_t n);
 hipError_t launch_rs_fold_1b_rows(const uint4 *mat, const uint4 *vec, uint64_t n, uint4 *partial, uint4 *out,
                                   hipStream_t s);
-// dst[i] = the 16 bytes at device address src[i] (src itself in device memory).
-hipError_t launch_gather16(const uint64_t *src, uint4 *dst, uint64_t n, hipStream_t s);
 hipError_t launch_pack_witness(const uint64_t *words, uint64_t n_words, uint4 *out, uint64_t n_elems,
                                hipStream_t s);
 
@@ -198,46 +196,77 @@ struct FriOracleView {
     int log_len, tree_log_len;
 };
 
+// One opened oracle of a query phase on the device: a codeword of
+// 2^(tree_log_len + log_coset) elements, its Merkle tree over 2^tree_log_len
+// cosets, the depth of the layer sent whole (optimal_verify_layer), and the
+// granules of one query's opening: 2^log_coset values, then two per branch digest.
+struct QueryOracle {
+    const uint4 *cw;
+    const uint4 *tree;
+    int tree_log_len, log_coset, depth;
+    uint32_t n_gran;
+};
+constexpr int MAX_OPENED = B64_MAX_FOLD_ARITIES + 1;
+// A job's query phase as the 16-byte granules of the tape it writes, in tape
+// order: the terminal codeword (term_gran), the layer of every opened oracle
+// (layers_gran), then per query index the opening of each opened oracle
+// (per_query).  idx_at and out_at: offsets of the job's indices and of its
+// granules in the arrays a launch shares.
+struct QueryArgs {
+    const uint4 *term;
+    uint64_t term_gran, layers_gran, n_gran;
+    uint64_t idx_at, out_at;
+    uint32_t nq, per_query;
+    int n_opened;
+    QueryOracle o[MAX_OPENED];
+};
+
 // The query phase of FRIFoldProver::finish_proof (prover fri/fold.rs:221-244,
 // query.rs:51-134), in steps so that many proofs can share device round trips:
-//   prepare    -- the oracles (the interleaved codeword, then every committed one);
-//   fetch      -- enqueue the copies of the terminal codeword and the opened
-//                 layers into host memory (fetch_bytes() bytes);
-//   transcribe (after those copies) -- decommit them, sample the query indices
-//                 and list every opening as 16-byte granules (device addresses, src);
-//   openings   -- decommit the gathered granules (src.size() * 16 bytes).
+//   prepare -- the oracles (the interleaved codeword, then every committed one);
+//   sample  -- the query indices.  Decommitments are not observed, so every
+//              index can be sampled before any of the phase's tape is written;
+//   args    -- the device arguments of the tape (idx_at = out_at = 0).
+// One launch_fri_query then writes the phase's whole tape, which is
+// decommitted as one piece.
 struct FriQuery {
     b64_fri_params p;
     FriOracleView term;
     std::vector<FriOracleView> opened;
     std::vector<int> depths;
-    std::vector<uint64_t> src;
+    std::vector<uint64_t> idx;
     int prepare(const b64_fri_params &params, const FriOracleView &first, const FriOracleView *committed,
                 size_t n_committed);
-    size_t fetch_bytes() const;
-    int fetch(hipStream_t s, uint8_t *dst) const;
-    int transcribe(b64_transcript *t, const uint8_t *fetched);
-    void openings(b64_transcript *t, const uint8_t *granules) const;
+    int sample(b64_transcript *t);
+    int args(QueryArgs *a) const;
 };
-// out[i] = the 16 bytes at device address src[i], for i < n: one launch, one copy,
-// then a stream synchronize.
-int gather_granules(const uint64_t *src, size_t n, uint8_t *out, hipStream_t s);
+// One launch writing the query-phase tapes of n_jobs jobs: d_jobs a device
+// array of their arguments, d_start[j] the first block of job j (BLOCK granules
+// per block), `blocks` the total; job j reads d_idx + idx_at and writes
+// d_out + out_at.
+hipError_t launch_fri_query(const QueryArgs *d_jobs, const uint32_t *d_start, int n_jobs, uint32_t blocks,
+                            const uint64_t *d_idx, uint4 *d_out, hipStream_t s);
+// Blocks of one job in that launch.
+uint32_t fri_query_blocks(const QueryArgs &a);
 
 // Batched proofs (b64_batch_prove) as a resumable sequence of phases: enqueue a
 // phase (nothing waits), wait for the stream by any means, then step() does the
-// host part (roots into transcripts, challenges, the next phase's tables),
-// appends the jobs whose proofs it completed to `finished` (indices into the
-// batch, may be null) and reports when every proof is complete.  Buffers
-// persist across batches.
+// host part (roots into transcripts, challenges, the next phase's tables) and
+// reports whether a next phase exists; enqueue that one, then drain() decommits
+// the completed phase's query tapes while the device runs it, and appends the
+// jobs it completed to `finished` (indices into the batch, may be null).  Every
+// proof is complete once step() reports no next phase and drain() has run.
+// Buffers persist across batches.
 struct ProveBuffers;
 struct ProveBatch;
-// reserve: bytes of stage tables and fetched data (twice that of results) to
+// reserve: bytes of stage tables (twice that of results and query tapes) to
 // allocate up front, in stream order on s for the device part (0: none).
 ProveBuffers *prove_buffers_create(size_t reserve, hipStream_t s);
 void prove_buffers_destroy(ProveBuffers *b);
 int prove_batch_create(const b64_prove_job *jobs, uint32_t n_jobs, ProveBuffers *buf, ProveBatch **out);
 int prove_batch_enqueue(ProveBatch *pb, hipStream_t s);
-int prove_batch_step(ProveBatch *pb, bool *done, std::vector<uint32_t> *finished);
+int prove_batch_step(ProveBatch *pb, bool *more);
+void prove_batch_drain(ProveBatch *pb, std::vector<uint32_t> *finished);
 void prove_batch_destroy(ProveBatch *pb);
 
 // Caller-owned buffers for a batch plan (the serving loop reuses one per stream

@@ -1661,19 +1661,68 @@ hipError_t launch_pack_witness(const uint64_t *words, uint64_t n_words, uint4 *o
 }
 
 // ---------------------------------------------------------------------------
-// Gather of 16-byte granules from device addresses: dst[i] = *src[i].  The FRI
-// query phase lists every opened coset value and branch digest in tape order,
-// so the whole opening proof comes back in one copy.
+// FRI query phase (prover fri/fold.rs:221-244, query.rs:51-134): one thread per
+// 16-byte granule of a job's tape -- the terminal codeword, the optimal layer of
+// every opened oracle, then per query index each oracle's coset values and
+// branch digests (prove_coset_opening, query.rs:112-134; siblings as in
+// BinaryMerkleTree::branch, binary_merkle_tree.rs:154-169).  The tree of an
+// oracle stores its levels leaves first, so level l starts at node
+// 2^(t+1) - 2^(t+1-l) and the layer of 2^depth digests at 2^(t+1) - 2^(depth+1).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_gather16(const uint64_t *__restrict__ src, uint4 *__restrict__ dst,
-                                                    uint64_t n) {
-    const uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) dst[i] = *reinterpret_cast<const uint4 *>(src[i]);
+__global__ __launch_bounds__(BLOCK) void k_fri_query_multi(const QueryArgs *__restrict__ jobs,
+                                                           const uint32_t *__restrict__ start, int n_jobs,
+                                                           const uint64_t *__restrict__ idx, uint4 *__restrict__ out) {
+    const int j = __builtin_amdgcn_readfirstlane(job_of_block2(start, n_jobs, blockIdx.x));
+    const QueryArgs &a = jobs[j];
+    uint64_t i = (uint64_t)(blockIdx.x - start[j]) * BLOCK + threadIdx.x;
+    if (i >= a.n_gran) return;
+    uint4 *dst = out + a.out_at + i;
+    if (i < a.term_gran) {
+        *dst = a.term[i];
+        return;
+    }
+    i -= a.term_gran;
+    if (i < a.layers_gran) {
+        for (int k = 0; k < a.n_opened; k++) {
+            const QueryOracle &o = a.o[k];
+            const uint64_t n = 2ull << o.depth;  // 2^depth digests
+            if (i < n) {
+                *dst = o.tree[2 * ((2ull << o.tree_log_len) - n) + i];
+                return;
+            }
+            i -= n;
+        }
+        return;
+    }
+    const uint32_t r = (uint32_t)(i - a.layers_gran);
+    const uint32_t q = r / a.per_query;
+    uint32_t rr = r - q * a.per_query;
+    uint64_t cur = idx[a.idx_at + q];
+    for (int k = 0; k < a.n_opened; k++) {
+        const QueryOracle &o = a.o[k];
+        if (k > 0) cur >>= o.log_coset;
+        if (rr < o.n_gran) {
+            cur &= (1ull << o.tree_log_len) - 1;  // the host has checked the range; this keeps reads in bounds
+            const uint32_t nv = 1u << o.log_coset;
+            if (rr < nv) {
+                *dst = o.cw[(cur << o.log_coset) + rr];
+                return;
+            }
+            const uint32_t b = rr - nv, lev = b >> 1;
+            const uint64_t node = (((1ull << lev) - 1) << (o.tree_log_len + 1 - lev)) | ((cur >> lev) ^ 1);
+            *dst = o.tree[2 * node + (b & 1)];
+            return;
+        }
+        rr -= o.n_gran;
+    }
 }
 
-hipError_t launch_gather16(const uint64_t *src, uint4 *dst, uint64_t n, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    k_gather16<<<grid_for(n), BLOCK, 0, s>>>(src, dst, n);
+uint32_t fri_query_blocks(const QueryArgs &a) { return (uint32_t)((a.n_gran + BLOCK - 1) / BLOCK); }
+
+hipError_t launch_fri_query(const QueryArgs *d_jobs, const uint32_t *d_start, int n_jobs, uint32_t blocks,
+                            const uint64_t *d_idx, uint4 *d_out, hipStream_t s) {
+    if (n_jobs == 0 || blocks == 0) return hipSuccess;
+    k_fri_query_multi<<<blocks, BLOCK, 0, s>>>(d_jobs, d_start, n_jobs, d_idx, d_out);
     return hipGetLastError();
 }
 

@@ -280,13 +280,18 @@ struct b64_server {
         return rc;
     }
 
-    // The host part of a proof batch's completed phase, then its next phase
-    // enqueued (called without mu, like launch).
+    // The host part of a proof batch's completed phase, its next phase
+    // enqueued, then the completed query tapes decommitted while that runs
+    // (called without mu, like launch).
     int advance(Slot *slot, bool *done, std::vector<uint32_t> *finished) {
-        B64_TRY(b64::prove_batch_step(slot->prove, done, finished));
-        if (*done) return B64_OK;
-        B64_TRY(b64::prove_batch_enqueue(slot->prove, slot->stream));
-        if (hipEventRecord(slot->done_ev, slot->stream) != hipSuccess) return B64_ERR_HIP;
+        bool more = false;
+        B64_TRY(b64::prove_batch_step(slot->prove, &more));
+        if (more) {
+            B64_TRY(b64::prove_batch_enqueue(slot->prove, slot->stream));
+            if (hipEventRecord(slot->done_ev, slot->stream) != hipSuccess) return B64_ERR_HIP;
+        }
+        b64::prove_batch_drain(slot->prove, finished);
+        *done = !more;
         return B64_OK;
     }
 
