@@ -4,6 +4,10 @@
 #include <stddef.h>
 #include <stdint.h>
 #include <string.h>
+#if !defined(__HIP_DEVICE_COMPILE__)
+#include <cpuid.h>
+#include <immintrin.h>
+#endif
 
 namespace b64 {
 namespace host {
@@ -70,20 +74,26 @@ class Sha256 {
             }
         }
     }
+    // The digest of everything absorbed so far (the state is left as it was):
+    // the pending bytes, 0x80, zeros and the 64-bit bit length, in one block or two.
     void finalize(uint8_t out[32]) const {
-        Sha256 c = *this;
-        const uint64_t bits = c.total_ * 8;
-        const uint8_t pad = 0x80, z = 0;
-        c.update(&pad, 1);
-        while (c.len_ != 56) c.update(&z, 1);
-        uint8_t len[8];
-        for (int i = 0; i < 8; i++) len[i] = (uint8_t)(bits >> (56 - 8 * i));
-        c.update(len, 8);
+        uint32_t h[8];
+        memcpy(h, h_, sizeof(h));
+        uint8_t blk[128];
+        memcpy(blk, buf_, len_);
+        size_t n = len_;
+        blk[n++] = 0x80;
+        const size_t end = n <= 56 ? 64 : 128;
+        memset(blk + n, 0, end - 8 - n);
+        const uint64_t bits = total_ * 8;
+        for (int i = 0; i < 8; i++) blk[end - 8 + i] = (uint8_t)(bits >> (56 - 8 * i));
+        block(h, blk);
+        if (end == 128) block(h, blk + 64);
         for (int i = 0; i < 8; i++) {
-            out[4 * i] = (uint8_t)(c.h_[i] >> 24);
-            out[4 * i + 1] = (uint8_t)(c.h_[i] >> 16);
-            out[4 * i + 2] = (uint8_t)(c.h_[i] >> 8);
-            out[4 * i + 3] = (uint8_t)c.h_[i];
+            out[4 * i] = (uint8_t)(h[i] >> 24);
+            out[4 * i + 1] = (uint8_t)(h[i] >> 16);
+            out[4 * i + 2] = (uint8_t)(h[i] >> 8);
+            out[4 * i + 3] = (uint8_t)h[i];
         }
     }
     static void digest(const uint8_t *p, size_t n, uint8_t out[32]) {
@@ -95,6 +105,68 @@ class Sha256 {
    private:
     static inline uint32_t ror(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
     static void block(uint32_t h[8], const uint8_t *b) {
+#if !defined(__HIP_DEVICE_COMPILE__)
+        if (sha_ni()) {
+            block_ni(h, b);
+            return;
+        }
+#endif
+        block_scalar(h, b);
+    }
+#if !defined(__HIP_DEVICE_COMPILE__)
+    // The x86 SHA extensions (with SSSE3 and SSE4.1), detected once.
+    static bool sha_ni() {
+        static const bool have = [] {
+            unsigned a, b, c, d;
+            if (!__get_cpuid(1, &a, &b, &c, &d) || !(c & bit_SSSE3) || !(c & bit_SSE4_1)) return false;
+            if (!__get_cpuid_count(7, 0, &a, &b, &c, &d)) return false;
+            return (b & bit_SHA) != 0;
+        }();
+        return have;
+    }
+    // One block with sha256rnds2 (two rounds each) on the state split as
+    // (A,B,E,F) / (C,D,G,H), the schedule four words at a time from
+    // sha256msg1 / sha256msg2 (Intel's SHA extensions reference flow).
+    __attribute__((target("sha,sse4.1,ssse3"))) static void block_ni(uint32_t h[8], const uint8_t *b) {
+        static const uint32_t K[64] = {
+            0x428a2f98, 0x71374491, 0xb5c0fbcf, 0xe9b5dba5, 0x3956c25b, 0x59f111f1, 0x923f82a4, 0xab1c5ed5,
+            0xd807aa98, 0x12835b01, 0x243185be, 0x550c7dc3, 0x72be5d74, 0x80deb1fe, 0x9bdc06a7, 0xc19bf174,
+            0xe49b69c1, 0xefbe4786, 0x0fc19dc6, 0x240ca1cc, 0x2de92c6f, 0x4a7484aa, 0x5cb0a9dc, 0x76f988da,
+            0x983e5152, 0xa831c66d, 0xb00327c8, 0xbf597fc7, 0xc6e00bf3, 0xd5a79147, 0x06ca6351, 0x14292967,
+            0x27b70a85, 0x2e1b2138, 0x4d2c6dfc, 0x53380d13, 0x650a7354, 0x766a0abb, 0x81c2c92e, 0x92722c85,
+            0xa2bfe8a1, 0xa81a664b, 0xc24b8b70, 0xc76c51a3, 0xd192e819, 0xd6990624, 0xf40e3585, 0x106aa070,
+            0x19a4c116, 0x1e376c08, 0x2748774c, 0x34b0bcb5, 0x391c0cb3, 0x4ed8aa4a, 0x5b9cca4f, 0x682e6ff3,
+            0x748f82ee, 0x78a5636f, 0x84c87814, 0x8cc70208, 0x90befffa, 0xa4506ceb, 0xbef9a3f7, 0xc67178f2};
+        const __m128i bswap = _mm_set_epi64x(0x0c0d0e0f08090a0bLL, 0x0405060700010203LL);
+        const __m128i cdab = _mm_shuffle_epi32(_mm_loadu_si128((const __m128i *)&h[0]), 0xB1);
+        const __m128i efgh = _mm_shuffle_epi32(_mm_loadu_si128((const __m128i *)&h[4]), 0x1B);
+        __m128i abef = _mm_alignr_epi8(cdab, efgh, 8);
+        __m128i cdgh = _mm_blend_epi16(efgh, cdab, 0xF0);
+        const __m128i abef0 = abef, cdgh0 = cdgh;
+        __m128i w[4];
+        for (int g = 0; g < 16; g++) {
+            __m128i x;
+            if (g < 4) {
+                x = _mm_shuffle_epi8(_mm_loadu_si128((const __m128i *)(b + 16 * g)), bswap);
+            } else {
+                x = _mm_sha256msg1_epu32(w[g & 3], w[(g + 1) & 3]);
+                x = _mm_add_epi32(x, _mm_alignr_epi8(w[(g + 3) & 3], w[(g + 2) & 3], 4));
+                x = _mm_sha256msg2_epu32(x, w[(g + 3) & 3]);
+            }
+            w[g & 3] = x;
+            const __m128i m = _mm_add_epi32(x, _mm_loadu_si128((const __m128i *)&K[4 * g]));
+            cdgh = _mm_sha256rnds2_epu32(cdgh, abef, m);
+            abef = _mm_sha256rnds2_epu32(abef, cdgh, _mm_shuffle_epi32(m, 0x0E));
+        }
+        abef = _mm_add_epi32(abef, abef0);
+        cdgh = _mm_add_epi32(cdgh, cdgh0);
+        const __m128i feba = _mm_shuffle_epi32(abef, 0x1B);
+        const __m128i dchg = _mm_shuffle_epi32(cdgh, 0xB1);
+        _mm_storeu_si128((__m128i *)&h[0], _mm_blend_epi16(feba, dchg, 0xF0));
+        _mm_storeu_si128((__m128i *)&h[4], _mm_alignr_epi8(dchg, feba, 8));
+    }
+#endif
+    static void block_scalar(uint32_t h[8], const uint8_t *b) {
         static const uint32_t K[64] = {
             0x428a2f98, 0x71374491, 0xb5c0fbcf, 0xe9b5dba5, 0x3956c25b, 0x59f111f1, 0x923f82a4, 0xab1c5ed5,
             0xd807aa98, 0x12835b01, 0x243185be, 0x550c7dc3, 0x72be5d74, 0x80deb1fe, 0x9bdc06a7, 0xc19bf174,
