@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -978,6 +979,63 @@ struct PooledBuffers {
     }
 };
 
+// Parts of a b64_batch_prove call, each with its own stream.
+constexpr int PROVE_PARTS = 2;
+
+// A side stream (non-blocking, default priority) and an event for a part of
+// b64_batch_prove's jobs, pooled per device like the buffers.
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t ev = nullptr;
+};
+std::vector<SideStream> g_side_pool[POOL_DEVICES];
+
+struct PooledSide {
+    int dev = -1;
+    SideStream side;
+    bool held = false;
+    int acquire() {
+        HIP_TRY(hipGetDevice(&dev));
+        if (dev < 0 || dev >= POOL_DEVICES) return B64_ERR_NO_DEVICE;
+        {
+            std::lock_guard<std::mutex> g(g_prove_pool_mu);
+            if (!g_side_pool[dev].empty()) {
+                side = g_side_pool[dev].back();
+                g_side_pool[dev].pop_back();
+                held = true;
+                return B64_OK;
+            }
+        }
+        HIP_TRY(hipStreamCreateWithFlags(&side.s, hipStreamNonBlocking));
+        if (hipEventCreateWithFlags(&side.ev, hipEventDisableTiming) != hipSuccess) {
+            (void)hipStreamDestroy(side.s);
+            return B64_ERR_HIP;
+        }
+        held = true;
+        return B64_OK;
+    }
+    ~PooledSide() {
+        if (!held) return;
+        std::lock_guard<std::mutex> g(g_prove_pool_mu);
+        g_side_pool[dev].push_back(side);
+    }
+};
+
+// One batch's phases on one stream: the host part of a completed phase, the
+// next phase enqueued, then the completed phase's query tapes decommitted.
+struct PhaseDriver {
+    b64::ProveBatch *pb;
+    hipStream_t s;
+    bool more = true;
+    int turn() {
+        if (hipStreamSynchronize(s) != hipSuccess) return B64_ERR_HIP;
+        B64_TRY(pb->step(&more));
+        const int rc = more ? pb->enqueue(s) : B64_OK;
+        pb->drain(nullptr);
+        return rc;
+    }
+};
+
 }  // namespace
 
 extern "C" int b64_batch_prove(const b64_prove_job *jobs, uint32_t n_jobs, void *stream) {
@@ -985,22 +1043,50 @@ extern "C" int b64_batch_prove(const b64_prove_job *jobs, uint32_t n_jobs, void 
     for (uint32_t j = 0; j < n_jobs; j++)
         if (!jobs[j].d_msg || !jobs[j].d_workspace || !jobs[j].transcript) return B64_ERR_INVALID_ARGS;
     const hipStream_t s = as_stream(stream);
-    PooledBuffers pool;
-    B64_TRY(pool.acquire());
-    b64::ProveBatch *raw = nullptr;
-    B64_TRY(b64::prove_batch_create(jobs, n_jobs, pool.buf, &raw));
-    std::unique_ptr<b64::ProveBatch> pb(raw);
-    int rc = pb->enqueue(s);
-    if (rc == B64_OK && hipStreamSynchronize(s) != hipSuccess) rc = B64_ERR_HIP;
-    while (rc == B64_OK) {
-        bool more = false;
-        rc = pb->step(&more);
-        if (rc == B64_OK && more) rc = pb->enqueue(s);
-        pb->drain(nullptr);  // while the device runs the next phase
-        if (rc != B64_OK || !more) break;
-        if (hipStreamSynchronize(s) != hipSuccess) rc = B64_ERR_HIP;
+    // The jobs in PROVE_PARTS parts (dealt round-robin in size order, so every
+    // part has work in every phase), each on a stream of its own: the host part
+    // of one part's phase runs while the device works on the others'.
+    const int parts = n_jobs < (uint32_t)PROVE_PARTS ? (int)n_jobs : PROVE_PARTS;
+    std::vector<uint32_t> order(n_jobs);
+    for (uint32_t j = 0; j < n_jobs; j++) order[j] = j;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return jobs[a].log_msg_len > jobs[b].log_msg_len; });
+    std::vector<b64_prove_job> part[PROVE_PARTS];
+    for (uint32_t i = 0; i < n_jobs; i++) part[i % parts].push_back(jobs[order[i]]);
+    PooledBuffers pool[PROVE_PARTS];
+    PooledSide side[PROVE_PARTS];
+    hipStream_t st[PROVE_PARTS];
+    for (int h = 0; h < parts; h++) {
+        B64_TRY(pool[h].acquire());
+        st[h] = s;
+        if (h > 0) {
+            // a side stream starts after the work the caller queued on s (the messages)
+            B64_TRY(side[h].acquire());
+            st[h] = side[h].side.s;
+            HIP_TRY(hipEventRecord(side[h].side.ev, s));
+            HIP_TRY(hipStreamWaitEvent(st[h], side[h].side.ev, 0));
+        }
+    }
+    std::unique_ptr<b64::ProveBatch> pb[PROVE_PARTS];
+    PhaseDriver drv[PROVE_PARTS];
+    for (int h = 0; h < parts; h++) {
+        b64::ProveBatch *raw = nullptr;
+        B64_TRY(b64::prove_batch_create(part[h].data(), (uint32_t)part[h].size(), pool[h].buf, &raw));
+        pb[h].reset(raw);
+        drv[h] = {raw, st[h]};
+    }
+    int rc = B64_OK;
+    for (int h = 0; h < parts && rc == B64_OK; h++) rc = pb[h]->enqueue(st[h]);
+    for (bool more = true; rc == B64_OK && more;) {
+        more = false;
+        for (int h = 0; h < parts && rc == B64_OK; h++)
+            if (drv[h].more) {
+                rc = drv[h].turn();
+                more = more || drv[h].more;
+            }
     }
     // the buffers go back to the pool only once nothing of this call is in flight
-    if (rc != B64_OK) (void)hipStreamSynchronize(s);
+    if (rc != B64_OK)
+        for (int h = 0; h < parts; h++) (void)hipStreamSynchronize(st[h]);
     return rc;
 }
