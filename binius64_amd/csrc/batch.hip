@@ -139,7 +139,7 @@ struct Stage {
 
 // Slot numbering: 0..15 NTT passes, 16 leaves+fold, 17..24 tree-0 steps, then
 // per folded oracle k (base 32 + 16k): 0 fold_codeword, 1 16-element leaves,
-// 2 other leaves, 3.. tree steps.
+// 2 other leaves, 3 few long leaves (one block each), 4.. tree steps.
 constexpr int SLOT_LEAF16 = 16, SLOT_TREE0 = 17, SLOT_ORACLE = 32;
 
 inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + 255) / 256); }
@@ -260,10 +260,14 @@ int oracle_stages(const Layout &L, uint8_t *ws, int k, const uint64_t *chal, std
         st.leaf.n_leaves = n_leaves;
         st.leaf.batch = 1ull << L.ocoset[k];
         st.leaf.digests = tk;
+        if (b64::leaf_hash_long_fits(st.leaf.batch, n_leaves)) {
+            st.kind = b64::MultiKind::LeafHashLong;
+            st.slot = base + 3;  // one kernel kind per slot
+        }
     }
-    st.blocks = blocks_for(n_leaves);
+    st.blocks = st.kind == b64::MultiKind::LeafHashLong ? (uint32_t)n_leaves : blocks_for(n_leaves);
     out.push_back(st);
-    tree_stages(tk, n_leaves, base + 3, out, roots);
+    tree_stages(tk, n_leaves, base + 4, out, roots);
     return B64_OK;
 }
 
@@ -341,7 +345,8 @@ int build_launches(const SlotMap &slots, std::vector<uint8_t> &host, std::vector
                 case b64::MultiKind::NttT4U9:
                 case b64::MultiKind::NttT4U10: a = &x.nttj; sz = sizeof(x.nttj); break;
                 case b64::MultiKind::Leaf16Fold: a = &x.leaf16; sz = sizeof(x.leaf16); break;
-                case b64::MultiKind::LeafHash: a = &x.leaf; sz = sizeof(x.leaf); break;
+                case b64::MultiKind::LeafHash:
+                case b64::MultiKind::LeafHashLong: a = &x.leaf; sz = sizeof(x.leaf); break;
                 case b64::MultiKind::Subtree: a = &x.sub; sz = sizeof(x.sub); break;
                 case b64::MultiKind::FoldCodeword: a = &x.fold; sz = sizeof(x.fold); break;
                 case b64::MultiKind::FoldInterleaved: a = &x.foldil; sz = sizeof(x.foldil); break;
@@ -700,15 +705,26 @@ int grow_device(uint8_t **d, size_t *cap, size_t need, hipStream_t s) {
     return B64_OK;
 }
 
-int grow_pinned(uint8_t **h, size_t *cap, size_t need) {
+int grow_pinned(uint8_t **h, size_t *cap, size_t need, unsigned flags = hipHostMallocDefault) {
     if (need <= *cap) return B64_OK;
     const size_t n = grown(*cap, need);
     if (*h) HIP_TRY(hipHostFree(*h));
     *h = nullptr;
     *cap = 0;
-    HIP_TRY(hipHostMalloc((void **)h, n, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)h, n, flags));
     *cap = n;
     return B64_OK;
+}
+
+// Phase tables up to this size go up through a copy kernel reading pinned host
+// memory: a DMA copy before the phase's first kernel costs ~30 us of hand-over
+// between the copy engine and the compute queue at every phase.
+constexpr size_t KERNEL_UPLOAD_MAX = 1 << 20;
+
+__global__ __launch_bounds__(256) void k_upload16(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                  uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 }  // namespace
@@ -716,10 +732,12 @@ int grow_pinned(uint8_t **h, size_t *cap, size_t need) {
 // Buffers of a batch's phases; they only grow, so a caller that keeps one set
 // (the server, per stream slot) stops allocating once it has seen its largest
 // batch.  Only touched between phases, after the previous phase has completed.
+// The pinned stage and output buffers are coherent host memory: kernels read
+// the stage tables and write the roots there directly.
 struct b64::ProveBuffers {
     uint8_t *d_tables = nullptr, *h_stage = nullptr;  // stage tables (and gather sources)
-    uint8_t *d_out = nullptr;                         // roots, then query tapes
-    uint8_t *h_out[2] = {};                           // alternate phases' copies of d_out
+    uint8_t *d_out = nullptr;                         // query tapes
+    uint8_t *h_out[2] = {};                           // alternate phases' roots and query tapes
     size_t d_tables_cap = 0, h_stage_cap = 0, d_out_cap = 0, h_out_cap[2] = {};
     ~ProveBuffers() {
         if (d_tables) (void)hipFree(d_tables);
@@ -840,25 +858,35 @@ struct b64::ProveBatch {
     int enqueue(hipStream_t s) {
         const size_t n_roots = in_round.size(), out_bytes = granules_off() + 16 * n_gran;
         if (!host.empty()) {
-            B64_TRY(grow_device(&buf->d_tables, &buf->d_tables_cap, host.size(), s));
-            B64_TRY(grow_pinned(&buf->h_stage, &buf->h_stage_cap, host.size()));
+            const size_t up = (host.size() + 15) & ~(size_t)15;
+            B64_TRY(grow_device(&buf->d_tables, &buf->d_tables_cap, up, s));
+            B64_TRY(grow_pinned(&buf->h_stage, &buf->h_stage_cap, up, hipHostMallocCoherent));
             memcpy(buf->h_stage, host.data(), host.size());
-            HIP_TRY(hipMemcpyAsync(buf->d_tables, buf->h_stage, host.size(), hipMemcpyHostToDevice, s));
+            if (up <= KERNEL_UPLOAD_MAX) {
+                const uint32_t n = (uint32_t)(up / 16);
+                k_upload16<<<(n + 255) / 256, 256, 0, s>>>((const uint4 *)buf->h_stage, (uint4 *)buf->d_tables, n);
+                HIP_TRY(hipGetLastError());
+            } else {
+                HIP_TRY(hipMemcpyAsync(buf->d_tables, buf->h_stage, host.size(), hipMemcpyHostToDevice, s));
+            }
         }
-        B64_TRY(grow_device(&buf->d_out, &buf->d_out_cap, out_bytes, s));
-        B64_TRY(grow_pinned(&buf->h_out[ob], &buf->h_out_cap[ob], out_bytes));
+        B64_TRY(grow_device(&buf->d_out, &buf->d_out_cap, 16 * n_gran, s));
+        B64_TRY(grow_pinned(&buf->h_out[ob], &buf->h_out_cap[ob], out_bytes, hipHostMallocCoherent));
         if (n_roots) {
             B64_TRY(enqueue_launches(launches, buf->d_tables, s));
+            // the roots go straight to the (coherent) host buffer
             k_gather_roots<<<(unsigned)((n_roots + 63) / 64), 64, 0, s>>>(
-                (const uint4 *const *)(buf->d_tables + roots_off), (uint4 *)buf->d_out, (int)n_roots);
+                (const uint4 *const *)(buf->d_tables + roots_off), (uint4 *)buf->h_out[ob], (int)n_roots);
             HIP_TRY(hipGetLastError());
         }
-        if (!in_query.empty())
+        if (!in_query.empty()) {
             HIP_TRY(b64::launch_fri_query((const b64::QueryArgs *)(buf->d_tables + q_args_off),
                                           (const uint32_t *)(buf->d_tables + q_start_off), (int)in_query.size(),
                                           q_blocks, (const uint64_t *)(buf->d_tables + q_idx_off),
-                                          (uint4 *)(buf->d_out + granules_off()), s));
-        if (out_bytes) HIP_TRY(hipMemcpyAsync(buf->h_out[ob], buf->d_out, out_bytes, hipMemcpyDeviceToHost, s));
+                                          (uint4 *)buf->d_out, s));
+            HIP_TRY(hipMemcpyAsync(buf->h_out[ob] + granules_off(), buf->d_out, 16 * n_gran, hipMemcpyDeviceToHost,
+                                   s));
+        }
         return B64_OK;
     }
 
@@ -937,10 +965,10 @@ b64::ProveBuffers *b64::prove_buffers_create(size_t reserve, hipStream_t s) {
     if (!b) return nullptr;
     if (reserve &&
         (grow_device(&b->d_tables, &b->d_tables_cap, reserve, s) != B64_OK ||
-         grow_pinned(&b->h_stage, &b->h_stage_cap, reserve) != B64_OK ||
+         grow_pinned(&b->h_stage, &b->h_stage_cap, reserve, hipHostMallocCoherent) != B64_OK ||
          grow_device(&b->d_out, &b->d_out_cap, 2 * reserve, s) != B64_OK ||
-         grow_pinned(&b->h_out[0], &b->h_out_cap[0], 2 * reserve) != B64_OK ||
-         grow_pinned(&b->h_out[1], &b->h_out_cap[1], 2 * reserve) != B64_OK))
+         grow_pinned(&b->h_out[0], &b->h_out_cap[0], 2 * reserve, hipHostMallocCoherent) != B64_OK ||
+         grow_pinned(&b->h_out[1], &b->h_out_cap[1], 2 * reserve, hipHostMallocCoherent) != B64_OK))
         return nullptr;
     return b.release();
 }

@@ -107,6 +107,7 @@ enum class MultiKind {
     NttPass,  // multi-job LDS-tile kernel with 32-row tables (NttPassArgs)
     NttR16, NttR16Msg, NttR16W8, NttR16W8Msg, NttT4U9, NttT4U10,  // (NttJobArgs)
     Leaf16Fold, LeafHash, Subtree, FoldCodeword,
+    LeafHashLong,  // one block per leaf, for few leaves of 32..128 elements (LeafArgs)
     FoldInterleaved  // fold_interleaved of batch 2^4 (FoldIlArgs)
 };
 
@@ -143,6 +144,9 @@ hipError_t launch_ntt(const DeviceDomain &dd, uint4 *data, int log_d, int skip_e
                       bool inverse, hipStream_t s, const uint4 *msg = nullptr, int msg_log = 0);
 hipError_t launch_leaf_hash(const uint4 *elems, uint64_t n_leaves, int log_batch, uint64_t batch,
                             uint4 *digests, hipStream_t s);
+// Whether leaves of `batch` elements, n_leaves of them, go to the one-block-per-leaf
+// kernel (MultiKind::LeafHashLong, one block per leaf).
+bool leaf_hash_long_fits(uint64_t batch, uint64_t n_leaves);
 hipError_t launch_merkle_levels(uint4 *tree, uint64_t n_leaves, const CompressIV &iv, hipStream_t s);
 // 16-element leaves; with folded != nullptr also fold_interleaved with 4 challenges.
 hipError_t launch_leaf16_fold(const uint4 *cw, uint64_t n_leaves, uint4 *digests, uint4 *folded, const uint4 *chal,

@@ -1038,6 +1038,63 @@ __device__ __forceinline__ void leaf_hash_core(const LeafArgs &a, uint64_t blk) 
 
 __global__ __launch_bounds__(BLOCK) void k_leaf_hash(LeafArgs a) { leaf_hash_core(a, blockIdx.x); }
 
+// Few long leaves (the cosets of a small FRI oracle, e.g. the last one's 2
+// leaves of 128 elements): each leaf is a chain of 9..33 compressions, which
+// one lane of an otherwise idle wave runs at ~4 clocks per instruction.  So one
+// 64-lane block per leaf: its lanes first put K[i] + W[i] of every block of the
+// leaf (data blocks and the padding block) in LDS -- the message schedules are
+// independent across blocks -- then lane 0 runs only the rounds, with the
+// schedule off its chain (14 VALU per round instead of ~22).
+constexpr int LONG_LEAF_MAX_BLOCKS = 33;  // 128 elements + the padding block
+
+bool leaf_hash_long_fits(uint64_t batch, uint64_t n_leaves) {
+    return batch % 4 == 0 && batch >= 32 && batch / 4 + 1 <= LONG_LEAF_MAX_BLOCKS && n_leaves <= 256;
+}
+
+__device__ __forceinline__ void leaf_hash_long_core(const LeafArgs &a, uint64_t leaf,
+                                                    uint32_t (*__restrict__ kw)[64]) {
+    const int nb = (int)(a.batch >> 2);  // data blocks; block nb is the padding block
+    const uint4 *p = a.elems + leaf * a.batch;
+    for (int b = threadIdx.x; b <= nb; b += 64) {
+        uint32_t w[16];
+        if (b < nb) {
+            load_block_words(w, p + 4 * b);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16; q++) w[q] = 0;
+            w[0] = 0x80000000u;
+            const uint64_t bits = a.batch * 128ull;
+            w[14] = (uint32_t)(bits >> 32);
+            w[15] = (uint32_t)bits;
+        }
+#pragma unroll
+        for (int i = 0; i < 64; i++) {
+            uint32_t wi;
+            if (i < 16) {
+                wi = w[i];
+            } else {
+                const uint32_t w15 = w[(i - 15) & 15], w2 = w[(i - 2) & 15];
+                const uint32_t s0 = xor3(rotr(w15, 7), rotr(w15, 18), w15 >> 3);
+                const uint32_t s1 = xor3(rotr(w2, 17), rotr(w2, 19), w2 >> 10);
+                wi = w[i & 15] + s0 + w[(i - 7) & 15] + s1;
+                w[i & 15] = wi;
+            }
+            kw[b][i] = wi + SHA_K[i];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    uint32_t st[8];
+    sha256_init(st);
+    for (int b = 0; b <= nb; b++) sha256_rounds_kw(st, kw[b]);
+    store_digest_be(a.digests + 2 * leaf, st);
+}
+
+__global__ __launch_bounds__(64) void k_leaf_hash_long(LeafArgs a) {
+    __shared__ __attribute__((aligned(16))) uint32_t kw[LONG_LEAF_MAX_BLOCKS][64];
+    leaf_hash_long_core(a, blockIdx.x, kw);
+}
+
 // Many jobs in one launch: block b belongs to the job j with
 // start[j] <= b < start[j+1] and works on that job's block b - start[j].
 __device__ __forceinline__ int job_of_block2(const uint32_t *__restrict__ start, int n_jobs, uint32_t b) {
@@ -1054,6 +1111,14 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_hash_multi(const LeafArgs *__res
                                                            const uint32_t *__restrict__ start, int n_jobs) {
     const int j = job_of_block2(start, n_jobs, blockIdx.x);
     leaf_hash_core(jobs[j], blockIdx.x - start[j]);
+}
+
+// One block per leaf over many jobs (every job's leaves pass leaf_hash_long_fits).
+__global__ __launch_bounds__(64) void k_leaf_hash_long_multi(const LeafArgs *__restrict__ jobs,
+                                                             const uint32_t *__restrict__ start, int n_jobs) {
+    __shared__ __attribute__((aligned(16))) uint32_t kw[LONG_LEAF_MAX_BLOCKS][64];
+    const int j = job_of_block2(start, n_jobs, blockIdx.x);
+    leaf_hash_long_core(jobs[j], blockIdx.x - start[j], kw);
 }
 
 // Leaves of 16 elements (the interleaved batch 2^4 of the commit) hashed and,
@@ -1133,7 +1198,10 @@ hipError_t launch_leaf_hash(const uint4 *elems, uint64_t n_leaves, int, uint64_t
     a.n_leaves = n_leaves;
     a.batch = batch;
     a.digests = digests;
-    k_leaf_hash<<<grid_for(n_leaves), BLOCK, 0, s>>>(a);
+    if (leaf_hash_long_fits(batch, n_leaves))
+        k_leaf_hash_long<<<(unsigned)n_leaves, 64, 0, s>>>(a);
+    else
+        k_leaf_hash<<<grid_for(n_leaves), BLOCK, 0, s>>>(a);
     return hipGetLastError();
 }
 
@@ -1620,6 +1688,9 @@ hipError_t launch_multi(MultiKind kind, const void *d_jobs, const uint32_t *d_st
             break;
         case MultiKind::LeafHash:
             k_leaf_hash_multi<<<blocks, BLOCK, 0, s>>>((const LeafArgs *)d_jobs, d_start, n_jobs);
+            break;
+        case MultiKind::LeafHashLong:
+            k_leaf_hash_long_multi<<<blocks, 64, 0, s>>>((const LeafArgs *)d_jobs, d_start, n_jobs);
             break;
         case MultiKind::Subtree:
             k_merkle_subtree_multi<<<blocks, BLOCK, 0, s>>>((const SubtreeArgs *)d_jobs, d_start, n_jobs);

@@ -233,7 +233,10 @@ def test_encode_dimension_mismatch(b64):
 
 
 @pytest.mark.parametrize("n_elems,batch", [(1, 1), (2, 1), (16, 16), (64, 4), (1 << 12, 16), (256, 2),
-                                           (1 << 10, 128), (8, 8), (12, 3)])
+                                           (1 << 10, 128), (8, 8), (12, 3),
+                                           # leaves of 32..128 elements: one block per leaf up to
+                                           # 256 leaves, one thread per leaf above
+                                           (1 << 11, 32), (1 << 14, 64), (1 << 15, 64), (1 << 8, 128)])
 def test_merkle_commit(b64, orc, n_elems, batch):
     elems = orc.random_elems(60 + n_elems, n_elems)
     root, tree = b64.BinaryMerkleTreeProver.commit(b64.DeviceBuffer.from_host(elems), n_elems, batch)
