@@ -137,7 +137,7 @@ struct Stage {
     Stage() {}
 };
 
-// Slot numbering: 0..15 NTT passes, 16 leaves+fold, 17..24 tree-0 steps, then
+// Slot numbering: 0..15 NTT passes, 16 leaves+fold, 17..30 tree-0 steps, then
 // per folded oracle k (base 32 + 16k): 0 fold_codeword, 1 16-element leaves,
 // 2 other leaves, 3 few long leaves (one block each), 4.. tree steps.
 constexpr int SLOT_LEAF16 = 16, SLOT_TREE0 = 17, SLOT_ORACLE = 32;
@@ -200,24 +200,28 @@ void leaf16_stage(const Layout &L, uint8_t *ws, const uint64_t *chal, std::vecto
 }
 
 // The tree levels above 2^k leaves' digests, from slot0 on; its root is listed.
-void tree_stages(uint4 *tree, uint64_t n_leaves, int slot0, std::vector<Stage> &out,
-                 std::vector<const uint4 *> &roots) {
-    uint64_t offs[16], ns[16];
-    int lv[16];
-    const int steps = b64::merkle_reduce_steps(n_leaves, offs, ns, lv);
+// Fails when the steps would run past max_slot (the next stage's slot).
+int tree_stages(uint4 *tree, uint64_t n_leaves, int slot0, int max_slot, std::vector<Stage> &out,
+                std::vector<const uint4 *> &roots) {
+    uint64_t offs[32], ns[32];
+    int lv[32], qd[32];
+    if (slot0 + b64::merkle_reduce_steps(n_leaves, nullptr, nullptr, nullptr, nullptr) > max_slot)
+        return B64_ERR_INVALID_ARGS;
+    const int steps = b64::merkle_reduce_steps(n_leaves, offs, ns, lv, qd);
     for (int i = 0; i < steps; i++) {
         Stage st;
         st.slot = slot0 + i;
-        st.kind = b64::MultiKind::Subtree;
+        st.kind = qd[i] ? b64::MultiKind::MerkleQuad : b64::MultiKind::Subtree;
         st.sub.tree = tree;
         st.sub.in_off = offs[i];
         st.sub.n_in = ns[i];
         st.sub.levels = lv[i];
         st.sub.iv = job_iv();
-        st.blocks = (uint32_t)(ns[i] / b64::merkle_chunk(ns[i]));
+        st.blocks = b64::merkle_step_blocks(ns[i], qd[i]);
         out.push_back(st);
     }
     roots.push_back(tree + 2 * (2 * n_leaves - 2));
+    return B64_OK;
 }
 
 // Folded oracle k: for k > 0 fold_codeword of oracle k-1 with chal (the
@@ -267,7 +271,7 @@ int oracle_stages(const Layout &L, uint8_t *ws, int k, const uint64_t *chal, std
     }
     st.blocks = st.kind == b64::MultiKind::LeafHashLong ? (uint32_t)n_leaves : blocks_for(n_leaves);
     out.push_back(st);
-    tree_stages(tk, n_leaves, base + 4, out, roots);
+    B64_TRY(tree_stages(tk, n_leaves, base + 4, base + 16, out, roots));
     return B64_OK;
 }
 
@@ -277,7 +281,7 @@ int job_stages(const b64_job_desc &jd, const Layout &L, std::vector<Stage> &out,
     B64_TRY(ntt_stages(L, jd.d_msg, jd.log_msg_len, (uint4 *)(ws + L.cw), out));
     // leaves of the codeword + interleaved fold into oracle 0, then tree 0
     leaf16_stage(L, ws, jd.challenges, out);
-    tree_stages((uint4 *)(ws + L.tree0), 1ull << L.rs, SLOT_TREE0, out, roots);
+    B64_TRY(tree_stages((uint4 *)(ws + L.tree0), 1ull << L.rs, SLOT_TREE0, SLOT_ORACLE - 1, out, roots));
     // folded oracles: fold (k > 0), leaves, tree
     int chal_at = L.lbs;
     for (int k = 0; k < L.n_oracles; k++) {
@@ -347,7 +351,8 @@ int build_launches(const SlotMap &slots, std::vector<uint8_t> &host, std::vector
                 case b64::MultiKind::Leaf16Fold: a = &x.leaf16; sz = sizeof(x.leaf16); break;
                 case b64::MultiKind::LeafHash:
                 case b64::MultiKind::LeafHashLong: a = &x.leaf; sz = sizeof(x.leaf); break;
-                case b64::MultiKind::Subtree: a = &x.sub; sz = sizeof(x.sub); break;
+                case b64::MultiKind::Subtree:
+                case b64::MultiKind::MerkleQuad: a = &x.sub; sz = sizeof(x.sub); break;
                 case b64::MultiKind::FoldCodeword: a = &x.fold; sz = sizeof(x.fold); break;
                 case b64::MultiKind::FoldInterleaved: a = &x.foldil; sz = sizeof(x.foldil); break;
             }
@@ -790,7 +795,7 @@ struct b64::ProveBatch {
             if (first) {
                 B64_TRY(ntt_stages(L, J.jd.d_msg, J.jd.log_msg_len, (uint4 *)(ws + L.cw), stg));
                 leaf16_stage(L, ws, nullptr, stg);
-                tree_stages((uint4 *)(ws + L.tree0), 1ull << L.rs, SLOT_TREE0, stg, roots);
+                B64_TRY(tree_stages((uint4 *)(ws + L.tree0), 1ull << L.rs, SLOT_TREE0, SLOT_ORACLE - 1, stg, roots));
             } else if (J.due) {
                 const b64_fri_params &p = L.si->p;
                 if (J.k >= L.n_oracles) return B64_ERR_INVALID_ARGS;

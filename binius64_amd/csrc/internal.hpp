@@ -43,6 +43,9 @@ constexpr int MERKLE_RED_LOG = 9;  // a subtree block reduces a chunk of up to 2
 // time, so most compressions run in full wavefronts.
 constexpr int MERKLE_WIDE_LOG = 13;
 constexpr int MERKLE_WIDE_LEVELS = 4;
+// Levels of at least 2^MERKLE_QUAD_LOG nodes are reduced two at a time by one
+// thread per 4 nodes (k_merkle_quad: no LDS, no barrier, every lane busy).
+constexpr int MERKLE_QUAD_LOG = 18;
 // Nodes per block of a subtree launch over a level of n_in nodes.
 inline uint64_t merkle_chunk(uint64_t n_in) {
     return n_in < (1ull << MERKLE_RED_LOG) ? n_in : (1ull << MERKLE_RED_LOG);
@@ -108,6 +111,7 @@ enum class MultiKind {
     NttR16, NttR16Msg, NttR16W8, NttR16W8Msg, NttT4U9, NttT4U10,  // (NttJobArgs)
     Leaf16Fold, LeafHash, Subtree, FoldCodeword,
     LeafHashLong,  // one block per leaf, for few leaves of 32..128 elements (LeafArgs)
+    MerkleQuad,    // two tree levels, one thread per 4 input nodes (SubtreeArgs, levels = 2)
     FoldInterleaved  // fold_interleaved of batch 2^4 (FoldIlArgs)
 };
 
@@ -157,7 +161,10 @@ hipError_t launch_fold_codeword(const DeviceDomain &dd, const uint4 *cw, uint4 *
                                 const uint4 *chal_host, int n_chal, hipStream_t s);
 // Subtree-reduction schedule of a tree with n_leaves leaves: per launch the
 // input level offset (in digests), its node count and the levels reduced.
-int merkle_reduce_steps(uint64_t n_leaves, uint64_t *offs, uint64_t *ns, int *levels);
+// quad[i] != 0 marks a two-level step for k_merkle_quad.
+int merkle_reduce_steps(uint64_t n_leaves, uint64_t *offs, uint64_t *ns, int *levels, int *quad);
+// Blocks of one step's launch.
+uint32_t merkle_step_blocks(uint64_t n_in, int quad);
 // FRIFoldProver::execute_fold_round without the final wait: the fold, the
 // Merkle commit and, on a committed round, the copy of the root into pinned
 // host memory (*root_host, valid once the stream is synchronised).

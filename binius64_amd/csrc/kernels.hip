@@ -1279,18 +1279,58 @@ __global__ __launch_bounds__(BLOCK) void k_merkle_subtree_multi(const SubtreeArg
     subtree_core(jobs[j], blockIdx.x - start[j], dig);
 }
 
-int merkle_reduce_steps(uint64_t n_leaves, uint64_t *offs, uint64_t *ns, int *levels_out) {
+// Two levels per thread: thread t compresses input nodes 4t..4t+3 into 2 nodes
+// of the next level and those into 1 node of the level after.  The two first
+// compressions are independent, so a lane keeps two chains in flight.
+__device__ __forceinline__ void merkle_quad_core(const SubtreeArgs &a, uint64_t t) {
+    if (t >= (a.n_in >> 2)) return;
+    const uint4 *src = a.tree + 2 * (a.in_off + 4 * t);
+    uint4 x[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) x[i] = src[i];
+    uint4 p0, p1, q0, q1, r0, r1;
+    node_compress(x[0], x[1], x[2], x[3], a.iv, p0, p1);
+    node_compress(x[4], x[5], x[6], x[7], a.iv, q0, q1);
+    uint4 *l1 = a.tree + 2 * (a.in_off + a.n_in + 2 * t);
+    l1[0] = p0;
+    l1[1] = p1;
+    l1[2] = q0;
+    l1[3] = q1;
+    node_compress(p0, p1, q0, q1, a.iv, r0, r1);
+    uint4 *l2 = a.tree + 2 * (a.in_off + a.n_in + (a.n_in >> 1) + t);
+    l2[0] = r0;
+    l2[1] = r1;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_merkle_quad(SubtreeArgs a) {
+    merkle_quad_core(a, (uint64_t)blockIdx.x * BLOCK + threadIdx.x);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_merkle_quad_multi(const SubtreeArgs *__restrict__ jobs,
+                                                             const uint32_t *__restrict__ start, int n_jobs) {
+    const int j = job_of_block2(start, n_jobs, blockIdx.x);
+    merkle_quad_core(jobs[j], (uint64_t)(blockIdx.x - start[j]) * BLOCK + threadIdx.x);
+}
+
+uint32_t merkle_step_blocks(uint64_t n_in, int quad) {
+    return quad ? (uint32_t)(((n_in >> 2) + BLOCK - 1) / BLOCK) : (uint32_t)(n_in / merkle_chunk(n_in));
+}
+
+int merkle_reduce_steps(uint64_t n_leaves, uint64_t *offs, uint64_t *ns, int *levels_out, int *quad) {
     uint64_t off = 0, n = n_leaves;
     int steps = 0;
     while (n > 1) {
         int log_n = 0;
         while ((1ull << log_n) < n) log_n++;
-        const int levels = log_n > MERKLE_WIDE_LOG ? MERKLE_WIDE_LEVELS
-                           : (log_n < MERKLE_RED_LOG ? log_n : MERKLE_RED_LOG);
+        const bool q = log_n >= MERKLE_QUAD_LOG;
+        const int levels = q ? 2
+                             : log_n > MERKLE_WIDE_LOG ? MERKLE_WIDE_LEVELS
+                                                       : (log_n < MERKLE_RED_LOG ? log_n : MERKLE_RED_LOG);
         if (offs) {
             offs[steps] = off;
             ns[steps] = n;
             levels_out[steps] = levels;
+            quad[steps] = q ? 1 : 0;
         }
         steps++;
         for (int l = 0; l < levels; l++) {
@@ -1302,9 +1342,10 @@ int merkle_reduce_steps(uint64_t n_leaves, uint64_t *offs, uint64_t *ns, int *le
 }
 
 hipError_t launch_merkle_levels(uint4 *tree, uint64_t n_leaves, const CompressIV &iv, hipStream_t s) {
-    uint64_t offs[16], ns[16];
-    int lv[16];
-    const int steps = merkle_reduce_steps(n_leaves, offs, ns, lv);
+    uint64_t offs[32], ns[32];
+    int lv[32], qd[32];
+    if (merkle_reduce_steps(n_leaves, nullptr, nullptr, nullptr, nullptr) > 32) return hipErrorInvalidValue;
+    const int steps = merkle_reduce_steps(n_leaves, offs, ns, lv, qd);
     for (int i = 0; i < steps; i++) {
         SubtreeArgs a;
         a.tree = tree;
@@ -1312,7 +1353,10 @@ hipError_t launch_merkle_levels(uint4 *tree, uint64_t n_leaves, const CompressIV
         a.n_in = ns[i];
         a.levels = lv[i];
         a.iv = iv;
-        k_merkle_subtree<<<(unsigned)(ns[i] / merkle_chunk(ns[i])), BLOCK, 0, s>>>(a);
+        if (qd[i])
+            k_merkle_quad<<<merkle_step_blocks(ns[i], 1), BLOCK, 0, s>>>(a);
+        else
+            k_merkle_subtree<<<merkle_step_blocks(ns[i], 0), BLOCK, 0, s>>>(a);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -1694,6 +1738,9 @@ hipError_t launch_multi(MultiKind kind, const void *d_jobs, const uint32_t *d_st
             break;
         case MultiKind::Subtree:
             k_merkle_subtree_multi<<<blocks, BLOCK, 0, s>>>((const SubtreeArgs *)d_jobs, d_start, n_jobs);
+            break;
+        case MultiKind::MerkleQuad:
+            k_merkle_quad_multi<<<blocks, BLOCK, 0, s>>>((const SubtreeArgs *)d_jobs, d_start, n_jobs);
             break;
         case MultiKind::FoldCodeword: {
             const FoldCwArgs *j = (const FoldCwArgs *)d_jobs;

@@ -236,7 +236,10 @@ def test_encode_dimension_mismatch(b64):
                                            (1 << 10, 128), (8, 8), (12, 3),
                                            # leaves of 32..128 elements: one block per leaf up to
                                            # 256 leaves, one thread per leaf above
-                                           (1 << 11, 32), (1 << 14, 64), (1 << 15, 64), (1 << 8, 128)])
+                                           (1 << 11, 32), (1 << 14, 64), (1 << 15, 64), (1 << 8, 128),
+                                           # 2^18 and 2^19 leaves: the widest levels two at a time
+                                           # per thread, then the subtree steps
+                                           (1 << 18, 1), (1 << 20, 2)])
 def test_merkle_commit(b64, orc, n_elems, batch):
     elems = orc.random_elems(60 + n_elems, n_elems)
     root, tree = b64.BinaryMerkleTreeProver.commit(b64.DeviceBuffer.from_host(elems), n_elems, batch)
