@@ -34,8 +34,8 @@ HBM_PEAK_GBS = 8000.0
 #    each scaled by the same counter on a 512 MiB device copy) and VALU issue rate
 #    (SQ_INSTS_VALU per CU per clock), averaged over the passes of one forward NTT;
 #  - per commit step: the VALU wave-instructions of all its kernels.
-TRAFFIC = {(25, 3, 4, 9, 16, 1): {"bytes": 1082724185, "valu_issue_per_cu_clk": 0.719,
-                                  "step_valu_wave_insts": 1055436096}}
+TRAFFIC = {(25, 3, 4, 9, 16, 1): {"bytes": 1082729497, "valu_issue_per_cu_clk": 0.726,
+                                  "step_valu_wave_insts": 1051501888}}
 # MI355X VALU issue peak: one wave64 instruction per CU per clock, 256 CUs, 2.4 GHz
 VALU_PEAK_WAVE_INSTS_PER_S = 256 * 2.4e9
 # LDS: a wave-wide ds_read_b128 occupies a CU's LDS for 4 clocks (256 B/clock;
