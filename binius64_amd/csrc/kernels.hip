@@ -1086,7 +1086,20 @@ __device__ __forceinline__ void leaf_hash_long_core(const LeafArgs &a, uint64_t 
     if (threadIdx.x != 0) return;
     uint32_t st[8];
     sha256_init(st);
-    for (int b = 0; b <= nb; b++) sha256_rounds_kw(st, kw[b]);
+    // each block's 64 words come into registers while the previous block's
+    // rounds run (an LDS read per round would put its latency on the chain)
+    uint4 cur[16], nxt[16];
+    const uint4 *k4 = reinterpret_cast<const uint4 *>(&kw[0][0]);
+#pragma unroll
+    for (int q = 0; q < 16; q++) cur[q] = k4[q];
+    for (int b = 0; b <= nb; b++) {
+        const int bn = b < nb ? b + 1 : b;  // the last block re-reads itself (unused)
+#pragma unroll
+        for (int q = 0; q < 16; q++) nxt[q] = k4[16 * bn + q];
+        sha256_rounds_kw(st, cur);
+#pragma unroll
+        for (int q = 0; q < 16; q++) cur[q] = nxt[q];
+    }
     store_digest_be(a.digests + 2 * leaf, st);
 }
 

@@ -64,15 +64,17 @@ __device__ __forceinline__ void sha256_compress(uint32_t s[8], uint32_t w[16]) {
     s[7] += h;
 }
 
-// The 64 rounds of one block with K[i] + W[i] precomputed (kw, in LDS), added
-// into state s[].
-__device__ __forceinline__ void sha256_rounds_kw(uint32_t s[8], const uint32_t *__restrict__ kw) {
+// The 64 rounds of one block with K[i] + W[i] precomputed (kw, in registers:
+// kw[i / 4] holds words i..i+3), added into state s[].
+__device__ __forceinline__ void sha256_rounds_kw(uint32_t s[8], const uint4 (&kw)[16]) {
     uint32_t a = s[0], b = s[1], c = s[2], d = s[3], e = s[4], f = s[5], g = s[6], h = s[7];
 #pragma unroll
     for (int i = 0; i < 64; i++) {
+        const uint4 q = kw[i >> 2];
+        const uint32_t k = (i & 3) == 0 ? q.x : (i & 3) == 1 ? q.y : (i & 3) == 2 ? q.z : q.w;
         const uint32_t S1 = xor3(rotr(e, 6), rotr(e, 11), rotr(e, 25));
         const uint32_t ch = bsel(f, g, e);
-        const uint32_t t1 = h + S1 + ch + kw[i];
+        const uint32_t t1 = h + S1 + ch + k;
         const uint32_t S0 = xor3(rotr(a, 2), rotr(a, 13), rotr(a, 22));
         const uint32_t mj = maj3(a, b, c);
         h = g;
