@@ -291,6 +291,32 @@ int job_stages(const b64_job_desc &jd, const Layout &L, std::vector<Stage> &out,
     return B64_OK;
 }
 
+// Tables up to this size go up through a copy kernel reading coherent pinned
+// host memory: a DMA copy before a phase's first kernel costs ~30 us of
+// hand-over between the copy engine and the compute queue.
+constexpr size_t KERNEL_UPLOAD_MAX = 1 << 20;
+
+__global__ __launch_bounds__(256) void k_upload16(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                  uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// Tables of `bytes` from host (coherent pinned, at least bytes rounded up to 16)
+// to dev (as large), in order on s.
+int upload_tables(const uint8_t *host, uint8_t *dev, size_t bytes, hipStream_t s) {
+    const size_t up = (bytes + 15) & ~(size_t)15;
+    if (up <= KERNEL_UPLOAD_MAX) {
+        const uint32_t n = (uint32_t)(up / 16);
+        if (n) k_upload16<<<(n + 255) / 256, 256, 0, s>>>((const uint4 *)host, (uint4 *)dev, n);
+        HIP_TRY(hipGetLastError());
+        return B64_OK;
+    }
+    HIP_TRY(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, s));
+    return B64_OK;
+}
+
+// The roots of a plan's jobs to dst (device memory, or coherent pinned host memory).
 __global__ void k_gather_roots(const uint4 *const *__restrict__ src, uint4 *__restrict__ dst, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
@@ -384,8 +410,7 @@ struct b64_batch_plan {
     int n_roots = 0;
     uint8_t *d_tables = nullptr;
     const uint4 **d_root_src = nullptr;
-    uint4 *d_roots = nullptr;
-    uint8_t *h_roots = nullptr;  // pinned
+    uint8_t *h_roots = nullptr;  // coherent pinned host memory: the roots are written there by a kernel
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool use_graph = false;
@@ -400,7 +425,6 @@ struct b64_batch_plan {
         if (graph) (void)hipGraphDestroy(graph);
         if (owns_buffers) {
             if (d_tables) (void)hipFree(d_tables);
-            if (d_roots) (void)hipFree(d_roots);
             if (h_roots) (void)hipHostFree(h_roots);
         }
     }
@@ -408,7 +432,7 @@ struct b64_batch_plan {
     int enqueue(hipStream_t s) const {
         B64_TRY(enqueue_launches(launches, d_tables, s));
         if (n_roots) {
-            k_gather_roots<<<(n_roots + 63) / 64, 64, 0, s>>>(d_root_src, d_roots, n_roots);
+            k_gather_roots<<<(n_roots + 63) / 64, 64, 0, s>>>(d_root_src, (uint4 *)h_roots, n_roots);
             HIP_TRY(hipGetLastError());
         }
         return B64_OK;
@@ -451,11 +475,25 @@ int b64_job_oracle(const b64_job_desc *job, uint32_t k, const void **d_codeword,
 
 }  // extern "C"
 
+// A job's root addresses, passed by value to the kernel that writes them out.
+struct RootPtrs {
+    const uint4 *p[B64_MAX_FOLD_ARITIES + 2];
+};
+
+__global__ void k_roots_out(RootPtrs src, int n, uint4 *__restrict__ dst) {
+    const int i = threadIdx.x;
+    if (i < n) {
+        dst[2 * i] = src.p[i][0];
+        dst[2 * i + 1] = src.p[i][1];
+    }
+}
+
 // Plain path: the job's stages as single-job launches on `s` (the per-size
-// NTT schedule, register-resident passes included), then the copies of its
-// roots into roots_dst (host memory; pinned for the copies to stay
-// asynchronous).  Nothing waits.
-static int job_enqueue(const b64_job_desc *job, hipStream_t s, uint8_t *roots_dst) {
+// NTT schedule, register-resident passes included), then its roots into
+// roots_dst: written by one kernel when roots_dst is coherent pinned host
+// memory (direct), else copied one by one (host memory; pinned for the copies
+// to stay asynchronous).  Nothing waits.
+static int job_enqueue(const b64_job_desc *job, hipStream_t s, uint8_t *roots_dst, bool direct = false) {
     if (!job || !job->d_msg || !job->d_workspace || !job->challenges || !roots_dst) return B64_ERR_INVALID_ARGS;
     SizeInfo *si = nullptr;
     B64_TRY(size_info(job->log_msg_len, &si));
@@ -489,6 +527,13 @@ static int job_enqueue(const b64_job_desc *job, hipStream_t s, uint8_t *roots_ds
         HIP_TRY(b64::launch_leaf_hash((const uint4 *)(ws + L.oracle[k]), n_leaves, 0, 1ull << L.ocoset[k],
                                       (uint4 *)(ws + L.tree[k]), s));
         HIP_TRY(b64::launch_merkle_levels((uint4 *)(ws + L.tree[k]), n_leaves, job_iv(), s));
+    }
+    if (direct && roots.size() <= B64_MAX_FOLD_ARITIES + 2) {
+        RootPtrs rp;
+        for (size_t r = 0; r < roots.size(); r++) rp.p[r] = roots[r];
+        k_roots_out<<<1, 64, 0, s>>>(rp, (int)roots.size(), (uint4 *)roots_dst);
+        HIP_TRY(hipGetLastError());
+        return B64_OK;
     }
     for (size_t r = 0; r < roots.size(); r++)
         HIP_TRY(hipMemcpyAsync(roots_dst + 32 * r, roots[r], 32, hipMemcpyDeviceToHost, s));
@@ -538,7 +583,7 @@ int b64_batch_plan_build(const b64_job_desc *jobs, uint32_t n_jobs, int use_grap
             plan->owns_buffers = false;
             plan->h_roots = st->h_roots;
         } else {
-            HIP_TRY(hipHostMalloc((void **)&plan->h_roots, 32ull * plan->n_roots, hipHostMallocDefault));
+            HIP_TRY(hipHostMalloc((void **)&plan->h_roots, 32ull * plan->n_roots, hipHostMallocCoherent));
         }
         *out = guard.release();
         return B64_OK;
@@ -569,19 +614,18 @@ int b64_batch_plan_build(const b64_job_desc *jobs, uint32_t n_jobs, int use_grap
     const size_t root_src_off = put_bytes(host, root_src.data(), root_src.size() * sizeof(void *));
     if (st) {
         // caller-owned buffers, uploaded in stream order (no allocation, no device sync)
-        if (host.size() > st->tables_cap || (size_t)plan->n_roots > st->roots_cap) return B64_ERR_BUFFER_TOO_SMALL;
+        if (((host.size() + 15) & ~(size_t)15) > st->tables_cap || (size_t)plan->n_roots > st->roots_cap)
+            return B64_ERR_BUFFER_TOO_SMALL;
         plan->owns_buffers = false;
         plan->d_tables = st->d_tables;
-        plan->d_roots = st->d_roots;
         plan->h_roots = st->h_roots;
         memcpy(st->h_stage, host.data(), host.size());
-        HIP_TRY(hipMemcpyAsync(plan->d_tables, st->h_stage, host.size(), hipMemcpyHostToDevice, upload_stream));
+        B64_TRY(upload_tables(st->h_stage, plan->d_tables, host.size(), upload_stream));
     } else {
         HIP_TRY(hipMalloc(&plan->d_tables, host.size() ? host.size() : 16));
         HIP_TRY(hipMemcpy(plan->d_tables, host.data(), host.size(), hipMemcpyHostToDevice));
-        HIP_TRY(hipMalloc(&plan->d_roots, 32ull * (plan->n_roots ? plan->n_roots : 1)));
         HIP_TRY(hipHostMalloc((void **)&plan->h_roots, 32ull * (plan->n_roots ? plan->n_roots : 1),
-                              hipHostMallocDefault));
+                              hipHostMallocCoherent));
     }
     plan->d_root_src = (const uint4 **)(plan->d_tables + root_src_off);
     if (use_graph) {
@@ -610,13 +654,12 @@ extern "C" {
 int b64_batch_plan_enqueue(b64_batch_plan *plan, void *stream) {
     if (!plan) return B64_ERR_INVALID_ARGS;
     hipStream_t s = as_stream(stream);
-    if (plan->single) return job_enqueue(&plan->jobs[0], s, plan->h_roots);
+    // h_roots is coherent pinned host memory: the roots are written there by a kernel
+    if (plan->single) return job_enqueue(&plan->jobs[0], s, plan->h_roots, true);
     if (plan->use_graph)
         HIP_TRY(hipGraphLaunch(plan->exec, s));
     else
         B64_TRY(plan->enqueue(s));
-    if (plan->n_roots)
-        HIP_TRY(hipMemcpyAsync(plan->h_roots, plan->d_roots, 32ull * plan->n_roots, hipMemcpyDeviceToHost, s));
     return B64_OK;
 }
 
@@ -719,17 +762,6 @@ int grow_pinned(uint8_t **h, size_t *cap, size_t need, unsigned flags = hipHostM
     HIP_TRY(hipHostMalloc((void **)h, n, flags));
     *cap = n;
     return B64_OK;
-}
-
-// Phase tables up to this size go up through a copy kernel reading pinned host
-// memory: a DMA copy before the phase's first kernel costs ~30 us of hand-over
-// between the copy engine and the compute queue at every phase.
-constexpr size_t KERNEL_UPLOAD_MAX = 1 << 20;
-
-__global__ __launch_bounds__(256) void k_upload16(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
-                                                  uint32_t n) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) dst[i] = src[i];
 }
 
 }  // namespace
@@ -867,13 +899,7 @@ struct b64::ProveBatch {
             B64_TRY(grow_device(&buf->d_tables, &buf->d_tables_cap, up, s));
             B64_TRY(grow_pinned(&buf->h_stage, &buf->h_stage_cap, up, hipHostMallocCoherent));
             memcpy(buf->h_stage, host.data(), host.size());
-            if (up <= KERNEL_UPLOAD_MAX) {
-                const uint32_t n = (uint32_t)(up / 16);
-                k_upload16<<<(n + 255) / 256, 256, 0, s>>>((const uint4 *)buf->h_stage, (uint4 *)buf->d_tables, n);
-                HIP_TRY(hipGetLastError());
-            } else {
-                HIP_TRY(hipMemcpyAsync(buf->d_tables, buf->h_stage, host.size(), hipMemcpyHostToDevice, s));
-            }
+            B64_TRY(upload_tables(buf->h_stage, buf->d_tables, host.size(), s));
         }
         B64_TRY(grow_device(&buf->d_out, &buf->d_out_cap, 16 * n_gran, s));
         B64_TRY(grow_pinned(&buf->h_out[ob], &buf->h_out_cap[ob], out_bytes, hipHostMallocCoherent));

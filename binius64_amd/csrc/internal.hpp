@@ -284,10 +284,9 @@ void prove_batch_destroy(ProveBatch *pb);
 // slot, so building a plan neither allocates nor synchronizes the device).
 struct BatchStorage {
     uint8_t *d_tables;  // device, tables_cap bytes
-    uint8_t *h_stage;   // pinned host staging for the table upload, tables_cap bytes
+    uint8_t *h_stage;   // coherent pinned host staging for the table upload (a kernel reads it), tables_cap bytes
     size_t tables_cap;
-    uint4 *d_roots;     // device, roots_cap digests
-    uint8_t *h_roots;   // pinned host, roots_cap digests
+    uint8_t *h_roots;   // coherent pinned host (hipHostMallocCoherent), roots_cap digests: kernels write the roots there
     size_t roots_cap;
 };
 

@@ -146,7 +146,6 @@ struct b64_server {
             if (s.done_ev) (void)hipEventDestroy(s.done_ev);
             if (s.stream) (void)hipStreamDestroy(s.stream);
             if (s.st.d_tables) (void)hipFree(s.st.d_tables);
-            if (s.st.d_roots) (void)hipFree(s.st.d_roots);
             if (s.st.h_stage) (void)hipHostFree(s.st.h_stage);
             if (s.st.h_roots) (void)hipHostFree(s.st.h_roots);
         }
@@ -382,10 +381,9 @@ int b64_server_create(const b64_server_config *cfg, b64_server **out) {
         s.st.tables_cap = TABLE_BYTES;
         s.st.roots_cap = ROOTS_CAP;
         if (hipMalloc(&s.st.d_tables, TABLE_BYTES) != hipSuccess) return B64_ERR_OUT_OF_MEMORY;
-        if (hipMalloc(&s.st.d_roots, 32 * ROOTS_CAP) != hipSuccess) return B64_ERR_OUT_OF_MEMORY;
-        if (hipHostMalloc((void **)&s.st.h_stage, TABLE_BYTES, hipHostMallocDefault) != hipSuccess)
+        if (hipHostMalloc((void **)&s.st.h_stage, TABLE_BYTES, hipHostMallocCoherent) != hipSuccess)
             return B64_ERR_OUT_OF_MEMORY;
-        if (hipHostMalloc((void **)&s.st.h_roots, 32 * ROOTS_CAP, hipHostMallocDefault) != hipSuccess)
+        if (hipHostMalloc((void **)&s.st.h_roots, 32 * ROOTS_CAP, hipHostMallocCoherent) != hipSuccess)
             return B64_ERR_OUT_OF_MEMORY;
         // proof batches' phase buffers: sized for a few dozen jobs' query phases
         // up front, so growth stays rare once the server runs
