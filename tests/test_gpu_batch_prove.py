@@ -55,7 +55,9 @@ def rand_msg(seed, n):
     return np.random.default_rng(seed).integers(0, 2 ** 63, size=(1 << n, 2), dtype=np.uint64)
 
 
-@pytest.mark.parametrize("sizes", [[8], [12], [8, 9, 10, 11, 12, 13, 14, 15, 16], [16, 9, 13, 9, 16]])
+# one job (one stream), two jobs (one per part), odd and even counts split over the two parts
+@pytest.mark.parametrize("sizes", [[8], [12], [12, 13], [10, 10], [8, 9, 10, 11, 12, 13, 14, 15, 16],
+                                   [16, 9, 13, 9, 16]])
 def test_batch_prove_matches_oracle(b64, orc, sizes):
     say("batch_prove sizes", sizes)
     msgs = [rand_msg(300 + i, n) for i, n in enumerate(sizes)]
