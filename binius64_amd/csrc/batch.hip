@@ -292,8 +292,8 @@ int job_stages(const b64_job_desc &jd, const Layout &L, std::vector<Stage> &out,
 }
 
 // Tables up to this size go up through a copy kernel reading coherent pinned
-// host memory: a DMA copy before a phase's first kernel costs ~30 us of
-// hand-over between the copy engine and the compute queue.
+// host memory: in a kernel trace, a DMA copy ahead of a phase's first kernel
+// showed ~30 us between the copy's end and the kernel's start.
 constexpr size_t KERNEL_UPLOAD_MAX = 1 << 20;
 
 __global__ __launch_bounds__(256) void k_upload16(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
