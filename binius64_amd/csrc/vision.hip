@@ -13,20 +13,30 @@
 //  * B^-1: B's linear part inverted over GF(2) (Gauss-Jordan on its 128 x 128
 //    bit matrix), plus the constant B^-1(0) = L^-1(c0);
 //  * window tables (the 4-bit layout of gf128_dev.hpp's win_mul_perm) for the
-//    F2-linear maps the S-box needs: x^(2^k) for k = 1, 3, 6, 12, 24 (the
-//    Itoh-Tsujii inversion chain) and the linear parts of B and B^-1.
+//    F2-linear maps the S-box needs: x^(2^64) and the linear parts of B and
+//    B^-1 (32 rows each), the Frobenius maps of the subfield chain below on the
+//    words that determine a subfield element (16, 8 and 8 rows), and the
+//    inverses of GF(2^8)^*.
 //
 // On the device one lane owns one state.  The S-box's batch inversion
 // (verifier/math/src/batch_invert.rs: invert_or_zero of every element) uses one
 // field inversion per S-box for the whole state (prefix products, then back
-// substitution); the inversion is a^(2^128-2) by the Itoh-Tsujii chain, 10
-// general products and 12 table maps.  The seven tables (56 KiB) and the round
-// constants sit in LDS for the whole launch (two blocks per CU).
+// substitution).  The inversion goes down the subfields by norms:
+// n1 = a a^(2^64) lies in GF(2^64), n2 = n1 n1^(2^32) in GF(2^32),
+// n3 = n2 n2^(2^16) in GF(2^16), n4 = n3 n3^(2^8) in GF(2^8), whose inverse is a
+// table entry; then a^-1 = a^(2^64) n1^(2^32) n2^(2^16) n3^(2^8) n4^-1.  That is
+// 8 general products and 57 table reads (the Itoh-Tsujii chain a^(2^128-2) takes
+// 10 products and 12 full maps, 384 reads).  In the GHASH representation an
+// element of GF(2^64) is fixed by its words 0 and 2, one of GF(2^32) or GF(2^16)
+// by word 0 and one of GF(2^8) by nibbles 0 and 4, so the subfield maps read
+// only those nibbles.  The tables (36 KiB) and the round constants sit in LDS
+// for the whole launch.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
 #include <mutex>
 #include <new>
+#include <utility>
 
 #include "abi_util.hpp"
 #include "gf128_dev.hpp"
@@ -102,15 +112,21 @@ constexpr u64x2 CONSTANT6[6] = {
 constexpr int ROUNDS = 8;
 constexpr int RC_ROWS = 1 + 2 * ROUNDS;
 constexpr int MAXM = 6;
-constexpr int N_TABLES = 7;
-// T_Pk: x -> x^(2^k)
-enum Table { T_P1 = 0, T_P3, T_P6, T_P12, T_P24, T_BFWD, T_BINV };
+// full 32-row window tables; T_P64: x -> x^(2^64)
+constexpr int N_TABLES = 3;
+enum Table { T_P64 = 0, T_BFWD, T_BINV };
+// Partial tables after them (uint4 offsets, each 256-byte aligned): rows for
+// the nibbles that fix a subfield element, entry n of row r at slot n ^ (r & 15).
+constexpr int PT_F32 = N_TABLES * WIN_ENTRIES;  // x^(2^32) on GF(2^64): rows 0..7 word 0, 8..15 word 2
+constexpr int PT_F16 = PT_F32 + 16 * 16;        // x^(2^16) on GF(2^32): word 0
+constexpr int PT_F8 = PT_F16 + 8 * 16;          // x^(2^8) on GF(2^16): word 0
+constexpr int PT_I8 = PT_F8 + 8 * 16;           // x^-1 on GF(2^8)^*, by nibble 0 | nibble 4 << 4 of word 0
 constexpr int VBLOCK = 256;
-// Two blocks of VBLOCK threads share a CU (58 KiB of LDS each), so kernels are
+// Two blocks of VBLOCK threads share a CU (37 KiB of LDS each), so kernels are
 // held to two waves per SIMD: at most 256 VGPRs and AGPRs together.
 #define VIS_KERNEL __global__ __launch_bounds__(VBLOCK) __attribute__((amdgpu_waves_per_eu(2)))
 // device data of one width: tables, then round constants (RC_ROWS * MAXM), then c0 and B^-1(0)
-constexpr int DEV_TAB = N_TABLES * WIN_ENTRIES;
+constexpr int DEV_TAB = PT_I8 + 256;
 constexpr int DEV_RC = DEV_TAB;
 constexpr int DEV_C = DEV_RC + RC_ROWS * MAXM;
 constexpr int DEV_ELEMS = DEV_C + 2;
@@ -220,16 +236,160 @@ struct HostVision {
     }
 };
 
-// Window table of a linear map given by its column images col[j] = L(x^j):
-// row k, entry n = L(n x^(4k)), stored at slot n ^ (k & 15) of the row.
-void window_table(const u128 *col, uint4 *tab) {
-    for (int k = 0; k < 32; k++)
+// Window table of a linear map given by its column images col[j] = L(e_j) for
+// the input bits e_j it reads: row k, entry n = L(n e_(4k)), stored at slot
+// n ^ (k & 15) of the row.
+void window_table(const u128 *col, uint4 *tab, int rows = 32) {
+    for (int k = 0; k < rows; k++)
         for (int n = 0; n < 16; n++) {
             u128 v = 0;
             for (int b = 0; b < 4; b++)
                 if ((n >> b) & 1) v ^= col[4 * k + b];
             tab[k * 16 + (n ^ (k & 15))] = u4(v);
         }
+}
+
+// ---- the subfield chain of the S-box inversion --------------------------------
+// F2-linear maps as their 128 column images.
+struct Lin {
+    u128 col[128];
+    u128 operator()(u128 x) const {
+        u128 r = 0;
+        for (int j = 0; j < 128; j++)
+            if ((x >> j) & 1) r ^= col[j];
+        return r;
+    }
+    Lin then(const Lin &g) const {  // g after this
+        Lin r;
+        for (int j = 0; j < 128; j++) r.col[j] = g(col[j]);
+        return r;
+    }
+};
+
+// A basis of GF(2^d) inside GF(2^128): the kernel of x -> x^(2^d) + x (Gauss
+// elimination over F2, tracking which input bits sum to each reduced vector).
+int subfield_basis(const Lin &frob_d, u128 *basis) {
+    u128 piv_v[128], piv_c[128];
+    bool has[128] = {};
+    int n = 0;
+    for (int j = 0; j < 128; j++) {
+        u128 v = frob_d.col[j] ^ ((u128)1 << j), c = (u128)1 << j;
+        while (v) {
+            int h = 127;
+            while (!((v >> h) & 1)) h--;
+            if (!has[h]) {
+                has[h] = true;
+                piv_v[h] = v;
+                piv_c[h] = c;
+                break;
+            }
+            v ^= piv_v[h];
+            c ^= piv_c[h];
+        }
+        if (!v) basis[n++] = c;
+    }
+    return n;
+}
+
+// Given a basis of a d-dimensional subspace V and a projection p onto d bits,
+// lift[j] = the element of V whose projection is bit j alone.  False when p is
+// not one-to-one on V.
+template <class Proj>
+bool lift_basis(const u128 *basis, int d, Proj p, u128 *lift) {
+    u128 pv[128], el[128];
+    for (int i = 0; i < d; i++) {
+        pv[i] = p(basis[i]);
+        el[i] = basis[i];
+    }
+    for (int j = 0; j < d; j++) {
+        int r = j;
+        while (r < d && !((pv[r] >> j) & 1)) r++;
+        if (r == d) return false;
+        std::swap(pv[r], pv[j]);
+        std::swap(el[r], el[j]);
+        for (int i = 0; i < d; i++)
+            if (i != j && ((pv[i] >> j) & 1)) {
+                pv[i] ^= pv[j];
+                el[i] ^= el[j];
+            }
+    }
+    for (int j = 0; j < d; j++) lift[j] = el[j];
+    return true;
+}
+
+inline u128 proj_w02(u128 x) { return (x & 0xFFFFFFFFu) | (((x >> 64) & 0xFFFFFFFFu) << 32); }
+inline u128 proj_w0(u128 x) { return x & 0xFFFFFFFFu; }
+inline uint32_t idx_i8(uint32_t w0) { return (w0 & 0xFu) | ((w0 >> 12) & 0xF0u); }
+inline u128 proj_i8(u128 x) { return idx_i8((uint32_t)x); }
+
+inline u128 from4(uint4 v) {
+    return ((u128)v.w << 96) | ((u128)v.z << 64) | ((u128)v.y << 32) | v.x;
+}
+// What the device's lookups compute: rows row0.. of a table on the nibbles of w.
+inline u128 rows_apply(const uint4 *tab, int row0, uint32_t w, int rows = 8) {
+    u128 r = 0;
+    for (int k = 0; k < rows; k++) {
+        const int row = row0 + k;
+        r ^= from4(tab[row * 16 + (((w >> (4 * k)) & 15u) ^ (uint32_t)(row & 15))]);
+    }
+    return r;
+}
+
+// The inversion's tables: T_P64 and PT_F32 .. PT_I8 of the layout above, then a
+// check of the chain, exactly as the device runs it, on a few elements.
+bool inversion_tables(uint4 *out) {
+    Lin f8;
+    for (int j = 0; j < 128; j++) {
+        u128 x = (u128)1 << j;
+        for (int k = 0; k < 8; k++) x = host::gf_mul(x, x);
+        f8.col[j] = x;
+    }
+    const Lin f16 = f8.then(f8), f32 = f16.then(f16), f64 = f32.then(f32);
+    window_table(f64.col, out + T_P64 * WIN_ENTRIES);
+    u128 b64[128], b32[128], b8[128], l64[64], l32[32], l8[8];
+    if (subfield_basis(f64, b64) != 64 || subfield_basis(f32, b32) != 32 || subfield_basis(f8, b8) != 8)
+        return false;
+    if (!lift_basis(b64, 64, proj_w02, l64) || !lift_basis(b32, 32, proj_w0, l32) ||
+        !lift_basis(b8, 8, proj_i8, l8))
+        return false;
+    u128 col[64];
+    for (int j = 0; j < 64; j++) col[j] = f32(l64[j]);
+    window_table(col, out + PT_F32, 16);
+    for (int j = 0; j < 32; j++) col[j] = f16(l32[j]);
+    window_table(col, out + PT_F16, 8);
+    // GF(2^16) lies in GF(2^32), so word 0 fixes its elements as well
+    for (int j = 0; j < 32; j++) col[j] = f8(l32[j]);
+    window_table(col, out + PT_F8, 8);
+    for (uint32_t i = 0; i < 256; i++) {
+        u128 v = 0;
+        for (int j = 0; j < 8; j++)
+            if ((i >> j) & 1) v ^= l8[j];
+        // v^254 = v^-1 on GF(2^8)^*
+        u128 inv = 1, sq = v;
+        for (int e = 254; e; e >>= 1) {
+            if (e & 1) inv = host::gf_mul(inv, sq);
+            sq = host::gf_mul(sq, sq);
+        }
+        out[PT_I8 + i] = u4(i ? inv : 0);
+    }
+    u128 a = ((u128)0x9e3779b97f4a7c15ull << 64) | 0xbf58476d1ce4e5b9ull;
+    for (int t = 0; t < 8; t++, a = host::gf_mul(a, a) ^ t) {
+        const u128 t64 = rows_apply(out + T_P64 * WIN_ENTRIES, 0, (uint32_t)a) ^
+                         rows_apply(out + T_P64 * WIN_ENTRIES, 8, (uint32_t)(a >> 32)) ^
+                         rows_apply(out + T_P64 * WIN_ENTRIES, 16, (uint32_t)(a >> 64)) ^
+                         rows_apply(out + T_P64 * WIN_ENTRIES, 24, (uint32_t)(a >> 96));
+        const u128 n1 = host::gf_mul(a, t64);
+        const u128 t32 = rows_apply(out + PT_F32, 0, (uint32_t)n1) ^ rows_apply(out + PT_F32, 8, (uint32_t)(n1 >> 64));
+        const u128 n2 = host::gf_mul(n1, t32);
+        const u128 t16 = rows_apply(out + PT_F16, 0, (uint32_t)n2);
+        const u128 n3 = host::gf_mul(n2, t16);
+        const u128 t8 = rows_apply(out + PT_F8, 0, (uint32_t)n3);
+        const u128 n4 = host::gf_mul(n3, t8);
+        const u128 i8 = from4(out[PT_I8 + idx_i8((uint32_t)n4)]);
+        const u128 inv = host::gf_mul(host::gf_mul(host::gf_mul(host::gf_mul(t64, t32), t16), t8), i8);
+        if (a != 0 && host::gf_mul(a, inv) != 1) return false;
+    }
+    return true;
 }
 
 bool build_device_data(int m, uint4 *out) {
@@ -241,16 +401,8 @@ bool build_device_data(int m, uint4 *out) {
         hv.round_constants(INITIAL4, MATRIX4, CONSTANT4);
     else
         hv.round_constants(INITIAL6, MATRIX6, CONSTANT6);
+    if (!inversion_tables(out)) return false;
     u128 col[128];
-    const int pows[5] = {1, 3, 6, 12, 24};
-    for (int t = 0; t < 5; t++) {
-        for (int j = 0; j < 128; j++) {
-            u128 x = (u128)1 << j;
-            for (int k = 0; k < pows[t]; k++) x = hv.sqr(x);
-            col[j] = x;
-        }
-        window_table(col, out + t * WIN_ENTRIES);
-    }
     for (int j = 0; j < 128; j++) col[j] = hv.b_fwd_linear((u128)1 << j);
     window_table(col, out + T_BFWD * WIN_ENTRIES);
     window_table(hv.binv_cols, out + T_BINV * WIN_ENTRIES);
@@ -416,23 +568,28 @@ __device__ __forceinline__ uint4 gf_mul_inv_x(uint4 v) {
                       (v.w >> 1) ^ (m & 0x80000000u));
 }
 
-// a^(2^128 - 2): b_k = a^(2^k - 1), b_(i+j) = b_i^(2^j) b_j along the
-// addition chain 1, 2, 3, 6, 12, 24, 48, 96, 120, 126, 127 (10 products;
-// x^(2^48) as two x^(2^24) maps), then one squaring.
+// a^-1 for a != 0 down the subfield chain (norms into GF(2^64), GF(2^32),
+// GF(2^16), GF(2^8), a table inverse there, and the Frobenius images multiplied
+// back up).  The subfield maps and the GF(2^8) entry are read by every lane of a
+// state alike; only the full map and the products go through the ops policy.
 template <class Ops>
 __device__ uint4 vis_inv(const Ops &o, uint32_t tabs, uint4 a) {
-    const uint4 b1 = a;
-    const uint4 b2 = o.mul(o.map(tabs, T_P1, b1), b1);
-    const uint4 b3 = o.mul(o.map(tabs, T_P1, b2), b1);
-    const uint4 b6 = o.mul(o.map(tabs, T_P3, b3), b3);
-    const uint4 b12 = o.mul(o.map(tabs, T_P6, b6), b6);
-    const uint4 b24 = o.mul(o.map(tabs, T_P12, b12), b12);
-    const uint4 b48 = o.mul(o.map(tabs, T_P24, b24), b24);
-    const uint4 b96 = o.mul(o.map(tabs, T_P24, o.map(tabs, T_P24, b48)), b48);
-    const uint4 b120 = o.mul(o.map(tabs, T_P24, b96), b24);
-    const uint4 b126 = o.mul(o.map(tabs, T_P6, b120), b6);
-    const uint4 b127 = o.mul(o.map(tabs, T_P1, b126), b1);
-    return o.map(tabs, T_P1, b127);
+    const uint4 z = make_uint4(0, 0, 0, 0);
+    const uint32_t f32 = tabs + PT_F32 * 16, f16 = tabs + PT_F16 * 16, f8 = tabs + PT_F8 * 16;
+    const uint4 t64 = o.map(tabs, T_P64, a);
+    const uint4 n1 = o.mul(a, t64);
+    const uint4 t32 = win_word_perm(f32, n1.z, 8, win_word_perm(f32, n1.x, 0, z));
+    const uint4 n2 = o.mul(n1, t32);
+    const uint4 p32 = o.mul(t64, t32);
+    const uint4 t16 = win_word_perm(f16, n2.x, 0, z);
+    const uint4 n3 = o.mul(n2, t16);
+    const uint4 p16 = o.mul(p32, t16);
+    const uint4 t8 = win_word_perm(f8, n3.x, 0, z);
+    const uint4 n4 = o.mul(n3, t8);
+    const uint4 p8 = o.mul(p16, t8);
+    const uint32_t i8 = tabs + (uint32_t)(PT_I8 * 16) + (((n4.x << 4) & 0xF0u) | ((n4.x >> 8) & 0xF00u));
+    const u32x4 e = *(const lds_cu32x4 *)(uintptr_t)i8;
+    return o.mul(p8, make_uint4(e.x, e.y, e.z, e.w));
 }
 
 // invert_or_zero of every element with one inversion (zeros enter as ones and
@@ -509,7 +666,7 @@ __device__ void vis_permute(const VisionLds &v, uint4 (&s)[M], const Ops &o = Op
     }
 }
 
-// Stages one width's tables and constants into LDS (58 KiB).
+// Stages one width's tables and constants into LDS (37 KiB).
 __device__ VisionLds vis_stage(const uint4 *__restrict__ dev, uint4 *lds) {
     for (int e = threadIdx.x; e < DEV_ELEMS; e += VBLOCK) lds[e] = dev[e];
     __syncthreads();
