@@ -21,13 +21,9 @@ static inline u128 clmul64(uint64_t a, uint64_t b) {
     return r;
 }
 
-// GHASH product (verifier/field/src/ghash.rs:188-195): schoolbook carry-less
-// product, then reduction by x^128 = x^7 + x^2 + x + 1.
-static inline u128 gf_mul(u128 a, u128 b) {
-    const uint64_t a0 = (uint64_t)a, a1 = (uint64_t)(a >> 64);
-    const uint64_t b0 = (uint64_t)b, b1 = (uint64_t)(b >> 64);
-    const u128 lo = clmul64(a0, b0), hi = clmul64(a1, b1);
-    const u128 mid = clmul64(a0, b1) ^ clmul64(a1, b0);
+// Reduction of the 256-bit product (lo, mid at x^64, hi at x^128) by
+// x^128 = x^7 + x^2 + x + 1.
+static inline u128 gf_reduce(u128 lo, u128 mid, u128 hi) {
     uint64_t v0 = (uint64_t)lo, v1 = (uint64_t)(lo >> 64) ^ (uint64_t)mid;
     uint64_t v2 = (uint64_t)hi ^ (uint64_t)(mid >> 64), v3 = (uint64_t)(hi >> 64);
     v1 ^= v3 ^ (v3 << 1) ^ (v3 << 2) ^ (v3 << 7);
@@ -35,6 +31,40 @@ static inline u128 gf_mul(u128 a, u128 b) {
     v0 ^= v2 ^ (v2 << 1) ^ (v2 << 2) ^ (v2 << 7);
     v1 ^= (v2 >> 63) ^ (v2 >> 62) ^ (v2 >> 57);
     return ((u128)v1 << 64) | v0;
+}
+
+#if !defined(__HIP_DEVICE_COMPILE__)
+// The x86 carry-less multiply (PCLMULQDQ), detected once.
+static inline bool have_pclmul() {
+    static const bool have = [] {
+        unsigned a, b, c, d;
+        return __get_cpuid(1, &a, &b, &c, &d) && (c & bit_PCLMUL) != 0;
+    }();
+    return have;
+}
+__attribute__((target("pclmul,sse2"))) static inline u128 gf_mul_pclmul(u128 a, u128 b) {
+    const __m128i x = _mm_set_epi64x((long long)(uint64_t)(a >> 64), (long long)(uint64_t)a);
+    const __m128i y = _mm_set_epi64x((long long)(uint64_t)(b >> 64), (long long)(uint64_t)b);
+    const __m128i l = _mm_clmulepi64_si128(x, y, 0x00), h = _mm_clmulepi64_si128(x, y, 0x11);
+    const __m128i m = _mm_xor_si128(_mm_clmulepi64_si128(x, y, 0x10), _mm_clmulepi64_si128(x, y, 0x01));
+    u128 lo, mid, hi;
+    memcpy(&lo, &l, 16);
+    memcpy(&mid, &m, 16);
+    memcpy(&hi, &h, 16);
+    return gf_reduce(lo, mid, hi);
+}
+#endif
+
+// GHASH product (verifier/field/src/ghash.rs:188-195): schoolbook carry-less
+// product, then reduction by x^128 = x^7 + x^2 + x + 1.  PCLMULQDQ where the
+// CPU has it (as the reference's x86 backend), else a shift-and-XOR loop.
+static inline u128 gf_mul(u128 a, u128 b) {
+#if !defined(__HIP_DEVICE_COMPILE__)
+    if (have_pclmul()) return gf_mul_pclmul(a, b);
+#endif
+    const uint64_t a0 = (uint64_t)a, a1 = (uint64_t)(a >> 64);
+    const uint64_t b0 = (uint64_t)b, b1 = (uint64_t)(b >> 64);
+    return gf_reduce(clmul64(a0, b0), clmul64(a0, b1) ^ clmul64(a1, b0), clmul64(a1, b1));
 }
 
 // a^(2^128 - 2): the inverse, and 0 for 0 (invert_or_zero).
