@@ -139,8 +139,13 @@ struct Stage {
 
 // Slot numbering: 0..15 NTT passes, 16 leaves+fold, 17..30 tree-0 steps, then
 // per folded oracle k (base 32 + 16k): 0 fold_codeword, 1 16-element leaves,
-// 2 other leaves, 3 few long leaves (one block each), 4.. tree steps.
+// 2 other leaves, 4.. tree steps.  Few long leaves (one block each: only a
+// terminal oracle has them, the last stage of its job) and their tree steps go
+// after every oracle's slots, so the jobs' terminal oracles, whatever their k,
+// share one launch: each is a chain of ~33 compressions per leaf that leaves the
+// device nearly idle for ~0.1 ms.
 constexpr int SLOT_LEAF16 = 16, SLOT_TREE0 = 17, SLOT_ORACLE = 32;
+constexpr int SLOT_TERMINAL = SLOT_ORACLE + 16 * (B64_MAX_FOLD_ARITIES + 1);
 
 inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + 255) / 256); }
 
@@ -266,12 +271,14 @@ int oracle_stages(const Layout &L, uint8_t *ws, int k, const uint64_t *chal, std
         st.leaf.digests = tk;
         if (b64::leaf_hash_long_fits(st.leaf.batch, n_leaves)) {
             st.kind = b64::MultiKind::LeafHashLong;
-            st.slot = base + 3;  // one kernel kind per slot
+            st.slot = SLOT_TERMINAL;  // one kernel kind per slot
         }
     }
-    st.blocks = st.kind == b64::MultiKind::LeafHashLong ? (uint32_t)n_leaves : blocks_for(n_leaves);
+    const bool terminal = st.kind == b64::MultiKind::LeafHashLong;
+    st.blocks = terminal ? (uint32_t)n_leaves : blocks_for(n_leaves);
     out.push_back(st);
-    B64_TRY(tree_stages(tk, n_leaves, base + 4, base + 16, out, roots));
+    const int tree_base = terminal ? SLOT_TERMINAL - 3 : base;  // tree steps from tree_base + 4
+    B64_TRY(tree_stages(tk, n_leaves, tree_base + 4, tree_base + 16, out, roots));
     return B64_OK;
 }
 
