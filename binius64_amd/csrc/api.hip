@@ -217,6 +217,10 @@ static bool tuning_table(const char *field, bool *batch, const char **knob) {
 
 int b64_config_set(const char *key, int64_t value) {
     if (!key) return B64_ERR_INVALID_ARGS;
+    if (!strcmp(key, "host.pclmul")) {
+        b64::host::g_pclmul_allowed.store(value != 0);
+        return B64_OK;
+    }
     if (!strcmp(key, "ntt.clear_sizes")) {
         (void)b64::ntt_tuning_for(0);  // initialises the table
         for (auto &t : b64::g_ntt_tuning_size) t.variant = -1;
@@ -264,6 +268,10 @@ int b64_config_set(const char *key, int64_t value) {
 
 int b64_config_get(const char *key, int64_t *value) {
     if (!key || !value) return B64_ERR_INVALID_ARGS;
+    if (!strcmp(key, "host.pclmul")) {
+        *value = b64::host::use_pclmul() ? 1 : 0;
+        return B64_OK;
+    }
     char field[32];
     int log_d;
     bool batch;

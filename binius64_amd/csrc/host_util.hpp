@@ -4,6 +4,8 @@
 #include <stddef.h>
 #include <stdint.h>
 #include <string.h>
+
+#include <atomic>
 #if !defined(__HIP_DEVICE_COMPILE__)
 #include <cpuid.h>
 #include <immintrin.h>
@@ -33,8 +35,11 @@ static inline u128 gf_reduce(u128 lo, u128 mid, u128 hi) {
     return ((u128)v1 << 64) | v0;
 }
 
+// The x86 carry-less multiply (PCLMULQDQ), detected once; the config key
+// "host.pclmul" = 0 turns its use off (the loop below then runs, as on a CPU
+// without it).
+inline std::atomic<bool> g_pclmul_allowed{true};
 #if !defined(__HIP_DEVICE_COMPILE__)
-// The x86 carry-less multiply (PCLMULQDQ), detected once.
 static inline bool have_pclmul() {
     static const bool have = [] {
         unsigned a, b, c, d;
@@ -42,6 +47,15 @@ static inline bool have_pclmul() {
     }();
     return have;
 }
+#endif
+static inline bool use_pclmul() {
+#if !defined(__HIP_DEVICE_COMPILE__)
+    return have_pclmul() && g_pclmul_allowed.load(std::memory_order_relaxed);
+#else
+    return false;
+#endif
+}
+#if !defined(__HIP_DEVICE_COMPILE__)
 __attribute__((target("pclmul,sse2"))) static inline u128 gf_mul_pclmul(u128 a, u128 b) {
     const __m128i x = _mm_set_epi64x((long long)(uint64_t)(a >> 64), (long long)(uint64_t)a);
     const __m128i y = _mm_set_epi64x((long long)(uint64_t)(b >> 64), (long long)(uint64_t)b);
@@ -60,7 +74,7 @@ __attribute__((target("pclmul,sse2"))) static inline u128 gf_mul_pclmul(u128 a, 
 // CPU has it (as the reference's x86 backend), else a shift-and-XOR loop.
 static inline u128 gf_mul(u128 a, u128 b) {
 #if !defined(__HIP_DEVICE_COMPILE__)
-    if (have_pclmul()) return gf_mul_pclmul(a, b);
+    if (use_pclmul()) return gf_mul_pclmul(a, b);
 #endif
     const uint64_t a0 = (uint64_t)a, a1 = (uint64_t)(a >> 64);
     const uint64_t b0 = (uint64_t)b, b1 = (uint64_t)(b >> 64);

@@ -107,8 +107,12 @@ int b64_stream_synchronize(void *stream);
  * for the NTT passes of batch plans: variants 0-1 run every pass in the multi-job
  * LDS-tile kernel (max_k, tile_log, tab_min, tab_group), variants 3-4 run each
  * pass in the multi-job form of the kernel a single-job launch would pick (register
- * passes, the one-round small-table tile), the rest in the table kernel.  Returns INVALID_ARGS for an unknown key or value.  Settings
- * are process-wide; change them only while no call is running. */
+ * passes, the one-round small-table tile), the rest in the table kernel.
+ * "host.pclmul" (default 1) lets the host's field products use PCLMULQDQ where
+ * the CPU has it; 0 selects the portable shift-and-XOR loop (same results);
+ * reading it reports whether PCLMULQDQ is in use.  Returns INVALID_ARGS for an
+ * unknown key or value.  Settings are process-wide; change them only while no
+ * call is running. */
 int b64_config_set(const char *key, int64_t value);
 int b64_config_get(const char *key, int64_t *value);
 

@@ -39,14 +39,27 @@ def test_version_and_status_strings(b64):
     assert b"subspace" in b64.lib().b64_status_string(2)
 
 
-def test_host_field_matches_oracle(b64, orc):
-    a = orc.random_elems(1, 64)
-    b = orc.random_elems(2, 64)
-    want = orc.to_ints(orc.gf_mul(a, b))
-    got = [b64.gf_mul(x, y) for x, y in zip(orc.to_ints(a), orc.to_ints(b))]
-    assert got == want
-    inv = [b64.gf_inv(x) for x in orc.to_ints(a)]
-    assert inv == orc.to_ints(orc.gf_inv(a))
+@pytest.mark.parametrize("pclmul", [1, 0])
+def test_host_field_matches_oracle(b64, orc, pclmul):
+    """The host's products (domain set-up, ring-switch claim, sumcheck Horner) on
+    PCLMULQDQ and on the portable loop ("host.pclmul" 0) both equal the oracle's."""
+    was = b64.config_get("host.pclmul")
+    b64.config_set("host.pclmul", pclmul)
+    try:
+        if pclmul == 0:
+            assert b64.config_get("host.pclmul") == 0
+        a = orc.random_elems(1, 64)
+        b = orc.random_elems(2, 64)
+        edge = [0, 1, 2, 0x87, 1 << 127, (1 << 128) - 1]
+        xs = orc.to_ints(a) + edge + edge
+        ys = orc.to_ints(b) + edge + edge[::-1]
+        want = orc.to_ints(orc.gf_mul(orc.to_elems(xs), orc.to_elems(ys)))
+        got = [b64.gf_mul(x, y) for x, y in zip(xs, ys)]
+        assert got == want
+        inv = [b64.gf_inv(x) for x in orc.to_ints(a)]
+        assert inv == orc.to_ints(orc.gf_inv(a))
+    finally:
+        b64.config_set("host.pclmul", was)
 
 
 @pytest.mark.parametrize("basis", [None, [5, 7, 22, 95, 0x1234, 0xABCDEF0123]])
